@@ -1,0 +1,365 @@
+"""ctypes bindings of the native libraries (include/dhgpu.h, include/diagham_host.h).
+
+The product path is native: importing this module loads ``libdhgpu.so`` (HIP kernels for
+gfx950) and ``libdiagham_host.so`` (C++ host layer) from this directory and raises if they
+are missing.  There is no Python or CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_GPU = os.path.join(_HERE, "libdhgpu.so")
+LIB_HOST = os.path.join(_HERE, "libdiagham_host.so")
+
+DH_OK, DH_E_ARG, DH_E_HIP, DH_E_NOMEM, DH_E_STATE, DH_E_RANGE = 0, -1, -2, -3, -4, -5
+INDEX_BSEARCH, INDEX_RANK = 0, 1
+
+c_int, c_int64, c_double, c_void_p, c_uint64, c_size_t, c_char_p = (
+    ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t,
+    ctypes.c_char_p)
+P_D = ctypes.POINTER(ctypes.c_double)
+P_I = ctypes.POINTER(ctypes.c_int)
+P_I64 = ctypes.POINTER(ctypes.c_int64)
+P_U64 = ctypes.POINTER(ctypes.c_uint64)
+PP = ctypes.POINTER(ctypes.c_void_p)
+
+_gpu = None
+_host = None
+
+# name: (restype, argtypes) of every entry point of include/dhgpu.h
+GPU_SIGNATURES = {
+    "dh_create": (c_int, [PP, c_int]),
+    "dh_destroy": (c_int, [c_void_p]),
+    "dh_last_error": (c_char_p, [c_void_p]),
+    "dh_set_stream": (c_int, [c_void_p, c_void_p]),
+    "dh_synchronize": (c_int, [c_void_p]),
+    "dh_device_info": (c_int, [c_void_p, P_I, c_char_p]),
+    "dh_malloc": (c_int, [c_void_p, PP, c_size_t]),
+    "dh_free": (c_int, [c_void_p, c_void_p]),
+    "dh_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "dh_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "dh_memset": (c_int, [c_void_p, c_void_p, c_int, c_size_t]),
+    "dh_fermion_sphere_create": (c_int, [c_void_p, PP, c_int, c_int, c_int]),
+    "dh_fermion_sphere_dimension": (c_int64, [c_int, c_int, c_int]),
+    "dh_spin_chain_create": (c_int, [c_void_p, PP, c_int, c_int]),
+    "dh_basis_destroy": (c_int, [c_void_p]),
+    "dh_basis_dimension": (c_int64, [c_void_p]),
+    "dh_basis_get_states": (c_int, [c_void_p, P_U64]),
+    "dh_basis_find_state_index": (c_int, [c_void_p, P_U64, c_int64, c_int, P_I64]),
+    "dh_sphere_hamiltonian_create": (c_int, [c_void_p, PP, c_void_p, P_D, P_I, c_int]),
+    "dh_sphere_hamiltonian_set_channels": (c_int, [c_void_p, P_D, c_int, P_D]),
+    "dh_spin_chain_hamiltonian_create": (c_int, [c_void_p, PP, c_void_p, P_D, P_D, c_int]),
+    "dh_hamiltonian_destroy": (c_int, [c_void_p]),
+    "dh_hamiltonian_set_shift": (c_int, [c_void_p, c_double]),
+    "dh_hamiltonian_dimension": (c_int64, [c_void_p]),
+    "dh_hamiltonian_set_launch": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "dh_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int64, c_int64]),
+    "dh_hamiltonian_count_elements": (c_int, [c_void_p, P_I64]),
+    "dh_hamiltonian_row_elements": (c_int, [c_void_p, c_int64, P_I64, P_D, c_int64, P_I64]),
+    "dh_sphere_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, P_D, c_void_p]),
+    "dh_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
+    "dh_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
+    "dh_block_rmul": (c_int, [c_void_p, c_void_p, c_int64, c_int, P_D]),
+    "dh_block_get_column": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
+    "dh_block_set_column": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
+    "dh_axpby": (c_int, [c_void_p, c_double, c_void_p, c_double, c_void_p, c_int64]),
+    "dh_fill_random": (c_int, [c_void_p, c_void_p, c_int64, c_uint64]),
+}
+
+HOST_SIGNATURES = {
+    "dhh_last_error": (c_char_p, []),
+    "dhh_arch_create": (c_int, [PP, c_int]),
+    "dhh_arch_destroy": (c_int, [c_void_p]),
+    "dhh_arch_context": (c_void_p, [c_void_p]),
+    "dhh_sphere_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_D]),
+    "dhh_spin_problem_create": (c_int, [c_void_p, PP, c_int, c_int, P_D, P_D, c_int]),
+    "dhh_problem_destroy": (c_int, [c_void_p]),
+    "dhh_problem_basis": (c_void_p, [c_void_p]),
+    "dhh_problem_hamiltonian": (c_void_p, [c_void_p]),
+    "dhh_problem_channels": (c_int, [c_void_p, P_I, c_int]),
+    "dhh_lanczos": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_uint64, P_D, P_I, P_I, P_D, c_void_p,
+                            c_int]),
+    "dhh_coulomb_pseudopotentials": (c_int, [c_int, c_int, P_D]),
+    "dhh_clebsch_gordan": (c_double, [c_int, c_int, c_int, c_int, c_int]),
+    "dhh_sphere_factors": (c_int, [c_int, P_D, P_D, P_I, P_I]),
+    "dhh_sphere_channel_table": (c_int, [c_int, P_I, P_D, c_int]),
+    "dhh_read_pseudopotentials": (c_int, [c_char_p, c_int, P_D]),
+    "dhh_band_eigenvalues": (c_int, [c_int, c_int, P_D, P_D]),
+}
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def gpu():
+    global _gpu
+    if _gpu is None:
+        if not os.path.exists(LIB_GPU):
+            raise RuntimeError("libdhgpu.so is not built: run `python -m diagham_amd.build`")
+        _gpu = ctypes.CDLL(LIB_GPU, mode=ctypes.RTLD_GLOBAL)
+        _bind(_gpu, GPU_SIGNATURES)
+    return _gpu
+
+
+def host():
+    global _host
+    if _host is None:
+        gpu()
+        if not os.path.exists(LIB_HOST):
+            raise RuntimeError("libdiagham_host.so is not built: run `python -m diagham_amd.build`")
+        _host = ctypes.CDLL(LIB_HOST, mode=ctypes.RTLD_GLOBAL)
+        _bind(_host, HOST_SIGNATURES)
+    return _host
+
+
+class DHError(RuntimeError):
+    pass
+
+
+def dptr(a):
+    return a.ctypes.data_as(P_D)
+
+
+# ---------------------------------------------------------------------------- host-only helpers
+def coulomb_pseudopotentials(nbr_flux, round_to_file=True):
+    out = np.zeros(nbr_flux + 1)
+    host().dhh_coulomb_pseudopotentials(nbr_flux, int(round_to_file), dptr(out))
+    return out
+
+
+def clebsch_gordan(j1, j2, m1, m2, j):
+    return host().dhh_clebsch_gordan(j1, j2, m1, m2, j)
+
+
+def sphere_factors(lz_max, pseudo):
+    pseudo = np.ascontiguousarray(pseudo, dtype=np.float64)
+    n_orb = lz_max + 1
+    cap = n_orb * n_orb * n_orb
+    f = np.zeros(cap)
+    off = np.zeros(n_orb * (n_orb - 1) // 2 + 1, np.int32)
+    n = ctypes.c_int(cap)
+    rc = host().dhh_sphere_factors(lz_max, dptr(pseudo), dptr(f), off.ctypes.data_as(P_I), ctypes.byref(n))
+    if rc != 0:
+        raise DHError("dhh_sphere_factors failed: %d" % rc)
+    return f[:n.value].copy(), off
+
+
+def sphere_channel_table(lz_max):
+    n_orb = lz_max + 1
+    npairs = n_orb * (n_orb - 1) // 2
+    cap = npairs * (lz_max + 1)
+    mom = np.zeros(lz_max + 1, np.int32)
+    tab = np.zeros(cap)
+    nc = host().dhh_sphere_channel_table(lz_max, mom.ctypes.data_as(P_I), dptr(tab), cap)
+    return mom[:nc].copy(), tab[:npairs * nc].reshape(npairs, nc).copy()
+
+
+def band_eigenvalues(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    ev = np.zeros(n)
+    rc = host().dhh_band_eigenvalues(n, b, dptr(a), dptr(ev))
+    if rc != 0:
+        raise DHError(host().dhh_last_error().decode())
+    return ev
+
+
+def fermion_sphere_dimension(n, total_lz, lz_max):
+    return gpu().dh_fermion_sphere_dimension(n, total_lz, lz_max)
+
+
+# ---------------------------------------------------------------------------- GPU objects
+class Arch:
+    """GPUArchitecture: one device context."""
+
+    def __init__(self, device=0):
+        h = c_void_p()
+        rc = host().dhh_arch_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise DHError("dhh_arch_create failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+        self.h = h
+        self.ctx = host().dhh_arch_context(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            host().dhh_arch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise DHError("%s failed (%d): %s" % (what, rc, gpu().dh_last_error(self.ctx).decode()))
+
+    def device_info(self):
+        n = ctypes.c_int()
+        name = ctypes.create_string_buffer(64)
+        self.check(gpu().dh_device_info(self.ctx, ctypes.byref(n), name), "dh_device_info")
+        return n.value, name.value.decode()
+
+    def sync(self):
+        self.check(gpu().dh_synchronize(self.ctx), "dh_synchronize")
+
+
+class DeviceBuffer:
+    """Device memory from dh_malloc holding float64 (or raw bytes)."""
+
+    def __init__(self, arch, n, dtype=np.float64):
+        self.arch = arch
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = c_void_p()
+        arch.check(gpu().dh_malloc(arch.ctx, ctypes.byref(p), max(self.n, 1) * self.dtype.itemsize), "dh_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arch, a):
+        a = np.ascontiguousarray(a)
+        b = cls(arch, a.size, a.dtype)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        self.arch.check(gpu().dh_memcpy_h2d(self.arch.ctx, self.ptr, a.ctypes.data, a.nbytes), "h2d")
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        self.arch.check(gpu().dh_memcpy_d2h(self.arch.ctx, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def zero(self):
+        self.arch.check(gpu().dh_memset(self.arch.ctx, self.ptr, 0, self.n * self.dtype.itemsize), "memset")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            gpu().dh_free(self.arch.ctx, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Problem:
+    """A sector: Hilbert space + Hamiltonian built by the C++ host layer."""
+
+    def __init__(self, arch, handle, kind):
+        self.arch = arch
+        self.h = handle
+        self.kind = kind
+        self.ham = host().dhh_problem_hamiltonian(handle)
+        self.basis = host().dhh_problem_basis(handle)
+        self.dim = gpu().dh_basis_dimension(self.basis)
+
+    @classmethod
+    def sphere(cls, arch, n, total_lz, lz_max, pseudo):
+        pseudo = np.ascontiguousarray(pseudo, dtype=np.float64)
+        assert pseudo.size >= lz_max + 1
+        h = c_void_p()
+        rc = host().dhh_sphere_problem_create(arch.h, ctypes.byref(h), n, total_lz, lz_max, dptr(pseudo))
+        if rc != 0:
+            raise DHError("sphere problem: " + host().dhh_last_error().decode())
+        p = cls(arch, h, "sphere")
+        p.n, p.total_lz, p.lz_max = n, total_lz, lz_max
+        return p
+
+    @classmethod
+    def spin_chain(cls, arch, length, sz, j=1.0, jz=1.0, periodic=True):
+        J = np.full(length, j, np.float64) if np.isscalar(j) else np.ascontiguousarray(j, np.float64)
+        Jz = np.full(length, jz, np.float64) if np.isscalar(jz) else np.ascontiguousarray(jz, np.float64)
+        h = c_void_p()
+        rc = host().dhh_spin_problem_create(arch.h, ctypes.byref(h), length, sz, dptr(J), dptr(Jz), int(periodic))
+        if rc != 0:
+            raise DHError("spin problem: " + host().dhh_last_error().decode())
+        p = cls(arch, h, "spin")
+        p.length = length
+        return p
+
+    def close(self):
+        if getattr(self, "h", None):
+            host().dhh_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- basis
+    def states(self):
+        out = np.zeros(self.dim, np.uint64)
+        self.arch.check(gpu().dh_basis_get_states(self.basis, out.ctypes.data_as(P_U64)), "get_states")
+        return out
+
+    def find_state_index(self, states, mode=INDEX_RANK):
+        states = np.ascontiguousarray(states, dtype=np.uint64)
+        out = np.zeros(states.size, np.int64)
+        self.arch.check(gpu().dh_basis_find_state_index(self.basis, states.ctypes.data_as(P_U64), states.size,
+                                                        mode, out.ctypes.data_as(P_I64)), "find_state_index")
+        return out
+
+    # -- Hamiltonian
+    def set_shift(self, shift):
+        self.arch.check(gpu().dh_hamiltonian_set_shift(self.ham, shift), "set_shift")
+
+    def set_launch(self, lanes_per_row=1, block_size=256, index_mode=INDEX_RANK, lds_tables=1):
+        self.arch.check(gpu().dh_hamiltonian_set_launch(self.ham, lanes_per_row, block_size, index_mode,
+                                                        lds_tables), "set_launch")
+
+    def hv_device(self, d_x, d_y, k=1, add=False, first=0, count=-1):
+        self.arch.check(gpu().dh_hv(self.ham, d_x, d_y, k, int(add), first, count), "dh_hv")
+
+    def hv(self, x, k=1):
+        """H x for host arrays (x: dim*k, [row][k] layout); convenience for tests."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        dx = DeviceBuffer.from_host(self.arch, x)
+        dy = DeviceBuffer(self.arch, x.size)
+        self.hv_device(dx.ptr, dy.ptr, k)
+        return dy.download()
+
+    def count_elements(self):
+        n = ctypes.c_int64()
+        self.arch.check(gpu().dh_hamiltonian_count_elements(self.ham, ctypes.byref(n)), "count_elements")
+        return n.value
+
+    def row_elements(self, row, max_out=100000):
+        t = np.zeros(max_out, np.int64)
+        c = np.zeros(max_out)
+        n = ctypes.c_int64()
+        self.arch.check(gpu().dh_hamiltonian_row_elements(self.ham, row, t.ctypes.data_as(P_I64), dptr(c),
+                                                          max_out, ctypes.byref(n)), "row_elements")
+        assert n.value <= max_out
+        return t[:n.value].copy(), c[:n.value].copy()
+
+    def channels(self):
+        mom = np.zeros(64, np.int32)
+        n = host().dhh_problem_channels(self.h, mom.ctypes.data_as(P_I), 64)
+        return mom[:n].copy()
+
+    def adjoint_device(self, d_psi, d_phi, d_hpsi=None):
+        nc = len(self.channels())
+        g = np.zeros(nc)
+        self.arch.check(gpu().dh_sphere_adjoint(self.ham, d_psi, d_phi, dptr(g), d_hpsi), "dh_sphere_adjoint")
+        return g
+
+    # -- Lanczos
+    def lanczos(self, nbr_eigen, block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000,
+                d_vectors=None, nbr_vectors=0, allow_unconverged=False):
+        ev = np.zeros(nbr_eigen)
+        prods, blocks, secs = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = host().dhh_lanczos(self.h, nbr_eigen, block_size, max_iter, precision, seed, dptr(ev),
+                                ctypes.byref(prods), ctypes.byref(blocks), ctypes.byref(secs), d_vectors,
+                                nbr_vectors)
+        if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
+            raise DHError("dhh_lanczos failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+        return {"eigenvalues": ev, "products": prods.value, "blocks": blocks.value, "seconds": secs.value,
+                "converged": rc == 0}

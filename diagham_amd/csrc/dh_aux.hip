@@ -1,0 +1,401 @@
+// Index lookup, matrix-element enumeration and the adjoint (bilinear) kernel.
+#include <algorithm>
+#include <cstring>
+
+#include "dh_index.h"
+#include "dh_internal.h"
+
+// ----------------------------------------------------------------------------- FindStateIndex
+__global__ void fs_find_kernel(const uint64_t* __restrict__ in, int64_t n, int64_t* __restrict__ out,
+                               const uint64_t* __restrict__ states, int64_t dim, int nbrOrbitals, int N, int S,
+                               int mode, const int* __restrict__ lut, int lutShift, const uint32_t* __restrict__ lo,
+                               const uint32_t* __restrict__ hi, int rankBits) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t s = in[i];
+  int sum = 0;
+  for (uint64_t r = s; r; r &= r - 1) sum += __builtin_ctzll(r);
+  bool valid = (nbrOrbitals >= 64 || (s >> nbrOrbitals) == 0) && __popcll(s) == N && sum == S;
+  if (!valid) {
+    out[i] = dim;
+    return;
+  }
+  out[i] = (mode == DH_INDEX_BSEARCH) ? dh_find_bsearch(s, lut, lutShift, states) : dh_find_rank(s, lo, hi, rankBits, dim);
+}
+
+__global__ void spin_find_kernel(const uint64_t* __restrict__ in, int64_t n, int64_t* __restrict__ out,
+                                 const uint32_t* __restrict__ words, int64_t dim, int L, int nbrUp, int mode,
+                                 const int* __restrict__ lut, int lutShift, const uint32_t* __restrict__ binom) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t w = in[i];
+  uint32_t mask = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
+  uint32_t s = (uint32_t)w & mask;
+  // reference words keep the bits L..31 at 1 and nothing above bit 31
+  bool valid = (w >> 32) == 0 && (((uint32_t)w | mask) == 0xffffffffu) && __popc(s) == nbrUp;
+  if (!valid) {
+    out[i] = dim;
+    return;
+  }
+  out[i] = (mode == DH_INDEX_BSEARCH) ? dh_spin_find_bsearch(s, L, lut, lutShift, words)
+                                     : dh_spin_find_rank(s, L, binom, dim);
+}
+
+extern "C" int dh_basis_find_state_index(dh_basis* b, const uint64_t* states, int64_t n, int mode, int64_t* out) {
+  if (!b || (n && (!states || !out)) || n < 0) return DH_E_ARG;
+  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK) return dh_fail(b->ctx, DH_E_ARG, "mode");
+  if (n == 0) return DH_OK;
+  dh_context* c = b->ctx;
+  uint64_t* d_in;
+  int64_t* d_out;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&d_in, sizeof(uint64_t) * n))) return rc;
+  if ((rc = dh_malloc(c, (void**)&d_out, sizeof(int64_t) * n))) {
+    hipFree(d_in);
+    return rc;
+  }
+  hipMemcpyAsync(d_in, states, sizeof(uint64_t) * n, hipMemcpyHostToDevice, c->stream);
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (b->kind == DH_BASIS_FERMION_SPHERE)
+    hipLaunchKernelGGL(fs_find_kernel, dim3(blocks), dim3(256), 0, c->stream, d_in, n, d_out, b->d_states, b->dim,
+                       b->nbrOrbitals, b->nbrFermions, b->orbitalSum, mode, b->d_lut, b->lutShift, b->d_rankLo,
+                       b->d_rankHi, b->rankBits);
+  else
+    hipLaunchKernelGGL(spin_find_kernel, dim3(blocks), dim3(256), 0, c->stream, d_in, n, d_out, b->d_words, b->dim,
+                       b->chainLength, b->nbrUp, mode, b->d_lut, b->lutShift, b->d_binom);
+  hipMemcpyAsync(out, d_out, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_in);
+  hipFree(d_out);
+  if (e != hipSuccess) return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+  return DH_OK;
+}
+
+// ----------------------------------------------------------------------------- element enumeration
+// mode 0: count elements per row (out_count[row]); mode 1: list the elements of one row
+__global__ void fs_elements_kernel(const uint64_t* __restrict__ states, int64_t dim, int nbrOrbitals,
+                                   const double* __restrict__ factors, const int* __restrict__ pairOffset,
+                                   const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, int rankBits,
+                                   int64_t rowBegin, int64_t rowEnd, int64_t* __restrict__ counts,
+                                   int64_t* __restrict__ targets, double* __restrict__ coefs, int64_t maxOut) {
+  int64_t row = rowBegin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rowEnd) return;
+  uint64_t s = states[row];
+  int64_t n = 0;
+  for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+    int n1 = __builtin_ctzll(r1);
+    int pc1 = __popcll(s >> n1);
+    for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+      int n2 = __builtin_ctzll(r2);
+      int sgnA = pc1 + __popcll(s >> n2);
+      uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+      int sum = n1 + n2;
+      int m3lo = (sum >> 1) + 1;
+      int m3hi = min(sum, nbrOrbitals - 1);
+      int fbase = pairOffset[(n1 * (n1 - 1)) / 2 + n2] - m3lo;
+      for (int m3 = m3lo; m3 <= m3hi; ++m3) {
+        int m4 = sum - m3;
+        uint64_t cm = (1ull << m3) | (1ull << m4);
+        if (t & cm) continue;
+        if (targets) {
+          int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          double coef = factors[fbase + m3];
+          if (n < maxOut) {
+            targets[n] = dh_find_rank(t | cm, lo, hi, rankBits, dim);
+            coefs[n] = (sg & 1) ? -coef : coef;
+          }
+        }
+        ++n;
+      }
+    }
+  }
+  if (counts) counts[row - rowBegin] = n;
+  if (targets && row == rowBegin) counts[0] = n;
+}
+
+__global__ void spin_elements_kernel(const uint32_t* __restrict__ words, int64_t dim, int L, int periodic,
+                                     const uint32_t* __restrict__ binom, const double* __restrict__ halfJ,
+                                     int64_t rowBegin, int64_t rowEnd, int64_t* __restrict__ counts,
+                                     int64_t* __restrict__ targets, double* __restrict__ coefs, int64_t maxOut,
+                                     double diagValue) {
+  int64_t row = rowBegin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rowEnd) return;
+  uint32_t mask = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
+  uint32_t s = words[row] & mask;
+  int64_t n = 0;
+  if (targets && n < maxOut) {
+    targets[n] = row;
+    coefs[n] = diagValue;
+  }
+  ++n;  // the diagonal element
+  for (int i = 0; i < L - 1; ++i) {
+    uint32_t pr = (s >> i) & 3u;
+    if (pr == 1u || pr == 2u) {
+      if (targets && n < maxOut) {
+        targets[n] = dh_spin_find_rank(s ^ (3u << i), L, binom, dim);
+        coefs[n] = halfJ[i];
+      }
+      ++n;
+    }
+  }
+  if (periodic && (((s >> (L - 1)) ^ s) & 1u)) {
+    if (targets && n < maxOut) {
+      targets[n] = dh_spin_find_rank(s ^ (1u | (1u << (L - 1))), L, binom, dim);
+      coefs[n] = halfJ[L - 1];
+    }
+    ++n;
+  }
+  if (counts) counts[row - rowBegin] = n;
+  if (targets && row == rowBegin) counts[0] = n;
+}
+
+__global__ void sum_i64_kernel(const int64_t* __restrict__ v, int64_t n, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[1024 / 64];
+  unsigned long long s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += (unsigned long long)v[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (unsigned w = 0; w < blockDim.x / 64; ++w) t += red[w];
+    atomicAdd(out, t);
+  }
+}
+
+static double spin_diag_value(const dh_hamiltonian* h, uint32_t s) {
+  int L = h->basis->chainLength;
+  double d = h->shift;
+  for (int b = 0; b < L - 1; ++b) d += h->Jz[b] * ((((s >> b) ^ (s >> (b + 1))) & 1u) ? -0.25 : 0.25);
+  if (h->periodic) d += h->Jz[L - 1] * ((((s >> (L - 1)) ^ s) & 1u) ? -0.25 : 0.25);
+  return d;
+}
+
+extern "C" int dh_hamiltonian_count_elements(dh_hamiltonian* h, int64_t* nnz) {
+  if (!h || !nnz) return DH_E_ARG;
+  dh_context* c = h->ctx;
+  dh_basis* b = h->basis;
+  const int64_t chunk = 1 << 24;
+  int64_t* d_counts;
+  unsigned long long* d_total;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::min<int64_t>(chunk, b->dim)))) return rc;
+  if ((rc = dh_malloc(c, (void**)&d_total, sizeof(unsigned long long)))) {
+    hipFree(d_counts);
+    return rc;
+  }
+  hipMemsetAsync(d_total, 0, sizeof(unsigned long long), c->stream);
+  double* d_halfJ = nullptr;
+  if (h->kind == DH_HAM_SPIN) {
+    double hj[32];
+    for (int i = 0; i < 32; ++i) hj[i] = 0.5 * h->J[i];
+    dh_malloc(c, (void**)&d_halfJ, sizeof(hj));
+    hipMemcpyAsync(d_halfJ, hj, sizeof(hj), hipMemcpyHostToDevice, c->stream);
+  }
+  for (int64_t r0 = 0; r0 < b->dim; r0 += chunk) {
+    int64_t r1 = std::min(b->dim, r0 + chunk);
+    unsigned blocks = (unsigned)((r1 - r0 + 255) / 256);
+    if (h->kind == DH_HAM_SPHERE)
+      hipLaunchKernelGGL(fs_elements_kernel, dim3(blocks), dim3(256), 0, c->stream, b->d_states, b->dim,
+                         b->nbrOrbitals, h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi, b->rankBits, r0, r1,
+                         d_counts, (int64_t*)nullptr, (double*)nullptr, (int64_t)0);
+    else
+      hipLaunchKernelGGL(spin_elements_kernel, dim3(blocks), dim3(256), 0, c->stream, b->d_words, b->dim,
+                         b->chainLength, h->periodic, b->d_binom, d_halfJ, r0, r1, d_counts, (int64_t*)nullptr,
+                         (double*)nullptr, (int64_t)0, 0.0);
+    hipLaunchKernelGGL(sum_i64_kernel, dim3(256), dim3(256), 0, c->stream, d_counts, r1 - r0, d_total);
+  }
+  unsigned long long total = 0;
+  hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_counts);
+  hipFree(d_total);
+  if (d_halfJ) hipFree(d_halfJ);
+  if (e != hipSuccess) return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+  *nnz = (int64_t)total;
+  return DH_OK;
+}
+
+extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64_t* targets, double* coefs,
+                                           int64_t maxOut, int64_t* n) {
+  if (!h || !n || (maxOut > 0 && (!targets || !coefs)) || maxOut < 0) return DH_E_ARG;
+  dh_context* c = h->ctx;
+  dh_basis* b = h->basis;
+  if (row < 0 || row >= b->dim) return dh_fail(c, DH_E_ARG, "row outside the space");
+  int64_t cap = std::max<int64_t>(maxOut, 1);
+  int64_t *d_t, *d_n;
+  double* d_c;
+  double* d_halfJ = nullptr;
+  dh_malloc(c, (void**)&d_t, sizeof(int64_t) * cap);
+  dh_malloc(c, (void**)&d_c, sizeof(double) * cap);
+  dh_malloc(c, (void**)&d_n, sizeof(int64_t));
+  if (h->kind == DH_HAM_SPHERE) {
+    hipLaunchKernelGGL(fs_elements_kernel, dim3(1), dim3(64), 0, c->stream, b->d_states, b->dim, b->nbrOrbitals,
+                       h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi, b->rankBits, row, row + 1, d_n, d_t,
+                       d_c, maxOut);
+  } else {
+    double hj[32];
+    for (int i = 0; i < 32; ++i) hj[i] = 0.5 * h->J[i];
+    dh_malloc(c, (void**)&d_halfJ, sizeof(hj));
+    hipMemcpyAsync(d_halfJ, hj, sizeof(hj), hipMemcpyHostToDevice, c->stream);
+    uint64_t w = 0;
+    hipMemcpyAsync(&w, b->d_words + row, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    uint32_t mask = (b->chainLength == 32) ? 0xffffffffu : ((1u << b->chainLength) - 1u);
+    double dv = spin_diag_value(h, (uint32_t)w & mask);
+    hipLaunchKernelGGL(spin_elements_kernel, dim3(1), dim3(64), 0, c->stream, b->d_words, b->dim, b->chainLength,
+                       h->periodic, b->d_binom, d_halfJ, row, row + 1, d_n, d_t, d_c, maxOut, dv);
+  }
+  int64_t cnt = 0;
+  hipMemcpyAsync(&cnt, d_n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  int64_t m = std::min(cnt, maxOut);
+  if (m > 0) {
+    hipMemcpyAsync(targets, d_t, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(coefs, d_c, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_t);
+  hipFree(d_c);
+  hipFree(d_n);
+  if (d_halfJ) hipFree(d_halfJ);
+  if (e != hipSuccess) return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+  *n = cnt;
+  return DH_OK;
+}
+
+// ----------------------------------------------------------------------------- adjoint
+// With the interaction split by channel (F(a->b) = sum_c V_c (-4) cg[a][c] cg[b][c]):
+//   g[c] = sum_r psi[r] (H_c phi)[r],   H_c phi gathered like the product,
+// accumulated per row in registers, reduced per block; optionally hpsi = H psi in the
+// same pass (the vector-Jacobian product of x -> H x).
+template <int NC>
+__global__ void __launch_bounds__(256) sphere_adjoint_kernel(const uint64_t* __restrict__ states, int64_t dim,
+                                                             int nbrOrbitals, int nbrPairs, int nbrChannels,
+                                                             const double* __restrict__ cg,
+                                                             const double* __restrict__ factors,
+                                                             const int* __restrict__ pairOffset,
+                                                             const uint32_t* __restrict__ lo,
+                                                             const uint32_t* __restrict__ hi, int rankBits,
+                                                             const double* __restrict__ psi,
+                                                             const double* __restrict__ phi, double shift,
+                                                             double* __restrict__ hpsi, double* __restrict__ partial) {
+  extern __shared__ double sCg[];  // [nbrPairs * nbrChannels]
+  __shared__ double red[256 / 64][NC];
+  for (int i = threadIdx.x; i < nbrPairs * nbrChannels; i += blockDim.x) sCg[i] = cg[i];
+  __syncthreads();
+  double g[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) g[c] = 0.0;
+  const int64_t nbrTiles = (dim + blockDim.x - 1) / blockDim.x;
+  for (int64_t tile = blockIdx.x; tile < nbrTiles; tile += gridDim.x) {
+    int64_t row = tile * blockDim.x + threadIdx.x;
+    if (row >= dim) continue;
+    uint64_t s = states[row];
+    double h[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) h[c] = 0.0;
+    double hp = 0.0;
+    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+      int n1 = __builtin_ctzll(r1);
+      int pc1 = __popcll(s >> n1);
+      for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+        int n2 = __builtin_ctzll(r2);
+        int sgnA = pc1 + __popcll(s >> n2);
+        uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+        int sum = n1 + n2;
+        int m3lo = (sum >> 1) + 1;
+        int m3hi = min(sum, nbrOrbitals - 1);
+        int pa = (n1 * (n1 - 1)) / 2 + n2;
+        int fbase = pairOffset[pa] - m3lo;
+        double w[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) w[c] = 0.0;
+        for (int m3 = m3lo; m3 <= m3hi; ++m3) {
+          int m4 = sum - m3;
+          uint64_t cm = (1ull << m3) | (1ull << m4);
+          if (t & cm) continue;
+          int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          int64_t idx = dh_find_rank(t | cm, lo, hi, rankBits, dim);
+          double p = phi[idx];
+          if (sg & 1) p = -p;
+          const double* cb = sCg + ((m3 * (m3 - 1)) / 2 + m4) * nbrChannels;
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            if (c < nbrChannels) w[c] += cb[c] * p;
+          if (hpsi) {
+            double q = psi[idx];
+            hp += ((sg & 1) ? -factors[fbase + m3] : factors[fbase + m3]) * q;
+          }
+        }
+        const double* ca = sCg + pa * nbrChannels;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (c < nbrChannels) h[c] += ca[c] * w[c];
+      }
+    }
+    double pr = psi[row];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) g[c] += -4.0 * pr * h[c];
+    if (hpsi) hpsi[row] = hp + shift * pr;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double v = g[c];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][threadIdx.x];
+    partial[(int64_t)blockIdx.x * NC + threadIdx.x] = s;
+  }
+}
+
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nbrBlocks, int width,
+                                       double* __restrict__ out) {
+  int c = threadIdx.x;
+  if (c >= width) return;
+  double s = 0.0;
+  for (int b = 0; b < nbrBlocks; ++b) s += partial[(int64_t)b * width + c];
+  out[c] = s;
+}
+
+template <int NC>
+static int adjoint_launch(dh_hamiltonian* h, const double* psi, const double* phi, double* g, double* hpsi) {
+  dh_context* c = h->ctx;
+  dh_basis* b = h->basis;
+  size_t lds = sizeof(double) * h->nbrPairs * h->nbrChannels;
+  if (lds > 150 * 1024) return dh_fail(c, DH_E_RANGE, "channel table exceeds LDS");
+  auto kern = sphere_adjoint_kernel<NC>;
+  if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int perCU = 1;
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 256, lds);
+  int64_t tiles = (b->dim + 255) / 256;
+  int nb = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * std::max(perCU, 1)));
+  double* scratch;
+  int rc = dh_scratch(c, sizeof(double) * ((size_t)nb * NC + NC), &scratch);
+  if (rc) return rc;
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, c->stream, b->d_states, b->dim, b->nbrOrbitals, h->nbrPairs,
+                     h->nbrChannels, h->d_cg, h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi, b->rankBits,
+                     psi, phi, h->shift, hpsi, scratch);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(64), 0, c->stream, scratch, nb, NC, scratch + (size_t)nb * NC);
+  DH_HIP(c, hipGetLastError());
+  double out[NC];
+  DH_HIP(c, hipMemcpyAsync(out, scratch + (size_t)nb * NC, sizeof(double) * NC, hipMemcpyDeviceToHost, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < h->nbrChannels; ++i) g[i] = out[i];
+  return DH_OK;
+}
+
+extern "C" int dh_sphere_adjoint(dh_hamiltonian* h, const double* psi, const double* phi, double* g, double* hpsi) {
+  if (!h || !psi || !phi || !g) return DH_E_ARG;
+  if (h->kind != DH_HAM_SPHERE) return dh_fail(h->ctx, DH_E_ARG, "not a sphere Hamiltonian");
+  if (!h->d_cg || h->nbrChannels <= 0) return dh_fail(h->ctx, DH_E_STATE, "channels not set");
+  if (h->nbrChannels <= 8) return adjoint_launch<8>(h, psi, phi, g, hpsi);
+  if (h->nbrChannels <= 16) return adjoint_launch<16>(h, psi, phi, g, hpsi);
+  return adjoint_launch<32>(h, psi, phi, g, hpsi);
+}

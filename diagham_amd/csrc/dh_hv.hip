@@ -1,0 +1,477 @@
+// On-the-fly Hamiltonian-vector products (gather form) for gfx950.
+//
+// Reference loops replaced:
+//   fermions on the sphere: AbstractQHEOnSphereHamiltonian::LowLevelAddMultiply / HermitianLowLevelAddMultiply /
+//     LowLevelMultipleAddMultiply (FQHE/src/Hamiltonian/AbstractQHEOnSphereHamiltonian.cc:410, :2563, :1217),
+//     with FermionOnSphere::AA (:790), AdAd (:949) and FindStateIndex (:1431) inlined;
+//   spin-1/2 chain: SpinChainHamiltonian::LowLevelAddMultiply (Spin/src/Hamiltonian/SpinChainHamiltonian.cc:257)
+//     with Spin1_2Chain::SmiSpj (:655) and SziSzj (:561).
+//
+// The reference scatters (y[target] += H[target,i] x[i]).  These kernels gather instead:
+// H is symmetric, so row r of H is the list of terms generated from state r, and
+//   y[r] = sum_terms coef * x[target] + shift * x[r]
+// is written once per destination row, with no atomics.  One row per group of
+// `LPR` lanes (LPR = 1: one destination row per lane); the state word stays in
+// registers, fermionic signs come from popcounts, factors are staged in LDS.
+#include <algorithm>
+#include <cstring>
+
+#include "dh_index.h"
+#include "dh_internal.h"
+
+enum { MODE_BSEARCH = 0, MODE_RANK_LDS = 1, MODE_RANK_GLOBAL = 2 };
+
+struct SphereArgs {
+  const uint64_t* states;
+  int64_t dim;
+  int nbrOrbitals, nbrFactors, nbrPairs;
+  const double* factors;
+  const int* pairOffset;
+  const int* lut;
+  int lutShift, lutSize, lutStride;  // staged LUT entry i = lut[i * lutStride]
+  const uint32_t* rankLo;
+  const uint32_t* rankHi;
+  int rankBits, rankLoSize, rankHiSize;
+  double shift;
+};
+
+// ----------------------------------------------------------------------------- sphere H.v
+template <int K>
+__device__ __forceinline__ void gather_fma(double* acc, const double* __restrict__ x, int64_t idx, double coef) {
+  const double* xr = x + idx * K;
+  if constexpr (K % 2 == 0) {
+#pragma unroll
+    for (int v = 0; v < K; v += 2) {
+      double2 p = *reinterpret_cast<const double2*>(xr + v);
+      acc[v] += coef * p.x;
+      acc[v + 1] += coef * p.y;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] += coef * xr[v];
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void sphere_stage(const SphereArgs& a, double*& sF, int*& sOff, int*& sLut,
+                                             uint32_t*& sLo, uint32_t*& sHi) {
+  extern __shared__ double smem[];
+  sF = smem;
+  sOff = reinterpret_cast<int*>(sF + a.nbrFactors);
+  int* tail = sOff + a.nbrPairs + 1;
+  for (int i = threadIdx.x; i < a.nbrFactors; i += blockDim.x) sF[i] = a.factors[i];
+  for (int i = threadIdx.x; i <= a.nbrPairs; i += blockDim.x) sOff[i] = a.pairOffset[i];
+  if constexpr (MODE == MODE_BSEARCH) {
+    sLut = tail;
+    for (int i = threadIdx.x; i < a.lutSize; i += blockDim.x) sLut[i] = a.lut[(int64_t)i * a.lutStride];
+  } else if constexpr (MODE == MODE_RANK_LDS) {
+    sLo = reinterpret_cast<uint32_t*>(tail);
+    sHi = sLo + a.rankLoSize;
+    for (int i = threadIdx.x; i < a.rankLoSize; i += blockDim.x) sLo[i] = a.rankLo[i];
+    for (int i = threadIdx.x; i < a.rankHiSize; i += blockDim.x) sHi[i] = a.rankHi[i];
+  } else {
+    sLo = const_cast<uint32_t*>(a.rankLo);
+    sHi = const_cast<uint32_t*>(a.rankHi);
+  }
+  __syncthreads();
+}
+
+template <int MODE>
+__device__ __forceinline__ int64_t sphere_find(const SphereArgs& a, uint64_t tgt, const int* sLut,
+                                               const uint32_t* sLo, const uint32_t* sHi) {
+  if constexpr (MODE == MODE_BSEARCH)
+    return dh_find_bsearch(tgt, sLut, a.lutShift, a.states);
+  else
+    return dh_find_rank(tgt, sLo, sHi, a.rankBits, a.dim);
+}
+
+template <int K, int LPR, int MODE>
+__global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const double* __restrict__ x,
+                                                         double* __restrict__ y, int add, int64_t first,
+                                                         int64_t count) {
+  double* sF;
+  int *sOff, *sLut = nullptr;
+  uint32_t *sLo = nullptr, *sHi = nullptr;
+  sphere_stage<MODE>(a, sF, sOff, sLut, sLo, sHi);
+  const int lane = threadIdx.x % LPR;
+  const int64_t rowsPerBlock = blockDim.x / LPR;
+  const int64_t nbrTiles = (count + rowsPerBlock - 1) / rowsPerBlock;
+  const unsigned vb = dh_xcd_block(blockIdx.x, gridDim.x);
+  for (int64_t tile = vb; tile < nbrTiles; tile += gridDim.x) {
+    const int64_t local = tile * rowsPerBlock + threadIdx.x / LPR;
+    const bool active = local < count;
+    const int64_t row = first + (active ? local : 0);
+    const uint64_t s = active ? a.states[row] : 0ull;
+    double acc[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] = 0.0;
+    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+      const int n1 = __builtin_ctzll(r1);
+      const int pc1 = __popcll(s >> n1);
+      for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+        const int n2 = __builtin_ctzll(r2);
+        const int sgnA = pc1 + __popcll(s >> n2);
+        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+        const int sum = n1 + n2;
+        const int m3lo = (sum >> 1) + 1;
+        const int m3hi = min(sum, a.nbrOrbitals - 1);
+        const int fbase = sOff[(n1 * (n1 - 1)) / 2 + n2] - m3lo;
+        for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
+          const int m4 = sum - m3;
+          const uint64_t cm = (1ull << m3) | (1ull << m4);
+          if (t & cm) continue;
+          const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          const int64_t idx = sphere_find<MODE>(a, t | cm, sLut, sLo, sHi);
+          double coef = sF[fbase + m3];
+          if (sg & 1) coef = -coef;
+          gather_fma<K>(acc, x, idx, coef);
+        }
+      }
+    }
+    if constexpr (LPR > 1) {
+#pragma unroll
+      for (int v = 0; v < K; ++v)
+        for (int o = LPR / 2; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o, LPR);
+    }
+    if (active && lane == 0) {
+      const double* xr = x + row * K;
+      double* yr = y + row * K;
+#pragma unroll
+      for (int v = 0; v < K; ++v) yr[v] = (add ? yr[v] : 0.0) + acc[v] + a.shift * xr[v];
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------- spin H.v
+struct SpinArgs {
+  const uint32_t* words;
+  int64_t dim;
+  int L, periodic;
+  const uint32_t* binom;
+  const int* lut;
+  int lutShift, lutSize;
+  double shift;
+  double halfJ[32], Jz[32];
+};
+
+template <int K, int MODE>
+__global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double* __restrict__ x,
+                                                       double* __restrict__ y, int add, int64_t first,
+                                                       int64_t count) {
+  __shared__ uint32_t sBinom[33 * 33];
+  extern __shared__ int sLut[];
+  for (int i = threadIdx.x; i < 33 * 33; i += blockDim.x) sBinom[i] = a.binom[i];
+  if constexpr (MODE == MODE_BSEARCH)
+    for (int i = threadIdx.x; i < a.lutSize; i += blockDim.x) sLut[i] = a.lut[i];
+  __syncthreads();
+  const int L = a.L;
+  const uint32_t mask = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
+  const int64_t nbrTiles = (count + blockDim.x - 1) / blockDim.x;
+  const unsigned vb = dh_xcd_block(blockIdx.x, gridDim.x);
+  for (int64_t tile = vb; tile < nbrTiles; tile += gridDim.x) {
+    const int64_t local = tile * blockDim.x + threadIdx.x;
+    if (local >= count) continue;
+    const int64_t row = first + local;
+    const uint32_t s = a.words[row] & mask;
+    double diag = a.shift;
+    for (int b = 0; b < L - 1; ++b) diag += a.Jz[b] * ((((s >> b) ^ (s >> (b + 1))) & 1u) ? -0.25 : 0.25);
+    if (a.periodic) diag += a.Jz[L - 1] * ((((s >> (L - 1)) ^ s) & 1u) ? -0.25 : 0.25);
+    double acc[K];
+    {
+      const double* xr = x + row * K;
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v] = diag * xr[v];
+    }
+    for (int i = 0; i < L - 1; ++i) {
+      const uint32_t pr = (s >> i) & 3u;
+      if (pr == 1u || pr == 2u) {
+        int64_t idx;
+        if constexpr (MODE == MODE_BSEARCH) {
+          idx = dh_spin_find_bsearch(s ^ (3u << i), L, sLut, a.lutShift, a.words);
+        } else {
+          const int kc = __popc(s >> i);
+          const int64_t d = sBinom[(L - 2 - i) * 33 + (kc - 1)];
+          idx = (pr == 1u) ? row + d : row - d;
+          idx = idx < 0 ? 0 : (idx >= a.dim ? a.dim - 1 : idx);
+        }
+        gather_fma<K>(acc, x, idx, a.halfJ[i]);
+      }
+    }
+    if (a.periodic && (((s >> (L - 1)) ^ s) & 1u)) {
+      const uint32_t t = s ^ (1u | (1u << (L - 1)));
+      int64_t idx;
+      if constexpr (MODE == MODE_BSEARCH)
+        idx = dh_spin_find_bsearch(t, L, sLut, a.lutShift, a.words);
+      else
+        idx = dh_spin_find_rank(t, L, sBinom, a.dim);
+      gather_fma<K>(acc, x, idx, a.halfJ[L - 1]);
+    }
+    double* yr = y + row * K;
+#pragma unroll
+    for (int v = 0; v < K; ++v) yr[v] = (add ? yr[v] : 0.0) + acc[v];
+  }
+}
+
+// ----------------------------------------------------------------------------- host plumbing
+static SphereArgs sphere_args(const dh_hamiltonian* h, int lutBitsUsed) {
+  const dh_basis* b = h->basis;
+  SphereArgs a;
+  a.states = b->d_states;
+  a.dim = b->dim;
+  a.nbrOrbitals = b->nbrOrbitals;
+  a.nbrFactors = h->nbrFactors;
+  a.nbrPairs = h->nbrPairs;
+  a.factors = h->d_factors;
+  a.pairOffset = h->d_pairOffset;
+  a.lut = b->d_lut;
+  int drop = b->lutBits - lutBitsUsed;
+  a.lutStride = 1 << drop;
+  a.lutShift = b->lutShift + drop;
+  a.lutSize = (1 << lutBitsUsed) + 1;
+  a.rankLo = b->d_rankLo;
+  a.rankHi = b->d_rankHi;
+  a.rankBits = b->rankBits;
+  a.rankLoSize = (int)b->rankLoSize;
+  a.rankHiSize = (int)b->rankHiSize;
+  a.shift = h->shift;
+  return a;
+}
+
+static const size_t kLdsLimit = 160 * 1024;
+
+// LDS bytes of the staged tables for a given mode
+static size_t sphere_lds(const dh_hamiltonian* h, int mode, int lutBitsUsed) {
+  size_t bytes = sizeof(double) * h->nbrFactors + sizeof(int) * (h->nbrPairs + 1);
+  bytes = (bytes + 7) & ~size_t(7);
+  if (mode == MODE_BSEARCH) bytes += sizeof(int) * ((size_t(1) << lutBitsUsed) + 1);
+  if (mode == MODE_RANK_LDS) bytes += sizeof(uint32_t) * (h->basis->rankLoSize + h->basis->rankHiSize);
+  return bytes;
+}
+
+template <typename Kern>
+static int launch_persistent(dh_context* c, Kern kern, int blockSize, size_t lds, int64_t nbrTiles, dim3& grid) {
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, blockSize, lds) != hipSuccess || perCU <= 0)
+    perCU = 1;
+  int64_t g = std::min<int64_t>(nbrTiles, (int64_t)c->nbrCU * perCU);
+  grid = dim3((unsigned)std::max<int64_t>(g, 1));
+  return DH_OK;
+}
+
+template <int K, int LPR, int MODE>
+static int sphere_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+  dh_context* c = h->ctx;
+  int lutBitsUsed = std::min(h->basis->lutBits, 13);
+  SphereArgs a = sphere_args(h, lutBitsUsed);
+  size_t lds = sphere_lds(h, MODE, lutBitsUsed);
+  if (lds > kLdsLimit) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
+  auto kern = sphere_hv_kernel<K, LPR, MODE>;
+  if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int64_t rowsPerBlock = h->blockSize / LPR;
+  dim3 grid;
+  launch_persistent(c, kern, h->blockSize, lds, (count + rowsPerBlock - 1) / rowsPerBlock, grid);
+  hipLaunchKernelGGL(kern, grid, dim3(h->blockSize), lds, c->stream, a, x, y, add, first, count);
+  DH_HIP(c, hipGetLastError());
+  return DH_OK;
+}
+
+template <int K, int LPR>
+static int sphere_dispatch_mode(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first,
+                                int64_t count) {
+  if (h->indexMode == DH_INDEX_BSEARCH) return sphere_launch<K, LPR, MODE_BSEARCH>(h, x, y, add, first, count);
+  size_t ldsRank = sphere_lds(h, MODE_RANK_LDS, 0);
+  if (h->ldsTables && ldsRank <= 64 * 1024) return sphere_launch<K, LPR, MODE_RANK_LDS>(h, x, y, add, first, count);
+  return sphere_launch<K, LPR, MODE_RANK_GLOBAL>(h, x, y, add, first, count);
+}
+
+template <int K>
+static int sphere_dispatch_lpr(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first,
+                               int64_t count) {
+  switch (h->lanesPerRow) {
+    case 1: return sphere_dispatch_mode<K, 1>(h, x, y, add, first, count);
+    case 2: return sphere_dispatch_mode<K, 2>(h, x, y, add, first, count);
+    case 4: return sphere_dispatch_mode<K, 4>(h, x, y, add, first, count);
+    case 8: return sphere_dispatch_mode<K, 8>(h, x, y, add, first, count);
+    default: return dh_fail(h->ctx, DH_E_ARG, "lanes_per_row must be 1, 2, 4 or 8 for this Hamiltonian");
+  }
+}
+
+static SpinArgs spin_args(const dh_hamiltonian* h) {
+  const dh_basis* b = h->basis;
+  SpinArgs a;
+  a.words = b->d_words;
+  a.dim = b->dim;
+  a.L = b->chainLength;
+  a.periodic = h->periodic;
+  a.binom = b->d_binom;
+  a.lut = b->d_lut;
+  a.lutShift = b->lutShift;
+  a.lutSize = (1 << b->lutBits) + 1;
+  a.shift = h->shift;
+  for (int i = 0; i < 32; ++i) {
+    a.halfJ[i] = 0.5 * h->J[i];
+    a.Jz[i] = h->Jz[i];
+  }
+  return a;
+}
+
+template <int K, int MODE>
+static int spin_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+  dh_context* c = h->ctx;
+  SpinArgs a = spin_args(h);
+  size_t lds = (MODE == MODE_BSEARCH) ? sizeof(int) * (size_t)a.lutSize : 0;
+  auto kern = spin_hv_kernel<K, MODE>;
+  dim3 grid;
+  launch_persistent(c, kern, h->blockSize, lds, (count + h->blockSize - 1) / h->blockSize, grid);
+  hipLaunchKernelGGL(kern, grid, dim3(h->blockSize), lds, c->stream, a, x, y, add, first, count);
+  DH_HIP(c, hipGetLastError());
+  return DH_OK;
+}
+
+template <int K>
+static int spin_dispatch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+  if (h->indexMode == DH_INDEX_BSEARCH) return spin_launch<K, MODE_BSEARCH>(h, x, y, add, first, count);
+  return spin_launch<K, MODE_RANK_GLOBAL>(h, x, y, add, first, count);
+}
+
+extern "C" int dh_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count) {
+  if (!h || !x || !y) return DH_E_ARG;
+  dh_context* c = h->ctx;
+  int64_t dim = h->basis->dim;
+  if (count < 0) count = dim - first;
+  if (first < 0 || first + count > dim) return dh_fail(c, DH_E_ARG, "row range outside the space");
+  if (count == 0) return DH_OK;
+  if (h->kind == DH_HAM_SPHERE) {
+    if (!h->d_factors) return dh_fail(c, DH_E_STATE, "factors not set");
+    switch (k) {
+      case 1: return sphere_dispatch_lpr<1>(h, x, y, add, first, count);
+      case 2: return sphere_dispatch_lpr<2>(h, x, y, add, first, count);
+      case 4: return sphere_dispatch_lpr<4>(h, x, y, add, first, count);
+      case 8: return sphere_dispatch_lpr<8>(h, x, y, add, first, count);
+      case 16: return sphere_dispatch_lpr<16>(h, x, y, add, first, count);
+      default: return dh_fail(c, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+    }
+  }
+  if (h->kind == DH_HAM_SPIN) {
+    switch (k) {
+      case 1: return spin_dispatch<1>(h, x, y, add, first, count);
+      case 2: return spin_dispatch<2>(h, x, y, add, first, count);
+      case 4: return spin_dispatch<4>(h, x, y, add, first, count);
+      case 8: return spin_dispatch<8>(h, x, y, add, first, count);
+      case 16: return spin_dispatch<16>(h, x, y, add, first, count);
+      default: return dh_fail(c, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+    }
+  }
+  return dh_fail(c, DH_E_STATE, "unknown Hamiltonian kind");
+}
+
+// ----------------------------------------------------------------------------- Hamiltonian objects
+extern "C" int dh_sphere_hamiltonian_create(dh_context* c, dh_hamiltonian** out, dh_basis* b, const double* factors,
+                                            const int* pairOffset, int nbrFactors) {
+  if (!c || !out || !b || !factors || !pairOffset) return DH_E_ARG;
+  *out = nullptr;
+  if (b->kind != DH_BASIS_FERMION_SPHERE) return dh_fail(c, DH_E_ARG, "basis is not a fermion-on-sphere basis");
+  const int nOrb = b->nbrOrbitals;
+  const int nPairs = nOrb * (nOrb - 1) / 2;
+  // the layout is fixed by the reference tables: check it
+  int expect = 0;
+  for (int m1 = 0; m1 < nOrb; ++m1)
+    for (int m2 = 0; m2 < m1; ++m2) {
+      int a = m1 * (m1 - 1) / 2 + m2;
+      if (pairOffset[a] != expect) return dh_fail(c, DH_E_ARG, "pair_offset does not match the factor layout");
+      int sum = m1 + m2;
+      expect += std::min(sum, nOrb - 1) - (sum / 2 + 1) + 1;
+    }
+  if (pairOffset[nPairs] != expect || nbrFactors != expect)
+    return dh_fail(c, DH_E_ARG, "nbr_factors does not match the factor layout");
+  dh_hamiltonian* h = new dh_hamiltonian();
+  h->ctx = c;
+  h->basis = b;
+  h->kind = DH_HAM_SPHERE;
+  h->nbrFactors = nbrFactors;
+  h->nbrPairs = nPairs;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&h->d_factors, sizeof(double) * std::max(nbrFactors, 1))) ||
+      (rc = dh_malloc(c, (void**)&h->d_pairOffset, sizeof(int) * (nPairs + 1)))) {
+    dh_hamiltonian_destroy(h);
+    return rc;
+  }
+  DH_HIP(c, hipMemcpyAsync(h->d_factors, factors, sizeof(double) * nbrFactors, hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipMemcpyAsync(h->d_pairOffset, pairOffset, sizeof(int) * (nPairs + 1), hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  h->lanesPerRow = (b->dim < (int64_t)c->nbrCU * 256) ? 4 : 1;
+  h->indexMode = DH_INDEX_RANK;
+  *out = h;
+  return DH_OK;
+}
+
+extern "C" int dh_sphere_hamiltonian_set_channels(dh_hamiltonian* h, const double* cg, int nbrChannels,
+                                                  const double* weights) {
+  if (!h || !cg || !weights || nbrChannels <= 0) return DH_E_ARG;
+  if (h->kind != DH_HAM_SPHERE) return dh_fail(h->ctx, DH_E_ARG, "not a sphere Hamiltonian");
+  if (nbrChannels > 32) return dh_fail(h->ctx, DH_E_RANGE, "at most 32 channels");
+  dh_context* c = h->ctx;
+  if (h->d_cg) {
+    hipFree(h->d_cg);
+    h->d_cg = nullptr;
+  }
+  int rc = dh_malloc(c, (void**)&h->d_cg, sizeof(double) * h->nbrPairs * nbrChannels);
+  if (rc) return rc;
+  DH_HIP(c, hipMemcpyAsync(h->d_cg, cg, sizeof(double) * h->nbrPairs * nbrChannels, hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  h->nbrChannels = nbrChannels;
+  h->channelWeights.assign(weights, weights + nbrChannels);
+  return DH_OK;
+}
+
+extern "C" int dh_spin_chain_hamiltonian_create(dh_context* c, dh_hamiltonian** out, dh_basis* b, const double* j,
+                                                const double* jz, int periodic) {
+  if (!c || !out || !b || !j || !jz) return DH_E_ARG;
+  *out = nullptr;
+  if (b->kind != DH_BASIS_SPIN_CHAIN) return dh_fail(c, DH_E_ARG, "basis is not a spin chain");
+  dh_hamiltonian* h = new dh_hamiltonian();
+  h->ctx = c;
+  h->basis = b;
+  h->kind = DH_HAM_SPIN;
+  h->periodic = periodic ? 1 : 0;
+  int nb = periodic ? b->chainLength : b->chainLength - 1;
+  for (int i = 0; i < nb; ++i) {
+    h->J[i] = j[i];
+    h->Jz[i] = jz[i];
+  }
+  h->indexMode = DH_INDEX_RANK;
+  *out = h;
+  return DH_OK;
+}
+
+extern "C" int dh_hamiltonian_destroy(dh_hamiltonian* h) {
+  if (!h) return DH_E_ARG;
+  dh_context* c = h->ctx;
+  hipStreamSynchronize(c->stream);
+  if (h->d_factors) hipFree(h->d_factors);
+  if (h->d_pairOffset) hipFree(h->d_pairOffset);
+  if (h->d_cg) hipFree(h->d_cg);
+  delete h;
+  return DH_OK;
+}
+
+extern "C" int dh_hamiltonian_set_shift(dh_hamiltonian* h, double shift) {
+  if (!h) return DH_E_ARG;
+  h->shift = shift;
+  return DH_OK;
+}
+
+extern "C" int64_t dh_hamiltonian_dimension(const dh_hamiltonian* h) { return h ? h->basis->dim : -1; }
+
+extern "C" int dh_hamiltonian_set_launch(dh_hamiltonian* h, int lpr, int blockSize, int mode, int lds) {
+  if (!h) return DH_E_ARG;
+  if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8) return dh_fail(h->ctx, DH_E_ARG, "lanes_per_row in {1,2,4,8}");
+  if (h->kind == DH_HAM_SPIN && lpr != 1) return dh_fail(h->ctx, DH_E_ARG, "spin kernels use one row per lane");
+  if (blockSize < 64 || blockSize > 1024 || (blockSize % 64)) return dh_fail(h->ctx, DH_E_ARG, "block_size");
+  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK) return dh_fail(h->ctx, DH_E_ARG, "index_mode");
+  h->lanesPerRow = lpr;
+  h->blockSize = blockSize;
+  h->indexMode = mode;
+  h->ldsTables = lds;
+  return DH_OK;
+}

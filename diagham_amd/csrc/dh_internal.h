@@ -1,0 +1,105 @@
+// Internal structures shared by the libdhgpu translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dhgpu.h"
+
+struct dh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+  std::string lastError;
+  int nbrCU = 0;
+  char arch[64] = {0};
+  // reduction scratch (device) and pinned staging (host), grown on demand
+  double* d_scratch = nullptr;
+  size_t scratchBytes = 0;
+  double* h_pinned = nullptr;
+  size_t pinnedBytes = 0;
+};
+
+enum dh_basis_kind { DH_BASIS_FERMION_SPHERE = 1, DH_BASIS_SPIN_CHAIN = 2 };
+
+struct dh_basis {
+  dh_context* ctx = nullptr;
+  int kind = 0;
+  int64_t dim = 0;
+  // fermions on the sphere
+  int nbrFermions = 0, lzMax = 0, totalLz = 0, nbrOrbitals = 0, orbitalSum = 0;
+  uint64_t* d_states = nullptr;  // [dim] descending uint64
+  // spin chain
+  int chainLength = 0, nbrUp = 0, sz = 0;
+  uint32_t* d_words = nullptr;  // [dim] reference 32-bit words (bits >= L set)
+  // LUT for the binary-search lookup: lut[v] = number of states whose top lutBits bits
+  // (of the ordering key) are >= v, v in [0, 2^lutBits]
+  int lutBits = 0, lutShift = 0;
+  int* d_lut = nullptr;
+  // combinatorial rank tables (fermions): index = dim - 1 - (lo[s & (2^rankBits-1)] + hi[s >> rankBits])
+  int rankBits = 0;
+  int64_t rankLoSize = 0, rankHiSize = 0;
+  uint32_t* d_rankLo = nullptr;
+  uint32_t* d_rankHi = nullptr;
+  // binomial table C[n][k], n,k <= 32 (spin rank)
+  uint32_t* d_binom = nullptr;
+};
+
+enum dh_ham_kind { DH_HAM_SPHERE = 1, DH_HAM_SPIN = 2 };
+
+struct dh_hamiltonian {
+  dh_context* ctx = nullptr;
+  dh_basis* basis = nullptr;
+  int kind = 0;
+  double shift = 0.0;
+  // sphere two-body factors
+  int nbrFactors = 0, nbrPairs = 0;
+  double* d_factors = nullptr;
+  int* d_pairOffset = nullptr;  // [nbrPairs + 1]
+  // adjoint channel split
+  int nbrChannels = 0;
+  double* d_cg = nullptr;  // [nbrPairs * nbrChannels]
+  std::vector<double> channelWeights;
+  // spin chain couplings
+  double J[32] = {0}, Jz[32] = {0};
+  int periodic = 0;
+  // launch configuration
+  int lanesPerRow = 1, blockSize = 256, indexMode = DH_INDEX_RANK, ldsTables = 1;
+};
+
+// error helpers
+int dh_fail(dh_context* ctx, int code, const std::string& msg);
+#define DH_HIP(ctx, call)                                                                 \
+  do {                                                                                    \
+    hipError_t _e = (call);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return dh_fail((ctx), DH_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// scratch management (core)
+int dh_scratch(dh_context* ctx, size_t bytes, double** d_ptr);
+int dh_pinned(dh_context* ctx, size_t bytes, double** h_ptr);
+
+// XCD-aware block remap: MI355X hands consecutive workgroups to its 8 XCDs round robin;
+// remapping gives every XCD one contiguous range of blocks so neighbouring rows (whose
+// gather targets are close) share that XCD's L2.
+__device__ __forceinline__ unsigned dh_xcd_block(unsigned b, unsigned n) {
+  const unsigned nx = 8;
+  if (n < nx) return b;
+  unsigned per = n / nx, rem = n % nx;
+  unsigned x = b % nx, i = b / nx;
+  // XCD x owns blocks [x*per + min(x,rem), ...) of length per + (x < rem)
+  unsigned start = x * per + (x < rem ? x : rem);
+  return start + i;
+}
+
+// binomial C(n, k) for the host, 64-bit
+inline uint64_t dh_binom_host(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  uint64_t r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+  return r;
+}

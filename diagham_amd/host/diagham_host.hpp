@@ -1,0 +1,238 @@
+// C++ host side of the on-the-fly H.v + Lanczos path, shaped like DiagHam's own classes
+// (same names, argument meanings and call order) and driving the GPU through the C ABI of
+// include/dhgpu.h.  Vectors live in device memory for their whole life.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dhgpu.h"
+
+namespace dhh {
+
+// ------------------------------------------------------------------ math tools
+// ClebschGordanCoefficients (src/MathTools/ClebschGordanCoefficients.cc:225): all
+// <j1/2 m1/2 ; j2/2 m2/2 | j/2 (m1+m2)/2> for doubled angular momenta.  Evaluated with
+// Racah's closed formula in 113-bit floating point, so each value is correctly rounded to
+// double in practice (the reference uses the Schulten-Gordon recursion in double).
+class ClebschGordanCoefficients {
+ public:
+  ClebschGordanCoefficients(int j1, int j2);
+  // zero outside the allowed range, like GetCoefficient (:150)
+  double GetCoefficient(int m1, int m2, int j) const;
+
+ private:
+  int J1, J2;
+  std::vector<double> table;  // [(m1pos * (J2+1) + m2pos) * (J1+J2+1) + j]
+};
+
+// Lowest-Landau-level Coulomb pseudopotentials V_0..V_nbrFlux on the sphere in the units of
+// EvaluatePseudopotentials (FQHE/src/Tools/FQHESpectrum/PseudoPotentials.cc:35) with zero
+// layer separation, from the closed form
+//   V_m = 2/sqrt(S) C(4S-2L, 2S-L) C(4S+2L+2, 2S+L+1) / C(4S+2, 2S+1)^2,  L = 2S - m.
+std::vector<double> EvaluatePseudopotentials(int nbrFlux);
+// FQHESpherePseudopotentialTools.cc:53: "Pseudopotentials = V_0 V_1 ..." in a key = value file
+bool FQHESphereGetPseudopotentials(const std::string& fileName, int lzMax, std::vector<double>& pseudo);
+// round like the 14 significant digits CoulombPseudopotentials writes
+std::vector<double> RoundToFileDigits(const std::vector<double>& v);
+
+// ------------------------------------------------------------------ architecture
+// GPU counterpart of AbstractArchitecture (src/Architecture/AbstractArchitecture.h:45):
+// owns the device context and routes the Hamiltonian-multiply operations to dh_hv.
+class GPUArchitecture {
+ public:
+  explicit GPUArchitecture(int device = 0);
+  ~GPUArchitecture();
+  dh_context* Context() const { return ctx; }
+  int GetArchitectureID() const { return 0x100; }  // new ArchitectureIDs entry "GPU"
+  double* AllocateMemory(int64_t nbrDoubles);        // device vector storage
+  void FreeMemory(double* p);
+  void Check(int rc, const char* what) const;
+
+ private:
+  dh_context* ctx = nullptr;
+};
+
+// Device-resident block of k vectors in [row][k] layout (k = 1 is a RealVector).
+class DeviceVector {
+ public:
+  DeviceVector() = default;
+  DeviceVector(GPUArchitecture* arch, int64_t dim, int k = 1);
+  ~DeviceVector();
+  DeviceVector(const DeviceVector&) = delete;
+  DeviceVector& operator=(const DeviceVector&) = delete;
+  DeviceVector(DeviceVector&& o) noexcept;
+  DeviceVector& operator=(DeviceVector&& o) noexcept;
+  double* Data() const { return d; }
+  int64_t GetVectorDimension() const { return dim; }
+  int BlockSize() const { return k; }
+
+ private:
+  GPUArchitecture* arch = nullptr;
+  double* d = nullptr;
+  int64_t dim = 0;
+  int k = 1;
+};
+
+// ------------------------------------------------------------------ Hilbert spaces
+class AbstractHilbertSpace {
+ public:
+  virtual ~AbstractHilbertSpace();
+  int64_t GetHilbertSpaceDimension() const { return dh_basis_dimension(basis); }
+  dh_basis* Basis() const { return basis; }
+
+ protected:
+  dh_basis* basis = nullptr;
+};
+
+// FermionOnSphere(nbrFermions, totalLz, lzMax) (FQHE/src/HilbertSpace/FermionOnSphere.cc:76)
+class FermionOnSphere : public AbstractHilbertSpace {
+ public:
+  FermionOnSphere(GPUArchitecture* arch, int nbrFermions, int totalLz, int lzMax);
+  int GetNbrParticles() const { return NbrFermions; }
+  int GetLzMax() const { return LzMax; }
+  int GetTotalLz() const { return TotalLz; }
+
+ private:
+  int NbrFermions, TotalLz, LzMax;
+};
+
+// Spin1_2Chain(chainLength, sz, memory) (Spin/src/HilbertSpace/Spin1_2Chain.cc:113)
+class Spin1_2Chain : public AbstractHilbertSpace {
+ public:
+  Spin1_2Chain(GPUArchitecture* arch, int chainLength, int sz);
+  int GetChainLength() const { return ChainLength; }
+
+ private:
+  int ChainLength, Sz;
+};
+
+// ------------------------------------------------------------------ Hamiltonians
+// AbstractHamiltonian (src/Hamiltonian/AbstractHamiltonian.h:56) restricted to the
+// operations of this path.  LowLevelAddMultiply adds H x to y for rows
+// [firstComponent, firstComponent + nbrComponent).
+class AbstractHamiltonian {
+ public:
+  virtual ~AbstractHamiltonian();
+  int64_t GetHilbertSpaceDimension() const { return dh_hamiltonian_dimension(ham); }
+  void ShiftHamiltonian(double shift);
+  double GetShift() const { return Shift; }
+  bool IsHermitian() const { return true; }
+  DeviceVector& LowLevelAddMultiply(DeviceVector& vSource, DeviceVector& vDestination, int64_t firstComponent,
+                                    int64_t nbrComponent);
+  DeviceVector& LowLevelMultiply(DeviceVector& vSource, DeviceVector& vDestination, int64_t firstComponent,
+                                 int64_t nbrComponent);
+  // block products: the k vectors of a DeviceVector block share every index lookup
+  DeviceVector& LowLevelMultipleAddMultiply(DeviceVector& vSources, DeviceVector& vDestinations);
+  DeviceVector& LowLevelMultipleMultiply(DeviceVector& vSources, DeviceVector& vDestinations);
+  dh_hamiltonian* Handle() const { return ham; }
+  GPUArchitecture* Architecture() const { return arch; }
+
+ protected:
+  GPUArchitecture* arch = nullptr;
+  dh_hamiltonian* ham = nullptr;
+  double Shift = 0.0;
+};
+
+// ParticleOnSphereGenericHamiltonian (FQHE/src/Hamiltonian/ParticleOnSphereGenericHamiltonian.cc:68)
+// for fermions: two-body interaction from pseudopotentials V_0..V_lzMax.
+class ParticleOnSphereGenericHamiltonian : public AbstractHamiltonian {
+ public:
+  ParticleOnSphereGenericHamiltonian(FermionOnSphere* particles, int nbrParticles, int lzmax,
+                                     const std::vector<double>& pseudoPotential, GPUArchitecture* architecture);
+  // the factor tables as uploaded (full list, pair-major, m3 ascending)
+  const std::vector<double>& InteractionFactors() const { return Factors; }
+  const std::vector<int>& PairOffsets() const { return PairOffset; }
+  // adjoint channels: relative angular momenta m of the interaction split
+  const std::vector<int>& ChannelMomenta() const { return ChannelM; }
+  // g[c] = <psi|H_c|phi> per channel; if d_hpsi, also H psi
+  void Bilinears(const double* d_psi, const double* d_phi, double* g, double* d_hpsi);
+
+  // EvaluateInteractionFactors (:493) in the device layout; exposed for tests
+  static void EvaluateInteractionFactors(int lzMax, const std::vector<double>& pseudo, std::vector<double>& factors,
+                                         std::vector<int>& pairOffset);
+  static void EvaluateChannelTable(int lzMax, std::vector<int>& channelM, std::vector<double>& cg);
+
+ private:
+  FermionOnSphere* Particles;
+  int LzMax;
+  std::vector<double> PseudoPotential, Factors, ChannelCG;
+  std::vector<int> PairOffset, ChannelM;
+};
+
+// SpinChainHamiltonian (Spin/src/Hamiltonian/SpinChainHamiltonian.cc:97)
+class SpinChainHamiltonian : public AbstractHamiltonian {
+ public:
+  SpinChainHamiltonian(Spin1_2Chain* chain, int nbrSpin, const std::vector<double>& j, const std::vector<double>& jz,
+                       bool periodicBoundaryConditions, GPUArchitecture* architecture);
+};
+
+// ------------------------------------------------------------------ operations
+// VectorHamiltonianMultiplyOperation (src/Architecture/ArchitectureOperation/VectorHamiltonianMultiplyOperation.cc):
+// on the GPU architecture the whole row range runs as one device launch.
+class VectorHamiltonianMultiplyOperation {
+ public:
+  VectorHamiltonianMultiplyOperation(AbstractHamiltonian* hamiltonian, DeviceVector* source, DeviceVector* destination);
+  bool ApplyOperation(GPUArchitecture* architecture);
+
+ private:
+  AbstractHamiltonian* Hamiltonian;
+  DeviceVector *Source, *Destination;
+};
+
+// ------------------------------------------------------------------ Lanczos
+struct LanczosResult {
+  std::vector<double> eigenvalues;
+  int iterations = 0;        // vector products (block: blocks * k)
+  int blockIterations = 0;   // Lanczos steps
+  bool converged = false;
+  double seconds = 0.0;
+};
+
+// FullReorthogonalizedLanczosAlgorithm (src/LanczosAlgorithm/FullReorthogonalizedLanczosAlgorithm.cc)
+// and FullReorthogonalizedBlockLanczosAlgorithm (FullReorthogonalizedBlockLanczosAlgorithm.cc):
+// block size 1 is the plain algorithm.  The Krylov basis stays on the device; every new block is
+// orthogonalised against all previous ones (two classical Gram-Schmidt passes).  Convergence as
+// TestConvergence (:325, strong form): every wanted eigenvalue changes by less than
+// precision * |value| between two consecutive checks.
+class FullReorthogonalizedBlockLanczosAlgorithm {
+ public:
+  FullReorthogonalizedBlockLanczosAlgorithm(GPUArchitecture* architecture, AbstractHamiltonian* hamiltonian,
+                                            int nbrEigenvalue, int blockSize, int maxIter, double precision = 1e-14);
+  ~FullReorthogonalizedBlockLanczosAlgorithm();
+  void InitializeLanczosAlgorithm(uint64_t seed);
+  void RunLanczosAlgorithm(int nbrIter);
+  bool TestConvergence();
+  std::vector<double> GetEigenvalues() const { return Current; }
+  // Ritz vectors of the nbr lowest eigenvalues into d_out (nbr * dim doubles, vector-major)
+  void GetEigenstates(int nbr, double* d_out);
+  LanczosResult Solve();
+  int NbrBlocks() const { return (int)A.size(); }
+
+ private:
+  void Diagonalize(std::vector<double>& evals, std::vector<double>* evecs);
+  void Orthonormalize(double* d_w, std::vector<double>& R);
+  GPUArchitecture* Arch;
+  AbstractHamiltonian* Hamiltonian;
+  int NbrEigenvalue, BlockSize, MaxIter;
+  double Precision;
+  int64_t Dim;
+  double* d_basis = nullptr;  // MaxBlocks+1 blocks of Dim*BlockSize
+  int MaxBlocks;
+  double* d_work = nullptr;
+  std::vector<std::vector<double>> A, B;  // diagonal and sub-diagonal k x k blocks
+  std::vector<double> Current, Previous;
+  bool HavePrevious = false;
+};
+
+// Tridiagonal QL with implicit shifts (RealTriDiagonalSymmetricMatrix.cc:687 semantics):
+// eigenvalues ascending; optional eigenvectors (column-major n x n) of the tridiagonal.
+void TridiagonalEigen(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& evals,
+                      std::vector<double>* evecs);
+// Symmetric band matrix (half-bandwidth b, dense row-major n x n) -> tridiagonal by Givens
+// rotations with bulge chasing; returns diagonal and off-diagonal, and accumulates Q if given.
+void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>& d, std::vector<double>& e,
+                       std::vector<double>* q);
+
+}  // namespace dhh

@@ -1,0 +1,105 @@
+// Interaction factors of the generic two-body Hamiltonian on the sphere and the spin chain.
+#include <quadmath.h>
+
+#include <cmath>
+#include <stdexcept>
+
+#include "diagham_host.hpp"
+
+namespace dhh {
+
+// ParticleOnSphereGenericHamiltonian::EvaluateInteractionFactors (fermions,
+// ParticleOnSphereGenericHamiltonian.cc:493-560):
+//   F(m1 m2 -> m3 m4) = -4 sum_L V_{lzMax-L} <m1 m2|L> <m3 m4|L>
+// over the pair angular momenta L whose relative momentum lzMax - L is odd, for every
+// annihilated pair m1 > m2 and created pair m3 > m4 = m1 + m2 - m3 (orbital indices,
+// l_z = index - lzMax/2).  Accumulated in 113-bit precision, then rounded.
+void ParticleOnSphereGenericHamiltonian::EvaluateInteractionFactors(int lzMax, const std::vector<double>& pseudo,
+                                                                     std::vector<double>& factors,
+                                                                     std::vector<int>& pairOffset) {
+  const int nOrb = lzMax + 1;
+  ClebschGordanCoefficients cg(lzMax, lzMax);
+  const int parity = (lzMax & 1) ? 0 : 1;  // L & 1 == parity  <=>  lzMax - L odd
+  factors.clear();
+  pairOffset.assign((size_t)nOrb * (nOrb - 1) / 2 + 1, 0);
+  for (int m1 = 0; m1 < nOrb; ++m1)
+    for (int m2 = 0; m2 < m1; ++m2) {
+      pairOffset[(size_t)m1 * (m1 - 1) / 2 + m2] = (int)factors.size();
+      int sum = m1 + m2;
+      int m3lo = sum / 2 + 1, m3hi = std::min(sum, lzMax);
+      int d1 = 2 * m1 - lzMax, d2 = 2 * m2 - lzMax;
+      for (int m3 = m3lo; m3 <= m3hi; ++m3) {
+        int m4 = sum - m3;
+        int d3 = 2 * m3 - lzMax, d4 = 2 * m4 - lzMax;
+        __float128 acc = 0;
+        for (int J = std::abs(d1 + d2); J <= 2 * lzMax; J += 2) {
+          int L = J >> 1;
+          if ((L & 1) != parity) continue;
+          acc += (__float128)pseudo[lzMax - L] * (__float128)cg.GetCoefficient(d1, d2, J) *
+                 (__float128)cg.GetCoefficient(d3, d4, J);
+        }
+        factors.push_back((double)(-4 * acc));
+      }
+    }
+  pairOffset.back() = (int)factors.size();
+}
+
+// channels of the adjoint split: one per relative momentum m (odd), the table holds
+// <m1 m2 | L> for every pair and L = lzMax - m
+void ParticleOnSphereGenericHamiltonian::EvaluateChannelTable(int lzMax, std::vector<int>& channelM,
+                                                              std::vector<double>& table) {
+  const int nOrb = lzMax + 1;
+  ClebschGordanCoefficients cg(lzMax, lzMax);
+  channelM.clear();
+  for (int m = 1; m <= lzMax; m += 2) channelM.push_back(m);
+  const int nc = (int)channelM.size();
+  const size_t nPairs = (size_t)nOrb * (nOrb - 1) / 2;
+  table.assign(nPairs * nc, 0.0);
+  for (int m1 = 0; m1 < nOrb; ++m1)
+    for (int m2 = 0; m2 < m1; ++m2) {
+      size_t a = (size_t)m1 * (m1 - 1) / 2 + m2;
+      int d1 = 2 * m1 - lzMax, d2 = 2 * m2 - lzMax;
+      for (int c = 0; c < nc; ++c) {
+        int L = lzMax - channelM[c];
+        table[a * nc + c] = cg.GetCoefficient(d1, d2, 2 * L);
+      }
+    }
+}
+
+ParticleOnSphereGenericHamiltonian::ParticleOnSphereGenericHamiltonian(FermionOnSphere* particles, int nbrParticles,
+                                                                       int lzmax, const std::vector<double>& pseudo,
+                                                                       GPUArchitecture* architecture)
+    : Particles(particles), LzMax(lzmax), PseudoPotential(pseudo) {
+  arch = architecture;
+  if ((int)pseudo.size() < lzmax + 1) throw std::invalid_argument("need lzMax + 1 pseudopotentials");
+  if (particles->GetNbrParticles() != nbrParticles || particles->GetLzMax() != lzmax)
+    throw std::invalid_argument("Hilbert space does not match the Hamiltonian parameters");
+  EvaluateInteractionFactors(lzmax, pseudo, Factors, PairOffset);
+  arch->Check(dh_sphere_hamiltonian_create(arch->Context(), &ham, particles->Basis(), Factors.data(),
+                                           PairOffset.data(), (int)Factors.size()),
+              "ParticleOnSphereGenericHamiltonian");
+  EvaluateChannelTable(lzmax, ChannelM, ChannelCG);
+  std::vector<double> weights;
+  for (int m : ChannelM) weights.push_back(pseudo[m]);
+  arch->Check(dh_sphere_hamiltonian_set_channels(ham, ChannelCG.data(), (int)ChannelM.size(), weights.data()),
+              "set channels");
+}
+
+void ParticleOnSphereGenericHamiltonian::Bilinears(const double* d_psi, const double* d_phi, double* g,
+                                                   double* d_hpsi) {
+  arch->Check(dh_sphere_adjoint(ham, d_psi, d_phi, g, d_hpsi), "Bilinears");
+}
+
+SpinChainHamiltonian::SpinChainHamiltonian(Spin1_2Chain* chain, int nbrSpin, const std::vector<double>& j,
+                                           const std::vector<double>& jz, bool periodic,
+                                           GPUArchitecture* architecture) {
+  arch = architecture;
+  int nb = periodic ? nbrSpin : nbrSpin - 1;
+  if ((int)j.size() < nb || (int)jz.size() < nb) throw std::invalid_argument("not enough couplings");
+  if (chain->GetChainLength() != nbrSpin) throw std::invalid_argument("chain length mismatch");
+  arch->Check(dh_spin_chain_hamiltonian_create(arch->Context(), &ham, chain->Basis(), j.data(), jz.data(),
+                                               periodic ? 1 : 0),
+              "SpinChainHamiltonian");
+}
+
+}  // namespace dhh

@@ -1,0 +1,177 @@
+// C facade of the host layer (include/diagham_host.h).
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/diagham_host.h"
+#include "diagham_host.hpp"
+
+using namespace dhh;
+
+struct dhh_arch {
+  std::unique_ptr<GPUArchitecture> arch;
+};
+
+struct dhh_problem {
+  dhh_arch* arch = nullptr;
+  std::unique_ptr<AbstractHilbertSpace> space;
+  std::unique_ptr<AbstractHamiltonian> hamiltonian;
+  std::vector<int> channels;
+};
+
+static thread_local std::string g_error;
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument& e) {
+    g_error = e.what();
+    return DH_E_ARG;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return DH_E_STATE;
+  }
+}
+
+extern "C" const char* dhh_last_error(void) { return g_error.c_str(); }
+
+extern "C" int dhh_arch_create(dhh_arch** out, int device) {
+  if (!out) return DH_E_ARG;
+  return guarded([&] {
+    auto* a = new dhh_arch();
+    a->arch.reset(new GPUArchitecture(device));
+    *out = a;
+    return DH_OK;
+  });
+}
+
+extern "C" int dhh_arch_destroy(dhh_arch* a) {
+  delete a;
+  return DH_OK;
+}
+
+extern "C" dh_context* dhh_arch_context(dhh_arch* a) { return a ? a->arch->Context() : nullptr; }
+
+extern "C" int dhh_sphere_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
+                                         const double* pseudo) {
+  if (!a || !out || !pseudo) return DH_E_ARG;
+  return guarded([&] {
+    auto p = std::make_unique<dhh_problem>();
+    p->arch = a;
+    auto* space = new FermionOnSphere(a->arch.get(), n, totalLz, lzMax);
+    p->space.reset(space);
+    std::vector<double> v(pseudo, pseudo + lzMax + 1);
+    auto* h = new ParticleOnSphereGenericHamiltonian(space, n, lzMax, v, a->arch.get());
+    p->hamiltonian.reset(h);
+    p->channels = h->ChannelMomenta();
+    *out = p.release();
+    return DH_OK;
+  });
+}
+
+extern "C" int dhh_spin_problem_create(dhh_arch* a, dhh_problem** out, int L, int sz, const double* j,
+                                       const double* jz, int periodic) {
+  if (!a || !out || !j || !jz) return DH_E_ARG;
+  return guarded([&] {
+    auto p = std::make_unique<dhh_problem>();
+    p->arch = a;
+    auto* chain = new Spin1_2Chain(a->arch.get(), L, sz);
+    p->space.reset(chain);
+    std::vector<double> J(j, j + L), Jz(jz, jz + L);
+    p->hamiltonian.reset(new SpinChainHamiltonian(chain, L, J, Jz, periodic != 0, a->arch.get()));
+    *out = p.release();
+    return DH_OK;
+  });
+}
+
+extern "C" int dhh_problem_destroy(dhh_problem* p) {
+  if (!p) return DH_E_ARG;
+  p->hamiltonian.reset();
+  p->space.reset();
+  delete p;
+  return DH_OK;
+}
+
+extern "C" dh_basis* dhh_problem_basis(dhh_problem* p) { return p ? p->space->Basis() : nullptr; }
+extern "C" dh_hamiltonian* dhh_problem_hamiltonian(dhh_problem* p) { return p ? p->hamiltonian->Handle() : nullptr; }
+
+extern "C" int dhh_problem_channels(dhh_problem* p, int* momenta, int capacity) {
+  if (!p) return DH_E_ARG;
+  int n = (int)p->channels.size();
+  for (int i = 0; i < n && i < capacity; ++i) momenta[i] = p->channels[i];
+  return n;
+}
+
+extern "C" int dhh_lanczos(dhh_problem* p, int nbrEigen, int blockSize, int maxIter, double precision, uint64_t seed,
+                           double* eigenvalues, int* products, int* blocks, double* seconds, double* d_vectors,
+                           int nbrVectors) {
+  if (!p || !eigenvalues || nbrEigen <= 0) return DH_E_ARG;
+  return guarded([&] {
+    FullReorthogonalizedBlockLanczosAlgorithm lanczos(p->arch->arch.get(), p->hamiltonian.get(), nbrEigen, blockSize,
+                                                      maxIter, precision);
+    lanczos.InitializeLanczosAlgorithm(seed);
+    LanczosResult r = lanczos.Solve();
+    for (int i = 0; i < nbrEigen; ++i) eigenvalues[i] = (i < (int)r.eigenvalues.size()) ? r.eigenvalues[i] : 0.0;
+    if (products) *products = r.iterations;
+    if (blocks) *blocks = r.blockIterations;
+    if (seconds) *seconds = r.seconds;
+    if (d_vectors && nbrVectors > 0) lanczos.GetEigenstates(nbrVectors, d_vectors);
+    return r.converged ? DH_OK : DH_E_STATE;
+  });
+}
+
+extern "C" int dhh_coulomb_pseudopotentials(int nbrFlux, int round, double* out) {
+  if (!out || nbrFlux < 0) return DH_E_ARG;
+  std::vector<double> v = EvaluatePseudopotentials(nbrFlux);
+  if (round) v = RoundToFileDigits(v);
+  std::memcpy(out, v.data(), sizeof(double) * v.size());
+  return DH_OK;
+}
+
+extern "C" double dhh_clebsch_gordan(int j1, int j2, int m1, int m2, int j) {
+  ClebschGordanCoefficients cg(j1, j2);
+  return cg.GetCoefficient(m1, m2, j);
+}
+
+extern "C" int dhh_sphere_factors(int lzMax, const double* pseudo, double* factors, int* pairOffset, int* nbr) {
+  if (!pseudo || !factors || !pairOffset || !nbr) return DH_E_ARG;
+  std::vector<double> v(pseudo, pseudo + lzMax + 1), f;
+  std::vector<int> off;
+  ParticleOnSphereGenericHamiltonian::EvaluateInteractionFactors(lzMax, v, f, off);
+  if ((int)f.size() > *nbr) return DH_E_RANGE;
+  std::memcpy(factors, f.data(), sizeof(double) * f.size());
+  std::memcpy(pairOffset, off.data(), sizeof(int) * off.size());
+  *nbr = (int)f.size();
+  return DH_OK;
+}
+
+extern "C" int dhh_sphere_channel_table(int lzMax, int* momenta, double* table, int capacity) {
+  std::vector<int> m;
+  std::vector<double> t;
+  ParticleOnSphereGenericHamiltonian::EvaluateChannelTable(lzMax, m, t);
+  if ((int)t.size() > capacity) return DH_E_RANGE;
+  std::memcpy(momenta, m.data(), sizeof(int) * m.size());
+  std::memcpy(table, t.data(), sizeof(double) * t.size());
+  return (int)m.size();
+}
+
+extern "C" int dhh_read_pseudopotentials(const char* file, int lzMax, double* out) {
+  if (!file || !out) return DH_E_ARG;
+  std::vector<double> v;
+  if (!FQHESphereGetPseudopotentials(file, lzMax, v)) return DH_E_ARG;
+  std::memcpy(out, v.data(), sizeof(double) * v.size());
+  return DH_OK;
+}
+
+extern "C" int dhh_band_eigenvalues(int n, int b, const double* a, double* ev) {
+  if (!a || !ev || n <= 0 || b < 1) return DH_E_ARG;
+  return guarded([&] {
+    std::vector<double> m(a, a + (size_t)n * n), d, e, w;
+    BandToTridiagonal(n, b, m, d, e, nullptr);
+    TridiagonalEigen(n, d, e, w, nullptr);
+    std::memcpy(ev, w.data(), sizeof(double) * n);
+    return DH_OK;
+  });
+}

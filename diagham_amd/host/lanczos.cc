@@ -1,0 +1,388 @@
+// Full-reorthogonalisation (block) Lanczos with a device-resident Krylov basis.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "diagham_host.hpp"
+
+namespace dhh {
+
+// ------------------------------------------------------------------ small dense eigen tools
+// Implicit-shift QL on a symmetric tridiagonal matrix (the textbook tql2 iteration).
+// d: diagonal (n), e: sub-diagonal (e[i] couples i and i+1, n-1 used).  evecs, if given,
+// receives the n x n eigenvector matrix column-major (column j <-> evals[j]).
+void TridiagonalEigen(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& evals,
+                      std::vector<double>* evecs) {
+  e.resize(n, 0.0);
+  if (n > 0) e[n - 1] = 0.0;
+  std::vector<double> z;
+  if (evecs) {
+    z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;  // z[row * n + col]
+  }
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    while (true) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-300 + 2.220446049250313e-16 * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 60) throw std::runtime_error("TridiagonalEigen: no convergence");
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i = m - 1;
+      bool underflow = false;
+      for (; i >= l; --i) {
+        double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        if (evecs) {
+          for (int k = 0; k < n; ++k) {
+            double zk1 = z[(size_t)k * n + i + 1], zk = z[(size_t)k * n + i];
+            z[(size_t)k * n + i + 1] = s * zk + c * zk1;
+            z[(size_t)k * n + i] = c * zk - s * zk1;
+          }
+        }
+      }
+      if (underflow && i >= l) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
+  evals.resize(n);
+  for (int i = 0; i < n; ++i) evals[i] = d[order[i]];
+  if (evecs) {
+    evecs->assign((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) (*evecs)[(size_t)j * n + k] = z[(size_t)k * n + order[j]];
+  }
+}
+
+// Givens rotation G on the plane (p, p+1) of a symmetric dense matrix: A <- G A G^T,
+// restricted to the index window [lo, hi] that holds every nonzero of rows p and p+1.
+static inline void rotate_sym(std::vector<double>& a, int n, int p, double c, double s, int lo, int hi) {
+  const int q = p + 1;
+  double* rp = &a[(size_t)p * n];
+  double* rq = &a[(size_t)q * n];
+  for (int j = lo; j <= hi; ++j) {
+    double x = rp[j], y = rq[j];
+    rp[j] = c * x + s * y;
+    rq[j] = -s * x + c * y;
+  }
+  for (int i = lo; i <= hi; ++i) {
+    double* ri = &a[(size_t)i * n];
+    double x = ri[p], y = ri[q];
+    ri[p] = c * x + s * y;
+    ri[q] = -s * x + c * y;
+  }
+}
+
+void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>& d, std::vector<double>& e,
+                       std::vector<double>* q) {
+  if (q) {
+    q->assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) (*q)[(size_t)i * n + i] = 1.0;  // q[row * n + col]
+  }
+  auto zero = [&](int p, int col) {
+    // annihilate a[p+1][col] against a[p][col] with a rotation in the plane (p, p+1)
+    double x = a[(size_t)p * n + col], y = a[(size_t)(p + 1) * n + col];
+    if (y == 0.0) return;
+    double r = std::hypot(x, y);
+    double c = x / r, s = y / r;
+    int lo = std::max(0, p - b - 1), hi = std::min(n - 1, p + 1 + b + 1);
+    rotate_sym(a, n, p, c, s, lo, hi);
+    a[(size_t)(p + 1) * n + col] = 0.0;
+    a[(size_t)col * n + p + 1] = 0.0;
+    if (q) {  // Q <- Q G^T : columns p, p+1
+      for (int i = 0; i < n; ++i) {
+        double* qi = &(*q)[(size_t)i * n];
+        double u = qi[p], v = qi[p + 1];
+        qi[p] = c * u + s * v;
+        qi[p + 1] = -s * u + c * v;
+      }
+    }
+  };
+  if (b > 1) {
+    for (int j = 0; j < n - 2; ++j)
+      for (int r = std::min(j + b, n - 1); r >= j + 2; --r) {
+        zero(r - 1, j);
+        // chase the bulge created at (r + b, r - 1)
+        for (int p = r; p + b < n; p += b) zero(p + b - 1, p - 1);
+      }
+  }
+  d.resize(n);
+  e.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) d[i] = a[(size_t)i * n + i];
+  for (int i = 0; i + 1 < n; ++i) e[i] = a[(size_t)(i + 1) * n + i];
+}
+
+// Cholesky G = R^T R (R upper, row-major k x k); false if G is not positive definite enough
+static bool cholesky_upper(int k, const std::vector<double>& G, std::vector<double>& R) {
+  R.assign((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) {
+    double s = G[(size_t)i * k + i];
+    for (int p = 0; p < i; ++p) s -= R[(size_t)p * k + i] * R[(size_t)p * k + i];
+    if (!(s > 1e-28 * std::max(1.0, std::fabs(G[(size_t)i * k + i])))) return false;
+    double rii = std::sqrt(s);
+    R[(size_t)i * k + i] = rii;
+    for (int j = i + 1; j < k; ++j) {
+      double t = G[(size_t)i * k + j];
+      for (int p = 0; p < i; ++p) t -= R[(size_t)p * k + i] * R[(size_t)p * k + j];
+      R[(size_t)i * k + j] = t / rii;
+    }
+  }
+  return true;
+}
+
+static std::vector<double> upper_inverse(int k, const std::vector<double>& R) {
+  std::vector<double> X((size_t)k * k, 0.0);
+  for (int j = 0; j < k; ++j) {
+    X[(size_t)j * k + j] = 1.0 / R[(size_t)j * k + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int p = i + 1; p <= j; ++p) s += R[(size_t)i * k + p] * X[(size_t)p * k + j];
+      X[(size_t)i * k + j] = -s / R[(size_t)i * k + i];
+    }
+  }
+  return X;
+}
+
+static std::vector<double> matmul(int k, const std::vector<double>& A, const std::vector<double>& B) {
+  std::vector<double> C((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i)
+    for (int p = 0; p < k; ++p)
+      for (int j = 0; j < k; ++j) C[(size_t)i * k + j] += A[(size_t)i * k + p] * B[(size_t)p * k + j];
+  return C;
+}
+
+// ------------------------------------------------------------------ block Lanczos
+FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgorithm(
+    GPUArchitecture* architecture, AbstractHamiltonian* hamiltonian, int nbrEigenvalue, int blockSize, int maxIter,
+    double precision)
+    : Arch(architecture),
+      Hamiltonian(hamiltonian),
+      NbrEigenvalue(nbrEigenvalue),
+      BlockSize(blockSize),
+      MaxIter(maxIter),
+      Precision(precision) {
+  Dim = hamiltonian->GetHilbertSpaceDimension();
+  if (BlockSize != 1 && BlockSize != 2 && BlockSize != 4 && BlockSize != 8 && BlockSize != 16)
+    throw std::invalid_argument("block size must be 1, 2, 4, 8 or 16");
+  MaxBlocks = std::max(3, MaxIter / BlockSize + 1);
+  if ((int64_t)MaxBlocks * BlockSize > Dim) MaxBlocks = (int)std::max<int64_t>(2, Dim / BlockSize);
+  d_basis = Arch->AllocateMemory((int64_t)(MaxBlocks + 1) * Dim * BlockSize);
+}
+
+FullReorthogonalizedBlockLanczosAlgorithm::~FullReorthogonalizedBlockLanczosAlgorithm() {
+  Arch->FreeMemory(d_basis);
+}
+
+// W <- W R^{-1} with W = (new W) R, by two Cholesky-QR passes; false on rank deficiency
+void FullReorthogonalizedBlockLanczosAlgorithm::Orthonormalize(double* d_w, std::vector<double>& R) {
+  const int k = BlockSize;
+  dh_context* c = Arch->Context();
+  R.assign((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) R[(size_t)i * k + i] = 1.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<double> G((size_t)k * k), Rp;
+    Arch->Check(dh_block_dots(c, d_w, 1, d_w, Dim, k, G.data()), "gram");
+    if (!cholesky_upper(k, G, Rp)) {
+      R.clear();
+      return;
+    }
+    std::vector<double> inv = upper_inverse(k, Rp);
+    Arch->Check(dh_block_rmul(c, d_w, Dim, k, inv.data()), "rmul");
+    R = matmul(k, Rp, R);
+  }
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::InitializeLanczosAlgorithm(uint64_t seed) {
+  dh_context* c = Arch->Context();
+  A.clear();
+  B.clear();
+  HavePrevious = false;
+  Arch->Check(dh_fill_random(c, d_basis, Dim * BlockSize, seed), "random start");
+  std::vector<double> R;
+  Orthonormalize(d_basis, R);
+  if (R.empty()) throw std::runtime_error("random start block is rank deficient");
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter) {
+  const int k = BlockSize;
+  const int kk = k * k;
+  dh_context* c = Arch->Context();
+  for (int it = 0; it < nbrIter; ++it) {
+    int j = (int)A.size();  // current block X_j
+    if (j + 1 > MaxBlocks) return;
+    double* Xj = d_basis + (int64_t)j * Dim * k;
+    double* W = d_basis + (int64_t)(j + 1) * Dim * k;
+    Arch->Check(dh_hv(Hamiltonian->Handle(), Xj, W, k, 0, 0, -1), "H.v");
+    std::vector<double> Aj((size_t)kk, 0.0);
+    std::vector<double> C((size_t)(j + 1) * kk);
+    for (int pass = 0; pass < 2; ++pass) {
+      Arch->Check(dh_block_dots(c, d_basis, j + 1, W, Dim, k, C.data()), "reorth dots");
+      for (int i = 0; i < kk; ++i) Aj[i] += C[(size_t)j * kk + i];
+      for (auto& v : C) v = -v;
+      Arch->Check(dh_block_axpy(c, d_basis, j + 1, W, Dim, k, C.data()), "reorth axpy");
+    }
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        double s = 0.5 * (Aj[(size_t)p * k + q] + Aj[(size_t)q * k + p]);
+        Aj[(size_t)p * k + q] = Aj[(size_t)q * k + p] = s;
+      }
+    std::vector<double> R;
+    Orthonormalize(W, R);
+    if (R.empty()) {
+      // invariant subspace reached: restart with a random block orthogonal to the basis
+      // (the subspace-collapse branch of FullReorthogonalizedLanczosAlgorithm.cc:270)
+      Arch->Check(dh_fill_random(c, W, Dim * k, 0x5EEDull + 7919ull * (uint64_t)(j + 1)), "restart");
+      for (int pass = 0; pass < 2; ++pass) {
+        Arch->Check(dh_block_dots(c, d_basis, j + 1, W, Dim, k, C.data()), "restart dots");
+        for (auto& v : C) v = -v;
+        Arch->Check(dh_block_axpy(c, d_basis, j + 1, W, Dim, k, C.data()), "restart axpy");
+      }
+      Orthonormalize(W, R);
+      if (R.empty()) throw std::runtime_error("Lanczos restart failed");
+      R.assign((size_t)kk, 0.0);
+    }
+    A.push_back(Aj);
+    B.push_back(R);
+  }
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::Diagonalize(std::vector<double>& evals, std::vector<double>* evecs) {
+  const int k = BlockSize;
+  const int m = (int)A.size();
+  const int n = m * k;
+  std::vector<double> d, e;
+  std::vector<double> Q;
+  if (k == 1) {
+    d.resize(n);
+    e.assign(n, 0.0);
+    for (int i = 0; i < m; ++i) {
+      d[i] = A[i][0];
+      if (i + 1 < m) e[i] = B[i][0];
+    }
+  } else {
+    std::vector<double> T((size_t)n * n, 0.0);
+    for (int j = 0; j < m; ++j)
+      for (int p = 0; p < k; ++p)
+        for (int q = 0; q < k; ++q) {
+          T[(size_t)(j * k + p) * n + j * k + q] = A[j][(size_t)p * k + q];
+          if (j + 1 < m) {
+            double b = B[j][(size_t)p * k + q];  // T[j+1][j] block = B_j
+            T[(size_t)((j + 1) * k + p) * n + j * k + q] = b;
+            T[(size_t)(j * k + q) * n + (j + 1) * k + p] = b;
+          }
+        }
+    BandToTridiagonal(n, k, T, d, e, evecs ? &Q : nullptr);
+  }
+  std::vector<double> z;
+  TridiagonalEigen(n, d, e, evals, evecs ? &z : nullptr);
+  if (evecs) {
+    if (k == 1) {
+      *evecs = z;
+    } else {  // Ritz coefficients in the Krylov basis: Q * z
+      evecs->assign((size_t)n * n, 0.0);
+      for (int col = 0; col < n; ++col)
+        for (int row = 0; row < n; ++row) {
+          double s = 0.0;
+          for (int t = 0; t < n; ++t) s += Q[(size_t)row * n + t] * z[(size_t)col * n + t];
+          (*evecs)[(size_t)col * n + row] = s;
+        }
+    }
+  }
+}
+
+bool FullReorthogonalizedBlockLanczosAlgorithm::TestConvergence() {
+  std::vector<double> ev;
+  Diagonalize(ev, nullptr);
+  int n = (int)ev.size();
+  int want = std::min(NbrEigenvalue, n);
+  std::vector<double> cur(ev.begin(), ev.begin() + want);
+  bool ok = HavePrevious && n > NbrEigenvalue && (int)Previous.size() == want;
+  if (ok)
+    for (int i = 0; i < want; ++i)
+      if (std::fabs(cur[i] - Previous[i]) > Precision * std::fabs(cur[i])) {
+        ok = false;
+        break;
+      }
+  Previous = cur;
+  Current = cur;
+  HavePrevious = true;
+  return ok;
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::GetEigenstates(int nbr, double* d_out) {
+  const int k = BlockSize;
+  const int m = (int)A.size();
+  const int n = m * k;
+  std::vector<double> ev, vecs;
+  Diagonalize(ev, &vecs);
+  dh_context* c = Arch->Context();
+  nbr = std::min(nbr, n);
+  // Ritz vectors in groups of k: block [Dim][k] = V * S, then copied out column by column
+  double* tmp = Arch->AllocateMemory(Dim * k);
+  for (int g0 = 0; g0 < nbr; g0 += k) {
+    Arch->Check(dh_memset(c, tmp, 0, sizeof(double) * Dim * k), "clear");
+    std::vector<double> S((size_t)m * k * k, 0.0);
+    for (int b = 0; b < m; ++b)
+      for (int p = 0; p < k; ++p)
+        for (int q = 0; q < k && g0 + q < nbr; ++q) S[((size_t)b * k + p) * k + q] = vecs[(size_t)(g0 + q) * n + b * k + p];
+    Arch->Check(dh_block_axpy(c, d_basis, m, tmp, Dim, k, S.data()), "Ritz vectors");
+    for (int q = 0; q < k && g0 + q < nbr; ++q)
+      Arch->Check(dh_block_get_column(c, tmp, Dim, k, q, d_out + (int64_t)(g0 + q) * Dim), "extract");
+  }
+  Arch->Check(dh_synchronize(c), "sync");
+  Arch->FreeMemory(tmp);
+}
+
+LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
+  auto t0 = std::chrono::steady_clock::now();
+  LanczosResult res;
+  int first = std::max(2, (NbrEigenvalue + 2 + BlockSize - 1) / BlockSize);
+  RunLanczosAlgorithm(first);
+  while (true) {
+    bool conv = TestConvergence();
+    if (conv) {
+      res.converged = true;
+      break;
+    }
+    if ((int)A.size() >= MaxBlocks || (int64_t)A.size() * BlockSize >= Dim) {
+      res.converged = ((int64_t)A.size() * BlockSize >= Dim);
+      break;
+    }
+    RunLanczosAlgorithm(1);
+  }
+  Arch->Check(dh_synchronize(Arch->Context()), "sync");
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  res.eigenvalues = Current;
+  res.blockIterations = (int)A.size();
+  res.iterations = (int)A.size() * BlockSize;
+  return res;
+}
+
+}  // namespace dhh

@@ -1,0 +1,167 @@
+/*
+ * dhgpu.h -- C ABI of the MI355X (gfx950) on-the-fly Hamiltonian-vector product and
+ * the Lanczos kernels behind it.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative DH_E* code; the message of
+ *     the last failure on a context is in dh_last_error(ctx).  Nothing throws across
+ *     the ABI.
+ *   - Pointers named d_* are device pointers (hipMalloc / dh_malloc, or a torch tensor's
+ *     data_ptr()); all other pointers are host memory.
+ *   - Work is enqueued on the context's stream (dh_set_stream); results are ready after
+ *     dh_synchronize(ctx) (or a hipStreamSynchronize of that stream).
+ *   - One context must be used from one host thread at a time.
+ *   - Block vectors are interleaved [row][k]: component (row, v) of a block of k vectors
+ *     is at d_x[row * k + v].  k = 1 is a plain vector.
+ *   - Real FP64 throughout; complex vectors are interleaved (re, im) pairs.
+ *
+ * Reference interfaces replaced (paths relative to the DiagHam source tree) are cited
+ * per entry point.
+ */
+#ifndef DHGPU_H
+#define DHGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_OK 0
+#define DH_E_ARG -1      /* invalid argument (null pointer, size, mismatched handles) */
+#define DH_E_HIP -2      /* a HIP runtime call failed */
+#define DH_E_NOMEM -3    /* device allocation failed */
+#define DH_E_STATE -4    /* handle in the wrong state (e.g. H.v before factors are set) */
+#define DH_E_RANGE -5    /* size beyond a documented limit */
+
+typedef struct dh_context dh_context;
+typedef struct dh_basis dh_basis;
+typedef struct dh_hamiltonian dh_hamiltonian;
+
+/* index lookup used by the H.v kernels */
+#define DH_INDEX_BSEARCH 0 /* top-bit LUT staged in LDS, then binary search of the sorted basis */
+#define DH_INDEX_RANK 1    /* combinatorial rank: two table lookups, no basis probes */
+
+/* ---------------------------------------------------------------- context
+ * Replaces the per-operation thread fork/join of SMPArchitecture::SendJobs
+ * (src/Architecture/SMPArchitecture.cc:134): one context owns a device and a stream. */
+int dh_create(dh_context** ctx, int device);
+int dh_destroy(dh_context* ctx);
+const char* dh_last_error(dh_context* ctx);
+int dh_set_stream(dh_context* ctx, void* hip_stream);
+int dh_synchronize(dh_context* ctx);
+/* number of compute units and gfx name of the context's device (name >= 64 bytes) */
+int dh_device_info(dh_context* ctx, int* nbr_cu, char* name);
+
+/* Device vector storage; AbstractArchitecture::GetNewRealVector / AllocateMemory hooks
+ * (src/Architecture/AbstractArchitecture.h:150-246). */
+int dh_malloc(dh_context* ctx, void** d_ptr, size_t bytes);
+int dh_free(dh_context* ctx, void* d_ptr);
+int dh_memcpy_h2d(dh_context* ctx, void* d_dst, const void* src, size_t bytes);
+int dh_memcpy_d2h(dh_context* ctx, void* dst, const void* d_src, size_t bytes);
+int dh_memset(dh_context* ctx, void* d_dst, int value, size_t bytes);
+
+/* ---------------------------------------------------------------- bases
+ * FermionOnSphere(nbrFermions, totalLz, lzMax) constructor + GenerateStates +
+ * GenerateLookUpTable (FQHE/src/HilbertSpace/FermionOnSphere.cc:76, :1621, :1659).
+ * The basis is generated on the device by unranking in the reference's order (states
+ * sorted as descending unsigned 64-bit integers, bit i = orbital l_z = i - lzMax/2).
+ * lzMax <= 62. */
+int dh_fermion_sphere_create(dh_context* ctx, dh_basis** basis, int nbr_fermions, int total_lz, int lz_max);
+/* EvaluateHilbertSpaceDimension (FermionOnSphere.cc:1794); no device work. */
+int64_t dh_fermion_sphere_dimension(int nbr_fermions, int total_lz, int lz_max);
+
+/* Spin1_2Chain(chainLength, sz, memory) (Spin/src/HilbertSpace/Spin1_2Chain.cc:113, :331):
+ * states ordered as the reference (site 0 most significant, bits >= L kept at 1 in the
+ * 32-bit word).  chain_length <= 32. */
+int dh_spin_chain_create(dh_context* ctx, dh_basis** basis, int chain_length, int sz);
+
+int dh_basis_destroy(dh_basis* basis);
+int64_t dh_basis_dimension(const dh_basis* basis);
+/* copy the state words (StateDescription) to the host: 8 bytes per state */
+int dh_basis_get_states(dh_basis* basis, uint64_t* states);
+/* FindStateIndex (FermionOnSphere.cc:1431, Spin1_2Chain.cc:1002) for n states at once,
+ * evaluated by the device index lookup `mode`; out-of-space states give dimension. */
+int dh_basis_find_state_index(dh_basis* basis, const uint64_t* states, int64_t n, int mode, int64_t* indices);
+
+/* ---------------------------------------------------------------- Hamiltonians
+ * ParticleOnSphereGenericHamiltonian (FQHE/src/Hamiltonian/ParticleOnSphereGenericHamiltonian.cc:68)
+ * with the full (non-Hermitian-reduced) two-body factor list of EvaluateInteractionFactors
+ * (:493): for every annihilated pair m1 > m2 (pair index m1(m1-1)/2 + m2) the factors of
+ * the created pairs m3 > m4 = m1 + m2 - m3, m3 ascending, are factors[pair_offset[pair] +
+ * (m3 - (m1 + m2) / 2 - 1)].  nbr_factors = pair_offset[nbrPairs].  The host computes the
+ * factors; this call copies them to the device. */
+int dh_sphere_hamiltonian_create(dh_context* ctx, dh_hamiltonian** ham, dh_basis* basis,
+                                 const double* factors, const int* pair_offset, int nbr_factors);
+/* Per-channel factor split used by the adjoint kernel: the two-body factor of (pair a ->
+ * pair b) is sum_c V_c * (-4) * cg[a * nbr_channels + c] * cg[b * nbr_channels + c], with
+ * V_c = channel_weights[c]. */
+int dh_sphere_hamiltonian_set_channels(dh_hamiltonian* ham, const double* cg, int nbr_channels,
+                                       const double* channel_weights);
+
+/* SpinChainHamiltonian(chain, nbrSpin, j, jz, periodic) (Spin/src/Hamiltonian/SpinChainHamiltonian.cc:97):
+ * j and jz have chain_length entries (bond i couples sites i and i+1 mod L). */
+int dh_spin_chain_hamiltonian_create(dh_context* ctx, dh_hamiltonian** ham, dh_basis* basis,
+                                     const double* j, const double* jz, int periodic);
+
+int dh_hamiltonian_destroy(dh_hamiltonian* ham);
+/* AbstractHamiltonian::ShiftHamiltonian (src/Hamiltonian/AbstractHamiltonian.h) */
+int dh_hamiltonian_set_shift(dh_hamiltonian* ham, double shift);
+int64_t dh_hamiltonian_dimension(const dh_hamiltonian* ham);
+
+/* Launch shape of the H.v kernels.  lanes_per_row in {1,2,4,8,16,32,64} (64/lanes_per_row
+ * destination rows per wavefront); block_size a multiple of 64 up to 1024; index_mode
+ * DH_INDEX_*; lds_tables != 0 stages the index tables in LDS when they fit. */
+int dh_hamiltonian_set_launch(dh_hamiltonian* ham, int lanes_per_row, int block_size, int index_mode, int lds_tables);
+
+/* ---------------------------------------------------------------- products
+ * y = (add ? y : 0) + (H + shift) x for a block of k vectors ([row][k] layout).
+ * LowLevelAddMultiply / LowLevelMultiply (src/Hamiltonian/AbstractHamiltonian.h:265,
+ * AbstractHamiltonian.cc:468) for k = 1 and LowLevelMultipleAddMultiply
+ * (AbstractHamiltonian.h:344; FQHE AbstractQHEOnSphereHamiltonian.cc:1217) for k > 1;
+ * rows [first, first + count) of y are written (count = -1: all rows).
+ * k in {1, 2, 4, 8, 16}.  Gather form: every destination row is written once, no atomics. */
+int dh_hv(dh_hamiltonian* ham, const double* d_x, double* d_y, int k, int add, int64_t first, int64_t count);
+
+/* Number of matrix elements per product (full non-Hermitian-reduced terms with a target in
+ * the space, diagonal terms included), counted on the device. */
+int dh_hamiltonian_count_elements(dh_hamiltonian* ham, int64_t* nnz);
+
+/* Matrix elements of row `row` in the order the kernel visits them: (target index,
+ * coefficient).  Returns the count in *n (at most max_out are written). */
+int dh_hamiltonian_row_elements(dh_hamiltonian* ham, int64_t row, int64_t* targets, double* coefs,
+                                int64_t max_out, int64_t* n);
+
+/* ---------------------------------------------------------------- adjoint (no reference counterpart)
+ * g[c] = <psi| H_c |phi> for every channel c of dh_sphere_hamiltonian_set_channels, in one
+ * pass over the basis; if d_hpsi is not null it also receives H psi (the vector-Jacobian
+ * product of x -> H x for the cotangent psi).  Hellmann-Feynman companion of
+ * --energy-expectation (FQHE/src/Programs/FQHEOnSphere/FQHESphereFermionsTwoBodyGeneric.cc:323). */
+int dh_sphere_adjoint(dh_hamiltonian* ham, const double* d_psi, const double* d_phi, double* g, double* d_hpsi);
+
+/* ---------------------------------------------------------------- Lanczos BLAS
+ * Krylov bases are arrays of nbr_blocks blocks of k vectors, block b at d_basis + b * dim * k,
+ * each block in [row][k] layout.
+ * MultipleRealScalarProductOperation (src/Architecture/ArchitectureOperation/MultipleRealScalarProductOperation.cc:174):
+ *   c[(b * k + p) * k + q] = sum_row basis_b[row][p] * w[row][q]   (host result, nbr_blocks*k*k) */
+int dh_block_dots(dh_context* ctx, const double* d_basis, int nbr_blocks, const double* d_w, int64_t dim,
+                  int k, double* c);
+/* AddRealLinearCombinationOperation (AddRealLinearCombinationOperation.cc:157):
+ *   w[row][q] += sum_{b,p} basis_b[row][p] * c[(b * k + p) * k + q]   (c on the host) */
+int dh_block_axpy(dh_context* ctx, const double* d_basis, int nbr_blocks, double* d_w, int64_t dim, int k,
+                  const double* c);
+/* w[row][q] = sum_p w[row][p] * m[p * k + q]  (in-place right multiplication by a k x k host matrix) */
+int dh_block_rmul(dh_context* ctx, double* d_w, int64_t dim, int k, const double* m);
+/* copy column `col` of a [row][k] block into / out of a plain vector of dim doubles */
+int dh_block_get_column(dh_context* ctx, const double* d_w, int64_t dim, int k, int col, double* d_out);
+int dh_block_set_column(dh_context* ctx, double* d_w, int64_t dim, int k, int col, const double* d_in);
+/* y = a * x + b * y over n doubles */
+int dh_axpby(dh_context* ctx, double a, const double* d_x, double b, double* d_y, int64_t n);
+/* uniform [-1, 1) SplitMix64 values (seeded start vectors) */
+int dh_fill_random(dh_context* ctx, double* d_x, int64_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DHGPU_H */

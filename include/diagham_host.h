@@ -1,0 +1,73 @@
+/*
+ * diagham_host.h -- C entry points of the C++ host layer (libdiagham_host.so).
+ *
+ * The host layer mirrors DiagHam's classes on the Lanczos / H.v path (FermionOnSphere,
+ * Spin1_2Chain, ParticleOnSphereGenericHamiltonian, SpinChainHamiltonian, the
+ * full-reorthogonalisation Lanczos algorithms, a GPU architecture) and drives the GPU
+ * through include/dhgpu.h.  These C functions expose that layer to drivers that are not
+ * C++ (test harnesses, benchmarks).  Return codes follow dhgpu.h; messages of failures
+ * raised by the host layer are available from dhh_last_error().
+ */
+#ifndef DIAGHAM_HOST_H
+#define DIAGHAM_HOST_H
+
+#include <stdint.h>
+
+#include "dhgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dhh_arch dhh_arch;       /* GPUArchitecture */
+typedef struct dhh_problem dhh_problem; /* Hilbert space + Hamiltonian of one sector */
+
+const char* dhh_last_error(void);
+
+int dhh_arch_create(dhh_arch** arch, int device);
+int dhh_arch_destroy(dhh_arch* arch);
+dh_context* dhh_arch_context(dhh_arch* arch);
+
+/* FermionOnSphere(nbrFermions, totalLz, lzMax) + ParticleOnSphereGenericHamiltonian with
+ * pseudopotentials V_0..V_lzMax (FQHESphereFermionsTwoBodyGeneric.cc:278-306) */
+int dhh_sphere_problem_create(dhh_arch* arch, dhh_problem** problem, int nbr_fermions, int total_lz, int lz_max,
+                              const double* pseudopotentials);
+/* Spin1_2Chain(L, sz) + SpinChainHamiltonian(j, jz, periodic) (GenericOpenSpinChain.cc:333-355);
+ * j, jz: L entries */
+int dhh_spin_problem_create(dhh_arch* arch, dhh_problem** problem, int chain_length, int sz, const double* j,
+                            const double* jz, int periodic);
+int dhh_problem_destroy(dhh_problem* problem);
+dh_basis* dhh_problem_basis(dhh_problem* problem);
+dh_hamiltonian* dhh_problem_hamiltonian(dhh_problem* problem);
+/* relative angular momenta m of the adjoint channels (sphere problems); returns count */
+int dhh_problem_channels(dhh_problem* problem, int* momenta, int capacity);
+
+/* Full-reorthogonalisation (block) Lanczos to the nbr_eigen lowest eigenvalues
+ * (FullReorthogonalizedLanczosAlgorithm / FullReorthogonalizedBlockLanczosAlgorithm driven
+ * as in QHEOnSphereMainTask.cc:728-808).  block_size 1 = plain.  d_eigenvectors (device,
+ * nbr_vectors * dim doubles, vector-major) may be null.  Outputs: eigenvalues (nbr_eigen),
+ * vector products used, block steps, wall seconds (device work included). */
+int dhh_lanczos(dhh_problem* problem, int nbr_eigen, int block_size, int max_iter, double precision, uint64_t seed,
+                double* eigenvalues, int* products, int* blocks, double* seconds, double* d_eigenvectors,
+                int nbr_vectors);
+
+/* Host-side tables (no GPU needed) */
+/* LLL Coulomb pseudopotentials V_0..V_nbr_flux in CoulombPseudopotentials units; round != 0
+ * rounds to the 14 significant digits written to pseudopotential files */
+int dhh_coulomb_pseudopotentials(int nbr_flux, int round, double* out);
+double dhh_clebsch_gordan(int j1, int j2, int m1, int m2, int j);
+/* factors / pair offsets in the dh_sphere_hamiltonian_create layout; *nbr_factors is the
+ * capacity on input and the count on output */
+int dhh_sphere_factors(int lz_max, const double* pseudopotentials, double* factors, int* pair_offset,
+                       int* nbr_factors);
+/* per-pair, per-channel Clebsch-Gordan table of the adjoint split; returns channel count */
+int dhh_sphere_channel_table(int lz_max, int* momenta, double* table, int capacity);
+/* "Pseudopotentials = ..." file reader (FQHESpherePseudopotentialTools.cc:53) */
+int dhh_read_pseudopotentials(const char* file_name, int lz_max, double* out);
+/* eigenvalues of a symmetric band matrix (dense row-major n x n, half-bandwidth b) */
+int dhh_band_eigenvalues(int n, int b, const double* a, double* eigenvalues);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

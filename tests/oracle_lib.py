@@ -1,0 +1,248 @@
+"""ctypes view of the CPU oracle (oracle/liboracle.so) -- test infrastructure only.
+
+Only tests, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg use this
+module; the product path never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_ROOT, "oracle", "liboracle.so")
+_lib = None
+
+c_int, c_long, c_double, c_void_p, c_uint64, c_ulong = (
+    ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_ulong)
+P_D = ctypes.POINTER(ctypes.c_double)
+P_L = ctypes.POINTER(ctypes.c_long)
+P_I = ctypes.POINTER(ctypes.c_int)
+P_U64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _sig(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.dirname(_LIB_PATH)])
+    L = ctypes.CDLL(_LIB_PATH)
+    _sig(L, "or_fs_dimension", c_long, [c_int, c_int, c_int])
+    _sig(L, "or_fs_create", c_void_p, [c_int, c_int, c_int, c_ulong])
+    _sig(L, "or_fs_destroy", None, [c_void_p])
+    _sig(L, "or_fs_dim", c_long, [c_void_p])
+    _sig(L, "or_fs_states", P_U64, [c_void_p])
+    _sig(L, "or_fs_state_lzmax", P_I, [c_void_p])
+    _sig(L, "or_fs_find_state_index", c_long, [c_void_p, c_uint64, c_int])
+    _sig(L, "or_fs_adadaa", c_long, [c_void_p, c_long, c_int, c_int, c_int, c_int, P_D])
+    _sig(L, "or_fs_aa", c_double, [c_void_p, c_long, c_int, c_int])
+    _sig(L, "or_fs_adad", c_long, [c_void_p, c_int, c_int, P_D])
+    _sig(L, "or_cg_create", c_void_p, [c_int, c_int])
+    _sig(L, "or_cg_destroy", None, [c_void_p])
+    _sig(L, "or_cg_get", c_double, [c_void_p, c_int, c_int, c_int])
+    _sig(L, "or_coulomb_pseudopotentials", None, [c_int, P_D])
+    _sig(L, "or_coulomb_closed_form", None, [c_int, P_D])
+    _sig(L, "or_factors_create", c_void_p, [c_int, P_D])
+    _sig(L, "or_factors_hermitian", None, [c_void_p])
+    _sig(L, "or_factors_destroy", None, [c_void_p])
+    _sig(L, "or_factors_nbr_m12", c_int, [c_void_p])
+    _sig(L, "or_factors_nbr", c_int, [c_void_p])
+    _sig(L, "or_factors_get", None, [c_void_p, P_I, P_I, P_I, P_I, P_D])
+    _sig(L, "or_fs_hv", None, [c_void_p, c_void_p, P_D, P_D, c_double, c_int])
+    _sig(L, "or_fs_hv_smp", None, [c_void_p, c_void_p, P_D, P_D, c_double, c_int, c_int])
+    _sig(L, "or_fs_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, c_long])
+    _sig(L, "or_fs_count_elements", c_long, [c_void_p, c_void_p])
+    _sig(L, "or_spin_create", c_void_p, [c_int, c_int, c_int])
+    _sig(L, "or_spin_destroy", None, [c_void_p])
+    _sig(L, "or_spin_dim", c_long, [c_void_p])
+    _sig(L, "or_spin_states", P_U64, [c_void_p])
+    _sig(L, "or_spin_find_state_index", c_long, [c_void_p, c_uint64])
+    _sig(L, "or_spin_hv", None, [c_void_p, P_D, P_D, c_int, P_D, P_D])
+    _sig(L, "or_spin_hv_smp", None, [c_void_p, P_D, P_D, c_int, P_D, P_D, c_int])
+    _sig(L, "or_tridiag_eigenvalues", c_int, [c_int, P_D, P_D, c_int])
+    _sig(L, "or_fs_lanczos", c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_double,
+                                     c_uint64, c_int, P_D, P_D])
+    _sig(L, "or_spin_lanczos", c_int, [c_void_p, P_D, P_D, c_int, c_int, c_int, c_double, c_uint64,
+                                       c_int, P_D])
+    _sig(L, "or_splitmix_fill", None, [c_uint64, P_D, c_long])
+    _lib = L
+    return L
+
+
+def dptr(a):
+    return a.ctypes.data_as(P_D)
+
+
+def splitmix(seed, n):
+    out = np.empty(n, dtype=np.float64)
+    lib().or_splitmix_fill(seed, dptr(out), n)
+    return out
+
+
+class FermionSphere:
+    """FermionOnSphere restatement (FQHE/src/HilbertSpace/FermionOnSphere.cc)."""
+
+    def __init__(self, n, total_lz, lz_max, memory=9 << 20):
+        self.n, self.total_lz, self.lz_max = n, total_lz, lz_max
+        self.h = lib().or_fs_create(n, total_lz, lz_max, memory)
+        if not self.h:
+            raise ValueError("empty Hilbert space")
+        self.dim = lib().or_fs_dim(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_fs_destroy(self.h)
+            self.h = None
+
+    def states(self):
+        p = lib().or_fs_states(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.dim,)).copy()
+
+    def state_lzmax(self):
+        p = lib().or_fs_state_lzmax(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.dim,)).copy()
+
+    def find(self, state, lzmax):
+        return lib().or_fs_find_state_index(self.h, int(state), int(lzmax))
+
+    def adadaa(self, i, m1, m2, n1, n2):
+        c = c_double(0.0)
+        idx = lib().or_fs_adadaa(self.h, i, m1, m2, n1, n2, ctypes.byref(c))
+        return idx, c.value
+
+    def aa(self, i, n1, n2):
+        return lib().or_fs_aa(self.h, i, n1, n2)
+
+    def adad(self, m1, m2):
+        c = c_double(0.0)
+        idx = lib().or_fs_adad(self.h, m1, m2, ctypes.byref(c))
+        return idx, c.value
+
+
+class Factors:
+    """Interaction factor tables of ParticleOnSphereGenericHamiltonian (fermions)."""
+
+    def __init__(self, lz_max, pseudopotentials, hermitian=False):
+        v = np.ascontiguousarray(pseudopotentials, dtype=np.float64)
+        assert v.size == lz_max + 1
+        self.h = lib().or_factors_create(lz_max, dptr(v))
+        if hermitian:
+            lib().or_factors_hermitian(self.h)
+        self.lz_max = lz_max
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_factors_destroy(self.h)
+            self.h = None
+
+    def tables(self):
+        n12 = lib().or_factors_nbr_m12(self.h)
+        nf = lib().or_factors_nbr(self.h)
+        m1 = np.zeros(n12, np.int32)
+        m2 = np.zeros(n12, np.int32)
+        nm3 = np.zeros(n12, np.int32)
+        m3 = np.zeros(max(nf, 1), np.int32)
+        f = np.zeros(max(nf, 1), np.float64)
+        lib().or_factors_get(self.h, m1.ctypes.data_as(P_I), m2.ctypes.data_as(P_I),
+                             nm3.ctypes.data_as(P_I), m3.ctypes.data_as(P_I), dptr(f))
+        return m1, m2, nm3, m3[:nf], f[:nf]
+
+    @property
+    def nbr_factors(self):
+        return lib().or_factors_nbr(self.h)
+
+
+def coulomb_pseudopotentials(nbr_flux):
+    out = np.zeros(nbr_flux + 1)
+    lib().or_coulomb_pseudopotentials(nbr_flux, dptr(out))
+    return out
+
+
+def coulomb_closed_form(nbr_flux):
+    out = np.zeros(nbr_flux + 1)
+    lib().or_coulomb_closed_form(nbr_flux, dptr(out))
+    return out
+
+
+def round14(v):
+    """Values as written by CoulombPseudopotentials (File.precision(14), %.14g)."""
+    return np.array([float("%.14g" % x) for x in v])
+
+
+def fs_hv(space, factors, x, shift=0.0, hermitian=False, threads=1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    if threads > 1:
+        lib().or_fs_hv_smp(space.h, factors.h, dptr(x), dptr(y), shift, int(hermitian), threads)
+    else:
+        lib().or_fs_hv(space.h, factors.h, dptr(x), dptr(y), shift, int(hermitian))
+    return y
+
+
+def fs_row_elements(space, factors, row, max_out=100000):
+    t = np.zeros(max_out, np.int64)
+    c = np.zeros(max_out, np.float64)
+    n = lib().or_fs_row_elements(space.h, factors.h, row, t.ctypes.data_as(P_L), dptr(c), max_out)
+    assert n <= max_out
+    return t[:n], c[:n]
+
+
+def fs_count_elements(space, factors):
+    return lib().or_fs_count_elements(space.h, factors.h)
+
+
+def fs_lanczos(space, factors, n_eig, shift=0.0, hermitian=False, max_iter=3000, precision=1e-14,
+               seed=0x5EED0000, threads=8):
+    ev = np.zeros(n_eig)
+    it = lib().or_fs_lanczos(space.h, factors.h, shift, int(hermitian), n_eig, max_iter, precision,
+                             seed, threads, dptr(ev), None)
+    return ev, it
+
+
+class SpinChain:
+    """Spin1_2Chain restatement (Spin/src/HilbertSpace/Spin1_2Chain.cc)."""
+
+    def __init__(self, length, sz, memory=1000000):
+        self.length = length
+        self.h = lib().or_spin_create(length, sz, memory)
+        self.dim = lib().or_spin_dim(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_spin_destroy(self.h)
+            self.h = None
+
+    def states(self):
+        p = lib().or_spin_states(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.dim,)).copy()
+
+    def find(self, state):
+        return lib().or_spin_find_state_index(self.h, int(state))
+
+    def couplings(self, j=1.0, jz=1.0):
+        return np.full(self.length, j, np.float64), np.full(self.length, jz, np.float64)
+
+    def hv(self, x, j=1.0, jz=1.0, periodic=True, threads=1):
+        J, Jz = self.couplings(j, jz)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        if threads > 1:
+            lib().or_spin_hv_smp(self.h, dptr(J), dptr(Jz), int(periodic), dptr(x), dptr(y), threads)
+        else:
+            lib().or_spin_hv(self.h, dptr(J), dptr(Jz), int(periodic), dptr(x), dptr(y))
+        return y
+
+    def lanczos(self, n_eig, j=1.0, jz=1.0, periodic=True, max_iter=3000, precision=1e-14,
+                seed=0x5EED0000, threads=8):
+        J, Jz = self.couplings(j, jz)
+        ev = np.zeros(n_eig)
+        it = lib().or_spin_lanczos(self.h, dptr(J), dptr(Jz), int(periodic), n_eig, max_iter,
+                                   precision, seed, threads, dptr(ev))
+        return ev, it

@@ -1,0 +1,137 @@
+"""GPU parity of the fermions-on-the-sphere path against the oracle (oracle/liboracle.so)
+and the reference outputs in tests/golden.  Every call goes through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+SECTORS = [(4, 2, 8), (8, 0, 21), (8, 6, 21), (6, 4, 15), (10, 0, 27)]
+
+
+def coulomb(lz_max):
+    return O.round14(O.coulomb_pseudopotentials(lz_max))
+
+
+@pytest.fixture(scope="module")
+def cfg1(arch):
+    from diagham_amd import capi
+    p = capi.Problem.sphere(arch, 8, 0, 21, coulomb(21))
+    yield p
+    p.close()
+
+
+@pytest.mark.parametrize("n,total_lz,lz_max", SECTORS)
+def test_basis_bit_exact(arch, n, total_lz, lz_max):
+    from diagham_amd import capi
+    p = capi.Problem.sphere(arch, n, total_lz, lz_max, coulomb(lz_max))
+    ref = O.FermionSphere(n, total_lz, lz_max)
+    assert p.dim == ref.dim
+    assert np.array_equal(p.states(), ref.states())
+    p.close()
+
+
+def test_doc_fixture_basis(arch, golden):
+    from diagham_amd import capi
+    g = golden["fermion_sphere_doc_basis_n4_2s8_lz1"]
+    p = capi.Problem.sphere(arch, g["nbr_fermions"], g["total_lz"], g["lz_max"], coulomb(g["lz_max"]))
+    expect = [sum(1 << i for i, o in enumerate(row) if o) for row in g["occupations"]]
+    assert [int(s) for s in p.states()] == expect
+    p.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_find_state_index(cfg1, mode):
+    states = cfg1.states()
+    idx = cfg1.find_state_index(states, mode)
+    assert np.array_equal(idx, np.arange(cfg1.dim))
+    # states outside the sector (wrong particle number / wrong Lz / beyond lzMax) give dim
+    s0 = int(states[0])
+    bad = np.array([s0 | (1 << 30), s0 & (s0 - 1), 0], dtype=np.uint64)
+    assert np.all(cfg1.find_state_index(bad, mode) == cfg1.dim)
+
+
+def test_element_targets_bit_exact(cfg1):
+    ref_space = O.FermionSphere(8, 0, 21)
+    fac = O.Factors(21, coulomb(21))
+    rows = list(range(0, cfg1.dim, 97)) + [cfg1.dim - 1]
+    for r in rows:
+        t_gpu, c_gpu = cfg1.row_elements(r)
+        t_ref, c_ref = O.fs_row_elements(ref_space, fac, r)
+        o_gpu = np.lexsort((c_gpu, t_gpu))
+        o_ref = np.lexsort((c_ref, t_ref))
+        assert np.array_equal(t_gpu[o_gpu], t_ref[o_ref]), r
+        np.testing.assert_allclose(c_gpu[o_gpu], c_ref[o_ref], rtol=0, atol=1e-13)
+
+
+def test_element_count(cfg1, golden):
+    assert cfg1.count_elements() == golden["fermions_sphere_n8_2s21_lz0"]["full_matrix_elements"]
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2)])
+def test_hv_parity(cfg1, k, mode, lpr):
+    """H.X (k vectors, [row][k] layout) vs the oracle, 1e-12 relative norm per vector."""
+    ref_space = O.FermionSphere(8, 0, 21)
+    fac = O.Factors(21, coulomb(21))
+    X = np.stack([O.splitmix(100 + v, cfg1.dim) for v in range(k)], axis=1)
+    cfg1.set_launch(lanes_per_row=lpr, block_size=256, index_mode=mode)
+    Y = cfg1.hv(X.reshape(-1), k).reshape(cfg1.dim, k)
+    for v in range(k):
+        y_ref = O.fs_hv(ref_space, fac, X[:, v])
+        err = np.linalg.norm(Y[:, v] - y_ref) / np.linalg.norm(y_ref)
+        assert err < 1e-12, (k, mode, lpr, v, err)
+    cfg1.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
+
+
+def test_hv_shift_add_and_range(cfg1, arch):
+    from diagham_amd import capi
+    ref_space = O.FermionSphere(8, 0, 21)
+    fac = O.Factors(21, coulomb(21))
+    x = O.splitmix(7, cfg1.dim)
+    y0 = O.splitmix(8, cfg1.dim)
+    shift = -0.5 * 64 / (0.5 * 21)  # FQHESphereFermionsTwoBodyGeneric.cc:320
+    cfg1.set_shift(shift)
+    dx = capi.DeviceBuffer.from_host(arch, x)
+    dy = capi.DeviceBuffer.from_host(arch, y0)
+    cfg1.hv_device(dx.ptr, dy.ptr, 1, add=True)
+    got = dy.download()
+    ref = y0 + O.fs_hv(ref_space, fac, x, shift=shift)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+    # a row range writes only those rows
+    dy.upload(y0)
+    cfg1.hv_device(dx.ptr, dy.ptr, 1, add=False, first=100, count=1000)
+    got = dy.download()
+    full = O.fs_hv(ref_space, fac, x, shift=shift)
+    assert np.array_equal(got[:100], y0[:100]) and np.array_equal(got[1100:], y0[1100:])
+    assert np.linalg.norm(got[100:1100] - full[100:1100]) / np.linalg.norm(full[100:1100]) < 1e-12
+    cfg1.set_shift(0.0)
+
+
+def test_hermitian_reference_form_agrees(cfg1):
+    """The reference's Hermitian-halved scatter (HermitianLowLevelAddMultiply) and the gather agree."""
+    ref_space = O.FermionSphere(8, 0, 21)
+    fac_h = O.Factors(21, coulomb(21), hermitian=True)
+    x = O.splitmix(11, cfg1.dim)
+    y_ref = O.fs_hv(ref_space, fac_h, x, hermitian=True)
+    y = cfg1.hv(x)
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-12
+
+
+@pytest.mark.parametrize("block", [1, 4, 8])
+def test_lanczos_reference_eigenvalues(cfg1, golden, block):
+    ref = np.array(golden["fermions_sphere_n8_2s21_lz0"]["eigenvalues"])
+    r = cfg1.lanczos(10, block_size=block)
+    assert r["converged"]
+    np.testing.assert_allclose(r["eigenvalues"], ref, rtol=0, atol=1e-10)
+
+
+def test_lanczos_n10_2s27(arch, golden):
+    from diagham_amd import capi
+    p = capi.Problem.sphere(arch, 10, 0, 27, coulomb(27))
+    ref = np.array(golden["fermions_sphere_n10_2s27_lz0"]["eigenvalues"])
+    assert p.dim == golden["fermions_sphere_n10_2s27_lz0"]["dimension"]
+    r = p.lanczos(2, block_size=1)
+    np.testing.assert_allclose(r["eigenvalues"], ref, rtol=0, atol=1e-10)
+    p.close()

@@ -1,0 +1,129 @@
+"""CPU checks of the product's host layer and of the C ABI boundary (no GPU calls)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(dhh?_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header,lib", [("dhgpu.h", "libdhgpu.so"), ("diagham_host.h", "libdiagham_host.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    from diagham_amd import capi
+    path = os.path.join(ROOT, "diagham_amd", lib)
+    assert os.path.exists(path), "build the native libraries first"
+    out = subprocess.run(["nm", "-D", "--defined-only", path], stdout=subprocess.PIPE, text=True, check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if " T " in line)
+    decl = declared_functions(header)
+    assert decl, header
+    missing = [d for d in decl if d not in exported]
+    assert not missing, missing
+    # the Python bindings cover exactly the declared ABI
+    sigs = capi.GPU_SIGNATURES if lib == "libdhgpu.so" else capi.HOST_SIGNATURES
+    assert sorted(sigs) == decl
+
+
+def test_libraries_load_without_gpu():
+    from diagham_amd import capi
+    capi.gpu()
+    capi.host()
+
+
+def test_dimension_without_gpu(golden):
+    from diagham_amd import capi
+    assert capi.fermion_sphere_dimension(8, 0, 21) == golden["fermions_sphere_n8_2s21_lz0"]["dimension"]
+    assert capi.fermion_sphere_dimension(10, 0, 27) == golden["fermions_sphere_n10_2s27_lz0"]["dimension"]
+    assert capi.fermion_sphere_dimension(12, 0, 33) == 7764392
+    assert capi.fermion_sphere_dimension(8, 1, 21) == 0  # wrong parity
+
+
+@pytest.mark.parametrize("lz", [8, 21, 27, 33])
+def test_host_pseudopotentials_match_oracle(lz):
+    from diagham_amd import capi
+    a = capi.coulomb_pseudopotentials(lz, round_to_file=False)
+    b = O.coulomb_pseudopotentials(lz)
+    assert np.max(np.abs(a - b) / np.abs(b)) < 1e-13
+    assert np.array_equal(capi.coulomb_pseudopotentials(lz, True), O.round14(a))
+
+
+@pytest.mark.parametrize("lz", [8, 21, 27, 33])
+def test_host_factors_match_oracle(lz):
+    """Product factors (113-bit Racah CG) vs oracle factors (reference recursion)."""
+    from diagham_amd import capi
+    V = O.round14(O.coulomb_pseudopotentials(lz))
+    f, off = capi.sphere_factors(lz, V)
+    m1, m2, nm3, m3, F = O.Factors(lz, V).tables()
+    assert f.size == F.size
+    assert np.array_equal(np.diff(off), nm3)
+    assert np.max(np.abs(f - F)) < 1e-13 * np.max(np.abs(F))
+
+
+def test_channel_split_reconstructs_factors():
+    """sum_c V_c (-4) cg[a][c] cg[b][c] == factor(a -> b): the adjoint split is exact."""
+    from diagham_amd import capi
+    lz = 15
+    rng = np.random.default_rng(1)
+    V = rng.uniform(0.1, 1.0, lz + 1)
+    f, off = capi.sphere_factors(lz, V)
+    mom, tab = capi.sphere_channel_table(lz)
+    assert list(mom) == list(range(1, lz + 1, 2))
+    w = V[mom]
+    pos = 0
+    for m1 in range(lz + 1):
+        for m2 in range(m1):
+            a = m1 * (m1 - 1) // 2 + m2
+            s = m1 + m2
+            for m3 in range(s // 2 + 1, min(s, lz) + 1):
+                m4 = s - m3
+                b = m3 * (m3 - 1) // 2 + m4
+                rec = -4.0 * np.sum(w * tab[a] * tab[b])
+                assert abs(rec - f[pos]) < 1e-13
+                pos += 1
+    assert pos == f.size
+
+
+def test_host_clebsch_gordan_matches_oracle():
+    from diagham_amd import capi
+    cg = O.lib().or_cg_create(27, 27)
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        m1 = int(rng.integers(0, 28)) * 2 - 27
+        m2 = int(rng.integers(0, 28)) * 2 - 27
+        jmin = abs(m1 + m2)
+        J = jmin + 2 * int(rng.integers(0, (54 - jmin) // 2 + 1))
+        assert abs(capi.clebsch_gordan(27, 27, m1, m2, J) - O.lib().or_cg_get(cg, m1, m2, J)) < 1e-13
+    O.lib().or_cg_destroy(cg)
+
+
+@pytest.mark.parametrize("n,b", [(50, 1), (64, 4), (96, 8), (33, 16)])
+def test_band_eigenvalues(n, b):
+    from diagham_amd import capi
+    rng = np.random.default_rng(n + b)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    A[np.abs(np.subtract.outer(np.arange(n), np.arange(n))) > b] = 0.0
+    assert np.max(np.abs(capi.band_eigenvalues(A, b) - np.linalg.eigvalsh(A))) < 1e-12 * np.max(np.abs(A)) * n
+
+
+def test_pseudopotential_file_reader(tmp_path):
+    from diagham_amd import capi
+    import ctypes
+    lz = 9
+    v = capi.coulomb_pseudopotentials(lz)
+    p = tmp_path / "pp.dat"
+    p.write_text("# pseudopotentials on the sphere\nName = test\nPseudopotentials = %s\n" %
+                 " ".join("%.14g" % x for x in v))
+    out = np.zeros(lz + 1)
+    rc = capi.host().dhh_read_pseudopotentials(str(p).encode(), lz, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0 and np.array_equal(out, v)

@@ -1,0 +1,220 @@
+"""Benchmark of the on-the-fly H.v + Lanczos path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Headline workload: spin-1/2 Heisenberg chain, L = 28 sites, periodic, J = 1, Sz = 0
+(40,116,600 states).  One step = one H.v over the whole basis (the one-vector product).
+The JSON line carries matrix elements / s, the roofline of that kernel (algorithmic HBM
+bytes per launch from HIP events on the kernel's stream), the Lanczos time to the 4 lowest
+eigenvalues of that chain, a block-vs-plain Lanczos comparison, and the oracle's CPU rate on
+a bounded sample of the same product on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H·v matrix elements/s + Lanczos time-to-10-eigvals, block vs plain, %HBM peak"
+HBM_PEAK = 8.0e12
+
+
+class HipTimer:
+    """hipEvent pair on a given stream (the stream the kernels are launched on)."""
+
+    def __init__(self, stream):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.stream = stream
+        self.a, self.b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(self.a)) == 0
+        assert self.hip.hipEventCreate(ctypes.byref(self.b)) == 0
+
+    def start(self):
+        assert self.hip.hipEventRecord(self.a, self.stream) == 0
+
+    def stop_ms(self):
+        assert self.hip.hipEventRecord(self.b, self.stream) == 0
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), self.a, self.b) == 0
+        return float(ms.value)
+
+
+def make_stream():
+    hip = ctypes.CDLL("libamdhip64.so")
+    s = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(s)) == 0  # default priority
+    return s
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def spin_hv_bench(capi, arch, stream, L, steps, warmup, k=1):
+    p = capi.Problem.spin_chain(arch, L, 0, 1.0, 1.0, True)
+    p.set_launch(1, 256, capi.INDEX_RANK, 1)
+    nnz = p.count_elements()
+    x = capi.DeviceBuffer(arch, p.dim * k)
+    y = capi.DeviceBuffer(arch, p.dim * k)
+    capi.gpu().dh_fill_random(arch.ctx, x.ptr, p.dim * k, 0x5EED0001)
+    for _ in range(warmup):
+        p.hv_device(x.ptr, y.ptr, k)
+    arch.sync()
+    timer = HipTimer(stream)
+    t0 = time.perf_counter()
+    timer.start()
+    for _ in range(steps):
+        p.hv_device(x.ptr, y.ptr, k)
+    ms = timer.stop_ms()
+    arch.sync()
+    wall = time.perf_counter() - t0
+    x.free()
+    y.free()
+    out = {"dim": p.dim, "nnz": nnz, "ms_per_step": ms / steps, "wall_s": wall}
+    p.close()
+    return out
+
+
+def copy_calibration(capi, arch, stream, n=1 << 27, reps=10):
+    """Streaming copy of n doubles (the HBM-traffic calibration kernel): GB/s."""
+    x = capi.DeviceBuffer(arch, n)
+    y = capi.DeviceBuffer(arch, n)
+    capi.gpu().dh_fill_random(arch.ctx, x.ptr, n, 3)
+    for _ in range(2):
+        capi.gpu().dh_copy(arch.ctx, y.ptr, x.ptr, n)
+    timer = HipTimer(stream)
+    timer.start()
+    for _ in range(reps):
+        capi.gpu().dh_copy(arch.ctx, y.ptr, x.ptr, n)
+    ms = timer.stop_ms() / reps
+    x.free()
+    y.free()
+    return {"doubles": n, "ms": ms, "GBps": 16.0 * n / (ms * 1e-3) / 1e9}
+
+
+def lanczos_runs(capi, arch):
+    res = {}
+    # time to the 4 lowest eigenvalues of the L = 28 chain (plain, full reorthogonalisation)
+    p = capi.Problem.spin_chain(arch, 28, 0)
+    r = p.lanczos(4, block_size=1, max_iter=400)
+    res["heisenberg_L28_4_eigenvalues"] = {"seconds": r["seconds"], "products": r["products"],
+                                           "eigenvalues": r["eigenvalues"].tolist(), "converged": r["converged"]}
+    p.close()
+    # block vs plain to the 10 lowest eigenvalues of N = 8 fermions at 2S = 21, Lz = 0
+    pp = capi.coulomb_pseudopotentials(21, True)
+    p = capi.Problem.sphere(arch, 8, 0, 21, pp)
+    out = {}
+    for block in (1, 8):
+        r = p.lanczos(10, block_size=block)
+        out["k%d" % block] = {"seconds": r["seconds"], "products": r["products"], "blocks": r["blocks"],
+                              "e0": float(r["eigenvalues"][0]), "e9": float(r["eigenvalues"][9])}
+    out["speedup_block_over_plain"] = out["k1"]["seconds"] / out["k8"]["seconds"]
+    res["fermions_n8_2s21_10_eigenvalues"] = out
+    p.close()
+    return res
+
+
+def cpu_baseline(L=28, seconds_target=15.0):
+    """Oracle (CPU restatement of SpinChainHamiltonian::LowLevelAddMultiply, reference thread
+    split) on the host cores, over a bounded row sample of the same L = 28 product."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    s = O.SpinChain(L, 0)
+    J = np.ones(L)
+    x = O.splitmix(1, s.dim)
+    y = np.zeros(s.dim)
+    rows = min(s.dim, 1 << 18)
+    t0 = time.perf_counter()
+    O.lib().or_spin_hv_rows(s.h, O.dptr(J), O.dptr(J), 1, O.dptr(x), O.dptr(y), 0, rows, cores)
+    dt = time.perf_counter() - t0
+    # grow the sample to about seconds_target of CPU work
+    if dt < seconds_target / 4:
+        rows = int(min(s.dim, rows * max(1.0, seconds_target / max(dt, 1e-3))))
+        y[:] = 0.0
+        t0 = time.perf_counter()
+        O.lib().or_spin_hv_rows(s.h, O.dptr(J), O.dptr(J), 1, O.dptr(x), O.dptr(y), 0, rows, cores)
+        dt = time.perf_counter() - t0
+    states = s.states()[:rows] & ((1 << L) - 1)
+    rot = ((states >> 1) | ((states & 1) << (L - 1)))
+    anti = np.unpackbits((states ^ rot).astype(">u4").view(np.uint8)).sum()
+    elements = int(anti) + rows  # off-diagonal flips + diagonal
+    return {"value": elements / dt, "unit": "matrix_elements/s", "cores": cores, "kind": "port",
+            "sample": "oracle H.v over rows [0, %d) of the L=28 basis (%d matrix elements, %.1f s)" %
+                      (rows, elements, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--skip-lanczos", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--length", type=int, default=28)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from diagham_amd import capi
+    arch = capi.Arch(0)
+    stream = make_stream()
+    arch.check(capi.gpu().dh_set_stream(arch.ctx, stream), "dh_set_stream")
+    ncu, name = arch.device_info()
+
+    hv = spin_hv_bench(capi, arch, stream, args.length, args.steps, args.warmup, k=1)
+    dim, nnz, ms = hv["dim"], hv["nnz"], hv["ms_per_step"]
+    alg_bytes = dim * (8 + 8) + nnz * 8  # state word + y write per row, one x read per element
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    traffic = load_traffic("traffic_spin_hv_L%d.json" % args.length)
+    calib = copy_calibration(capi, arch, stream)
+    line = {
+        "metric": METRIC,
+        "value": nnz / (ms * 1e-3),
+        "unit": "matrix_elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform [-1,1) vector; exact Hamiltonian)",
+        "config": {"workload": "heisenberg_chain_L%d_periodic_J1_Sz0_one_vector_hv" % args.length,
+                   "dimension": dim, "matrix_elements_per_hv": nnz, "k": 1, "index_lookup": "rank",
+                   "device": name, "compute_units": ncu},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / (HBM_PEAK / 1e9),
+                     "traffic": (traffic or {}).get("bytes_per_launch"),
+                     "kernel": "spin_hv_kernel<1, RANK>", "algorithmic_bytes_per_launch": alg_bytes},
+        "copy_calibration": calib,
+    }
+    if not args.skip_lanczos:
+        line["lanczos"] = lanczos_runs(capi, arch)
+    if not args.skip_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.length)
+    arch.sync()
+    print(json.dumps(line))
+    sys.stdout.flush()
+
+
+if __name__ == "__main__":
+    main()

@@ -59,11 +59,23 @@ GPU_SIGNATURES = {
     "dh_hamiltonian_count_elements": (c_int, [c_void_p, P_I64]),
     "dh_hamiltonian_row_elements": (c_int, [c_void_p, c_int64, P_I64, P_D, c_int64, P_I64]),
     "dh_sphere_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, P_D, c_void_p]),
+    "dh_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, P_I, c_int]),
+    "dh_sector_set_destroy": (c_int, [c_void_p]),
+    "dh_sector_set_dimension": (c_int64, [c_void_p]),
+    "dh_sector_set_offsets": (c_int, [c_void_p, P_I64]),
+    "dh_sector_set_get_states": (c_int, [c_void_p, P_U64]),
+    "dh_sector_set_factors": (c_int, [c_void_p, P_D, P_I, c_int, c_double]),
+    "dh_sector_set_launch": (c_int, [c_void_p, c_int, c_int]),
+    "dh_sectors_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_char_p]),
+    "dh_sectors_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
+    "dh_sectors_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
+    "dh_sectors_block_rmul": (c_int, [c_void_p, c_void_p, c_int, c_char_p, P_D]),
     "dh_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_rmul": (c_int, [c_void_p, c_void_p, c_int64, c_int, P_D]),
     "dh_block_get_column": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
     "dh_block_set_column": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
+    "dh_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
     "dh_axpby": (c_int, [c_void_p, c_double, c_void_p, c_double, c_void_p, c_int64]),
     "dh_fill_random": (c_int, [c_void_p, c_void_p, c_int64, c_uint64]),
 }
@@ -79,8 +91,10 @@ HOST_SIGNATURES = {
     "dhh_problem_basis": (c_void_p, [c_void_p]),
     "dhh_problem_hamiltonian": (c_void_p, [c_void_p]),
     "dhh_problem_channels": (c_int, [c_void_p, P_I, c_int]),
-    "dhh_lanczos": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_uint64, P_D, P_I, P_I, P_D, c_void_p,
-                            c_int]),
+    "dhh_lanczos": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_uint64, P_D, P_I, P_I, P_D,
+                            c_void_p, c_int]),
+    "dhh_sectors_solve": (c_int, [c_void_p, c_int, c_int, P_I, c_int, P_D, c_int, c_int, c_int, c_double, c_uint64,
+                                  c_int, c_double, P_D, P_I, P_I, P_D]),
     "dhh_coulomb_pseudopotentials": (c_int, [c_int, c_int, P_D]),
     "dhh_clebsch_gordan": (c_double, [c_int, c_int, c_int, c_int, c_int]),
     "dhh_sphere_factors": (c_int, [c_int, P_D, P_D, P_I, P_I]),
@@ -353,10 +367,10 @@ class Problem:
 
     # -- Lanczos
     def lanczos(self, nbr_eigen, block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000,
-                d_vectors=None, nbr_vectors=0, allow_unconverged=False):
+                d_vectors=None, nbr_vectors=0, allow_unconverged=False, residual=0.0):
         ev = np.zeros(nbr_eigen)
         prods, blocks, secs = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        rc = host().dhh_lanczos(self.h, nbr_eigen, block_size, max_iter, precision, seed, dptr(ev),
+        rc = host().dhh_lanczos(self.h, nbr_eigen, block_size, max_iter, precision, residual, seed, dptr(ev),
                                 ctypes.byref(prods), ctypes.byref(blocks), ctypes.byref(secs), d_vectors,
                                 nbr_vectors)
         if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):

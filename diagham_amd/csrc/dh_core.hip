@@ -169,8 +169,8 @@ __global__ void __launch_bounds__(256) dh_block_dots_kernel(const double* __rest
                                                             int64_t rowsPerChunk, double* __restrict__ partial) {
   constexpr int KK = K * K;
   __shared__ double red[256 / 64][G * KK];
-  const int chunk = blockIdx.x;
-  const int b0 = blockIdx.y * G;
+  const int chunk = blockIdx.y;  // blocks of one chunk are adjacent: its W rows stay in L2
+  const int b0 = blockIdx.x * G;
   const int64_t rowBegin = (int64_t)chunk * rowsPerChunk;
   const int64_t rowEnd = min(dim, rowBegin + rowsPerChunk);
   double acc[G * KK];
@@ -233,7 +233,7 @@ static int launch_dots(dh_context* c, const double* V, int nb, const double* W, 
   int rc = dh_scratch(c, sizeof(double) * ((size_t)nbrChunks * total + total), &scratch);
   if (rc) return rc;
   double* d_out = scratch + (size_t)nbrChunks * total;
-  dim3 grid(nbrChunks, (nb + G - 1) / G);
+  dim3 grid((nb + G - 1) / G, nbrChunks);
   hipLaunchKernelGGL((dh_block_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, W, dim, rowsPerChunk,
                      scratch);
   hipLaunchKernelGGL(dh_reduce_chunks_kernel, dim3((total + 255) / 256), dim3(256), 0, c->stream, scratch,
@@ -454,6 +454,29 @@ extern "C" int dh_block_set_column(dh_context* c, double* w, int64_t dim, int k,
   if (dim == 0) return DH_OK;
   unsigned blocks = (unsigned)std::min<int64_t>((dim + 255) / 256, (int64_t)c->nbrCU * 16);
   hipLaunchKernelGGL(dh_set_column_kernel, dim3(blocks), dim3(256), 0, c->stream, w, dim, k, col, in);
+  DH_HIP(c, hipGetLastError());
+  return DH_OK;
+}
+
+// ----------------------------------------------------------------------------- copy
+// Streaming copy, 16 bytes per lane per iteration (reads n*8 bytes, writes n*8 bytes).
+__global__ void __launch_bounds__(256) dh_copy_kernel(double* __restrict__ y, const double* __restrict__ x,
+                                                      int64_t n) {
+  int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
+  for (; i + 1 < n; i += stride) {
+    double2 v = *reinterpret_cast<const double2*>(x + i);
+    *reinterpret_cast<double2*>(y + i) = v;
+  }
+  if (i < n) y[i] = x[i];
+}
+
+extern "C" int dh_copy(dh_context* c, double* y, const double* x, int64_t n) {
+  if (!c || !x || !y || n < 0) return DH_E_ARG;
+  if (n == 0) return DH_OK;
+  if (((uintptr_t)x | (uintptr_t)y) & 15) return dh_fail(c, DH_E_ARG, "dh_copy needs 16-byte aligned vectors");
+  unsigned blocks = (unsigned)std::min<int64_t>((n / 2 + 255) / 256 + 1, (int64_t)c->nbrCU * 32);
+  hipLaunchKernelGGL(dh_copy_kernel, dim3(blocks), dim3(256), 0, c->stream, y, x, n);
   DH_HIP(c, hipGetLastError());
   return DH_OK;
 }

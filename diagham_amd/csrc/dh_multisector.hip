@@ -1,0 +1,658 @@
+// Many Lz sectors of one fermions-on-the-sphere problem in single launches.
+//
+// The reference solves the sectors one after the other (FQHESphereFermionsTwoBodyGeneric.cc:270
+// loops over Lz, each with its own FermionOnSphere, Hamiltonian and Lanczos).  Here the
+// sectors of uneven dimension share one concatenated basis, one factor table and one low-bit
+// rank table; a product over every active sector is one persistent launch whose workgroups
+// stride over (sector, row-tile) work items, and the Lanczos vector operations are segmented
+// by sector so that every sector keeps its own Krylov space.
+//
+// Block vectors of the set: per sector a [D_s][k] block, sectors concatenated in order, so
+// global row g of sector s is offset_s + local row.
+#include <algorithm>
+#include <cstring>
+
+#include "dh_index.h"
+#include "dh_internal.h"
+#include "dh_sphere_row.h"
+
+struct dh_sector_set {
+  dh_context* ctx = nullptr;
+  int nbrFermions = 0, lzMax = 0, nbrOrbitals = 0, nbrSectors = 0;
+  std::vector<int> totalLz, orbitalSum;
+  std::vector<int64_t> offsets;  // nbrSectors + 1
+  int64_t totalDim = 0;
+  uint64_t* d_states = nullptr;
+  int rankBits = 0;
+  int64_t loSize = 0, hiSize = 0;  // hi table of sector s starts at s * hiSize
+  uint32_t* d_rankLo = nullptr;
+  uint32_t* d_rankHi = nullptr;
+  int64_t* d_offsets = nullptr;
+  double* d_factors = nullptr;
+  int* d_pairOffset = nullptr;
+  int nbrFactors = 0, nbrPairs = 0;
+  double shift = 0.0;
+  int lanesPerRow = 1, blockSize = 256;
+  // per-call work descriptor: [active sector ids (int64)] [tile prefix (int64)]
+  int64_t* d_work = nullptr;
+  size_t workCap = 0;
+  double* d_coef = nullptr;  // per-sector coefficient matrices for axpy / rmul
+  size_t coefCap = 0;
+};
+
+// device-side subset-count table (same recursion as the single-sector basis)
+static std::vector<int64_t> subset_table(int nOrb, int maxN, int& maxSum) {
+  maxSum = 0;
+  for (int i = 0; i < maxN; ++i) maxSum += nOrb - 1 - i;
+  std::vector<int64_t> t((size_t)(nOrb + 1) * (maxN + 1) * (maxSum + 1), 0);
+  auto idx = [&](int j, int n, int s) { return ((size_t)j * (maxN + 1) + n) * (maxSum + 1) + s; };
+  t[idx(0, 0, 0)] = 1;
+  for (int j = 0; j < nOrb; ++j)
+    for (int n = 0; n <= maxN; ++n)
+      for (int s = 0; s <= maxSum; ++s) {
+        int64_t v = t[idx(j, n, s)];
+        if (n >= 1 && s >= j) v += t[idx(j, n - 1, s - j)];
+        t[idx(j + 1, n, s)] = v;
+      }
+  return t;
+}
+
+__global__ void ms_unrank_kernel(uint64_t* __restrict__ states, int64_t dim, int nbrOrbitals, int nbrFermions,
+                                 int orbitalSum, const int64_t* __restrict__ T, int maxN, int maxSum) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  int64_t rem = i;
+  int n = nbrFermions, s = orbitalSum;
+  uint64_t state = 0;
+  for (int j = nbrOrbitals - 1; j >= 0 && n > 0; --j) {
+    int64_t c = 0;
+    if (s >= j && s - j <= maxSum) c = T[((int64_t)j * (maxN + 1) + (n - 1)) * (maxSum + 1) + (s - j)];
+    if (rem < c) {
+      state |= (1ull << j);
+      --n;
+      s -= j;
+    } else {
+      rem -= c;
+    }
+  }
+  states[i] = state;
+}
+
+extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, int lzMax, const int* totalLz,
+                                    int nbrSectors) {
+  if (!c || !out || !totalLz || nbrSectors <= 0) return DH_E_ARG;
+  *out = nullptr;
+  if (N <= 0 || lzMax < 0 || N > lzMax + 1) return dh_fail(c, DH_E_ARG, "need 0 < nbr_fermions <= lz_max + 1");
+  if (lzMax > 62) return dh_fail(c, DH_E_RANGE, "lz_max must be <= 62");
+  hipSetDevice(c->device);
+  const int nOrb = lzMax + 1;
+  int maxSum;
+  std::vector<int64_t> T = subset_table(nOrb, N, maxSum);
+  auto Tat = [&](int j, int n, int s) -> int64_t {
+    if (j < 0 || j > nOrb || n < 0 || n > N || s < 0 || s > maxSum) return 0;
+    return T[((size_t)j * (N + 1) + n) * (maxSum + 1) + s];
+  };
+  dh_sector_set* S = new dh_sector_set();
+  S->ctx = c;
+  S->nbrFermions = N;
+  S->lzMax = lzMax;
+  S->nbrOrbitals = nOrb;
+  S->nbrSectors = nbrSectors;
+  S->offsets.assign(nbrSectors + 1, 0);
+  for (int i = 0; i < nbrSectors; ++i) {
+    int twice = totalLz[i] + N * lzMax;
+    if (twice < 0 || (twice & 1)) {
+      delete S;
+      return dh_fail(c, DH_E_ARG, "a total_lz has the wrong parity or range");
+    }
+    int sum = twice >> 1;
+    int64_t d = Tat(nOrb, N, sum);
+    if (d <= 0 || d >= (1ll << 31)) {
+      delete S;
+      return dh_fail(c, DH_E_ARG, "a sector is empty or larger than 2^31");
+    }
+    S->totalLz.push_back(totalLz[i]);
+    S->orbitalSum.push_back(sum);
+    S->offsets[i + 1] = S->offsets[i] + d;
+  }
+  S->totalDim = S->offsets[nbrSectors];
+  S->rankBits = (nOrb + 1) / 2;
+  S->loSize = 1ll << S->rankBits;
+  S->hiSize = 1ll << (nOrb - S->rankBits);
+  // rank tables: one low table, one high table per sector
+  std::vector<uint32_t> lo(S->loSize, 0), hi((size_t)S->hiSize * nbrSectors, 0);
+  for (uint64_t l = 0; l < (uint64_t)S->loSize; ++l) {
+    int n = 0, s = 0;
+    int64_t acc = 0;
+    for (uint64_t r = l; r; r &= r - 1) {
+      int j = __builtin_ctzll(r);
+      ++n;
+      s += j;
+      acc += Tat(j, n, s);
+    }
+    lo[l] = (uint32_t)acc;
+  }
+  for (int sec = 0; sec < nbrSectors; ++sec)
+    for (uint64_t h = 0; h < (uint64_t)S->hiSize; ++h) {
+      int popH = __builtin_popcountll(h), sumH = 0;
+      for (uint64_t r = h; r; r &= r - 1) sumH += S->rankBits + __builtin_ctzll(r);
+      int n = N - popH, s = S->orbitalSum[sec] - sumH;
+      if (n < 0 || s < 0) continue;
+      int64_t acc = 0;
+      for (uint64_t r = h; r; r &= r - 1) {
+        int j = S->rankBits + __builtin_ctzll(r);
+        ++n;
+        s += j;
+        acc += Tat(j, n, s);
+      }
+      hi[(size_t)sec * S->hiSize + h] = (uint32_t)acc;
+    }
+  int rc;
+  int64_t* d_T = nullptr;
+  if ((rc = dh_malloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
+      (rc = dh_malloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
+    if (d_T) hipFree(d_T);
+    dh_sector_set_destroy(S);
+    return rc;
+  }
+  hipMemcpyAsync(d_T, T.data(), sizeof(int64_t) * T.size(), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(S->d_rankLo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(S->d_rankHi, hi.data(), sizeof(uint32_t) * hi.size(), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(S->d_offsets, S->offsets.data(), sizeof(int64_t) * (nbrSectors + 1), hipMemcpyHostToDevice,
+                 c->stream);
+  for (int sec = 0; sec < nbrSectors; ++sec) {
+    int64_t d = S->offsets[sec + 1] - S->offsets[sec];
+    hipLaunchKernelGGL(ms_unrank_kernel, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, c->stream,
+                       S->d_states + S->offsets[sec], d, nOrb, N, S->orbitalSum[sec], d_T, N, maxSum);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_T);
+  if (e != hipSuccess) {
+    dh_sector_set_destroy(S);
+    return dh_fail(c, DH_E_HIP, std::string("sector set: ") + hipGetErrorString(e));
+  }
+  *out = S;
+  return DH_OK;
+}
+
+extern "C" int dh_sector_set_destroy(dh_sector_set* S) {
+  if (!S) return DH_E_ARG;
+  dh_context* c = S->ctx;
+  hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)S->d_states, (void*)S->d_rankLo, (void*)S->d_rankHi, (void*)S->d_offsets,
+                  (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef})
+    if (p) hipFree(p);
+  delete S;
+  return DH_OK;
+}
+
+extern "C" int dh_sector_set_offsets(const dh_sector_set* S, int64_t* offsets) {
+  if (!S || !offsets) return DH_E_ARG;
+  std::memcpy(offsets, S->offsets.data(), sizeof(int64_t) * (S->nbrSectors + 1));
+  return DH_OK;
+}
+
+extern "C" int dh_sector_set_get_states(dh_sector_set* S, uint64_t* states) {
+  if (!S || !states) return DH_E_ARG;
+  dh_context* c = S->ctx;
+  DH_HIP(c, hipMemcpyAsync(states, S->d_states, sizeof(uint64_t) * S->totalDim, hipMemcpyDeviceToHost, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_sector_set_factors(dh_sector_set* S, const double* factors, const int* pairOffset, int nbrFactors,
+                                     double shift) {
+  if (!S || !factors || !pairOffset) return DH_E_ARG;
+  dh_context* c = S->ctx;
+  const int nOrb = S->nbrOrbitals, nPairs = nOrb * (nOrb - 1) / 2;
+  int expect = 0;
+  for (int m1 = 0; m1 < nOrb; ++m1)
+    for (int m2 = 0; m2 < m1; ++m2) {
+      if (pairOffset[m1 * (m1 - 1) / 2 + m2] != expect) return dh_fail(c, DH_E_ARG, "pair_offset layout");
+      int sum = m1 + m2;
+      expect += std::min(sum, nOrb - 1) - (sum / 2 + 1) + 1;
+    }
+  if (pairOffset[nPairs] != expect || nbrFactors != expect) return dh_fail(c, DH_E_ARG, "nbr_factors layout");
+  if (S->d_factors) hipFree(S->d_factors);
+  if (S->d_pairOffset) hipFree(S->d_pairOffset);
+  S->d_factors = nullptr;
+  S->d_pairOffset = nullptr;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&S->d_factors, sizeof(double) * std::max(1, nbrFactors))) ||
+      (rc = dh_malloc(c, (void**)&S->d_pairOffset, sizeof(int) * (nPairs + 1))))
+    return rc;
+  DH_HIP(c, hipMemcpyAsync(S->d_factors, factors, sizeof(double) * nbrFactors, hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_pairOffset, pairOffset, sizeof(int) * (nPairs + 1), hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  S->nbrFactors = nbrFactors;
+  S->nbrPairs = nPairs;
+  S->shift = shift;
+  return DH_OK;
+}
+
+extern "C" int dh_sector_set_launch(dh_sector_set* S, int lanesPerRow, int blockSize) {
+  if (!S) return DH_E_ARG;
+  if (lanesPerRow != 1 && lanesPerRow != 2 && lanesPerRow != 4 && lanesPerRow != 8)
+    return dh_fail(S->ctx, DH_E_ARG, "lanes_per_row in {1,2,4,8}");
+  if (blockSize < 64 || blockSize > 1024 || (blockSize % 64)) return dh_fail(S->ctx, DH_E_ARG, "block_size");
+  S->lanesPerRow = lanesPerRow;
+  S->blockSize = blockSize;
+  return DH_OK;
+}
+
+// work descriptor of the active sectors for tiles of `rowsPerTile` rows:
+// d_work[0..na) = sector ids, d_work[na..2na] = tile prefix
+static int upload_work(dh_sector_set* S, const unsigned char* active, int64_t rowsPerTile, int* nActive,
+                       int64_t* totalTiles) {
+  dh_context* c = S->ctx;
+  std::vector<int64_t> w;
+  std::vector<int64_t> prefix(1, 0);
+  for (int s = 0; s < S->nbrSectors; ++s) {
+    if (active && !active[s]) continue;
+    w.push_back(s);
+    int64_t d = S->offsets[s + 1] - S->offsets[s];
+    prefix.push_back(prefix.back() + (d + rowsPerTile - 1) / rowsPerTile);
+  }
+  int na = (int)w.size();
+  w.insert(w.end(), prefix.begin(), prefix.end());
+  size_t bytes = sizeof(int64_t) * w.size();
+  if (bytes > S->workCap) {
+    if (S->d_work) {
+      DH_HIP(c, hipStreamSynchronize(c->stream));
+      hipFree(S->d_work);
+    }
+    S->d_work = nullptr;
+    int rc = dh_malloc(c, (void**)&S->d_work, std::max<size_t>(bytes, 4096));
+    if (rc) return rc;
+    S->workCap = std::max<size_t>(bytes, 4096);
+  }
+  double* pin;
+  int rc = dh_pinned(c, bytes, &pin);
+  if (rc) return rc;
+  std::memcpy(pin, w.data(), bytes);
+  DH_HIP(c, hipMemcpyAsync(S->d_work, pin, bytes, hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));  // the pinned buffer is reused by the next call
+  *nActive = na;
+  *totalTiles = prefix.back();
+  return DH_OK;
+}
+
+struct SectorsArgs {
+  const uint64_t* states;
+  const int64_t* offsets;
+  int nbrOrbitals, nbrFactors, nbrPairs;
+  const double* factors;
+  const int* pairOffset;
+  const uint32_t* rankLo;
+  const uint32_t* rankHi;
+  int rankBits;
+  int64_t loSize, hiSize;
+  double shift;
+  const int64_t* work;  // active ids then prefix
+  int nActive;
+  int64_t totalTiles;
+};
+
+// tile -> position j in the active list (prefix in LDS)
+__device__ __forceinline__ int tile_sector(const int64_t* sPrefix, int nActive, int64_t tile) {
+  int lo = 0, hi = nActive;  // sPrefix[lo] <= tile < sPrefix[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (sPrefix[mid] <= tile)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int K, int LPR, bool LO_LDS>
+__global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const double* __restrict__ x,
+                                                          double* __restrict__ y, int add) {
+  extern __shared__ double smem[];
+  double* sF = smem;
+  int* sOff = reinterpret_cast<int*>(sF + a.nbrFactors);
+  int64_t* sWork = reinterpret_cast<int64_t*>(smem) + ((a.nbrFactors * 8 + (a.nbrPairs + 1) * 4 + 7) / 8);
+  uint32_t* sLo = reinterpret_cast<uint32_t*>(sWork + 2 * a.nActive + 1);
+  for (int i = threadIdx.x; i < a.nbrFactors; i += blockDim.x) sF[i] = a.factors[i];
+  for (int i = threadIdx.x; i <= a.nbrPairs; i += blockDim.x) sOff[i] = a.pairOffset[i];
+  for (int i = threadIdx.x; i < 2 * a.nActive + 1; i += blockDim.x) sWork[i] = a.work[i];
+  if constexpr (LO_LDS)
+    for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo[i];
+  __syncthreads();
+  const uint32_t* lo = LO_LDS ? sLo : a.rankLo;
+  const int64_t* sPrefix = sWork + a.nActive;
+  const int lane = threadIdx.x % LPR;
+  const int64_t rowsPerTile = blockDim.x / LPR;
+  for (int64_t tile = dh_xcd_block(blockIdx.x, gridDim.x); tile < a.totalTiles; tile += gridDim.x) {
+    const int j = tile_sector(sPrefix, a.nActive, tile);
+    const int sec = (int)sWork[j];
+    const int64_t off = a.offsets[sec];
+    const int64_t dim = a.offsets[sec + 1] - off;
+    const uint32_t* hi = a.rankHi + (int64_t)sec * a.hiSize;
+    const int64_t local = (tile - sPrefix[j]) * rowsPerTile + threadIdx.x / LPR;
+    const bool active = local < dim;
+    const int64_t g = off + (active ? local : 0);
+    const uint64_t s = active ? a.states[g] : 0ull;
+    double acc[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] = 0.0;
+    auto find = [&](uint64_t t) -> int64_t { return dh_find_rank(t, lo, hi, a.rankBits, dim); };
+    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x + off * K, acc);
+    if constexpr (LPR > 1) {
+#pragma unroll
+      for (int v = 0; v < K; ++v)
+        for (int o = LPR / 2; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o, LPR);
+    }
+    if (active && lane == 0) {
+      const double* xr = x + g * K;
+      double* yr = y + g * K;
+#pragma unroll
+      for (int v = 0; v < K; ++v) yr[v] = (add ? yr[v] : 0.0) + acc[v] + a.shift * xr[v];
+    }
+  }
+}
+
+static SectorsArgs sectors_args(dh_sector_set* S, int nActive, int64_t totalTiles) {
+  SectorsArgs a;
+  a.states = S->d_states;
+  a.offsets = S->d_offsets;
+  a.nbrOrbitals = S->nbrOrbitals;
+  a.nbrFactors = S->nbrFactors;
+  a.nbrPairs = S->nbrPairs;
+  a.factors = S->d_factors;
+  a.pairOffset = S->d_pairOffset;
+  a.rankLo = S->d_rankLo;
+  a.rankHi = S->d_rankHi;
+  a.rankBits = S->rankBits;
+  a.loSize = S->loSize;
+  a.hiSize = S->hiSize;
+  a.shift = S->shift;
+  a.work = S->d_work;
+  a.nActive = nActive;
+  a.totalTiles = totalTiles;
+  return a;
+}
+
+template <int K, int LPR>
+static int sectors_hv_launch(dh_sector_set* S, const double* x, double* y, int add, const unsigned char* active) {
+  dh_context* c = S->ctx;
+  int64_t rowsPerTile = S->blockSize / LPR;
+  int na;
+  int64_t tiles;
+  int rc = upload_work(S, active, rowsPerTile, &na, &tiles);
+  if (rc) return rc;
+  if (tiles == 0) return DH_OK;
+  size_t base = ((size_t)S->nbrFactors * 8 + (size_t)(S->nbrPairs + 1) * 4 + 7) / 8 * 8 + 8 * (2 * na + 1);
+  size_t withLo = base + 4 * S->loSize;
+  bool loLds = withLo <= 64 * 1024;
+  size_t lds = loLds ? withLo : base;
+  if (lds > 160 * 1024) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
+  SectorsArgs a = sectors_args(S, na, tiles);
+  auto launch = [&](auto kern) -> int {
+    if (lds > 64 * 1024)
+      DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int perCU = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, S->blockSize, lds) != hipSuccess || perCU < 1)
+      perCU = 1;
+    int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(S->blockSize), lds, c->stream, a, x, y, add);
+    DH_HIP(c, hipGetLastError());
+    return DH_OK;
+  };
+  if (loLds) return launch(sectors_hv_kernel<K, LPR, true>);
+  return launch(sectors_hv_kernel<K, LPR, false>);
+}
+
+template <int K>
+static int sectors_hv_lpr(dh_sector_set* S, const double* x, double* y, int add, const unsigned char* active) {
+  switch (S->lanesPerRow) {
+    case 1: return sectors_hv_launch<K, 1>(S, x, y, add, active);
+    case 2: return sectors_hv_launch<K, 2>(S, x, y, add, active);
+    case 4: return sectors_hv_launch<K, 4>(S, x, y, add, active);
+    default: return sectors_hv_launch<K, 8>(S, x, y, add, active);
+  }
+}
+
+extern "C" int dh_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add,
+                             const unsigned char* active) {
+  if (!S || !x || !y) return DH_E_ARG;
+  if (!S->d_factors) return dh_fail(S->ctx, DH_E_STATE, "factors not set");
+  switch (k) {
+    case 1: return sectors_hv_lpr<1>(S, x, y, add, active);
+    case 2: return sectors_hv_lpr<2>(S, x, y, add, active);
+    case 4: return sectors_hv_lpr<4>(S, x, y, add, active);
+    case 8: return sectors_hv_lpr<8>(S, x, y, add, active);
+    case 16: return sectors_hv_lpr<16>(S, x, y, add, active);
+    default: return dh_fail(S->ctx, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+  }
+}
+
+// ----------------------------------------------------------------------------- segmented BLAS
+// dots: partial[tile][b*KK + i] over the tile's rows, reduced per active sector afterwards
+template <int K, int G>
+__global__ void __launch_bounds__(256) sectors_dots_kernel(const double* __restrict__ V, int nbrBlocks,
+                                                           int64_t totalDim, const double* __restrict__ W,
+                                                           const int64_t* __restrict__ offsets,
+                                                           const int64_t* __restrict__ work, int nActive,
+                                                           int64_t rowsPerTile, double* __restrict__ partial) {
+  constexpr int KK = K * K;
+  __shared__ double red[256 / 64][G * KK];
+  const int64_t tile = blockIdx.y;  // blocks of one tile are adjacent: its W rows stay in L2
+  const int64_t* prefix = work + nActive;
+  int lo = 0, hi = nActive;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= tile) lo = mid; else hi = mid;
+  }
+  const int sec = (int)work[lo];
+  const int64_t off = offsets[sec], dim = offsets[sec + 1] - off;
+  const int64_t r0 = (tile - prefix[lo]) * rowsPerTile;
+  const int64_t r1 = min(dim, r0 + rowsPerTile);
+  const int b0 = blockIdx.x * G;
+  double acc[G * KK];
+#pragma unroll
+  for (int i = 0; i < G * KK; ++i) acc[i] = 0.0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const int64_t row = off + r;
+    double w[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) w[q] = W[row * K + q];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (b0 + g < nbrBlocks) {
+        const double* vb = V + (int64_t)(b0 + g) * totalDim * K + row * K;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+          double v = vb[p];
+#pragma unroll
+          for (int q = 0; q < K; ++q) acc[g * KK + p * K + q] += v * w[q];
+        }
+      }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < G * KK; ++i) {
+    double v = acc[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  const int64_t total = (int64_t)nbrBlocks * KK;
+  for (int i = threadIdx.x; i < G * KK; i += blockDim.x)
+    if (b0 + i / KK < nbrBlocks) {
+      double s = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][i];
+      partial[tile * total + (int64_t)b0 * KK + i] = s;
+    }
+}
+
+__global__ void sectors_reduce_kernel(const double* __restrict__ partial, const int64_t* __restrict__ work,
+                                      int nActive, int64_t total, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= total) return;
+  const int64_t* prefix = work + nActive;
+  double s = 0.0;
+  for (int64_t t = prefix[j]; t < prefix[j + 1]; ++t) s += partial[t * total + i];
+  out[(int64_t)j * total + i] = s;
+}
+
+template <int K>
+static int sectors_dots(dh_sector_set* S, const double* V, int nb, const double* W, const unsigned char* active,
+                        double* h_out) {
+  dh_context* c = S->ctx;
+  constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
+  const int64_t rowsPerTile = std::max<int64_t>(8192, (S->totalDim + 32767) / 32768);
+  int na;
+  int64_t tiles;
+  int rc = upload_work(S, active, rowsPerTile, &na, &tiles);
+  if (rc) return rc;
+  if (tiles == 0) return DH_OK;
+  const int64_t total = (int64_t)nb * K * K;
+  double* scratch;
+  if ((rc = dh_scratch(c, sizeof(double) * (size_t)(tiles * total + (int64_t)na * total), &scratch))) return rc;
+  double* d_out = scratch + tiles * total;
+  dim3 grid((unsigned)((nb + G - 1) / G), (unsigned)tiles);
+  hipLaunchKernelGGL((sectors_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, S->totalDim, W, S->d_offsets,
+                     S->d_work, na, rowsPerTile, scratch);
+  hipLaunchKernelGGL(sectors_reduce_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)na), dim3(256), 0,
+                     c->stream, scratch, S->d_work, na, total, d_out);
+  DH_HIP(c, hipGetLastError());
+  double* pin;
+  if ((rc = dh_pinned(c, sizeof(double) * na * total, &pin))) return rc;
+  DH_HIP(c, hipMemcpyAsync(pin, d_out, sizeof(double) * na * total, hipMemcpyDeviceToHost, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  std::memcpy(h_out, pin, sizeof(double) * na * total);
+  return DH_OK;
+}
+
+extern "C" int dh_sectors_block_dots(dh_sector_set* S, const double* V, int nb, const double* W, int k,
+                                     const unsigned char* active, double* out) {
+  if (!S || !V || !W || !out || nb <= 0) return DH_E_ARG;
+  switch (k) {
+    case 1: return sectors_dots<1>(S, V, nb, W, active, out);
+    case 2: return sectors_dots<2>(S, V, nb, W, active, out);
+    case 4: return sectors_dots<4>(S, V, nb, W, active, out);
+    case 8: return sectors_dots<8>(S, V, nb, W, active, out);
+    case 16: return sectors_dots<16>(S, V, nb, W, active, out);
+    default: return dh_fail(S->ctx, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+  }
+}
+
+// axpy / rmul: W[row][q] (+)= sum over (b, p) of V_b[row][p] C_j[(b*K+p)*K+q]; rmul uses W itself
+template <int K, bool RMUL>
+__global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restrict__ V, int nb, int64_t totalDim,
+                                                           double* __restrict__ W,
+                                                           const int64_t* __restrict__ offsets,
+                                                           const int64_t* __restrict__ work, int nActive,
+                                                           int64_t totalTiles, const double* __restrict__ C) {
+  constexpr int KK = K * K;
+  constexpr int SLAB = (4096 / KK) > 0 ? (4096 / KK) : 1;
+  __shared__ double sC[SLAB * KK];
+  const int64_t* prefix = work + nActive;
+  for (int64_t tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {
+    int lo = 0, hi = nActive;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= tile) lo = mid; else hi = mid;
+    }
+    const int sec = (int)work[lo];
+    const int64_t off = offsets[sec], dim = offsets[sec + 1] - off;
+    const int64_t r = (tile - prefix[lo]) * blockDim.x + threadIdx.x;
+    const bool active = r < dim;
+    const int64_t row = off + (active ? r : 0);
+    const double* Cj = C + (int64_t)lo * nb * KK;
+    double acc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q] = 0.0;
+    for (int s0 = 0; s0 < nb; s0 += SLAB) {
+      const int ns = min(SLAB, nb - s0);
+      __syncthreads();
+      for (int i = threadIdx.x; i < ns * KK; i += blockDim.x) sC[i] = Cj[(int64_t)s0 * KK + i];
+      __syncthreads();
+      if (active) {
+        for (int b = 0; b < ns; ++b) {
+          const double* vb = (RMUL ? W : V + (int64_t)(s0 + b) * totalDim * K) + row * K;
+#pragma unroll
+          for (int p = 0; p < K; ++p) {
+            double v = vb[p];
+#pragma unroll
+            for (int q = 0; q < K; ++q) acc[q] += v * sC[b * KK + p * K + q];
+          }
+        }
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        if (RMUL)
+          W[row * K + q] = acc[q];
+        else
+          W[row * K + q] += acc[q];
+      }
+    }
+  }
+}
+
+template <int K, bool RMUL>
+static int sectors_axpy(dh_sector_set* S, const double* V, int nb, double* W, const unsigned char* active,
+                        const double* h_c) {
+  dh_context* c = S->ctx;
+  int na;
+  int64_t tiles;
+  int rc = upload_work(S, active, 256, &na, &tiles);
+  if (rc) return rc;
+  if (tiles == 0) return DH_OK;
+  size_t bytes = sizeof(double) * (size_t)na * nb * K * K;
+  if (bytes > S->coefCap) {
+    if (S->d_coef) {
+      DH_HIP(c, hipStreamSynchronize(c->stream));
+      hipFree(S->d_coef);
+    }
+    S->d_coef = nullptr;
+    if ((rc = dh_malloc(c, (void**)&S->d_coef, bytes))) return rc;
+    S->coefCap = bytes;
+  }
+  double* pin;
+  if ((rc = dh_pinned(c, bytes, &pin))) return rc;
+  std::memcpy(pin, h_c, bytes);
+  DH_HIP(c, hipMemcpyAsync(S->d_coef, pin, bytes, hipMemcpyHostToDevice, c->stream));
+  int64_t grid = std::min<int64_t>(tiles, (int64_t)c->nbrCU * 8);
+  hipLaunchKernelGGL((sectors_axpy_kernel<K, RMUL>), dim3((unsigned)grid), dim3(256), 0, c->stream, V, nb,
+                     S->totalDim, W, S->d_offsets, S->d_work, na, tiles, S->d_coef);
+  DH_HIP(c, hipGetLastError());
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_sectors_block_axpy(dh_sector_set* S, const double* V, int nb, double* W, int k,
+                                     const unsigned char* active, const double* coef) {
+  if (!S || !V || !W || !coef || nb <= 0) return DH_E_ARG;
+  switch (k) {
+    case 1: return sectors_axpy<1, false>(S, V, nb, W, active, coef);
+    case 2: return sectors_axpy<2, false>(S, V, nb, W, active, coef);
+    case 4: return sectors_axpy<4, false>(S, V, nb, W, active, coef);
+    case 8: return sectors_axpy<8, false>(S, V, nb, W, active, coef);
+    case 16: return sectors_axpy<16, false>(S, V, nb, W, active, coef);
+    default: return dh_fail(S->ctx, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+  }
+}
+
+extern "C" int dh_sectors_block_rmul(dh_sector_set* S, double* W, int k, const unsigned char* active,
+                                     const double* m) {
+  if (!S || !W || !m) return DH_E_ARG;
+  switch (k) {
+    case 1: return sectors_axpy<1, true>(S, W, 1, W, active, m);
+    case 2: return sectors_axpy<2, true>(S, W, 1, W, active, m);
+    case 4: return sectors_axpy<4, true>(S, W, 1, W, active, m);
+    case 8: return sectors_axpy<8, true>(S, W, 1, W, active, m);
+    case 16: return sectors_axpy<16, true>(S, W, 1, W, active, m);
+    default: return dh_fail(S->ctx, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+  }
+}
+
+extern "C" int64_t dh_sector_set_dimension(const dh_sector_set* S) { return S ? S->totalDim : -1; }

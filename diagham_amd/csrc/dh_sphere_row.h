@@ -1,0 +1,58 @@
+// Per-row accumulation of the fermions-on-the-sphere product, shared by the single-sector
+// and the multi-sector kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dh_index.h"
+
+template <int K>
+__device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restrict__ x, int64_t idx, double coef) {
+  const double* xr = x + idx * K;
+  if constexpr (K % 2 == 0) {
+#pragma unroll
+    for (int v = 0; v < K; v += 2) {
+      double2 p = *reinterpret_cast<const double2*>(xr + v);
+      acc[v] += coef * p.x;
+      acc[v + 1] += coef * p.y;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] += coef * xr[v];
+  }
+}
+
+// Two-body terms of row state s: for every occupied pair n1 > n2 (annihilated as a_n1 a_n2,
+// sign = parity of the occupied orbitals at and above each), every created pair m3 > m4 with
+// m3 + m4 = n1 + n2 (created a+_m3 a+_m4 on the intermediate state) whose orbitals are free.
+// Lane `lane` of the row's LPR lanes takes m3 = m3lo + lane, m3lo + lane + LPR, ...
+// Target indices come from `find(target)`; x is indexed relative to `xbase`.
+template <int K, int LPR, typename Find>
+__device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const double* __restrict__ sF,
+                                              const int* __restrict__ sOff, int lane, Find find,
+                                              const double* __restrict__ xbase, double* acc) {
+  for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+    const int n1 = __builtin_ctzll(r1);
+    const int pc1 = __popcll(s >> n1);
+    for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+      const int n2 = __builtin_ctzll(r2);
+      const int sgnA = pc1 + __popcll(s >> n2);
+      const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+      const int sum = n1 + n2;
+      const int m3lo = (sum >> 1) + 1;
+      const int m3hi = min(sum, nbrOrbitals - 1);
+      const int fbase = sOff[(n1 * (n1 - 1)) / 2 + n2] - m3lo;
+      for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
+        const int m4 = sum - m3;
+        const uint64_t cm = (1ull << m3) | (1ull << m4);
+        if (t & cm) continue;
+        const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+        const int64_t idx = find(t | cm);
+        double coef = sF[fbase + m3];
+        if (sg & 1) coef = -coef;
+        dh_gather_fma<K>(acc, xbase, idx, coef);
+      }
+    }
+  }
+}

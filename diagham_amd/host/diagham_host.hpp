@@ -199,7 +199,8 @@ struct LanczosResult {
 class FullReorthogonalizedBlockLanczosAlgorithm {
  public:
   FullReorthogonalizedBlockLanczosAlgorithm(GPUArchitecture* architecture, AbstractHamiltonian* hamiltonian,
-                                            int nbrEigenvalue, int blockSize, int maxIter, double precision = 1e-14);
+                                            int nbrEigenvalue, int blockSize, int maxIter, double precision = 1e-14,
+                                            double residual = 0.0);
   ~FullReorthogonalizedBlockLanczosAlgorithm();
   void InitializeLanczosAlgorithm(uint64_t seed);
   void RunLanczosAlgorithm(int nbrIter);
@@ -213,10 +214,13 @@ class FullReorthogonalizedBlockLanczosAlgorithm {
  private:
   void Diagonalize(std::vector<double>& evals, std::vector<double>* evecs);
   void Orthonormalize(double* d_w, std::vector<double>& R);
+  bool Dense() const;
+  void DenseSolve();
+  bool Exhausted = false;
   GPUArchitecture* Arch;
   AbstractHamiltonian* Hamiltonian;
   int NbrEigenvalue, BlockSize, MaxIter;
-  double Precision;
+  double Precision, Residual;
   int64_t Dim;
   double* d_basis = nullptr;  // MaxBlocks+1 blocks of Dim*BlockSize
   int MaxBlocks;
@@ -226,10 +230,48 @@ class FullReorthogonalizedBlockLanczosAlgorithm {
   bool HavePrevious = false;
 };
 
+// All Lz sectors of one fermions-on-the-sphere problem solved together: the per-sector loop of
+// FQHESphereFermionsTwoBodyGeneric.cc:270 + QHEOnSphereMainTask becomes one block Lanczos per
+// sector whose steps advance every unconverged sector at once (one persistent H.v launch and
+// segmented reorthogonalisation per step).  Sectors too small for a Krylov block are
+// diagonalised densely.  Sectors are processed in waves whose Krylov bases fit `memoryBytes`.
+struct SectorResult {
+  int totalLz = 0;
+  int64_t dimension = 0;
+  std::vector<double> eigenvalues;
+  int products = 0;
+  bool converged = false;
+};
+
+class MultiSectorBlockLanczos {
+ public:
+  MultiSectorBlockLanczos(GPUArchitecture* architecture, int nbrFermions, int lzMax, const std::vector<int>& totalLz,
+                          const std::vector<double>& pseudoPotential, int nbrEigenvalue, int blockSize, int maxIter,
+                          double precision = 1e-14, size_t memoryBytes = 0);
+  std::vector<SectorResult> Solve(uint64_t seed, double* seconds = nullptr);
+  int NbrWaves() const { return Waves; }
+
+ private:
+  void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
+  GPUArchitecture* Arch;
+  int NbrFermions, LzMax, NbrEigenvalue, BlockSize, MaxIter;
+  double Precision;
+  size_t MemoryBytes;
+  int Waves = 0;
+  std::vector<int> TotalLz;
+  std::vector<double> Pseudo, Factors;
+  std::vector<int> PairOffset;
+};
+
 // Tridiagonal QL with implicit shifts (RealTriDiagonalSymmetricMatrix.cc:687 semantics):
 // eigenvalues ascending; optional eigenvectors (column-major n x n) of the tridiagonal.
 void TridiagonalEigen(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& evals,
                       std::vector<double>* evecs);
+// Eigenvalues (ascending) of the block tridiagonal Lanczos matrix with diagonal blocks A[0..m)
+// and sub-diagonal blocks B[0..m-1) (k x k, row-major); Ritz coefficients (column-major n x n)
+// if evecs is given.
+void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& B,
+                           int k, int m, std::vector<double>& evals, std::vector<double>* evecs);
 // Symmetric band matrix (half-bandwidth b, dense row-major n x n) -> tridiagonal by Givens
 // rotations with bulge chasing; returns diagonal and off-diagonal, and accumulates Q if given.
 void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>& d, std::vector<double>& e,

@@ -1,4 +1,5 @@
 // C facade of the host layer (include/diagham_host.h).
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -104,13 +105,13 @@ extern "C" int dhh_problem_channels(dhh_problem* p, int* momenta, int capacity) 
   return n;
 }
 
-extern "C" int dhh_lanczos(dhh_problem* p, int nbrEigen, int blockSize, int maxIter, double precision, uint64_t seed,
-                           double* eigenvalues, int* products, int* blocks, double* seconds, double* d_vectors,
-                           int nbrVectors) {
+extern "C" int dhh_lanczos(dhh_problem* p, int nbrEigen, int blockSize, int maxIter, double precision,
+                           double residual, uint64_t seed, double* eigenvalues, int* products, int* blocks,
+                           double* seconds, double* d_vectors, int nbrVectors) {
   if (!p || !eigenvalues || nbrEigen <= 0) return DH_E_ARG;
   return guarded([&] {
     FullReorthogonalizedBlockLanczosAlgorithm lanczos(p->arch->arch.get(), p->hamiltonian.get(), nbrEigen, blockSize,
-                                                      maxIter, precision);
+                                                      maxIter, precision, residual);
     lanczos.InitializeLanczosAlgorithm(seed);
     LanczosResult r = lanczos.Solve();
     for (int i = 0; i < nbrEigen; ++i) eigenvalues[i] = (i < (int)r.eigenvalues.size()) ? r.eigenvalues[i] : 0.0;
@@ -119,6 +120,48 @@ extern "C" int dhh_lanczos(dhh_problem* p, int nbrEigen, int blockSize, int maxI
     if (seconds) *seconds = r.seconds;
     if (d_vectors && nbrVectors > 0) lanczos.GetEigenstates(nbrVectors, d_vectors);
     return r.converged ? DH_OK : DH_E_STATE;
+  });
+}
+
+extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* totalLz, int nbrSectors,
+                                 const double* pseudo, int nbrEigen, int blockSize, int maxIter, double precision,
+                                 uint64_t seed, int batched, double memoryGB, double* eigenvalues, int* products,
+                                 int* converged, double* seconds) {
+  if (!a || !totalLz || !pseudo || !eigenvalues || nbrSectors <= 0 || nbrEigen <= 0) return DH_E_ARG;
+  return guarded([&] {
+    std::vector<int> lz(totalLz, totalLz + nbrSectors);
+    std::vector<double> v(pseudo, pseudo + lzMax + 1);
+    auto t0 = std::chrono::steady_clock::now();
+    bool all = true;
+    for (int i = 0; i < nbrSectors * nbrEigen; ++i) eigenvalues[i] = 0.0;
+    if (batched) {
+      MultiSectorBlockLanczos solver(a->arch.get(), n, lzMax, lz, v, nbrEigen, blockSize, maxIter, precision,
+                                     (size_t)(memoryGB * 1073741824.0));
+      std::vector<SectorResult> r = solver.Solve(seed);
+      for (int s = 0; s < nbrSectors; ++s) {
+        for (size_t e = 0; e < r[s].eigenvalues.size() && (int)e < nbrEigen; ++e)
+          eigenvalues[(size_t)s * nbrEigen + e] = r[s].eigenvalues[e];
+        if (products) products[s] = r[s].products;
+        if (converged) converged[s] = r[s].converged;
+        all = all && r[s].converged;
+      }
+    } else {
+      for (int s = 0; s < nbrSectors; ++s) {
+        FermionOnSphere space(a->arch.get(), n, lz[s], lzMax);
+        ParticleOnSphereGenericHamiltonian h(&space, n, lzMax, v, a->arch.get());
+        FullReorthogonalizedBlockLanczosAlgorithm lanczos(a->arch.get(), &h, nbrEigen, blockSize, maxIter, precision);
+        lanczos.InitializeLanczosAlgorithm(seed);
+        LanczosResult r = lanczos.Solve();
+        for (size_t e = 0; e < r.eigenvalues.size() && (int)e < nbrEigen; ++e)
+          eigenvalues[(size_t)s * nbrEigen + e] = r.eigenvalues[e];
+        if (products) products[s] = r.iterations;
+        if (converged) converged[s] = r.converged;
+        all = all && r.converged;
+      }
+    }
+    dh_synchronize(a->arch->Context());
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return all ? DH_OK : DH_E_STATE;
   });
 }
 

@@ -181,13 +181,14 @@ static std::vector<double> matmul(int k, const std::vector<double>& A, const std
 // ------------------------------------------------------------------ block Lanczos
 FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgorithm(
     GPUArchitecture* architecture, AbstractHamiltonian* hamiltonian, int nbrEigenvalue, int blockSize, int maxIter,
-    double precision)
+    double precision, double residual)
     : Arch(architecture),
       Hamiltonian(hamiltonian),
       NbrEigenvalue(nbrEigenvalue),
       BlockSize(blockSize),
       MaxIter(maxIter),
-      Precision(precision) {
+      Precision(precision),
+      Residual(residual) {
   Dim = hamiltonian->GetHilbertSpaceDimension();
   if (BlockSize != 1 && BlockSize != 2 && BlockSize != 4 && BlockSize != 8 && BlockSize != 16)
     throw std::invalid_argument("block size must be 1, 2, 4, 8 or 16");
@@ -224,6 +225,8 @@ void FullReorthogonalizedBlockLanczosAlgorithm::InitializeLanczosAlgorithm(uint6
   A.clear();
   B.clear();
   HavePrevious = false;
+  Exhausted = false;
+  if (Dense()) return;
   Arch->Check(dh_fill_random(c, d_basis, Dim * BlockSize, seed), "random start");
   std::vector<double> R;
   Orthonormalize(d_basis, R);
@@ -255,6 +258,13 @@ void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter)
       }
     std::vector<double> R;
     Orthonormalize(W, R);
+    if (R.empty() && (int64_t)(j + 1) * k >= Dim) {
+      // the Krylov space is the whole space: the block tridiagonal matrix is exact
+      Exhausted = true;
+      A.push_back(Aj);
+      B.push_back(std::vector<double>((size_t)kk, 0.0));
+      return;
+    }
     if (R.empty()) {
       // invariant subspace reached: restart with a random block orthogonal to the basis
       // (the subspace-collapse branch of FullReorthogonalizedLanczosAlgorithm.cc:270)
@@ -274,8 +284,11 @@ void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter)
 }
 
 void FullReorthogonalizedBlockLanczosAlgorithm::Diagonalize(std::vector<double>& evals, std::vector<double>* evecs) {
-  const int k = BlockSize;
-  const int m = (int)A.size();
+  BlockTridiagonalEigen(A, B, BlockSize, (int)A.size(), evals, evecs);
+}
+
+void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& B,
+                           int k, int m, std::vector<double>& evals, std::vector<double>* evecs) {
   const int n = m * k;
   std::vector<double> d, e;
   std::vector<double> Q;
@@ -333,6 +346,24 @@ bool FullReorthogonalizedBlockLanczosAlgorithm::TestConvergence() {
   Previous = cur;
   Current = cur;
   HavePrevious = true;
+  if (ok && Residual > 0.0) {
+    // |H v - theta v| = |B_last * (last k coefficients of the Ritz vector)|: the pending
+    // block X_m times B[m-1] is what the Krylov space has not yet absorbed
+    const int k = BlockSize;
+    const int m = (int)A.size();
+    std::vector<double> ev2, vecs;
+    Diagonalize(ev2, &vecs);
+    const std::vector<double>& Bl = B[m - 1];
+    for (int i = 0; i < want && ok; ++i) {
+      double r2 = 0.0;
+      for (int p = 0; p < k; ++p) {
+        double s = 0.0;
+        for (int q = 0; q < k; ++q) s += Bl[(size_t)p * k + q] * vecs[(size_t)i * n + (m - 1) * k + q];
+        r2 += s * s;
+      }
+      if (std::sqrt(r2) > Residual * std::fabs(ev2[i])) ok = false;
+    }
+  }
   return ok;
 }
 
@@ -360,21 +391,72 @@ void FullReorthogonalizedBlockLanczosAlgorithm::GetEigenstates(int nbr, double* 
   Arch->FreeMemory(tmp);
 }
 
+// Spaces too small for a Krylov block are diagonalised densely (H built column by column with
+// the GPU product), like the reference's full-diagonalisation branch for small dimensions
+// (QHEOnSphereMainTask.cc:397).
+bool FullReorthogonalizedBlockLanczosAlgorithm::Dense() const { return Dim <= std::max<int64_t>(64, 4 * BlockSize); }
+
+void FullReorthogonalizedBlockLanczosAlgorithm::DenseSolve() {
+  dh_context* c = Arch->Context();
+  const int n = (int)Dim;
+  double* x = Arch->AllocateMemory(n);
+  double* y = Arch->AllocateMemory(n);
+  std::vector<double> H((size_t)n * n), e(n, 0.0), col(n);
+  for (int i = 0; i < n; ++i) {
+    e[i] = 1.0;
+    Arch->Check(dh_memcpy_h2d(c, x, e.data(), sizeof(double) * n), "unit vector");
+    e[i] = 0.0;
+    Arch->Check(dh_hv(Hamiltonian->Handle(), x, y, 1, 0, 0, -1), "H.v");
+    Arch->Check(dh_memcpy_d2h(c, col.data(), y, sizeof(double) * n), "column");
+    for (int r = 0; r < n; ++r) H[(size_t)r * n + i] = col[r];
+  }
+  Arch->FreeMemory(x);
+  Arch->FreeMemory(y);
+  for (int r = 0; r < n; ++r)
+    for (int q = r + 1; q < n; ++q) H[(size_t)r * n + q] = H[(size_t)q * n + r] = 0.5 * (H[(size_t)r * n + q] + H[(size_t)q * n + r]);
+  std::vector<double> d, ee, w;
+  BandToTridiagonal(n, std::max(1, n - 1), H, d, ee, nullptr);
+  TridiagonalEigen(n, d, ee, w, nullptr);
+  Current.assign(w.begin(), w.begin() + std::min<int>(NbrEigenvalue, n));
+}
+
+// convergence checks of a block Lanczos cost a band reduction; past the first ones they are
+// spaced so that at most ~10% extra blocks are run after convergence
+static bool check_due(int blocks, int lastCheck, int blockSize) {
+  if (blockSize == 1 || lastCheck < 0) return true;
+  return blocks - lastCheck >= std::max(1, blocks / 10);
+}
+
 LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
   auto t0 = std::chrono::steady_clock::now();
   LanczosResult res;
+  if (Dense()) {
+    DenseSolve();
+    res.converged = true;
+    res.eigenvalues = Current;
+    res.iterations = (int)Dim;
+    res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return res;
+  }
   int first = std::max(2, (NbrEigenvalue + 2 + BlockSize - 1) / BlockSize);
   RunLanczosAlgorithm(first);
+  int lastCheck = -1;
   while (true) {
-    bool conv = TestConvergence();
-    if (conv) {
+    if (Exhausted) {
+      std::vector<double> ev;
+      Diagonalize(ev, nullptr);
+      Current.assign(ev.begin(), ev.begin() + std::min<int>(NbrEigenvalue, (int)ev.size()));
       res.converged = true;
       break;
     }
-    if ((int)A.size() >= MaxBlocks || (int64_t)A.size() * BlockSize >= Dim) {
-      res.converged = ((int64_t)A.size() * BlockSize >= Dim);
-      break;
+    if (check_due((int)A.size(), lastCheck, BlockSize)) {
+      lastCheck = (int)A.size();
+      if (TestConvergence()) {
+        res.converged = true;
+        break;
+      }
     }
+    if ((int)A.size() >= MaxBlocks) break;
     RunLanczosAlgorithm(1);
   }
   Arch->Check(dh_synchronize(Arch->Context()), "sync");

@@ -133,6 +133,35 @@ int dh_hamiltonian_count_elements(dh_hamiltonian* ham, int64_t* nnz);
 int dh_hamiltonian_row_elements(dh_hamiltonian* ham, int64_t row, int64_t* targets, double* coefs,
                                 int64_t max_out, int64_t* n);
 
+/* ---------------------------------------------------------------- many sectors in one launch
+ * The Lz loop of FQHESphereFermionsTwoBodyGeneric.cc:270 (one FermionOnSphere + Hamiltonian +
+ * Lanczos per sector, run one after the other) becomes one set of sectors: the bases are
+ * concatenated (sector s owns global rows [offsets[s], offsets[s+1])), block vectors of the set
+ * are the per-sector [D_s][k] blocks concatenated, and every call below covers all sectors with
+ * active[s] != 0 (active = null: all) in one persistent launch. */
+typedef struct dh_sector_set dh_sector_set;
+int dh_sector_set_create(dh_context* ctx, dh_sector_set** set, int nbr_fermions, int lz_max, const int* total_lz,
+                         int nbr_sectors);
+int dh_sector_set_destroy(dh_sector_set* set);
+int64_t dh_sector_set_dimension(const dh_sector_set* set);
+int dh_sector_set_offsets(const dh_sector_set* set, int64_t* offsets /* nbr_sectors + 1 */);
+int dh_sector_set_get_states(dh_sector_set* set, uint64_t* states);
+/* factors in the dh_sphere_hamiltonian_create layout (shared by every sector) and the shift */
+int dh_sector_set_factors(dh_sector_set* set, const double* factors, const int* pair_offset, int nbr_factors,
+                          double shift);
+int dh_sector_set_launch(dh_sector_set* set, int lanes_per_row, int block_size);
+/* y = (add ? y : 0) + (H + shift) x over the active sectors (LowLevelMultipleAddMultiply per sector) */
+int dh_sectors_hv(dh_sector_set* set, const double* d_x, double* d_y, int k, int add, const unsigned char* active);
+/* per active sector j (in sector order): c[j][(b*k+p)*k+q] = sum_rows basis_b[row][p] w[row][q];
+ * the basis holds nbr_blocks set-blocks, block b at d_basis + b * dimension * k */
+int dh_sectors_block_dots(dh_sector_set* set, const double* d_basis, int nbr_blocks, const double* d_w, int k,
+                          const unsigned char* active, double* c);
+/* w[row][q] += sum_{b,p} basis_b[row][p] c[j][(b*k+p)*k+q] for the rows of active sector j */
+int dh_sectors_block_axpy(dh_sector_set* set, const double* d_basis, int nbr_blocks, double* d_w, int k,
+                          const unsigned char* active, const double* c);
+/* w[row] <- w[row] * m[j] (k x k per active sector j) */
+int dh_sectors_block_rmul(dh_sector_set* set, double* d_w, int k, const unsigned char* active, const double* m);
+
 /* ---------------------------------------------------------------- adjoint (no reference counterpart)
  * g[c] = <psi| H_c |phi> for every channel c of dh_sphere_hamiltonian_set_channels, in one
  * pass over the basis; if d_hpsi is not null it also receives H psi (the vector-Jacobian
@@ -156,6 +185,8 @@ int dh_block_rmul(dh_context* ctx, double* d_w, int64_t dim, int k, const double
 /* copy column `col` of a [row][k] block into / out of a plain vector of dim doubles */
 int dh_block_get_column(dh_context* ctx, const double* d_w, int64_t dim, int k, int col, double* d_out);
 int dh_block_set_column(dh_context* ctx, double* d_w, int64_t dim, int k, int col, const double* d_in);
+/* y = x over n doubles by a streaming kernel (also the HBM-traffic calibration kernel) */
+int dh_copy(dh_context* ctx, double* d_y, const double* d_x, int64_t n);
 /* y = a * x + b * y over n doubles */
 int dh_axpby(dh_context* ctx, double a, const double* d_x, double b, double* d_y, int64_t n);
 /* uniform [-1, 1) SplitMix64 values (seeded start vectors) */

@@ -44,12 +44,27 @@ int dhh_problem_channels(dhh_problem* problem, int* momenta, int capacity);
 
 /* Full-reorthogonalisation (block) Lanczos to the nbr_eigen lowest eigenvalues
  * (FullReorthogonalizedLanczosAlgorithm / FullReorthogonalizedBlockLanczosAlgorithm driven
- * as in QHEOnSphereMainTask.cc:728-808).  block_size 1 = plain.  d_eigenvectors (device,
+ * as in QHEOnSphereMainTask.cc:728-808).  block_size 1 = plain.  Converged when every wanted
+ * eigenvalue changes by less than precision * |value| between checks and, if residual > 0,
+ * every wanted Ritz pair has |H v - theta v| < residual * |theta|.  d_eigenvectors (device,
  * nbr_vectors * dim doubles, vector-major) may be null.  Outputs: eigenvalues (nbr_eigen),
  * vector products used, block steps, wall seconds (device work included). */
-int dhh_lanczos(dhh_problem* problem, int nbr_eigen, int block_size, int max_iter, double precision, uint64_t seed,
-                double* eigenvalues, int* products, int* blocks, double* seconds, double* d_eigenvectors,
-                int nbr_vectors);
+int dhh_lanczos(dhh_problem* problem, int nbr_eigen, int block_size, int max_iter, double precision,
+                double residual, uint64_t seed, double* eigenvalues, int* products, int* blocks, double* seconds,
+                double* d_eigenvectors, int nbr_vectors);
+
+/* Every sector total_lz[0..nbr_sectors) of nbr_fermions fermions at lz_max with the given
+ * pseudopotentials, nbr_eigen lowest eigenvalues each: the Lz loop of
+ * FQHESphereFermionsTwoBodyGeneric.cc:270 with a (block) full-reorthogonalisation Lanczos
+ * per sector.  batched != 0 advances all sectors together (one persistent product launch
+ * and segmented vector operations per step, waves within memory_gb of Krylov storage);
+ * batched == 0 solves the sectors one after the other like the reference.  Outputs per
+ * sector: eigenvalues (nbr_sectors * nbr_eigen), vector products, converged flags; total
+ * wall seconds. */
+int dhh_sectors_solve(dhh_arch* arch, int nbr_fermions, int lz_max, const int* total_lz, int nbr_sectors,
+                      const double* pseudopotentials, int nbr_eigen, int block_size, int max_iter, double precision,
+                      uint64_t seed, int batched, double memory_gb, double* eigenvalues, int* products,
+                      int* converged, double* seconds);
 
 /* Host-side tables (no GPU needed) */
 /* LLL Coulomb pseudopotentials V_0..V_nbr_flux in CoulombPseudopotentials units; round != 0

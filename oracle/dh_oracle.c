@@ -1011,6 +1011,30 @@ void or_spin_hv(void* space, const double* J, const double* Jz, int periodic, co
   spin_hv_range(s, J, Jz, periodic, x, y, 0, s->dim);
 }
 
+/* rows [first, last) only, split over threads like or_spin_hv_smp: the bounded sample the
+ * CPU baseline times */
+void or_spin_hv_rows(void* space, const double* J, const double* Jz, int periodic, const double* x, double* y,
+                     long first, long last, int nbrThreads) {
+  spin_space* s = (spin_space*)space;
+  long dim = s->dim;
+  if (nbrThreads < 1) nbrThreads = 1;
+  double** priv = (double**)calloc(nbrThreads, sizeof(double*));
+  for (int t = 1; t < nbrThreads; ++t) priv[t] = (double*)calloc(dim, sizeof(double));
+#pragma omp parallel for num_threads(nbrThreads) schedule(static, 1)
+  for (int t = 0; t < nbrThreads; ++t) {
+    long step = (last - first) / nbrThreads;
+    long a = first + step * t;
+    long b = (t == nbrThreads - 1) ? last : a + step;
+    spin_hv_range(s, J, Jz, periodic, x, (t == 0) ? y : priv[t], a, b);
+  }
+  for (int t = 1; t < nbrThreads; ++t) {
+#pragma omp parallel for num_threads(nbrThreads)
+    for (long i = 0; i < dim; ++i) y[i] += priv[t][i];
+    free(priv[t]);
+  }
+  free(priv);
+}
+
 void or_spin_hv_smp(void* space, const double* J, const double* Jz, int periodic, const double* x, double* y,
                     int nbrThreads) {
   spin_space* s = (spin_space*)space;

@@ -58,6 +58,8 @@ void or_spin_hv(void* space, const double* j, const double* jz, int periodic,
                 const double* x, double* y);
 void or_spin_hv_smp(void* space, const double* j, const double* jz, int periodic,
                     const double* x, double* y, int nbrThreads);
+void or_spin_hv_rows(void* space, const double* j, const double* jz, int periodic,
+                     const double* x, double* y, long first, long last, int nbrThreads);
 
 /* ---------------- Lanczos ---------------- */
 int or_tridiag_eigenvalues(int n, double* diag, double* upper, int maxIter);

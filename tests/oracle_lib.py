@@ -66,6 +66,7 @@ def lib():
     _sig(L, "or_spin_find_state_index", c_long, [c_void_p, c_uint64])
     _sig(L, "or_spin_hv", None, [c_void_p, P_D, P_D, c_int, P_D, P_D])
     _sig(L, "or_spin_hv_smp", None, [c_void_p, P_D, P_D, c_int, P_D, P_D, c_int])
+    _sig(L, "or_spin_hv_rows", None, [c_void_p, P_D, P_D, c_int, P_D, P_D, c_long, c_long, c_int])
     _sig(L, "or_tridiag_eigenvalues", c_int, [c_int, P_D, P_D, c_int])
     _sig(L, "or_fs_lanczos", c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_double,
                                      c_uint64, c_int, P_D, P_D])

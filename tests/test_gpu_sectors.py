@@ -1,0 +1,166 @@
+"""Many Lz sectors in one launch: concatenated bases, persistent multi-sector product,
+segmented Lanczos vector operations and the batched block Lanczos, against the oracle and the
+sequential per-sector path."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+N, LZ = 6, 15
+SECTORS = [0, 2, 4, 10, 24, 40, 56, 58, 60]  # twice Lz; the last ones are tiny
+
+
+def coulomb(lz):
+    return O.round14(O.coulomb_pseudopotentials(lz))
+
+
+class SectorSet:
+    def __init__(self, arch, n, lz, sectors, pseudo):
+        from diagham_amd import capi
+        self.capi, self.arch = capi, arch
+        arr = np.array(sectors, np.int32)
+        h = ctypes.c_void_p()
+        arch.check(capi.gpu().dh_sector_set_create(arch.ctx, ctypes.byref(h), n, lz,
+                                                   arr.ctypes.data_as(capi.P_I), len(sectors)), "sector set")
+        self.h = h
+        self.off = np.zeros(len(sectors) + 1, np.int64)
+        capi.gpu().dh_sector_set_offsets(h, self.off.ctypes.data_as(capi.P_I64))
+        self.total = int(self.off[-1])
+        f, po = capi.sphere_factors(lz, pseudo)
+        arch.check(capi.gpu().dh_sector_set_factors(h, capi.dptr(f), po.ctypes.data_as(capi.P_I), f.size, 0.0),
+                   "factors")
+
+    def states(self):
+        out = np.zeros(self.total, np.uint64)
+        self.capi.gpu().dh_sector_set_get_states(self.h, out.ctypes.data_as(self.capi.P_U64))
+        return out
+
+    def hv(self, x, k, active=None, y0=None):
+        c = self.capi
+        dx = c.DeviceBuffer.from_host(self.arch, x)
+        dy = c.DeviceBuffer.from_host(self.arch, y0 if y0 is not None else np.zeros_like(x))
+        am = None if active is None else bytes(np.asarray(active, np.uint8))
+        self.arch.check(c.gpu().dh_sectors_hv(self.h, dx.ptr, dy.ptr, k, 0, am), "sectors hv")
+        return dy.download()
+
+    def close(self):
+        self.capi.gpu().dh_sector_set_destroy(self.h)
+
+
+@pytest.mark.parametrize("lpr", [1, 4])
+def test_sector_set_basis_and_products(arch, lpr):
+    V = coulomb(LZ)
+    S = SectorSet(arch, N, LZ, SECTORS, V)
+    S.capi.gpu().dh_sector_set_launch(S.h, lpr, 256)
+    states = S.states()
+    fac = O.Factors(LZ, V)
+    for k in (1, 4):
+        X = O.splitmix(9 + k, S.total * k)
+        active = [1] * len(SECTORS)
+        active[1] = 0
+        y0 = O.splitmix(99, S.total * k)
+        Y = S.hv(X, k, active, y0)
+        for i, lz in enumerate(SECTORS):
+            ref = O.FermionSphere(N, lz, LZ)
+            a, b = S.off[i], S.off[i + 1]
+            assert np.array_equal(states[a:b], ref.states())
+            if not active[i]:
+                assert np.array_equal(Y[a * k:b * k], y0[a * k:b * k])  # inactive sectors untouched
+                continue
+            Xs = X[a * k:b * k].reshape(-1, k)
+            Ys = Y[a * k:b * k].reshape(-1, k)
+            for v in range(k):
+                y_ref = O.fs_hv(ref, fac, np.ascontiguousarray(Xs[:, v]))
+                assert np.linalg.norm(Ys[:, v] - y_ref) <= 1e-12 * max(np.linalg.norm(y_ref), 1e-300), (lz, k, v)
+    S.close()
+
+
+def test_segmented_vector_operations(arch):
+    c = __import__("diagham_amd.capi", fromlist=["capi"])
+    S = SectorSet(arch, N, LZ, SECTORS, coulomb(LZ))
+    k, nb = 4, 3
+    T = S.total
+    rng = np.random.default_rng(2)
+    Vb = rng.standard_normal(nb * T * k)
+    W = rng.standard_normal(T * k)
+    dV = c.DeviceBuffer.from_host(arch, Vb)
+    dW = c.DeviceBuffer.from_host(arch, W)
+    active = np.ones(len(SECTORS), np.uint8)
+    active[3] = 0
+    am = bytes(active)
+    act = [i for i in range(len(SECTORS)) if active[i]]
+    out = np.zeros(len(act) * nb * k * k)
+    arch.check(c.gpu().dh_sectors_block_dots(S.h, dV.ptr, nb, dW.ptr, k, am, c.dptr(out)), "dots")
+    out = out.reshape(len(act), nb * k, k)
+    Vm = Vb.reshape(nb, T, k)
+    Wm = W.reshape(T, k)
+    for a, i in enumerate(act):
+        lo, hi = S.off[i], S.off[i + 1]
+        ref = np.concatenate([Vm[b, lo:hi].T @ Wm[lo:hi] for b in range(nb)], axis=0)
+        assert np.allclose(out[a], ref, rtol=1e-12, atol=1e-10)
+    C = rng.standard_normal((len(act), nb * k, k))
+    arch.check(c.gpu().dh_sectors_block_axpy(S.h, dV.ptr, nb, dW.ptr, k, am, c.dptr(np.ascontiguousarray(C))), "axpy")
+    W2 = dW.download().reshape(T, k)
+    for a, i in enumerate(act):
+        lo, hi = S.off[i], S.off[i + 1]
+        ref = Wm[lo:hi] + sum(Vm[b, lo:hi] @ C[a, b * k:(b + 1) * k] for b in range(nb))
+        assert np.allclose(W2[lo:hi], ref, rtol=1e-12, atol=1e-10)
+    lo, hi = S.off[3], S.off[4]
+    assert np.array_equal(W2[lo:hi], Wm[lo:hi])
+    M = rng.standard_normal((len(act), k, k))
+    dW.upload(W)
+    arch.check(c.gpu().dh_sectors_block_rmul(S.h, dW.ptr, k, am, c.dptr(np.ascontiguousarray(M))), "rmul")
+    W3 = dW.download().reshape(T, k)
+    for a, i in enumerate(act):
+        lo, hi = S.off[i], S.off[i + 1]
+        assert np.allclose(W3[lo:hi], Wm[lo:hi] @ M[a], rtol=1e-12, atol=1e-10)
+    S.close()
+
+
+def solve(arch, n, lz, sectors, nev, k, batched, max_iter=2000):
+    from diagham_amd import capi
+    V = capi.coulomb_pseudopotentials(lz, True)
+    arr = np.array(sectors, np.int32)
+    ev = np.zeros(len(sectors) * nev)
+    prods = np.zeros(len(sectors), np.int32)
+    conv = np.zeros(len(sectors), np.int32)
+    secs = ctypes.c_double()
+    rc = capi.host().dhh_sectors_solve(arch.h, n, lz, arr.ctypes.data_as(capi.P_I), len(sectors), capi.dptr(V), nev,
+                                       k, max_iter, 1e-14, 0x5EED0000, int(batched), 8.0, capi.dptr(ev),
+                                       prods.ctypes.data_as(capi.P_I), conv.ctypes.data_as(capi.P_I),
+                                       ctypes.byref(secs))
+    assert rc == 0, capi.host().dhh_last_error().decode()
+    return ev.reshape(len(sectors), nev), prods, conv, secs.value
+
+
+def test_batched_equals_sequential_and_oracle(arch):
+    sectors = list(range(0, 61, 2))  # every Lz >= 0 sector of N = 6, 2S = 15
+    nev = 5
+    eb, pb, cb, tb = solve(arch, N, LZ, sectors, nev, 4, True)
+    es, ps, cs, ts = solve(arch, N, LZ, sectors, nev, 1, False)
+    assert cb.all() and cs.all()
+    for i, lz in enumerate(sectors):
+        d = O.lib().or_fs_dimension(N, LZ, lz)
+        m = min(nev, d)
+        np.testing.assert_allclose(eb[i, :m], es[i, :m], rtol=0, atol=1e-10)
+        if d <= 400:  # oracle dense diagonalisation
+            ref = O.FermionSphere(N, lz, LZ)
+            fac = O.Factors(LZ, coulomb(LZ))
+            H = np.zeros((d, d))
+            for c in range(d):
+                e = np.zeros(d)
+                e[c] = 1.0
+                H[:, c] = O.fs_hv(ref, fac, e)
+            np.testing.assert_allclose(eb[i, :m], np.linalg.eigvalsh(H)[:m], rtol=0, atol=1e-10)
+
+
+def test_batched_reference_sector(arch, golden):
+    """the 2S = 21, N = 8 Lz = 0 sector solved inside a batch of sectors reproduces the reference"""
+    ref = np.array(golden["fermions_sphere_n8_2s21_lz0"]["eigenvalues"])
+    ev, prods, conv, secs = solve(arch, 8, 21, [0, 2, 4, 6], 10, 8, True)
+    assert conv.all()
+    np.testing.assert_allclose(ev[0], ref, rtol=0, atol=1e-10)

@@ -70,6 +70,8 @@ GPU_SIGNATURES = {
     "dh_sectors_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
     "dh_sectors_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
     "dh_sectors_block_rmul": (c_int, [c_void_p, c_void_p, c_int, c_char_p, P_D]),
+    "dh_momentum_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_I, P_I, c_int]),
+    "dh_sector_set_lattice_factors": (c_int, [c_void_p, c_int, P_I, P_I, P_D, c_double]),
     "dh_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_rmul": (c_int, [c_void_p, c_void_p, c_int64, c_int, P_D]),
@@ -95,6 +97,14 @@ HOST_SIGNATURES = {
                             c_void_p, c_int]),
     "dhh_sectors_solve": (c_int, [c_void_p, c_int, c_int, P_I, c_int, P_D, c_int, c_int, c_int, c_double, c_uint64,
                                   c_int, c_double, P_D, P_I, P_I, P_D]),
+    "dhh_autotune": (c_int, [c_void_p, P_I, c_int, c_int, P_D, P_I]),
+    "dhh_lattice_sectors_solve": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_double, c_double,
+                                          c_double, c_double, c_int, P_I, P_I, c_int, c_int, c_int, c_int, c_double,
+                                          c_uint64, c_double, P_D, P_I, P_I, P_D]),
+    "dhh_checkerboard_lower_band": (c_int, [c_int, c_int, c_double, c_double, c_double, c_double, c_double,
+                                            c_double, P_D, P_D]),
+    "dhh_checkerboard_factors": (c_int, [c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
+                                         c_int, P_I, P_I, P_I, P_D, P_I64]),
     "dhh_coulomb_pseudopotentials": (c_int, [c_int, c_int, P_D]),
     "dhh_clebsch_gordan": (c_double, [c_int, c_int, c_int, c_int, c_int]),
     "dhh_sphere_factors": (c_int, [c_int, P_D, P_D, P_I, P_I]),
@@ -364,6 +374,19 @@ class Problem:
         g = np.zeros(nc)
         self.arch.check(gpu().dh_sphere_adjoint(self.ham, d_psi, d_phi, dptr(g), d_hpsi), "dh_sphere_adjoint")
         return g
+
+    def autotune(self, ks=(1, 2, 4, 8, 16), repeats=5):
+        """Time every launch shape per block size; returns (table rows, best k) and leaves the
+        fastest launch of the best k configured (see dhh_autotune)."""
+        ks = np.ascontiguousarray(ks, dtype=np.int32)
+        table = np.zeros(6 * ks.size)
+        best = ctypes.c_int()
+        rc = host().dhh_autotune(self.h, ks.ctypes.data_as(P_I), ks.size, repeats, dptr(table), ctypes.byref(best))
+        if rc != 0:
+            raise DHError("dhh_autotune: " + host().dhh_last_error().decode())
+        rows = [dict(zip(("k", "lanes_per_row", "block_size", "index_mode", "ms", "elements_per_s"), r))
+                for r in table.reshape(-1, 6)]
+        return rows, best.value
 
     # -- Lanczos
     def lanczos(self, nbr_eigen, block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000,

@@ -70,6 +70,53 @@ struct dh_hamiltonian {
   int lanesPerRow = 1, blockSize = 256, indexMode = DH_INDEX_RANK, ldsTables = 1;
 };
 
+enum dh_sector_kind { DH_SECTORS_SPHERE = 1, DH_SECTORS_MOMENTUM = 2 };
+
+// A set of symmetry sectors of one problem with concatenated bases (sector s owns global
+// rows [offsets[s], offsets[s+1])).  Sphere sets are real (Lz sectors); momentum sets are
+// complex ((kx, ky) sectors, vectors stored as interleaved (re, im) pairs).
+struct dh_sector_set {
+  dh_context* ctx = nullptr;
+  int kind = DH_SECTORS_SPHERE;
+  int nbrFermions = 0, lzMax = 0, nbrOrbitals = 0, nbrSectors = 0;
+  std::vector<int> totalLz, orbitalSum;  // sphere: 2 Lz and orbital-index sum per sector
+  int nbrSiteX = 0, nbrSiteY = 0;        // momentum: lattice and (kx, ky) per sector
+  std::vector<int> kx, ky;
+  std::vector<int64_t> offsets;  // nbrSectors + 1
+  int64_t totalDim = 0;
+  uint64_t* d_states = nullptr;
+  int rankBits = 0;
+  int64_t loSize = 0, hiSize = 0;  // hi table of sector s starts at s * hiSize
+  uint32_t* d_rankLo = nullptr;
+  uint32_t* d_rankHi = nullptr;
+  int64_t* d_offsets = nullptr;
+  // sphere factors
+  double* d_factors = nullptr;
+  int* d_pairOffset = nullptr;
+  int nbrFactors = 0, nbrPairs = 0;
+  // momentum factors: per momentum-sum sector j its pair list and complex factor matrix
+  int nbrSums = 0;
+  int* d_pairInfo = nullptr;     // [pair id b(b-1)/2 + a] -> (sum << 16) | position
+  int* d_sumPairOff = nullptr;   // [nbrSums + 1] into d_sumPairs
+  int* d_sumPairs = nullptr;     // (a, b) packed as a | (b << 8)
+  int64_t* d_sumFactorOff = nullptr;  // [nbrSums + 1] complex entries
+  double* d_cfactors = nullptr;  // interleaved (re, im)
+  int64_t nbrComplexFactors = 0;
+  double shift = 0.0;
+  int lanesPerRow = 1, blockSize = 256;
+  // per-call work descriptor: [active sector ids (int64)] [tile prefix (int64)]
+  int64_t* d_work = nullptr;
+  size_t workCap = 0;
+  double* d_coef = nullptr;  // per-sector coefficient matrices for axpy / rmul
+  size_t coefCap = 0;
+};
+
+// work descriptor of the active sectors (dh_multisector.hip)
+int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_t rowsPerTile, int* nActive,
+                           int64_t* totalTiles);
+// complex product over a momentum set (dh_lattice.hip)
+int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active);
+
 // error helpers
 int dh_fail(dh_context* ctx, int code, const std::string& msg);
 #define DH_HIP(ctx, call)                                                                 \

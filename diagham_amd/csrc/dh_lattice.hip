@@ -1,0 +1,424 @@
+// Spinless fermions in one Chern band on a lattice, in momentum space: complex FP64 H.v.
+//
+// Reference path: FermionOnSquareLatticeMomentumSpace (FQHE/src/HilbertSpace/
+// FermionOnSquareLatticeMomentumSpace.cc:76, GenerateStates :290) with the FermionOnSphere
+// bit, sign and search conventions, and ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian::
+// EvaluateMNTwoBodyAddMultiplyComponent (FQHE/src/Hamiltonian/ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian.h:370):
+// for every momentum-sum sector j, every annihilated pair i1 = (a < b) of that sum and every
+// created pair i2 = (c < d) of the same sum, target += sign * F[j][i1, i2] * x.
+// The kernel gathers: H is Hermitian, so row r of H is the conjugate of the terms generated
+// from state r, y[r] = sum_terms sign * conj(F) * x[target].
+//
+// Orbital o = kx * Ny + ky; a sector is a total momentum class (Kx mod Nx, Ky mod Ny).  The
+// basis is unranked on the device and FindStateIndex is the combinatorial rank with the
+// momentum class taking the place of the orbital-index sum of the sphere.
+#include <algorithm>
+#include <cstring>
+
+#include "dh_index.h"
+#include "dh_internal.h"
+
+namespace {
+struct Classes {
+  int nx, ny;
+  int add(int a, int b) const { return (((a / ny + b / ny) % nx) * ny) + ((a % ny + b % ny) % ny); }
+  int sub(int a, int b) const { return (((a / ny - b / ny + nx) % nx) * ny) + ((a % ny - b % ny + ny) % ny); }
+};
+
+// T[(j * (N+1) + n) * C + c]: number of n-subsets of orbitals {0..j-1} with momentum class c
+std::vector<int64_t> class_table(int nOrb, int N, const Classes& cl) {
+  const int C = cl.nx * cl.ny;
+  std::vector<int64_t> t((size_t)(nOrb + 1) * (N + 1) * C, 0);
+  auto idx = [&](int j, int n, int c) { return ((size_t)j * (N + 1) + n) * C + c; };
+  t[idx(0, 0, 0)] = 1;
+  for (int j = 0; j < nOrb; ++j)
+    for (int n = 0; n <= N; ++n)
+      for (int c = 0; c < C; ++c) {
+        int64_t v = t[idx(j, n, c)];
+        if (n >= 1) v += t[idx(j, n - 1, cl.sub(c, j))];
+        t[idx(j + 1, n, c)] = v;
+      }
+  return t;
+}
+}  // namespace
+
+__device__ __forceinline__ int cls_sub(int a, int b, int nx, int ny) {
+  return (((a / ny - b / ny + nx) % nx) * ny) + ((a % ny - b % ny + ny) % ny);
+}
+
+__global__ void lattice_unrank_kernel(uint64_t* __restrict__ states, int64_t dim, int nOrb, int N, int target,
+                                      const int64_t* __restrict__ T, int nx, int ny) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const int C = nx * ny;
+  int64_t rem = i;
+  int n = N, c = target;
+  uint64_t state = 0;
+  for (int j = nOrb - 1; j >= 0 && n > 0; --j) {
+    int rest = cls_sub(c, j, nx, ny);
+    int64_t cnt = T[((int64_t)j * (N + 1) + (n - 1)) * C + rest];
+    if (rem < cnt) {
+      state |= (1ull << j);
+      --n;
+      c = rest;
+    } else {
+      rem -= cnt;
+    }
+  }
+  states[i] = state;
+}
+
+extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out, int N, int nx, int ny,
+                                             const int* kx, const int* ky, int nbrSectors) {
+  if (!c || !out || !kx || !ky || nbrSectors <= 0) return DH_E_ARG;
+  *out = nullptr;
+  const int nOrb = nx * ny;
+  if (nx <= 0 || ny <= 0 || N <= 0 || N > nOrb) return dh_fail(c, DH_E_ARG, "need 0 < nbr_fermions <= Nx * Ny");
+  if (nOrb > 62) return dh_fail(c, DH_E_RANGE, "Nx * Ny must be <= 62 (64-bit states)");
+  hipSetDevice(c->device);
+  Classes cl{nx, ny};
+  const int C = nOrb;
+  std::vector<int64_t> T = class_table(nOrb, N, cl);
+  auto Tat = [&](int j, int n, int cc) -> int64_t {
+    if (j < 0 || j > nOrb || n < 0 || n > N) return 0;
+    return T[((size_t)j * (N + 1) + n) * C + cc];
+  };
+  dh_sector_set* S = new dh_sector_set();
+  S->ctx = c;
+  S->kind = DH_SECTORS_MOMENTUM;
+  S->nbrFermions = N;
+  S->nbrOrbitals = nOrb;
+  S->lzMax = nOrb;
+  S->nbrSiteX = nx;
+  S->nbrSiteY = ny;
+  S->nbrSectors = nbrSectors;
+  S->offsets.assign(nbrSectors + 1, 0);
+  std::vector<int> target(nbrSectors);
+  for (int i = 0; i < nbrSectors; ++i) {
+    if (kx[i] < 0 || kx[i] >= nx || ky[i] < 0 || ky[i] >= ny) {
+      delete S;
+      return dh_fail(c, DH_E_ARG, "momentum sector outside the Brillouin zone");
+    }
+    target[i] = kx[i] * ny + ky[i];
+    int64_t d = Tat(nOrb, N, target[i]);
+    if (d <= 0 || d >= (1ll << 31)) {
+      delete S;
+      return dh_fail(c, DH_E_ARG, "a sector is empty or larger than 2^31");
+    }
+    S->kx.push_back(kx[i]);
+    S->ky.push_back(ky[i]);
+    S->offsets[i + 1] = S->offsets[i] + d;
+  }
+  S->totalDim = S->offsets[nbrSectors];
+  S->rankBits = (nOrb + 1) / 2;
+  S->loSize = 1ll << S->rankBits;
+  S->hiSize = 1ll << (nOrb - S->rankBits);
+  std::vector<uint32_t> lo(S->loSize, 0), hi((size_t)S->hiSize * nbrSectors, 0);
+  for (uint64_t l = 0; l < (uint64_t)S->loSize; ++l) {
+    int n = 0, cc = 0;
+    int64_t acc = 0;
+    for (uint64_t r = l; r; r &= r - 1) {
+      int j = __builtin_ctzll(r);
+      ++n;
+      cc = cl.add(cc, j);
+      acc += Tat(j, n, cc);
+    }
+    lo[l] = (uint32_t)acc;
+  }
+  for (int sec = 0; sec < nbrSectors; ++sec)
+    for (uint64_t h = 0; h < (uint64_t)S->hiSize; ++h) {
+      int popH = __builtin_popcountll(h), clsH = 0;
+      for (uint64_t r = h; r; r &= r - 1) clsH = cl.add(clsH, S->rankBits + __builtin_ctzll(r));
+      int n = N - popH;
+      if (n < 0) continue;
+      int cc = cl.sub(target[sec], clsH);
+      int64_t acc = 0;
+      for (uint64_t r = h; r; r &= r - 1) {
+        int j = S->rankBits + __builtin_ctzll(r);
+        ++n;
+        cc = cl.add(cc, j);
+        acc += Tat(j, n, cc);
+      }
+      hi[(size_t)sec * S->hiSize + h] = (uint32_t)acc;
+    }
+  int rc;
+  int64_t* d_T = nullptr;
+  if ((rc = dh_malloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
+      (rc = dh_malloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
+    if (d_T) hipFree(d_T);
+    dh_sector_set_destroy(S);
+    return rc;
+  }
+  hipMemcpyAsync(d_T, T.data(), sizeof(int64_t) * T.size(), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(S->d_rankLo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(S->d_rankHi, hi.data(), sizeof(uint32_t) * hi.size(), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(S->d_offsets, S->offsets.data(), sizeof(int64_t) * (nbrSectors + 1), hipMemcpyHostToDevice,
+                 c->stream);
+  for (int sec = 0; sec < nbrSectors; ++sec) {
+    int64_t d = S->offsets[sec + 1] - S->offsets[sec];
+    hipLaunchKernelGGL(lattice_unrank_kernel, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, c->stream,
+                       S->d_states + S->offsets[sec], d, nOrb, N, target[sec], d_T, nx, ny);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_T);
+  if (e != hipSuccess) {
+    dh_sector_set_destroy(S);
+    return dh_fail(c, DH_E_HIP, std::string("momentum sector set: ") + hipGetErrorString(e));
+  }
+  *out = S;
+  return DH_OK;
+}
+
+extern "C" int dh_sector_set_lattice_factors(dh_sector_set* S, int nbrSums, const int* sumPairCount,
+                                             const int* sumPairs, const double* factors, double shift) {
+  if (!S || !sumPairCount || !sumPairs || !factors) return DH_E_ARG;
+  dh_context* c = S->ctx;
+  if (S->kind != DH_SECTORS_MOMENTUM) return dh_fail(c, DH_E_ARG, "not a momentum sector set");
+  const int nOrb = S->nbrOrbitals;
+  if (nbrSums != S->nbrSiteX * S->nbrSiteY) return dh_fail(c, DH_E_ARG, "nbr_sums must be Nx * Ny");
+  const int nPairs = nOrb * (nOrb - 1) / 2;
+  std::vector<int> info(nPairs, -1), pairOff(nbrSums + 1, 0), packed;
+  std::vector<int64_t> facOff(nbrSums + 1, 0);
+  int pos = 0;
+  for (int j = 0; j < nbrSums; ++j) {
+    int n = sumPairCount[j];
+    if (n < 0 || n > 65535) return dh_fail(c, DH_E_ARG, "pair count");
+    pairOff[j + 1] = pairOff[j] + n;
+    facOff[j + 1] = facOff[j] + (int64_t)n * n;
+    for (int i = 0; i < n; ++i, ++pos) {
+      int a = sumPairs[2 * pos], b = sumPairs[2 * pos + 1];
+      if (a < 0 || b <= a || b >= nOrb) return dh_fail(c, DH_E_ARG, "pairs must be (a < b) orbitals");
+      int cls = (((a / S->nbrSiteY + b / S->nbrSiteY) % S->nbrSiteX) * S->nbrSiteY) +
+                ((a % S->nbrSiteY + b % S->nbrSiteY) % S->nbrSiteY);
+      if (cls != j) return dh_fail(c, DH_E_ARG, "pair listed under the wrong momentum sum");
+      int id = b * (b - 1) / 2 + a;
+      if (info[id] != -1) return dh_fail(c, DH_E_ARG, "pair listed twice");
+      info[id] = (j << 16) | i;
+      packed.push_back(a | (b << 8));
+    }
+  }
+  for (int id = 0; id < nPairs; ++id)
+    if (info[id] == -1) return dh_fail(c, DH_E_ARG, "every pair a < b must be listed once");
+  for (void* p : {(void*)S->d_pairInfo, (void*)S->d_sumPairOff, (void*)S->d_sumPairs, (void*)S->d_sumFactorOff,
+                  (void*)S->d_cfactors})
+    if (p) hipFree(p);
+  S->d_pairInfo = S->d_sumPairOff = S->d_sumPairs = nullptr;
+  S->d_sumFactorOff = nullptr;
+  S->d_cfactors = nullptr;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&S->d_pairInfo, sizeof(int) * nPairs)) ||
+      (rc = dh_malloc(c, (void**)&S->d_sumPairOff, sizeof(int) * (nbrSums + 1))) ||
+      (rc = dh_malloc(c, (void**)&S->d_sumPairs, sizeof(int) * packed.size())) ||
+      (rc = dh_malloc(c, (void**)&S->d_sumFactorOff, sizeof(int64_t) * (nbrSums + 1))) ||
+      (rc = dh_malloc(c, (void**)&S->d_cfactors, sizeof(double) * 2 * facOff.back())))
+    return rc;
+  DH_HIP(c, hipMemcpyAsync(S->d_pairInfo, info.data(), sizeof(int) * nPairs, hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_sumPairOff, pairOff.data(), sizeof(int) * (nbrSums + 1), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_sumPairs, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_sumFactorOff, facOff.data(), sizeof(int64_t) * (nbrSums + 1), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_cfactors, factors, sizeof(double) * 2 * facOff.back(), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  S->nbrSums = nbrSums;
+  S->nbrPairs = nPairs;
+  S->nbrComplexFactors = facOff.back();
+  S->shift = shift;
+  return DH_OK;
+}
+
+struct LatticeArgs {
+  const uint64_t* states;
+  const int64_t* offsets;
+  int nbrOrbitals, nbrPairs, nbrSums, totalPairs;
+  const int* pairInfo;
+  const int* sumPairOff;
+  const int* sumPairs;
+  const int64_t* sumFactorOff;
+  const double* factors;
+  const uint32_t* rankLo;
+  const uint32_t* rankHi;
+  int rankBits;
+  int64_t loSize, hiSize;
+  double shift;
+  const int64_t* work;
+  int nActive;
+  int64_t totalTiles;
+};
+
+__device__ __forceinline__ int lat_tile_sector(const int64_t* sPrefix, int nActive, int64_t tile) {
+  int lo = 0, hi = nActive;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (sPrefix[mid] <= tile)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// K complex vectors per row: x and y hold 2K doubles per row, (re, im) interleaved
+template <int K, int LPR, bool LO_LDS>
+__global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const double* __restrict__ x,
+                                                          double* __restrict__ y, int add) {
+  extern __shared__ int64_t smem64[];
+  int64_t* sWork = smem64;                                    // 2 nActive + 1
+  int64_t* sFacOff = sWork + 2 * a.nActive + 1;               // nbrSums + 1
+  int* sInfo = reinterpret_cast<int*>(sFacOff + a.nbrSums + 1);  // nbrPairs
+  int* sPairOff = sInfo + a.nbrPairs;                         // nbrSums + 1
+  int* sPairs = sPairOff + a.nbrSums + 1;                     // totalPairs
+  uint32_t* sLo = reinterpret_cast<uint32_t*>(sPairs + a.totalPairs);
+  for (int i = threadIdx.x; i < 2 * a.nActive + 1; i += blockDim.x) sWork[i] = a.work[i];
+  for (int i = threadIdx.x; i <= a.nbrSums; i += blockDim.x) {
+    sFacOff[i] = a.sumFactorOff[i];
+    sPairOff[i] = a.sumPairOff[i];
+  }
+  for (int i = threadIdx.x; i < a.nbrPairs; i += blockDim.x) sInfo[i] = a.pairInfo[i];
+  for (int i = threadIdx.x; i < a.totalPairs; i += blockDim.x) sPairs[i] = a.sumPairs[i];
+  if constexpr (LO_LDS)
+    for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo[i];
+  __syncthreads();
+  const uint32_t* lo = LO_LDS ? sLo : a.rankLo;
+  const int64_t* sPrefix = sWork + a.nActive;
+  const int lane = threadIdx.x % LPR;
+  const int64_t rowsPerTile = blockDim.x / LPR;
+  for (int64_t tile = dh_xcd_block(blockIdx.x, gridDim.x); tile < a.totalTiles; tile += gridDim.x) {
+    const int j = lat_tile_sector(sPrefix, a.nActive, tile);
+    const int sec = (int)sWork[j];
+    const int64_t off = a.offsets[sec];
+    const int64_t dim = a.offsets[sec + 1] - off;
+    const uint32_t* hi = a.rankHi + (int64_t)sec * a.hiSize;
+    const int64_t local = (tile - sPrefix[j]) * rowsPerTile + threadIdx.x / LPR;
+    const bool active = local < dim;
+    const int64_t g = off + (active ? local : 0);
+    const uint64_t s = active ? a.states[g] : 0ull;
+    double acc[2 * K];
+#pragma unroll
+    for (int v = 0; v < 2 * K; ++v) acc[v] = 0.0;
+    const double* xs = x + off * 2 * K;
+    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+      const int n1 = __builtin_ctzll(r1);  // lower annihilated orbital
+      for (uint64_t r2 = s & ~((2ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+        const int n2 = __builtin_ctzll(r2);  // higher annihilated orbital
+        // AA(n1, n2): a_n2 first, then a_n1
+        const int sgnA = __popcll(s >> n2) + __popcll((s ^ (1ull << n2)) >> n1);
+        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+        const int info = sInfo[(n2 * (n2 - 1)) / 2 + n1];
+        const int sum = info >> 16, i1 = info & 0xffff;
+        const int p0 = sPairOff[sum], np = sPairOff[sum + 1] - p0;
+        const double* F = a.factors + 2 * (sFacOff[sum] + (int64_t)i1 * np);
+        for (int i2 = lane; i2 < np; i2 += LPR) {
+          const int pk = sPairs[p0 + i2];
+          const int m1 = pk & 0xff, m2 = pk >> 8;  // m1 < m2
+          const uint64_t cm = (1ull << m1) | (1ull << m2);
+          if (t & cm) continue;
+          // AdAd(m1, m2): a+_m2 first, then a+_m1
+          const int sg = sgnA + __popcll(t >> m2) + __popcll((t | (1ull << m2)) >> m1);
+          const int64_t idx = dh_find_rank(t | cm, lo, hi, a.rankBits, dim);
+          double cr = F[2 * i2], ci = -F[2 * i2 + 1];  // conj(F)
+          if (sg & 1) {
+            cr = -cr;
+            ci = -ci;
+          }
+          const double* xr = xs + idx * 2 * K;
+#pragma unroll
+          for (int v = 0; v < K; ++v) {
+            double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
+            acc[2 * v] += cr * p.x - ci * p.y;
+            acc[2 * v + 1] += cr * p.y + ci * p.x;
+          }
+        }
+      }
+    }
+    if constexpr (LPR > 1) {
+#pragma unroll
+      for (int v = 0; v < 2 * K; ++v)
+        for (int o = LPR / 2; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o, LPR);
+    }
+    if (active && lane == 0) {
+      const double* xr = x + g * 2 * K;
+      double* yr = y + g * 2 * K;
+#pragma unroll
+      for (int v = 0; v < 2 * K; ++v) yr[v] = (add ? yr[v] : 0.0) + acc[v] + a.shift * xr[v];
+    }
+  }
+}
+
+template <int K, int LPR>
+static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add, const unsigned char* active) {
+  dh_context* c = S->ctx;
+  int64_t rowsPerTile = S->blockSize / LPR;
+  int na;
+  int64_t tiles;
+  int rc = dh_sectors_upload_work(S, active, rowsPerTile, &na, &tiles);
+  if (rc) return rc;
+  if (tiles == 0) return DH_OK;
+  LatticeArgs a;
+  a.states = S->d_states;
+  a.offsets = S->d_offsets;
+  a.nbrOrbitals = S->nbrOrbitals;
+  a.nbrPairs = S->nbrPairs;
+  a.nbrSums = S->nbrSums;
+  a.totalPairs = S->nbrPairs;
+  a.pairInfo = S->d_pairInfo;
+  a.sumPairOff = S->d_sumPairOff;
+  a.sumPairs = S->d_sumPairs;
+  a.sumFactorOff = S->d_sumFactorOff;
+  a.factors = S->d_cfactors;
+  a.rankLo = S->d_rankLo;
+  a.rankHi = S->d_rankHi;
+  a.rankBits = S->rankBits;
+  a.loSize = S->loSize;
+  a.hiSize = S->hiSize;
+  a.shift = S->shift;
+  a.work = S->d_work;
+  a.nActive = na;
+  a.totalTiles = tiles;
+  size_t base = 8 * (2 * (size_t)na + 1) + 8 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs +
+                4 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs;
+  base = (base + 7) & ~size_t(7);
+  size_t withLo = base + 4 * (size_t)S->loSize;
+  bool loLds = withLo <= 64 * 1024;
+  size_t lds = loLds ? withLo : base;
+  if (lds > 160 * 1024) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
+  auto launch = [&](auto kern) -> int {
+    if (lds > 64 * 1024)
+      DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int perCU = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, S->blockSize, lds) != hipSuccess || perCU < 1)
+      perCU = 1;
+    int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(S->blockSize), lds, c->stream, a, x, y, add);
+    DH_HIP(c, hipGetLastError());
+    return DH_OK;
+  };
+  if (loLds) return launch(lattice_hv_kernel<K, LPR, true>);
+  return launch(lattice_hv_kernel<K, LPR, false>);
+}
+
+template <int K>
+static int lattice_lpr(dh_sector_set* S, const double* x, double* y, int add, const unsigned char* active) {
+  switch (S->lanesPerRow) {
+    case 1: return lattice_launch<K, 1>(S, x, y, add, active);
+    case 2: return lattice_launch<K, 2>(S, x, y, add, active);
+    case 4: return lattice_launch<K, 4>(S, x, y, add, active);
+    default: return lattice_launch<K, 8>(S, x, y, add, active);
+  }
+}
+
+int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active) {
+  if (!S->d_cfactors) return dh_fail(S->ctx, DH_E_STATE, "lattice factors not set");
+  switch (k) {
+    case 1: return lattice_lpr<1>(S, x, y, add, active);
+    case 2: return lattice_lpr<2>(S, x, y, add, active);
+    case 4: return lattice_lpr<4>(S, x, y, add, active);
+    case 8: return lattice_lpr<8>(S, x, y, add, active);
+    default: return dh_fail(S->ctx, DH_E_ARG, "k must be 1, 2, 4 or 8 for complex vectors");
+  }
+}

@@ -16,30 +16,6 @@
 #include "dh_internal.h"
 #include "dh_sphere_row.h"
 
-struct dh_sector_set {
-  dh_context* ctx = nullptr;
-  int nbrFermions = 0, lzMax = 0, nbrOrbitals = 0, nbrSectors = 0;
-  std::vector<int> totalLz, orbitalSum;
-  std::vector<int64_t> offsets;  // nbrSectors + 1
-  int64_t totalDim = 0;
-  uint64_t* d_states = nullptr;
-  int rankBits = 0;
-  int64_t loSize = 0, hiSize = 0;  // hi table of sector s starts at s * hiSize
-  uint32_t* d_rankLo = nullptr;
-  uint32_t* d_rankHi = nullptr;
-  int64_t* d_offsets = nullptr;
-  double* d_factors = nullptr;
-  int* d_pairOffset = nullptr;
-  int nbrFactors = 0, nbrPairs = 0;
-  double shift = 0.0;
-  int lanesPerRow = 1, blockSize = 256;
-  // per-call work descriptor: [active sector ids (int64)] [tile prefix (int64)]
-  int64_t* d_work = nullptr;
-  size_t workCap = 0;
-  double* d_coef = nullptr;  // per-sector coefficient matrices for axpy / rmul
-  size_t coefCap = 0;
-};
-
 // device-side subset-count table (same recursion as the single-sector basis)
 static std::vector<int64_t> subset_table(int nOrb, int maxN, int& maxSum) {
   maxSum = 0;
@@ -183,7 +159,9 @@ extern "C" int dh_sector_set_destroy(dh_sector_set* S) {
   dh_context* c = S->ctx;
   hipStreamSynchronize(c->stream);
   for (void* p : {(void*)S->d_states, (void*)S->d_rankLo, (void*)S->d_rankHi, (void*)S->d_offsets,
-                  (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef})
+                  (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef,
+                  (void*)S->d_pairInfo, (void*)S->d_sumPairOff, (void*)S->d_sumPairs, (void*)S->d_sumFactorOff,
+                  (void*)S->d_cfactors})
     if (p) hipFree(p);
   delete S;
   return DH_OK;
@@ -245,7 +223,7 @@ extern "C" int dh_sector_set_launch(dh_sector_set* S, int lanesPerRow, int block
 
 // work descriptor of the active sectors for tiles of `rowsPerTile` rows:
 // d_work[0..na) = sector ids, d_work[na..2na] = tile prefix
-static int upload_work(dh_sector_set* S, const unsigned char* active, int64_t rowsPerTile, int* nActive,
+int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_t rowsPerTile, int* nActive,
                        int64_t* totalTiles) {
   dh_context* c = S->ctx;
   std::vector<int64_t> w;
@@ -383,7 +361,7 @@ static int sectors_hv_launch(dh_sector_set* S, const double* x, double* y, int a
   int64_t rowsPerTile = S->blockSize / LPR;
   int na;
   int64_t tiles;
-  int rc = upload_work(S, active, rowsPerTile, &na, &tiles);
+  int rc = dh_sectors_upload_work(S, active, rowsPerTile, &na, &tiles);
   if (rc) return rc;
   if (tiles == 0) return DH_OK;
   size_t base = ((size_t)S->nbrFactors * 8 + (size_t)(S->nbrPairs + 1) * 4 + 7) / 8 * 8 + 8 * (2 * na + 1);
@@ -420,6 +398,7 @@ static int sectors_hv_lpr(dh_sector_set* S, const double* x, double* y, int add,
 extern "C" int dh_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add,
                              const unsigned char* active) {
   if (!S || !x || !y) return DH_E_ARG;
+  if (S->kind == DH_SECTORS_MOMENTUM) return dh_lattice_sectors_hv(S, x, y, k, add, active);
   if (!S->d_factors) return dh_fail(S->ctx, DH_E_STATE, "factors not set");
   switch (k) {
     case 1: return sectors_hv_lpr<1>(S, x, y, add, active);
@@ -509,7 +488,7 @@ static int sectors_dots(dh_sector_set* S, const double* V, int nb, const double*
   const int64_t rowsPerTile = std::max<int64_t>(8192, (S->totalDim + 32767) / 32768);
   int na;
   int64_t tiles;
-  int rc = upload_work(S, active, rowsPerTile, &na, &tiles);
+  int rc = dh_sectors_upload_work(S, active, rowsPerTile, &na, &tiles);
   if (rc) return rc;
   if (tiles == 0) return DH_OK;
   const int64_t total = (int64_t)nb * K * K;
@@ -604,7 +583,7 @@ static int sectors_axpy(dh_sector_set* S, const double* V, int nb, double* W, co
   dh_context* c = S->ctx;
   int na;
   int64_t tiles;
-  int rc = upload_work(S, active, 256, &na, &tiles);
+  int rc = dh_sectors_upload_work(S, active, 256, &na, &tiles);
   if (rc) return rc;
   if (tiles == 0) return DH_OK;
   size_t bytes = sizeof(double) * (size_t)na * nb * K * K;

@@ -3,6 +3,7 @@
 // include/dhgpu.h.  Vectors live in device memory for their whole life.
 #pragma once
 
+#include <complex>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -267,6 +268,40 @@ class MultiSectorBlockLanczos {
 // eigenvalues ascending; optional eigenvectors (column-major n x n) of the tridiagonal.
 void TridiagonalEigen(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& evals,
                       std::vector<double>* evecs);
+// ------------------------------------------------------------------ Chern band (momentum space)
+// Lower band of TightBindingModelCheckerboardLattice as OneBodyBasis[k][0][alpha] (u[2k + alpha])
+// and the band energies (energies[2k + band]).
+void CheckerboardLowerBand(int nx, int ny, double t1, double t2, double tpp, double mus, double gx, double gy,
+                           std::vector<std::complex<double>>& u, std::vector<double>& energies);
+// ParticleOnLatticeCheckerboardLatticeSingleBandHamiltonian factors in the layout of
+// dh_sector_set_lattice_factors (complex entries interleaved in `factors`).
+void CheckerboardSingleBandFactors(int nx, int ny, const std::vector<std::complex<double>>& u, double uPotential,
+                                   double vPotential, bool flatBand, std::vector<int>& sumPairCount,
+                                   std::vector<int>& sumPairs, std::vector<double>& factors);
+
+// FCICheckerboardLatticeModel's (kx, ky) loop (FTI/src/Programs/FCI/FCICheckerboardLatticeModel.cc:412)
+// with FullReorthogonalizedComplex(Block)LanczosAlgorithm per sector, all sectors advanced together.
+class LatticeSectorsBlockLanczos {
+ public:
+  LatticeSectorsBlockLanczos(GPUArchitecture* architecture, int nbrFermions, int nx, int ny,
+                             const std::vector<int>& kx, const std::vector<int>& ky,
+                             const std::vector<int>& sumPairCount, const std::vector<int>& sumPairs,
+                             const std::vector<double>& factors, int nbrEigenvalue, int blockSize, int maxIter,
+                             double precision = 1e-14, size_t memoryBytes = 0);
+  std::vector<SectorResult> Solve(uint64_t seed, double* seconds = nullptr);
+  int NbrWaves() const { return Waves; }
+
+ private:
+  void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
+  GPUArchitecture* Arch;
+  int NbrFermions, NbrSiteX, NbrSiteY, NbrEigenvalue, BlockSize, MaxIter;
+  double Precision;
+  size_t MemoryBytes;
+  int Waves = 0;
+  std::vector<int> Kx, Ky, SumPairCount, SumPairs;
+  std::vector<double> Factors;
+};
+
 // Eigenvalues (ascending) of the block tridiagonal Lanczos matrix with diagonal blocks A[0..m)
 // and sub-diagonal blocks B[0..m-1) (k x k, row-major); Ritz coefficients (column-major n x n)
 // if evecs is given.

@@ -123,6 +123,70 @@ extern "C" int dhh_lanczos(dhh_problem* p, int nbrEigen, int blockSize, int maxI
   });
 }
 
+extern "C" int dhh_autotune(dhh_problem* p, const int* ks, int nbrKs, int repeats, double* table, int* bestK) {
+  if (!p || !ks || nbrKs <= 0 || !table) return DH_E_ARG;
+  return guarded([&] {
+    GPUArchitecture* arch = p->arch->arch.get();
+    dh_context* c = arch->Context();
+    dh_hamiltonian* h = p->hamiltonian->Handle();
+    const int64_t dim = p->hamiltonian->GetHilbertSpaceDimension();
+    int64_t nnz = 0;
+    arch->Check(dh_hamiltonian_count_elements(h, &nnz), "count elements");
+    const bool spin = dynamic_cast<SpinChainHamiltonian*>(p->hamiltonian.get()) != nullptr;
+    double bestRate = -1.0;
+    int bestIdx = 0;
+    std::vector<int> bestCfg(nbrKs * 4);
+    for (int i = 0; i < nbrKs; ++i) {
+      const int k = ks[i];
+      double* x = arch->AllocateMemory(dim * k);
+      double* y = arch->AllocateMemory(dim * k);
+      arch->Check(dh_fill_random(c, x, dim * k, 11), "random");
+      double bestMs = 1e300;
+      int cfg[4] = {1, 256, DH_INDEX_RANK, 1};
+      for (int mode : {DH_INDEX_RANK, DH_INDEX_BSEARCH})
+        for (int lpr : {1, 2, 4, 8}) {
+          if (spin && lpr != 1) continue;
+          for (int bs : {128, 256, 512}) {
+            if (dh_hamiltonian_set_launch(h, lpr, bs, mode, 1) != DH_OK) continue;
+            arch->Check(dh_hv(h, x, y, k, 0, 0, -1), "warm-up");
+            arch->Check(dh_synchronize(c), "sync");
+            auto t0 = std::chrono::steady_clock::now();
+            for (int r = 0; r < repeats; ++r) arch->Check(dh_hv(h, x, y, k, 0, 0, -1), "timed product");
+            arch->Check(dh_synchronize(c), "sync");
+            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                        std::max(1, repeats);
+            if (ms < bestMs) {
+              bestMs = ms;
+              cfg[0] = lpr;
+              cfg[1] = bs;
+              cfg[2] = mode;
+            }
+          }
+        }
+      arch->FreeMemory(x);
+      arch->FreeMemory(y);
+      double rate = (double)nnz * k / (bestMs * 1e-3);
+      double* row = table + 6 * i;
+      row[0] = k;
+      row[1] = cfg[0];
+      row[2] = cfg[1];
+      row[3] = cfg[2];
+      row[4] = bestMs;
+      row[5] = rate;
+      for (int q = 0; q < 4; ++q) bestCfg[4 * i + q] = cfg[q];
+      if (rate > bestRate) {
+        bestRate = rate;
+        bestIdx = i;
+      }
+    }
+    arch->Check(dh_hamiltonian_set_launch(h, bestCfg[4 * bestIdx], bestCfg[4 * bestIdx + 1],
+                                          bestCfg[4 * bestIdx + 2], 1),
+                "set launch");
+    if (bestK) *bestK = ks[bestIdx];
+    return DH_OK;
+  });
+}
+
 extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* totalLz, int nbrSectors,
                                  const double* pseudo, int nbrEigen, int blockSize, int maxIter, double precision,
                                  uint64_t seed, int batched, double memoryGB, double* eigenvalues, int* products,
@@ -163,6 +227,68 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return all ? DH_OK : DH_E_STATE;
   });
+}
+
+extern "C" int dhh_lattice_sectors_solve(dhh_arch* a, int n, int nx, int ny, double t1, double t2, double tpp,
+                                         double mus, double uPot, double vPot, int flat, const int* kx, const int* ky,
+                                         int nbrSectors, int nbrEigen, int blockSize, int maxIter, double precision,
+                                         uint64_t seed, double memoryGB, double* eigenvalues, int* products,
+                                         int* converged, double* seconds) {
+  if (!a || !kx || !ky || !eigenvalues || nbrSectors <= 0 || nbrEigen <= 0) return DH_E_ARG;
+  return guarded([&] {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::complex<double>> u;
+    std::vector<double> e, f;
+    std::vector<int> cnt, pairs;
+    CheckerboardLowerBand(nx, ny, t1, t2, tpp, mus, 0.0, 0.0, u, e);
+    CheckerboardSingleBandFactors(nx, ny, u, uPot, vPot, flat != 0, cnt, pairs, f);
+    LatticeSectorsBlockLanczos solver(a->arch.get(), n, nx, ny, std::vector<int>(kx, kx + nbrSectors),
+                                      std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
+                                      maxIter, precision, (size_t)(memoryGB * 1073741824.0));
+    std::vector<SectorResult> r = solver.Solve(seed);
+    bool all = true;
+    for (int s = 0; s < nbrSectors; ++s) {
+      for (int q = 0; q < nbrEigen; ++q)
+        eigenvalues[(size_t)s * nbrEigen + q] = q < (int)r[s].eigenvalues.size() ? r[s].eigenvalues[q] : 0.0;
+      if (products) products[s] = r[s].products;
+      if (converged) converged[s] = r[s].converged;
+      all = all && r[s].converged;
+    }
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return all ? DH_OK : DH_E_STATE;
+  });
+}
+
+extern "C" int dhh_checkerboard_lower_band(int nx, int ny, double t1, double t2, double tpp, double mus, double gx,
+                                           double gy, double* u, double* energies) {
+  if (!u || !energies || nx <= 0 || ny <= 0) return DH_E_ARG;
+  std::vector<std::complex<double>> uu;
+  std::vector<double> e;
+  CheckerboardLowerBand(nx, ny, t1, t2, tpp, mus, gx, gy, uu, e);
+  for (size_t i = 0; i < uu.size(); ++i) {
+    u[2 * i] = uu[i].real();
+    u[2 * i + 1] = uu[i].imag();
+  }
+  std::memcpy(energies, e.data(), sizeof(double) * e.size());
+  return DH_OK;
+}
+
+extern "C" int dhh_checkerboard_factors(int nx, int ny, double t1, double t2, double tpp, double mus, double uPot,
+                                        double vPot, int flat, int* sumPairCount, int* sumPairs, int* nbrPairs,
+                                        double* factors, int64_t* nbrFactors) {
+  if (!sumPairCount || !sumPairs || !nbrPairs || !factors || !nbrFactors) return DH_E_ARG;
+  std::vector<std::complex<double>> u;
+  std::vector<double> e, f;
+  std::vector<int> cnt, pairs;
+  CheckerboardLowerBand(nx, ny, t1, t2, tpp, mus, 0.0, 0.0, u, e);
+  CheckerboardSingleBandFactors(nx, ny, u, uPot, vPot, flat != 0, cnt, pairs, f);
+  if ((int)pairs.size() / 2 > *nbrPairs || (int64_t)f.size() / 2 > *nbrFactors) return DH_E_RANGE;
+  std::memcpy(sumPairCount, cnt.data(), sizeof(int) * cnt.size());
+  std::memcpy(sumPairs, pairs.data(), sizeof(int) * pairs.size());
+  std::memcpy(factors, f.data(), sizeof(double) * f.size());
+  *nbrPairs = (int)pairs.size() / 2;
+  *nbrFactors = (int64_t)f.size() / 2;
+  return DH_OK;
 }
 
 extern "C" int dhh_coulomb_pseudopotentials(int nbrFlux, int round, double* out) {

@@ -1,0 +1,435 @@
+// Chern-band lattice model in momentum space: one-body band, projected two-body factors and
+// a complex block Lanczos over many momentum sectors at once.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+
+#include "diagham_host.hpp"
+
+namespace dhh {
+
+using cplx = std::complex<double>;
+
+// TightBindingModelCheckerboardLattice (FTI/src/Tools/FTITightBinding/TightBindingModelCheckerboardLattice.cc:206,
+// the form used with --single-band): H(k) = [[d1 + d3, B1], [conj(B1), d1 - d3]] with
+// B1 = 4 t1 (cos(Kx/2) cos(Ky/2) cos(pi/4) + i sin(Kx/2) sin(Ky/2) sin(pi/4)), d1 = 4 t'' cos Kx cos Ky,
+// d3 = mu_s + 2 t2 (cos Kx - cos Ky), K = 2 pi (k + gamma) / N.  Lower-band eigenvector v(k) in closed
+// form, gauge v_0 >= 0 (v_1 >= 0 if v_0 = 0); returned as OneBodyBasis[k][0][alpha] = conj(v_alpha(k)).
+void CheckerboardLowerBand(int nx, int ny, double t1, double t2, double tpp, double mus, double gx, double gy,
+                           std::vector<cplx>& u, std::vector<double>& energies) {
+  u.assign((size_t)nx * ny * 2, 0.0);
+  energies.assign((size_t)nx * ny * 2, 0.0);
+  for (int kx = 0; kx < nx; ++kx)
+    for (int ky = 0; ky < ny; ++ky) {
+      const int index = kx * ny + ky;
+      const double Kx = 2.0 * M_PI * (kx + gx) / nx, Ky = 2.0 * M_PI * (ky + gy) / ny;
+      const cplx B1 = 4.0 * t1 * cplx(std::cos(0.5 * Kx) * std::cos(0.5 * Ky) * std::cos(0.25 * M_PI),
+                                      std::sin(0.5 * Kx) * std::sin(0.5 * Ky) * std::sin(0.25 * M_PI));
+      const double d1 = 4.0 * tpp * std::cos(Kx) * std::cos(Ky);
+      const double d3 = mus + 2.0 * t2 * (std::cos(Kx) - std::cos(Ky));
+      const double r = std::sqrt(d3 * d3 + std::norm(B1));
+      energies[2 * index] = d1 - r;
+      energies[2 * index + 1] = d1 + r;
+      cplx v0, v1;  // null vector of H - (d1 - r)
+      if (d3 >= 0.0) {
+        v0 = B1;
+        v1 = -(d3 + r);
+      } else {
+        v0 = r - d3;
+        v1 = -std::conj(B1);
+      }
+      double nrm = std::sqrt(std::norm(v0) + std::norm(v1));
+      if (nrm == 0.0) {
+        v0 = 1.0;
+        nrm = 1.0;
+      }
+      v0 /= nrm;
+      v1 /= nrm;
+      cplx phase = (std::abs(v0) > 1e-300) ? std::conj(v0) / std::abs(v0) : std::conj(v1) / std::abs(v1);
+      v0 *= phase;
+      v1 *= phase;
+      u[2 * index] = std::conj(v0);
+      u[2 * index + 1] = std::conj(v1);
+    }
+}
+
+// ParticleOnLatticeCheckerboardLatticeSingleBandHamiltonian::EvaluateInteractionFactors
+// (FTI/src/Hamiltonian/ParticleOnLatticeCheckerboardLatticeSingleBandHamiltonian.cc:126, fermions):
+// nearest-neighbour A-B repulsion U (and A-A / B-B repulsion V) projected on the band u.
+void CheckerboardSingleBandFactors(int nx, int ny, const std::vector<cplx>& u, double uPotential, double vPotential,
+                                   bool flatBand, std::vector<int>& sumPairCount, std::vector<int>& sumPairs,
+                                   std::vector<double>& factors) {
+  const int ns = nx * ny;
+  std::vector<std::vector<std::pair<int, int>>> pairs(ns);
+  for (int kx1 = 0; kx1 < nx; ++kx1)
+    for (int kx2 = 0; kx2 < nx; ++kx2)
+      for (int ky1 = 0; ky1 < ny; ++ky1)
+        for (int ky2 = 0; ky2 < ny; ++ky2) {
+          int i1 = kx1 * ny + ky1, i2 = kx2 * ny + ky2;
+          if (i1 < i2) pairs[((kx1 + kx2) % nx) * ny + ((ky1 + ky2) % ny)].push_back({i1, i2});
+        }
+  double factorU = 0.5 / ns;
+  if (!flatBand) factorU *= uPotential;
+  const double factorV = vPotential * 0.5 / ns;
+  auto K = [&](int i, int c) { return c == 0 ? 2.0 * M_PI * (i / ny) / nx : 2.0 * M_PI * (i % ny) / ny; };
+  auto AB = [&](int p1, int p2) {  // ComputeTwoBodyMatrixElementAB (:359)
+    return 2.0 * (std::cos(0.5 * (K(p2, 0) - K(p2, 1) + K(p1, 1) - K(p1, 0))) +
+                  std::cos(0.5 * (K(p2, 0) + K(p2, 1) - K(p1, 1) - K(p1, 0))));
+  };
+  auto AA = [&](int p1, int p2) {  // ComputeTwoBodyMatrixElementAA (:380)
+    return std::polar(1.0, K(p2, 0) - K(p1, 0)) + std::polar(1.0, K(p2, 1) - K(p1, 1));
+  };
+  auto U = [&](int i, int alpha) { return u[2 * i + alpha]; };
+  sumPairCount.assign(ns, 0);
+  sumPairs.clear();
+  factors.clear();
+  for (int s = 0; s < ns; ++s) {
+    const auto& P = pairs[s];
+    sumPairCount[s] = (int)P.size();
+    for (const auto& p : P) {
+      sumPairs.push_back(p.first);
+      sumPairs.push_back(p.second);
+    }
+    for (const auto& q1 : P)
+      for (const auto& q2 : P) {
+        const int I1 = q1.first, I2 = q1.second, I3 = q2.first, I4 = q2.second;
+        // (sign, a, b, c, d | form factor momenta): conj(u_a,A) u_b,A conj(u_c,B) u_d,B
+        const int t[4][7] = {{+1, I1, I3, I2, I4, I2, I4},
+                             {-1, I2, I3, I1, I4, I1, I4},
+                             {-1, I1, I4, I2, I3, I2, I3},
+                             {+1, I2, I4, I1, I3, I1, I3}};
+        cplx f = 0.0;
+        for (const auto& r : t) {
+          f += (double)r[0] * factorU * (std::conj(U(r[1], 0)) * U(r[2], 0) * std::conj(U(r[3], 1)) * U(r[4], 1)) *
+               AB(r[5], r[6]);
+          if (factorV != 0.0)
+            f += (double)r[0] * factorV *
+                 (std::conj(U(r[1], 0)) * U(r[2], 0) * std::conj(U(r[3], 0)) * U(r[4], 0) +
+                  std::conj(U(r[1], 1)) * U(r[2], 1) * std::conj(U(r[3], 1)) * U(r[4], 1)) *
+                 AA(r[5], r[6]);
+        }
+        f *= -2.0;
+        factors.push_back(f.real());
+        factors.push_back(f.imag());
+      }
+  }
+}
+
+namespace {
+bool cholesky_hermitian(int k, const std::vector<cplx>& G, std::vector<cplx>& R) {
+  R.assign((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) {
+    double s = G[(size_t)i * k + i].real();
+    for (int p = 0; p < i; ++p) s -= std::norm(R[(size_t)p * k + i]);
+    if (!(s > 1e-28 * std::max(1.0, std::abs(G[(size_t)i * k + i])))) return false;
+    double rii = std::sqrt(s);
+    R[(size_t)i * k + i] = rii;
+    for (int j = i + 1; j < k; ++j) {
+      cplx t = G[(size_t)i * k + j];
+      for (int p = 0; p < i; ++p) t -= std::conj(R[(size_t)p * k + i]) * R[(size_t)p * k + j];
+      R[(size_t)i * k + j] = t / rii;
+    }
+  }
+  return true;
+}
+
+std::vector<cplx> upper_inverse_c(int k, const std::vector<cplx>& R) {
+  std::vector<cplx> X((size_t)k * k, 0.0);
+  for (int j = 0; j < k; ++j) {
+    X[(size_t)j * k + j] = 1.0 / R[(size_t)j * k + j];
+    for (int i = j - 1; i >= 0; --i) {
+      cplx s = 0.0;
+      for (int p = i + 1; p <= j; ++p) s += R[(size_t)i * k + p] * X[(size_t)p * k + j];
+      X[(size_t)i * k + j] = -s / R[(size_t)i * k + i];
+    }
+  }
+  return X;
+}
+
+std::vector<cplx> matmul_c(int k, const std::vector<cplx>& A, const std::vector<cplx>& B) {
+  std::vector<cplx> C((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i)
+    for (int p = 0; p < k; ++p)
+      for (int j = 0; j < k; ++j) C[(size_t)i * k + j] += A[(size_t)i * k + p] * B[(size_t)p * k + j];
+  return C;
+}
+
+// real (2k x 2k) matrix acting on interleaved (re, im) rows: w_q += sum_p v_p c_pq
+void realify(int k, const cplx* C, double* M) {
+  const int K2 = 2 * k;
+  for (int p = 0; p < k; ++p)
+    for (int q = 0; q < k; ++q) {
+      cplx c = C[(size_t)p * k + q];
+      M[(size_t)(2 * p) * K2 + 2 * q] = c.real();
+      M[(size_t)(2 * p) * K2 + 2 * q + 1] = c.imag();
+      M[(size_t)(2 * p + 1) * K2 + 2 * q] = -c.imag();
+      M[(size_t)(2 * p + 1) * K2 + 2 * q + 1] = c.real();
+    }
+}
+
+// complex k x k block of V^H W from the real (2k x 2k) Gram block G of the interleaved rows
+void complexify(int k, const double* G, cplx* C) {
+  const int K2 = 2 * k;
+  for (int p = 0; p < k; ++p)
+    for (int q = 0; q < k; ++q) {
+      double rr = G[(size_t)(2 * p) * K2 + 2 * q], ii = G[(size_t)(2 * p + 1) * K2 + 2 * q + 1];
+      double ri = G[(size_t)(2 * p) * K2 + 2 * q + 1], ir = G[(size_t)(2 * p + 1) * K2 + 2 * q];
+      C[(size_t)p * k + q] = cplx(rr + ii, ri - ir);
+    }
+}
+
+// eigenvalues of the complex Hermitian block tridiagonal matrix, through its real 2n x 2n form
+void complex_block_eigen(const std::vector<std::vector<cplx>>& A, const std::vector<std::vector<cplx>>& B, int k,
+                         int m, std::vector<double>& evals) {
+  const int n = m * k, n2 = 2 * n;
+  std::vector<double> T((size_t)n2 * n2, 0.0);
+  auto put = [&](int i, int j, cplx v) {  // complex entry (i, j) -> real 2 x 2 block
+    T[(size_t)(2 * i) * n2 + 2 * j] = v.real();
+    T[(size_t)(2 * i) * n2 + 2 * j + 1] = -v.imag();
+    T[(size_t)(2 * i + 1) * n2 + 2 * j] = v.imag();
+    T[(size_t)(2 * i + 1) * n2 + 2 * j + 1] = v.real();
+  };
+  for (int j = 0; j < m; ++j)
+    for (int p = 0; p < k; ++p)
+      for (int q = 0; q < k; ++q) {
+        put(j * k + p, j * k + q, A[j][(size_t)p * k + q]);
+        if (j + 1 < m) {
+          cplx b = B[j][(size_t)p * k + q];  // T[j+1][j] block
+          put((j + 1) * k + p, j * k + q, b);
+          put(j * k + q, (j + 1) * k + p, std::conj(b));
+        }
+      }
+  std::vector<double> d, e, w;
+  BandToTridiagonal(n2, 2 * k + 1, T, d, e, nullptr);
+  TridiagonalEigen(n2, d, e, w, nullptr);
+  evals.resize(n);
+  for (int i = 0; i < n; ++i) evals[i] = w[2 * i];  // every eigenvalue appears twice
+}
+}  // namespace
+
+LatticeSectorsBlockLanczos::LatticeSectorsBlockLanczos(GPUArchitecture* architecture, int nbrFermions, int nx, int ny,
+                                                       const std::vector<int>& kx, const std::vector<int>& ky,
+                                                       const std::vector<int>& sumPairCount,
+                                                       const std::vector<int>& sumPairs,
+                                                       const std::vector<double>& factors, int nbrEigenvalue,
+                                                       int blockSize, int maxIter, double precision,
+                                                       size_t memoryBytes)
+    : Arch(architecture),
+      NbrFermions(nbrFermions),
+      NbrSiteX(nx),
+      NbrSiteY(ny),
+      NbrEigenvalue(nbrEigenvalue),
+      BlockSize(blockSize),
+      MaxIter(maxIter),
+      Precision(precision),
+      MemoryBytes(memoryBytes ? memoryBytes : (size_t)160 << 30),
+      Kx(kx),
+      Ky(ky),
+      SumPairCount(sumPairCount),
+      SumPairs(sumPairs),
+      Factors(factors) {
+  if (BlockSize != 1 && BlockSize != 2 && BlockSize != 4 && BlockSize != 8)
+    throw std::invalid_argument("complex block size must be 1, 2, 4 or 8");
+}
+
+std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, double* seconds) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int ns = (int)Kx.size();
+  std::vector<SectorResult> out(ns);
+  const int k = BlockSize;
+  const int maxBlocks = std::max(3, MaxIter / k + 1);
+  // sector dimensions from a one-sector set each is wasteful; use a set of all and split
+  dh_context* c = Arch->Context();
+  std::vector<int64_t> dims(ns);
+  {
+    dh_sector_set* S = nullptr;
+    Arch->Check(dh_momentum_sector_set_create(c, &S, NbrFermions, NbrSiteX, NbrSiteY, Kx.data(), Ky.data(), ns),
+                "momentum sectors");
+    std::vector<int64_t> off(ns + 1);
+    dh_sector_set_offsets(S, off.data());
+    for (int s = 0; s < ns; ++s) dims[s] = off[s + 1] - off[s];
+    dh_sector_set_destroy(S);
+  }
+  std::vector<int> wave;
+  size_t used = 0;
+  Waves = 0;
+  for (int s = 0; s < ns; ++s) {
+    out[s].dimension = dims[s];
+    if (dims[s] <= 4 * k) throw std::invalid_argument("momentum sector too small for the block size");
+    size_t need = (size_t)dims[s] * 2 * k * (maxBlocks + 2) * sizeof(double);
+    if (!wave.empty() && used + need > MemoryBytes) {
+      SolveWave(wave, seed, out);
+      ++Waves;
+      wave.clear();
+      used = 0;
+    }
+    wave.push_back(s);
+    used += need;
+  }
+  if (!wave.empty()) {
+    SolveWave(wave, seed, out);
+    ++Waves;
+  }
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return out;
+}
+
+void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_t seed,
+                                           std::vector<SectorResult>& out) {
+  dh_context* c = Arch->Context();
+  const int k = BlockSize, kk = k * k, K2 = 2 * k, KK2 = K2 * K2;
+  const int na = (int)sectors.size();
+  std::vector<int> kx, ky;
+  for (int s : sectors) {
+    kx.push_back(Kx[s]);
+    ky.push_back(Ky[s]);
+  }
+  dh_sector_set* S = nullptr;
+  Arch->Check(dh_momentum_sector_set_create(c, &S, NbrFermions, NbrSiteX, NbrSiteY, kx.data(), ky.data(), na),
+              "momentum sector set");
+  Arch->Check(dh_sector_set_lattice_factors(S, NbrSiteX * NbrSiteY, SumPairCount.data(), SumPairs.data(),
+                                            Factors.data(), 0.0),
+              "lattice factors");
+  std::vector<int64_t> off(na + 1);
+  dh_sector_set_offsets(S, off.data());
+  const int64_t total = off.back();
+  Arch->Check(dh_sector_set_launch(S, total < 4ll * 256 * 256 ? 4 : 1, 256), "launch");
+  int maxBlocks = 2;
+  std::vector<int> blockCap(na);
+  for (int i = 0; i < na; ++i) {
+    int64_t d = off[i + 1] - off[i];
+    blockCap[i] = (int)std::min<int64_t>(MaxIter / k + 1, (d + k - 1) / k + 1);
+    maxBlocks = std::max(maxBlocks, blockCap[i]);
+  }
+  const int64_t blockDoubles = total * K2;
+  double* basis = Arch->AllocateMemory((int64_t)(maxBlocks + 1) * blockDoubles);
+  auto blockPtr = [&](int j) { return basis + (int64_t)j * blockDoubles; };
+  Arch->Check(dh_fill_random(c, basis, blockDoubles, seed), "random start");
+
+  std::vector<unsigned char> active(na, 1);
+  std::vector<std::vector<std::vector<cplx>>> A(na), B(na);
+  std::vector<std::vector<double>> prev(na);
+  std::vector<int> lastCheck(na, -1);
+  std::vector<char> done(na, 0), exhausted(na, 0);
+
+  auto orthonormalize = [&](double* W, std::vector<std::vector<cplx>>& R) {
+    std::vector<int> act;
+    for (int i = 0; i < na; ++i)
+      if (active[i]) act.push_back(i);
+    R.assign(na, std::vector<cplx>());
+    for (int i : act) {
+      R[i].assign((size_t)kk, 0.0);
+      for (int p = 0; p < k; ++p) R[i][(size_t)p * k + p] = 1.0;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double> G((size_t)act.size() * KK2);
+      Arch->Check(dh_sectors_block_dots(S, W, 1, W, K2, active.data(), G.data()), "gram");
+      std::vector<double> M((size_t)act.size() * KK2, 0.0);
+      for (size_t a = 0; a < act.size(); ++a) {
+        int i = act[a];
+        std::vector<cplx> Gc((size_t)kk), Rp;
+        complexify(k, &G[a * KK2], Gc.data());
+        if (!R[i].empty() && cholesky_hermitian(k, Gc, Rp)) {
+          std::vector<cplx> inv = upper_inverse_c(k, Rp);
+          realify(k, inv.data(), &M[a * KK2]);
+          R[i] = matmul_c(k, Rp, R[i]);
+        } else {
+          R[i].clear();
+          for (int p = 0; p < K2; ++p) M[a * KK2 + (size_t)p * K2 + p] = 1.0;
+        }
+      }
+      Arch->Check(dh_sectors_block_rmul(S, W, K2, active.data(), M.data()), "rmul");
+    }
+  };
+
+  {
+    std::vector<std::vector<cplx>> R;
+    orthonormalize(blockPtr(0), R);
+    for (int i = 0; i < na; ++i)
+      if (R[i].empty()) throw std::runtime_error("random start block is rank deficient");
+  }
+  for (int j = 0;; ++j) {
+    std::vector<int> act;
+    for (int i = 0; i < na; ++i) active[i] = done[i] ? 0 : 1;
+    for (int i = 0; i < na; ++i)
+      if (!done[i]) act.push_back(i);
+    if (act.empty() || j + 1 > maxBlocks) break;
+    double* X = blockPtr(j);
+    double* W = blockPtr(j + 1);
+    Arch->Check(dh_sectors_hv(S, X, W, k, 0, active.data()), "lattice products");
+    const size_t per = (size_t)(j + 1) * KK2;
+    std::vector<double> G(act.size() * per), M(act.size() * per);
+    std::vector<std::vector<cplx>> Aj(na, std::vector<cplx>((size_t)kk, 0.0));
+    for (int pass = 0; pass < 2; ++pass) {
+      Arch->Check(dh_sectors_block_dots(S, basis, j + 1, W, K2, active.data(), G.data()), "reorth dots");
+      for (size_t a = 0; a < act.size(); ++a)
+        for (int b = 0; b <= j; ++b) {
+          std::vector<cplx> C((size_t)kk);
+          complexify(k, &G[a * per + (size_t)b * KK2], C.data());
+          if (b == j)
+            for (int q = 0; q < kk; ++q) Aj[act[a]][q] += C[q];
+          for (auto& v : C) v = -v;
+          realify(k, C.data(), &M[a * per + (size_t)b * KK2]);
+        }
+      Arch->Check(dh_sectors_block_axpy(S, basis, j + 1, W, K2, active.data(), M.data()), "reorth axpy");
+    }
+    std::vector<std::vector<cplx>> R;
+    orthonormalize(W, R);
+    for (int i : act) {
+      auto& a = Aj[i];
+      for (int p = 0; p < k; ++p) {
+        a[(size_t)p * k + p] = a[(size_t)p * k + p].real();
+        for (int q = p + 1; q < k; ++q) {
+          cplx s = 0.5 * (a[(size_t)p * k + q] + std::conj(a[(size_t)q * k + p]));
+          a[(size_t)p * k + q] = s;
+          a[(size_t)q * k + p] = std::conj(s);
+        }
+      }
+      A[i].push_back(a);
+      if (R[i].empty()) {
+        exhausted[i] = 1;
+        B[i].push_back(std::vector<cplx>((size_t)kk, 0.0));
+      } else {
+        B[i].push_back(R[i]);
+      }
+      out[sectors[i]].products += k;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t a = 0; a < act.size(); ++a) {
+      int i = act[a];
+      int m = (int)A[i].size();
+      bool last = exhausted[i] || m >= blockCap[i];
+      if (!last && !(k == 1 || lastCheck[i] < 0 || m - lastCheck[i] >= std::max(1, m / 10))) continue;
+      if (!last && m * k < NbrEigenvalue + 2) continue;
+      lastCheck[i] = m;
+      std::vector<double> ev;
+      complex_block_eigen(A[i], B[i], k, m, ev);
+      int want = std::min<int>(NbrEigenvalue, (int)ev.size());
+      std::vector<double> cur(ev.begin(), ev.begin() + want);
+      bool ok = !prev[i].empty() && (int)ev.size() > NbrEigenvalue && (int)prev[i].size() == want;
+      if (ok)
+        for (int e = 0; e < want; ++e)
+          if (std::fabs(cur[e] - prev[i][e]) > Precision * std::fabs(cur[e])) {
+            ok = false;
+            break;
+          }
+      prev[i] = cur;
+      SectorResult& r = out[sectors[i]];
+      r.eigenvalues = cur;
+      if (ok || (exhausted[i] && (int64_t)m * k >= off[i + 1] - off[i])) {
+        r.converged = true;
+        done[i] = 1;
+      } else if (last) {
+        done[i] = 1;
+      }
+    }
+  }
+  Arch->Check(dh_synchronize(c), "sync");
+  Arch->FreeMemory(basis);
+  dh_sector_set_destroy(S);
+}
+
+}  // namespace dhh

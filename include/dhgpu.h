@@ -162,6 +162,22 @@ int dh_sectors_block_axpy(dh_sector_set* set, const double* d_basis, int nbr_blo
 /* w[row] <- w[row] * m[j] (k x k per active sector j) */
 int dh_sectors_block_rmul(dh_sector_set* set, double* d_w, int k, const unsigned char* active, const double* m);
 
+/* Momentum sectors (kx[i], ky[i]) of nbr_fermions spinless fermions in one band of an Nx x Ny
+ * lattice: FermionOnSquareLatticeMomentumSpace (FQHE/src/HilbertSpace/FermionOnSquareLatticeMomentumSpace.cc:76),
+ * orbital kx * Ny + ky, states sorted as descending 64-bit integers, Nx * Ny <= 62.  Vectors of
+ * such a set are complex, interleaved (re, im): a block of k complex vectors is [row][k][2]
+ * doubles, and dh_sectors_block_* take 2k as their k. */
+int dh_momentum_sector_set_create(dh_context* ctx, dh_sector_set** set, int nbr_fermions, int nbr_site_x,
+                                  int nbr_site_y, const int* kx, const int* ky, int nbr_sectors);
+/* Two-body factors of ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian
+ * (FQHE/src/Hamiltonian/ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian.h:370):
+ * momentum sum j = 0..nbr_sums-1 (nbr_sums = Nx*Ny) lists sum_pair_count[j] orbital pairs
+ * (a < b) in sum_pairs (2 ints per pair, sums in order), and its complex factor matrix
+ * F_j[i1][i2] (annihilated pair i1, created pair i2) follows in factors, (re, im) interleaved,
+ * the matrices of the sums concatenated. */
+int dh_sector_set_lattice_factors(dh_sector_set* set, int nbr_sums, const int* sum_pair_count, const int* sum_pairs,
+                                  const double* factors, double shift);
+
 /* ---------------------------------------------------------------- adjoint (no reference counterpart)
  * g[c] = <psi| H_c |phi> for every channel c of dh_sphere_hamiltonian_set_channels, in one
  * pass over the basis; if d_hpsi is not null it also receives H psi (the vector-Jacobian

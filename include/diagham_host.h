@@ -53,6 +53,14 @@ int dhh_lanczos(dhh_problem* problem, int nbr_eigen, int block_size, int max_ite
                 double residual, uint64_t seed, double* eigenvalues, int* products, int* blocks, double* seconds,
                 double* d_eigenvectors, int nbr_vectors);
 
+/* Autotuner of the product launch: for each block size ks[i] it times dh_hv (after a warm-up)
+ * over lanes_per_row in {1,2,4,8} (64/lanes rows per wavefront), workgroup sizes {128,256,512}
+ * and both index lookups, then leaves the problem configured with the fastest launch of the
+ * block size best_k that maximises matrix elements * vectors per second.  Per block size the
+ * table row is: k, lanes_per_row, block_size, index_mode, milliseconds per product, element-
+ * vector products per second (6 doubles per row, nbr_ks rows). */
+int dhh_autotune(dhh_problem* problem, const int* ks, int nbr_ks, int repeats, double* table, int* best_k);
+
 /* Every sector total_lz[0..nbr_sectors) of nbr_fermions fermions at lz_max with the given
  * pseudopotentials, nbr_eigen lowest eigenvalues each: the Lz loop of
  * FQHESphereFermionsTwoBodyGeneric.cc:270 with a (block) full-reorthogonalisation Lanczos
@@ -65,6 +73,25 @@ int dhh_sectors_solve(dhh_arch* arch, int nbr_fermions, int lz_max, const int* t
                       const double* pseudopotentials, int nbr_eigen, int block_size, int max_iter, double precision,
                       uint64_t seed, int batched, double memory_gb, double* eigenvalues, int* products,
                       int* converged, double* seconds);
+
+/* FCICheckerboardLatticeModel --single-band (FTI/src/Programs/FCI/FCICheckerboardLatticeModel.cc:395-635):
+ * nbr_fermions fermions in the lower band of the checkerboard model (t1, t2, tpp, mus) on an
+ * Nx x Ny lattice with nearest-neighbour repulsion u_potential (and v_potential), projected;
+ * momentum sectors (kx[i], ky[i]), nbr_eigen lowest eigenvalues each, complex block Lanczos with
+ * block_size complex vectors, all sectors advanced together. */
+int dhh_lattice_sectors_solve(dhh_arch* arch, int nbr_fermions, int nbr_site_x, int nbr_site_y, double t1,
+                              double t2, double tpp, double mus, double u_potential, double v_potential,
+                              int flat_band, const int* kx, const int* ky, int nbr_sectors, int nbr_eigen,
+                              int block_size, int max_iter, double precision, uint64_t seed, double memory_gb,
+                              double* eigenvalues, int* products, int* converged, double* seconds);
+/* Host tables of that model: lower band (u: 2 complex per momentum, interleaved re/im, and
+ * energies: 2 per momentum) and the two-body factors in the dh_sector_set_lattice_factors layout
+ * (capacities in, counts out). */
+int dhh_checkerboard_lower_band(int nbr_site_x, int nbr_site_y, double t1, double t2, double tpp, double mus,
+                                double gamma_x, double gamma_y, double* u, double* energies);
+int dhh_checkerboard_factors(int nbr_site_x, int nbr_site_y, double t1, double t2, double tpp, double mus,
+                             double u_potential, double v_potential, int flat_band, int* sum_pair_count,
+                             int* sum_pairs, int* nbr_pairs, double* factors, int64_t* nbr_factors);
 
 /* Host-side tables (no GPU needed) */
 /* LLL Coulomb pseudopotentials V_0..V_nbr_flux in CoulombPseudopotentials units; round != 0

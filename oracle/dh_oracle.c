@@ -863,6 +863,346 @@ long or_fs_count_elements(void* space, void* factors) {
 }
 
 /* ======================================================================
+ * Fermions in momentum space: FQHE/src/HilbertSpace/FermionOnSquareLatticeMomentumSpace.cc
+ * (orbital index kx * Ny + ky, FermionOnSphere bit conventions, signs, LUT and search)
+ * ====================================================================== */
+typedef struct {
+  int nx, ny, kx, ky;
+} fm_params;
+
+/* FermionOnSquareLatticeMomentumSpace.cc:349 EvaluateHilbertSpaceDimension */
+static long fm_dimension(const fm_params* p, int n, int cKx, int cKy, int tKx, int tKy) {
+  if (cKy < 0) {
+    cKy = p->ny - 1;
+    cKx--;
+  }
+  if (n == 0) return (((tKx % p->nx) == p->kx) && ((tKy % p->ny) == p->ky)) ? 1 : 0;
+  if (cKx < 0) return 0;
+  long count = 0;
+  if (n == 1) {
+    for (int j = cKy; j >= 0; --j)
+      if ((((cKx + tKx) % p->nx) == p->kx) && (((j + tKy) % p->ny) == p->ky)) ++count;
+    for (int i = cKx - 1; i >= 0; --i)
+      for (int j = p->ny - 1; j >= 0; --j)
+        if ((((i + tKx) % p->nx) == p->kx) && (((j + tKy) % p->ny) == p->ky)) ++count;
+    return count;
+  }
+  count += fm_dimension(p, n - 1, cKx, cKy - 1, tKx + cKx, tKy + cKy);
+  count += fm_dimension(p, n, cKx, cKy - 1, tKx, tKy);
+  return count;
+}
+
+/* FermionOnSquareLatticeMomentumSpace.cc:290 GenerateStates */
+static long fm_generate(const fm_params* p, uint64_t* st, int n, int cKx, int cKy, int tKx, int tKy, long pos) {
+  if (cKy < 0) {
+    cKy = p->ny - 1;
+    cKx--;
+  }
+  if (n == 0) {
+    if (((tKx % p->nx) == p->kx) && ((tKy % p->ny) == p->ky)) {
+      st[pos] = 0x0ul;
+      return pos + 1;
+    }
+    return pos;
+  }
+  if (cKx < 0) return pos;
+  if (n == 1) {
+    for (int j = cKy; j >= 0; --j)
+      if ((((cKx + tKx) % p->nx) == p->kx) && (((j + tKy) % p->ny) == p->ky)) st[pos++] = 0x1ul << ((cKx * p->ny) + j);
+    for (int i = cKx - 1; i >= 0; --i)
+      for (int j = p->ny - 1; j >= 0; --j)
+        if ((((i + tKx) % p->nx) == p->kx) && (((j + tKy) % p->ny) == p->ky)) st[pos++] = 0x1ul << ((i * p->ny) + j);
+    return pos;
+  }
+  long tmp = fm_generate(p, st, n - 1, cKx, cKy - 1, tKx + cKx, tKy + cKy, pos);
+  uint64_t mask = 0x1ul << ((cKx * p->ny) + cKy);
+  for (; pos < tmp; ++pos) st[pos] |= mask;
+  return fm_generate(p, st, n, cKx, cKy - 1, tKx, tKy, pos);
+}
+
+/* FermionOnSquareLatticeMomentumSpace.cc:76 constructor: LzMax = Nx * Ny */
+void* or_fm_create(int nbrFermions, int nx, int ny, int kx, int ky, unsigned long memory) {
+  fm_params p = {nx, ny, kx, ky};
+  long dim = fm_dimension(&p, nbrFermions, nx - 1, ny - 1, 0, 0);
+  if (dim <= 0) return NULL;
+  fs_space* s = (fs_space*)calloc(1, sizeof(fs_space));
+  s->N = nbrFermions;
+  s->lzMax = nx * ny;
+  s->nbrLzValue = s->lzMax + 1;
+  s->dim = dim;
+  s->states = (uint64_t*)malloc(sizeof(uint64_t) * dim);
+  s->stateLzMax = (int*)malloc(sizeof(int) * dim);
+  fm_generate(&p, s->states, nbrFermions, nx - 1, ny - 1, 0, 0, 0);
+  int tmp = s->lzMax;
+  for (long i = 0; i < dim; ++i) {
+    while ((s->states[i] >> tmp) == 0x0ul) --tmp;
+    s->stateLzMax[i] = tmp;
+  }
+  fs_lookup_table(s, memory);
+  return s;
+}
+
+/* TightBindingModelCheckerboardLattice.cc:206 CoreComputeBandStructure (non-Bloch form):
+ *   H(k) = [[d1 + d3, B1], [conj(B1), d1 - d3]],
+ *   B1 = 4 t1 (cos(Kx/2) cos(Ky/2) cos(pi/4) + i sin(Kx/2) sin(Ky/2) sin(pi/4)),
+ *   d1 = 4 t'' cos Kx cos Ky,  d3 = mu_s + 2 t2 (cos Kx - cos Ky),  K = 2 pi (k + gamma) / N.
+ * The 2x2 problem is solved in closed form.  out_u[4 * index + (re0, im0, re1, im1)] holds the
+ * lower-band one-body coefficients OneBodyBasis[index][0][alpha] = conj(v_alpha(k)), v the
+ * normalised lower-band eigenvector with v_0 real >= 0 (v_1 real >= 0 when v_0 = 0); the
+ * reference's own phase comes from HermitianMatrix::Diagonalize and is not restated (spectra
+ * do not depend on it).  out_e[2 * index + band] = band energies. */
+void or_checkerboard_lower_band(int nx, int ny, double t1, double t2, double tpp, double mus, double gx, double gy,
+                                double* out_u, double* out_e) {
+  for (int kx = 0; kx < nx; ++kx)
+    for (int ky = 0; ky < ny; ++ky) {
+      int index = kx * ny + ky;
+      double Kx = 2.0 * M_PI * (kx + gx) / nx, Ky = 2.0 * M_PI * (ky + gy) / ny;
+      double br = 4.0 * t1 * cos(0.5 * Kx) * cos(0.5 * Ky) * cos(M_PI * 0.25);
+      double bi = 4.0 * t1 * sin(0.5 * Kx) * sin(0.5 * Ky) * sin(M_PI * 0.25);
+      double d1 = 4.0 * tpp * cos(Kx) * cos(Ky);
+      double d3 = mus + 2.0 * t2 * (cos(Kx) - cos(Ky));
+      double r = sqrt(d3 * d3 + br * br + bi * bi);
+      out_e[2 * index] = d1 - r;
+      out_e[2 * index + 1] = d1 + r;
+      double v0r, v0i, v1r, v1i;
+      if (d3 >= 0.0) { /* (B1, -(d3 + r)) */
+        v0r = br; v0i = bi; v1r = -(d3 + r); v1i = 0.0;
+      } else { /* (r - d3, -conj(B1)) */
+        v0r = r - d3; v0i = 0.0; v1r = -br; v1i = bi;
+      }
+      double nrm = sqrt(v0r * v0r + v0i * v0i + v1r * v1r + v1i * v1i);
+      if (nrm == 0.0) { v0r = 1.0; nrm = 1.0; }
+      v0r /= nrm; v0i /= nrm; v1r /= nrm; v1i /= nrm;
+      double a0 = sqrt(v0r * v0r + v0i * v0i);
+      double pr, pi; /* multiply by conj(phase) of the gauge-fixing component */
+      if (a0 > 1e-300) { pr = v0r / a0; pi = -v0i / a0; }
+      else { double a1 = sqrt(v1r * v1r + v1i * v1i); pr = v1r / a1; pi = -v1i / a1; }
+      double w0r = v0r * pr - v0i * pi, w0i = v0r * pi + v0i * pr;
+      double w1r = v1r * pr - v1i * pi, w1i = v1r * pi + v1i * pr;
+      out_u[4 * index + 0] = w0r; out_u[4 * index + 1] = -w0i;
+      out_u[4 * index + 2] = w1r; out_u[4 * index + 3] = -w1i;
+    }
+}
+
+/* ParticleOnLatticeCheckerboardLatticeSingleBandHamiltonian.cc:126 EvaluateInteractionFactors
+ * (fermions) with ComputeTwoBodyMatrixElementAB / AA (:359, :380) */
+typedef struct {
+  int nx, ny, nbrSums;
+  int* nbrPerSum;  /* [nbrSums] */
+  int** pairs;     /* [nbrSums][2 * nbrPerSum] (Index1 < Index2) */
+  double** fre;    /* [nbrSums][nbrPerSum^2] */
+  double** fim;
+} fm_factors;
+
+static void cmul(double ar, double ai, double br, double bi, double* cr, double* ci) {
+  double r = ar * br - ai * bi, i = ar * bi + ai * br;
+  *cr = r;
+  *ci = i;
+}
+
+void* or_fm_factors_create(int nx, int ny, const double* u, double uPotential, double vPotential, int flatBand) {
+  fm_factors* f = (fm_factors*)calloc(1, sizeof(fm_factors));
+  f->nx = nx;
+  f->ny = ny;
+  f->nbrSums = nx * ny;
+  f->nbrPerSum = (int*)calloc(f->nbrSums, sizeof(int));
+  for (int kx1 = 0; kx1 < nx; ++kx1)
+    for (int kx2 = 0; kx2 < nx; ++kx2)
+      for (int ky1 = 0; ky1 < ny; ++ky1)
+        for (int ky2 = 0; ky2 < ny; ++ky2) {
+          int i1 = kx1 * ny + ky1, i2 = kx2 * ny + ky2;
+          if (i1 < i2) ++f->nbrPerSum[(((kx1 + kx2) % nx) * ny) + ((ky1 + ky2) % ny)];
+        }
+  f->pairs = (int**)calloc(f->nbrSums, sizeof(int*));
+  f->fre = (double**)calloc(f->nbrSums, sizeof(double*));
+  f->fim = (double**)calloc(f->nbrSums, sizeof(double*));
+  int* fill = (int*)calloc(f->nbrSums, sizeof(int));
+  for (int i = 0; i < f->nbrSums; ++i) f->pairs[i] = (int*)malloc(sizeof(int) * 2 * (f->nbrPerSum[i] + 1));
+  for (int kx1 = 0; kx1 < nx; ++kx1)
+    for (int kx2 = 0; kx2 < nx; ++kx2)
+      for (int ky1 = 0; ky1 < ny; ++ky1)
+        for (int ky2 = 0; ky2 < ny; ++ky2) {
+          int i1 = kx1 * ny + ky1, i2 = kx2 * ny + ky2;
+          if (i1 < i2) {
+            int sum = (((kx1 + kx2) % nx) * ny) + ((ky1 + ky2) % ny);
+            f->pairs[sum][2 * fill[sum]] = i1;
+            f->pairs[sum][2 * fill[sum] + 1] = i2;
+            ++fill[sum];
+          }
+        }
+  free(fill);
+  double factorU = 0.5 / ((double)(nx * ny));
+  if (!flatBand) factorU *= uPotential;
+  double factorV = vPotential * 0.5 / ((double)(nx * ny));
+  /* Kx = projected momenta (untilted lattice, no twist) */
+#define KX(i) (2.0 * M_PI * ((i) / ny) / nx)
+#define KY(i) (2.0 * M_PI * ((i) % ny) / ny)
+#define UR(i, a) (u[4 * (i) + 2 * (a)])
+#define UI(i, a) (u[4 * (i) + 2 * (a) + 1])
+  for (int s = 0; s < f->nbrSums; ++s) {
+    int n = f->nbrPerSum[s];
+    f->fre[s] = (double*)calloc((size_t)n * n + 1, sizeof(double));
+    f->fim[s] = (double*)calloc((size_t)n * n + 1, sizeof(double));
+    int index = 0;
+    for (int j1 = 0; j1 < n; ++j1) {
+      int I1 = f->pairs[s][2 * j1], I2 = f->pairs[s][2 * j1 + 1];
+      for (int j2 = 0; j2 < n; ++j2) {
+        int I3 = f->pairs[s][2 * j2], I4 = f->pairs[s][2 * j2 + 1];
+        /* four AB terms: (+,I1,I3,I2,I4 | k2,k4) (-,I2,I3,I1,I4 | k1,k4) (-,I1,I4,I2,I3 | k2,k3) (+,I2,I4,I1,I3 | k1,k3) */
+        int ta[4][5] = {{+1, I1, I3, I2, I4}, {-1, I2, I3, I1, I4}, {-1, I1, I4, I2, I3}, {+1, I2, I4, I1, I3}};
+        int tk[4][2] = {{I2, I4}, {I1, I4}, {I2, I3}, {I1, I3}};
+        double accr = 0.0, acci = 0.0;
+        for (int t = 0; t < 4; ++t) {
+          int a = ta[t][1], b = ta[t][2], c = ta[t][3], d = ta[t][4];
+          double pr, pi;
+          cmul(UR(a, 0), -UI(a, 0), UR(b, 0), UI(b, 0), &pr, &pi);
+          cmul(pr, pi, UR(c, 1), -UI(c, 1), &pr, &pi);
+          cmul(pr, pi, UR(d, 1), UI(d, 1), &pr, &pi);
+          int p1 = tk[t][0], p2 = tk[t][1];
+          double m = 2.0 * (cos(0.5 * (KX(p2) - KY(p2) + KY(p1) - KX(p1))) + cos(0.5 * (KX(p2) + KY(p2) - KY(p1) - KX(p1))));
+          accr += ta[t][0] * factorU * pr * m;
+          acci += ta[t][0] * factorU * pi * m;
+        }
+        if (factorV != 0.0) {
+          for (int t = 0; t < 4; ++t) {
+            int a = ta[t][1], b = ta[t][2], c = ta[t][3], d = ta[t][4];
+            double sr = 0.0, si = 0.0;
+            for (int al = 0; al < 2; ++al) {
+              double pr, pi;
+              cmul(UR(a, al), -UI(a, al), UR(b, al), UI(b, al), &pr, &pi);
+              cmul(pr, pi, UR(c, al), -UI(c, al), &pr, &pi);
+              cmul(pr, pi, UR(d, al), UI(d, al), &pr, &pi);
+              sr += pr;
+              si += pi;
+            }
+            int p1 = tk[t][0], p2 = tk[t][1];
+            double mr = cos(KX(p2) - KX(p1)) + cos(KY(p2) - KY(p1));
+            double mi = sin(KX(p2) - KX(p1)) + sin(KY(p2) - KY(p1));
+            double qr, qi;
+            cmul(sr, si, mr, mi, &qr, &qi);
+            accr += ta[t][0] * factorV * qr;
+            acci += ta[t][0] * factorV * qi;
+          }
+        }
+        f->fre[s][index] = -2.0 * accr;
+        f->fim[s][index] = -2.0 * acci;
+        ++index;
+      }
+    }
+  }
+#undef KX
+#undef KY
+#undef UR
+#undef UI
+  return f;
+}
+
+void or_fm_factors_destroy(void* p) {
+  fm_factors* f = (fm_factors*)p;
+  if (!f) return;
+  for (int i = 0; i < f->nbrSums; ++i) {
+    free(f->pairs[i]);
+    free(f->fre[i]);
+    free(f->fim[i]);
+  }
+  free(f->pairs);
+  free(f->fre);
+  free(f->fim);
+  free(f->nbrPerSum);
+  free(f);
+}
+
+int or_fm_factors_sector_size(void* p, int sum) { return ((fm_factors*)p)->nbrPerSum[sum]; }
+void or_fm_factors_get(void* p, int sum, int* pairs, double* fre, double* fim) {
+  fm_factors* f = (fm_factors*)p;
+  int n = f->nbrPerSum[sum];
+  memcpy(pairs, f->pairs[sum], sizeof(int) * 2 * n);
+  memcpy(fre, f->fre[sum], sizeof(double) * n * n);
+  memcpy(fim, f->fim[sum], sizeof(double) * n * n);
+}
+
+/* ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian.h:370
+ * EvaluateMNTwoBodyAddMultiplyComponent (scatter) and :457 Hermitian variant; complex vectors are
+ * interleaved (re, im).  Flat band: no one-body term. */
+static void fm_hv_range(const fs_space* sp, const fm_factors* f, const double* x, double* y, int hermitian,
+                        long first, long last) {
+  long dim = sp->dim;
+  for (long index = first; index < last; ++index) {
+    double sumr = 0.0, sumi = 0.0;
+    for (int j = 0; j < f->nbrSums; ++j) {
+      int n = f->nbrPerSum[j];
+      const int* idx = f->pairs[j];
+      for (int i1 = 0; i1 < n; ++i1) {
+        uint64_t tmp;
+        int lz = 0;
+        double c3 = fs_aa_local(sp, index, idx[2 * i1], idx[2 * i1 + 1], &tmp, &lz);
+        if (c3 == 0.0) continue;
+        double c4r = x[2 * index] * c3, c4i = x[2 * index + 1] * c3;
+        const double* fr = &f->fre[j][i1 * n];
+        const double* fi = &f->fim[j][i1 * n];
+        for (int i2 = 0; i2 < n; ++i2) {
+          double c;
+          long target = fs_adad_local(sp, tmp, lz, idx[2 * i2], idx[2 * i2 + 1], &c);
+          if (target >= dim) continue;
+          if (hermitian) {
+            if (target > index) continue;
+            if (target < index) { /* TmpSum += x[target] * c * c3 * conj(F) */
+              double ar = x[2 * target], ai = x[2 * target + 1];
+              double pr, pi;
+              cmul(ar, ai, fr[i2], -fi[i2], &pr, &pi);
+              sumr += pr * c * c3;
+              sumi += pi * c * c3;
+            }
+          }
+          double pr, pi; /* y[target] += (c * F) * c4 */
+          cmul(c * fr[i2], c * fi[i2], c4r, c4i, &pr, &pi);
+          y[2 * target] += pr;
+          y[2 * target + 1] += pi;
+        }
+      }
+    }
+    if (hermitian) {
+      y[2 * index] += sumr;
+      y[2 * index + 1] += sumi;
+    }
+  }
+}
+
+void or_fm_hv(void* space, void* factors, const double* x, double* y, int hermitian) {
+  fs_space* s = (fs_space*)space;
+  fm_hv_range(s, (fm_factors*)factors, x, y, hermitian, 0, s->dim);
+}
+
+/* elements (target, real and imaginary coefficient) produced from source row `row` */
+long or_fm_row_elements(void* space, void* factors, long row, long* targets, double* cre, double* cim, long maxOut) {
+  const fs_space* sp = (const fs_space*)space;
+  const fm_factors* f = (const fm_factors*)factors;
+  long cnt = 0;
+  for (int j = 0; j < f->nbrSums; ++j) {
+    int n = f->nbrPerSum[j];
+    const int* idx = f->pairs[j];
+    for (int i1 = 0; i1 < n; ++i1) {
+      uint64_t tmp;
+      int lz = 0;
+      double c3 = fs_aa_local(sp, row, idx[2 * i1], idx[2 * i1 + 1], &tmp, &lz);
+      if (c3 == 0.0) continue;
+      for (int i2 = 0; i2 < n; ++i2) {
+        double c;
+        long target = fs_adad_local(sp, tmp, lz, idx[2 * i2], idx[2 * i2 + 1], &c);
+        if (target >= sp->dim) continue;
+        if (cnt < maxOut) {
+          targets[cnt] = target;
+          cre[cnt] = c * c3 * f->fre[j][i1 * n + i2];
+          cim[cnt] = c * c3 * f->fim[j][i1 * n + i2];
+        }
+        ++cnt;
+      }
+    }
+  }
+  return cnt;
+}
+
+/* ======================================================================
  * Spin-1/2 chain: Spin/src/HilbertSpace/Spin1_2Chain.cc, Spin/src/Hamiltonian/SpinChainHamiltonian.cc
  * ====================================================================== */
 typedef struct {

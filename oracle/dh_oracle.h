@@ -48,6 +48,17 @@ void or_fs_hv_smp(void* space, void* factors, const double* x, double* y, double
 long or_fs_row_elements(void* space, void* factors, long row, long* targets, double* coefs, long maxOut);
 long or_fs_count_elements(void* space, void* factors);
 
+/* ---------------- fermions in momentum space (Chern band) ---------------- */
+void* or_fm_create(int nbrFermions, int nx, int ny, int kx, int ky, unsigned long memory); /* or_fs_* accessors apply */
+void or_checkerboard_lower_band(int nx, int ny, double t1, double t2, double tpp, double mus, double gx, double gy,
+                                double* out_u, double* out_e);
+void* or_fm_factors_create(int nx, int ny, const double* u, double uPotential, double vPotential, int flatBand);
+void or_fm_factors_destroy(void* factors);
+int or_fm_factors_sector_size(void* factors, int sum);
+void or_fm_factors_get(void* factors, int sum, int* pairs, double* fre, double* fim);
+void or_fm_hv(void* space, void* factors, const double* x, double* y, int hermitian);
+long or_fm_row_elements(void* space, void* factors, long row, long* targets, double* cre, double* cim, long maxOut);
+
 /* ---------------- spin-1/2 chain ---------------- */
 void* or_spin_create(int chainLength, int sz, int memorySize);
 void or_spin_destroy(void* space);

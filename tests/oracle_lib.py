@@ -247,3 +247,93 @@ class SpinChain:
         it = lib().or_spin_lanczos(self.h, dptr(J), dptr(Jz), int(periodic), n_eig, max_iter,
                                    precision, seed, threads, dptr(ev))
         return ev, it
+
+
+def _bind_momentum(L):
+    if getattr(L, "_momentum_bound", False):
+        return L
+    _sig(L, "or_fm_create", c_void_p, [c_int, c_int, c_int, c_int, c_int, c_ulong])
+    _sig(L, "or_checkerboard_lower_band", None, [c_int, c_int, c_double, c_double, c_double, c_double, c_double,
+                                                 c_double, P_D, P_D])
+    _sig(L, "or_fm_factors_create", c_void_p, [c_int, c_int, P_D, c_double, c_double, c_int])
+    _sig(L, "or_fm_factors_destroy", None, [c_void_p])
+    _sig(L, "or_fm_factors_sector_size", c_int, [c_void_p, c_int])
+    _sig(L, "or_fm_factors_get", None, [c_void_p, c_int, P_I, P_D, P_D])
+    _sig(L, "or_fm_hv", None, [c_void_p, c_void_p, P_D, P_D, c_int])
+    _sig(L, "or_fm_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, P_D, c_long])
+    L._momentum_bound = True
+    return L
+
+
+def checkerboard_lower_band(nx, ny, t1=1.0, t2=0.5, tpp=-0.5, mus=0.0, gx=0.0, gy=0.0):
+    """OneBodyBasis[k][0][alpha] (lower band) as complex (nx*ny, 2), band energies (nx*ny, 2)."""
+    L = _bind_momentum(lib())
+    u = np.zeros(nx * ny * 4)
+    e = np.zeros(nx * ny * 2)
+    L.or_checkerboard_lower_band(nx, ny, t1, t2, tpp, mus, gx, gy, dptr(u), dptr(e))
+    return (u[0::2] + 1j * u[1::2]).reshape(nx * ny, 2), e.reshape(nx * ny, 2)
+
+
+class MomentumFermions:
+    """FermionOnSquareLatticeMomentumSpace restatement."""
+
+    def __init__(self, n, nx, ny, kx, ky, memory=9 << 20):
+        L = _bind_momentum(lib())
+        self.n, self.nx, self.ny, self.kx, self.ky = n, nx, ny, kx, ky
+        self.h = L.or_fm_create(n, nx, ny, kx, ky, memory)
+        if not self.h:
+            raise ValueError("empty Hilbert space")
+        self.dim = L.or_fs_dim(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_fs_destroy(self.h)
+            self.h = None
+
+    def states(self):
+        p = lib().or_fs_states(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.dim,)).copy()
+
+
+class LatticeFactors:
+    """ParticleOnLatticeCheckerboardLatticeSingleBandHamiltonian factor tables (fermions)."""
+
+    def __init__(self, nx, ny, u, u_potential=1.0, v_potential=0.0, flat_band=True):
+        L = _bind_momentum(lib())
+        uu = np.zeros(nx * ny * 4)
+        uu[0::2] = np.asarray(u).reshape(-1).real
+        uu[1::2] = np.asarray(u).reshape(-1).imag
+        self.nx, self.ny = nx, ny
+        self.h = L.or_fm_factors_create(nx, ny, dptr(uu), u_potential, v_potential, int(flat_band))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_fm_factors_destroy(self.h)
+            self.h = None
+
+    def sector(self, s):
+        L = lib()
+        n = L.or_fm_factors_sector_size(self.h, s)
+        pairs = np.zeros(2 * n, np.int32)
+        fr = np.zeros(n * n + 1)
+        fi = np.zeros(n * n + 1)
+        L.or_fm_factors_get(self.h, s, pairs.ctypes.data_as(P_I), dptr(fr), dptr(fi))
+        return pairs.reshape(n, 2), (fr[:n * n] + 1j * fi[:n * n]).reshape(n, n)
+
+
+def fm_hv(space, factors, x, hermitian=False):
+    """H x for a complex vector x (numpy complex128)."""
+    xr = np.empty(2 * space.dim)
+    xr[0::2], xr[1::2] = np.real(x), np.imag(x)
+    y = np.zeros(2 * space.dim)
+    lib().or_fm_hv(space.h, factors.h, dptr(xr), dptr(y), int(hermitian))
+    return y[0::2] + 1j * y[1::2]
+
+
+def fm_row_elements(space, factors, row, max_out=20000):
+    t = np.zeros(max_out, np.int64)
+    cr = np.zeros(max_out)
+    ci = np.zeros(max_out)
+    n = lib().or_fm_row_elements(space.h, factors.h, row, t.ctypes.data_as(P_L), dptr(cr), dptr(ci), max_out)
+    assert n <= max_out
+    return t[:n], cr[:n] + 1j * ci[:n]

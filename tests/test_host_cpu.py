@@ -127,3 +127,36 @@ def test_pseudopotential_file_reader(tmp_path):
     out = np.zeros(lz + 1)
     rc = capi.host().dhh_read_pseudopotentials(str(p).encode(), lz, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     assert rc == 0 and np.array_equal(out, v)
+
+
+def test_host_checkerboard_band_and_factors_match_oracle():
+    """Product host tables of the Chern-band model vs the oracle restatement."""
+    import ctypes
+    from diagham_amd import capi
+    nx, ny = 6, 5
+    u = np.zeros(nx * ny * 4)
+    e = np.zeros(nx * ny * 2)
+    assert capi.host().dhh_checkerboard_lower_band(nx, ny, 1.0, 0.5, -0.5, 0.0, 0.0, 0.0, capi.dptr(u), capi.dptr(e)) == 0
+    uo, eo = O.checkerboard_lower_band(nx, ny, 1.0, 0.5, -0.5)
+    assert np.max(np.abs((u[0::2] + 1j * u[1::2]).reshape(-1, 2) - uo)) < 1e-14
+    assert np.max(np.abs(e.reshape(-1, 2) - eo)) < 1e-14
+    npairs = nx * ny * (nx * ny - 1) // 2
+    cnt = np.zeros(nx * ny, np.int32)
+    pairs = np.zeros(2 * npairs, np.int32)
+    f = np.zeros(2 * npairs * npairs)
+    npr, nf = ctypes.c_int(npairs), ctypes.c_int64(npairs * npairs)
+    assert capi.host().dhh_checkerboard_factors(nx, ny, 1.0, 0.5, -0.5, 0.0, 1.0, 0.0, 1, cnt.ctypes.data_as(capi.P_I),
+                                                pairs.ctypes.data_as(capi.P_I), ctypes.byref(npr), capi.dptr(f),
+                                                ctypes.byref(nf)) == 0
+    fac = O.LatticeFactors(nx, ny, uo)
+    pos, fpos = 0, 0
+    for s in range(nx * ny):
+        p_ref, F_ref = fac.sector(s)
+        m = cnt[s]
+        assert m == len(p_ref)
+        assert np.array_equal(pairs[2 * pos:2 * (pos + m)].reshape(m, 2), p_ref)
+        F = (f[2 * fpos:2 * (fpos + m * m):2] + 1j * f[2 * fpos + 1:2 * (fpos + m * m):2]).reshape(m, m)
+        assert np.max(np.abs(F - F_ref)) < 1e-14
+        assert np.max(np.abs(F - F.conj().T)) < 1e-14  # Hermitian factor matrix per momentum sum
+        pos += m
+        fpos += m * m

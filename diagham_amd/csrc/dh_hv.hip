@@ -18,6 +18,7 @@
 
 #include "dh_index.h"
 #include "dh_internal.h"
+#include "dh_sphere_row.h"
 
 enum { MODE_BSEARCH = 0, MODE_RANK_LDS = 1, MODE_RANK_GLOBAL = 2 };
 
@@ -105,29 +106,8 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
     double acc[K];
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
-    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
-      const int n1 = __builtin_ctzll(r1);
-      const int pc1 = __popcll(s >> n1);
-      for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
-        const int n2 = __builtin_ctzll(r2);
-        const int sgnA = pc1 + __popcll(s >> n2);
-        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
-        const int sum = n1 + n2;
-        const int m3lo = (sum >> 1) + 1;
-        const int m3hi = min(sum, a.nbrOrbitals - 1);
-        const int fbase = sOff[(n1 * (n1 - 1)) / 2 + n2] - m3lo;
-        for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
-          const int m4 = sum - m3;
-          const uint64_t cm = (1ull << m3) | (1ull << m4);
-          if (t & cm) continue;
-          const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
-          const int64_t idx = sphere_find<MODE>(a, t | cm, sLut, sLo, sHi);
-          double coef = sF[fbase + m3];
-          if (sg & 1) coef = -coef;
-          gather_fma<K>(acc, x, idx, coef);
-        }
-      }
-    }
+    auto find = [&](uint64_t t) -> int64_t { return sphere_find<MODE>(a, t, sLut, sLo, sHi); };
+    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x, row, acc);
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)
@@ -151,21 +131,33 @@ struct SpinArgs {
   const int* lut;
   int lutShift, lutSize;
   double shift;
+  double jzSum;  // sum of Jz over the bonds of the chain
   double halfJ[32], Jz[32];
 };
 
+// Only antiparallel bonds contribute off-diagonal terms (and lower the diagonal by Jz/2 each):
+// they are the set bits of s ^ (s >> 1).  Their target indices are computed G at a time before
+// the G gathers of x are issued, so each lane keeps G independent loads in flight instead of
+// waiting on one gather per bond.
 template <int K, int MODE>
 __global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double* __restrict__ x,
                                                        double* __restrict__ y, int add, int64_t first,
                                                        int64_t count) {
+  constexpr int G = K == 1 ? 8 : (K == 2 ? 4 : 2);
   __shared__ uint32_t sBinom[33 * 33];
+  __shared__ double sHalfJ[32], sJz[32];
   extern __shared__ int sLut[];
   for (int i = threadIdx.x; i < 33 * 33; i += blockDim.x) sBinom[i] = a.binom[i];
+  if (threadIdx.x < 32) {
+    sHalfJ[threadIdx.x] = a.halfJ[threadIdx.x];
+    sJz[threadIdx.x] = a.Jz[threadIdx.x];
+  }
   if constexpr (MODE == MODE_BSEARCH)
     for (int i = threadIdx.x; i < a.lutSize; i += blockDim.x) sLut[i] = a.lut[i];
   __syncthreads();
   const int L = a.L;
   const uint32_t mask = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
+  const uint32_t openBonds = (1u << (L - 1)) - 1u;  // bonds (i, i+1), i < L-1
   const int64_t nbrTiles = (count + blockDim.x - 1) / blockDim.x;
   const unsigned vb = dh_xcd_block(blockIdx.x, gridDim.x);
   for (int64_t tile = vb; tile < nbrTiles; tile += gridDim.x) {
@@ -173,38 +165,67 @@ __global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double*
     if (local >= count) continue;
     const int64_t row = first + local;
     const uint32_t s = a.words[row] & mask;
-    double diag = a.shift;
-    for (int b = 0; b < L - 1; ++b) diag += a.Jz[b] * ((((s >> b) ^ (s >> (b + 1))) & 1u) ? -0.25 : 0.25);
-    if (a.periodic) diag += a.Jz[L - 1] * ((((s >> (L - 1)) ^ s) & 1u) ? -0.25 : 0.25);
+    uint32_t anti = (s ^ (s >> 1)) & openBonds;
+    double diag = a.shift + 0.25 * a.jzSum;
+    for (uint32_t r = anti; r; r &= r - 1) diag -= 0.5 * sJz[__builtin_ctz(r)];
+    const bool wrap = a.periodic && (((s >> (L - 1)) ^ s) & 1u);
+    if (wrap) diag -= 0.5 * sJz[L - 1];
     double acc[K];
     {
       const double* xr = x + row * K;
 #pragma unroll
       for (int v = 0; v < K; ++v) acc[v] = diag * xr[v];
     }
-    for (int i = 0; i < L - 1; ++i) {
-      const uint32_t pr = (s >> i) & 3u;
-      if (pr == 1u || pr == 2u) {
-        int64_t idx;
-        if constexpr (MODE == MODE_BSEARCH) {
-          idx = dh_spin_find_bsearch(s ^ (3u << i), L, sLut, a.lutShift, a.words);
-        } else {
-          const int kc = __popc(s >> i);
-          const int64_t d = sBinom[(L - 2 - i) * 33 + (kc - 1)];
-          idx = (pr == 1u) ? row + d : row - d;
-          idx = idx < 0 ? 0 : (idx >= a.dim ? a.dim - 1 : idx);
+    while (anti) {
+      int64_t idx[G];
+      double coef[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        idx[g] = row;  // padding entries gather the row itself with a zero coefficient
+        coef[g] = 0.0;
+        if (anti) {
+          const int i = __builtin_ctz(anti);
+          anti &= anti - 1;
+          if constexpr (MODE == MODE_BSEARCH) {
+            idx[g] = dh_spin_find_bsearch(s ^ (3u << i), L, sLut, a.lutShift, a.words);
+          } else {
+            const int kc = __popc(s >> i);
+            const int64_t d = sBinom[(L - 2 - i) * 33 + (kc - 1)];
+            int64_t t = ((s >> i) & 1u) ? row + d : row - d;
+            idx[g] = t < 0 ? 0 : (t >= a.dim ? a.dim - 1 : t);
+          }
+          coef[g] = sHalfJ[i];
         }
-        gather_fma<K>(acc, x, idx, a.halfJ[i]);
       }
+      double v[G][K];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const double* xr = x + idx[g] * K;
+        if constexpr (K % 2 == 0) {
+#pragma unroll
+          for (int q = 0; q < K; q += 2) {
+            double2 p = *reinterpret_cast<const double2*>(xr + q);
+            v[g][q] = p.x;
+            v[g][q + 1] = p.y;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < K; ++q) v[g][q] = xr[q];
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] += coef[g] * v[g][q];
     }
-    if (a.periodic && (((s >> (L - 1)) ^ s) & 1u)) {
+    if (wrap) {
       const uint32_t t = s ^ (1u | (1u << (L - 1)));
       int64_t idx;
       if constexpr (MODE == MODE_BSEARCH)
         idx = dh_spin_find_bsearch(t, L, sLut, a.lutShift, a.words);
       else
         idx = dh_spin_find_rank(t, L, sBinom, a.dim);
-      gather_fma<K>(acc, x, idx, a.halfJ[L - 1]);
+      gather_fma<K>(acc, x, idx, sHalfJ[L - 1]);
     }
     double* yr = y + row * K;
 #pragma unroll
@@ -308,9 +329,11 @@ static SpinArgs spin_args(const dh_hamiltonian* h) {
   a.lutShift = b->lutShift;
   a.lutSize = (1 << b->lutBits) + 1;
   a.shift = h->shift;
+  a.jzSum = 0.0;
   for (int i = 0; i < 32; ++i) {
     a.halfJ[i] = 0.5 * h->J[i];
     a.Jz[i] = h->Jz[i];
+    if (i < a.L - 1 || (i == a.L - 1 && a.periodic)) a.jzSum += h->Jz[i];
   }
   return a;
 }

@@ -1,0 +1,412 @@
+// Block Lanczos recurrence with full reorthogonalisation, shared by the single-space algorithm
+// (FullReorthogonalizedBlockLanczosAlgorithm.cc) and the drivers that advance many sectors
+// together.  The Krylov basis stays on the device; the host keeps the small k x k blocks.
+//
+// Orthogonalisation.  The new block W = H X_j goes through two rounds of "project out the basis,
+// then Cholesky-QR of the block" (block classical Gram-Schmidt with reorthogonalisation).  The
+// second projection removes what the first Cholesky-QR amplified when the columns of W were
+// nearly dependent, so the basis stays orthonormal to rounding level.
+//
+// Rank deficiency.  A column is deficient when its Cholesky pivot is negligible against its own
+// norm (it depends on the previous columns of the block), or when a projection leaves only
+// rounding noise of what it had before (it lies in the span of the basis).  Such a column carries
+// no new direction: either the Krylov space of the start block became invariant, or the sector
+// has fewer than k directions left.  It gets a random vector orthogonalised against the basis and
+// the rest of the block (the restart branch of FullReorthogonalizedLanczosAlgorithm.cc:270).
+// When no direction is left it is zeroed and marked dead: it stays decoupled from the block
+// tridiagonal matrix (zero rows in the coupling blocks) and its diagonal entry is replaced by a
+// value above the Gershgorin bound of the matrix, so it never appears among the wanted
+// eigenvalues.  A block whose columns are all dead means the Krylov space of the sector is
+// complete and the projected matrix is exact.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+namespace dhh {
+
+using cplx = std::complex<double>;
+
+// Device operations of the recurrence.  A block is `rows` x KR doubles, row-major (a block of k
+// complex vectors interleaves real and imaginary parts: KR = 2k).  Rows are split in segments
+// (sectors); an operation touches only the segments flagged in `active`, and the per-segment host
+// matrices (KR x KR each) are packed for the active segments in increasing order.
+struct BlockOps {
+  virtual ~BlockOps() {}
+  virtual void Product(const double* X, double* W, const unsigned char* active) = 0;  // W = H X
+  // out[a][b] = V_b^T W over segment a (b < nb)
+  virtual void Dots(const double* V, int nb, const double* W, const unsigned char* active, double* out) = 0;
+  // W += sum_b V_b C[a][b] over segment a
+  virtual void Axpy(const double* V, int nb, double* W, const unsigned char* active, const double* C) = 0;
+  // W <- W M[a] over segment a
+  virtual void Rmul(double* W, const unsigned char* active, const double* M) = 0;
+  virtual void Random(double* W, uint64_t seed) = 0;  // the whole block
+};
+
+template <class T>
+struct KrylovScalar;
+
+template <>
+struct KrylovScalar<double> {
+  static constexpr int Real = 1;
+  static double Norm(double x) { return x * x; }
+  static double Conj(double x) { return x; }
+  // k x k coefficient block from the real Gram block of the device layout, and back
+  static void FromGram(int k, const double* G, double* C) { std::memcpy(C, G, sizeof(double) * k * k); }
+  static void ToDevice(int k, const double* C, double* M) { std::memcpy(M, C, sizeof(double) * k * k); }
+};
+
+template <>
+struct KrylovScalar<cplx> {
+  static constexpr int Real = 2;
+  static double Norm(cplx x) { return std::norm(x); }
+  static cplx Conj(cplx x) { return std::conj(x); }
+  // C[p][q] = <v_p, w_q> from the real (2k x 2k) Gram block of interleaved rows
+  static void FromGram(int k, const double* G, cplx* C) {
+    const int K2 = 2 * k;
+    for (int p = 0; p < k; ++p)
+      for (int q = 0; q < k; ++q) {
+        double rr = G[(size_t)(2 * p) * K2 + 2 * q], ii = G[(size_t)(2 * p + 1) * K2 + 2 * q + 1];
+        double ri = G[(size_t)(2 * p) * K2 + 2 * q + 1], ir = G[(size_t)(2 * p + 1) * K2 + 2 * q];
+        C[(size_t)p * k + q] = cplx(rr + ii, ri - ir);
+      }
+  }
+  // real (2k x 2k) matrix acting on interleaved rows: w_q += sum_p v_p c_pq
+  static void ToDevice(int k, const cplx* C, double* M) {
+    const int K2 = 2 * k;
+    for (int p = 0; p < k; ++p)
+      for (int q = 0; q < k; ++q) {
+        cplx c = C[(size_t)p * k + q];
+        M[(size_t)(2 * p) * K2 + 2 * q] = c.real();
+        M[(size_t)(2 * p) * K2 + 2 * q + 1] = c.imag();
+        M[(size_t)(2 * p + 1) * K2 + 2 * q] = -c.imag();
+        M[(size_t)(2 * p + 1) * K2 + 2 * q + 1] = c.real();
+      }
+  }
+};
+
+// Eigenvalues (ascending) of the (Hermitian) block tridiagonal matrix with diagonal blocks
+// A[0..m) and sub-diagonal blocks B[0..m-1); Ritz coefficients (column-major n x n) for real
+// matrices only.
+void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& B,
+                           int k, int m, std::vector<double>& evals, std::vector<double>* evecs);
+void BlockTridiagonalEigen(const std::vector<std::vector<cplx>>& A, const std::vector<std::vector<cplx>>& B, int k,
+                           int m, std::vector<double>& evals, std::vector<cplx>* evecs);
+
+template <class T>
+class SegmentedBlockLanczos {
+ public:
+  using S = KrylovScalar<T>;
+
+  // storage: (maxBlocks + 2) device blocks of blockDoubles (the last one is scratch)
+  SegmentedBlockLanczos(BlockOps* ops, const std::vector<int64_t>& dims, int k, int maxBlocks, double* storage,
+                        int64_t blockDoubles)
+      : Ops(ops),
+        Dims(dims),
+        K(k),
+        KR(k * S::Real),
+        MaxBlocks(maxBlocks),
+        Storage(storage),
+        BlockDoubles(blockDoubles),
+        A(dims.size()),
+        B(dims.size()),
+        DeadCols(dims.size()),
+        Previous(dims.size()),
+        LastCheck(dims.size(), -1) {}
+
+  double* Block(int j) const { return Storage + (int64_t)j * BlockDoubles; }
+  int BlockSize() const { return K; }
+  int Steps() const { return StepCount; }
+  int NbrBlocks(int s) const { return (int)A[s].size(); }
+  const std::vector<std::vector<T>>& Diagonal(int s) const { return A[s]; }
+  const std::vector<std::vector<T>>& Coupling(int s) const { return B[s]; }
+
+  // random orthonormal start block X_0 in every segment
+  void Start(uint64_t seed) {
+    NextSeed = seed * 0x9E3779B97F4A7C15ull + 0x5EED;
+    Ops->Random(Block(0), seed);
+    std::vector<int> act(Dims.size());
+    for (size_t s = 0; s < Dims.size(); ++s) act[s] = (int)s;
+    std::vector<Rounds> r;
+    Orthonormalize(Block(0), 0, act, r);
+    for (size_t s = 0; s < Dims.size(); ++s) {
+      A[s].clear();
+      B[s].clear();
+      DeadCols[s].assign(1, Dead(r[s].good));
+      Previous[s].clear();
+      LastCheck[s] = -1;
+    }
+    StepCount = 0;
+  }
+
+  // one Lanczos step X_j -> (A_j, B_j, X_{j+1}) in every segment flagged in `active`
+  void Step(const std::vector<unsigned char>& active) {
+    const int j = StepCount;
+    if (j + 1 > MaxBlocks) throw std::runtime_error("Krylov storage exhausted");
+    std::vector<int> act;
+    for (size_t s = 0; s < active.size(); ++s)
+      if (active[s]) act.push_back((int)s);
+    double* W = Block(j + 1);
+    Ops->Product(Block(j), W, active.data());
+    std::vector<Rounds> r;
+    Orthonormalize(W, j + 1, act, r);
+    for (size_t a = 0; a < act.size(); ++a) {
+      const int s = act[a];
+      std::vector<T>& h = r[a].coef;
+      for (int p = 0; p < K; ++p) {
+        h[(size_t)p * K + p] = T(std::real(h[(size_t)p * K + p]));
+        for (int q = p + 1; q < K; ++q) {
+          T v = 0.5 * (h[(size_t)p * K + q] + S::Conj(h[(size_t)q * K + p]));
+          h[(size_t)p * K + q] = v;
+          h[(size_t)q * K + p] = S::Conj(v);
+        }
+      }
+      A[s].push_back(h);
+      B[s].push_back(r[a].R);
+      DeadCols[s].push_back(Dead(r[a].good));
+    }
+    ++StepCount;
+  }
+
+  // the newest block of s has no live column: the Krylov space of s is complete
+  bool Exhausted(int s) const {
+    const std::vector<char>& d = DeadCols[s].back();
+    return std::all_of(d.begin(), d.end(), [](char x) { return x != 0; });
+  }
+
+  // orthonormal Krylov vectors of s behind the projected matrix (its meaningful dimension)
+  int64_t Live(int s) const {
+    int64_t n = 0;
+    for (size_t b = 0; b < A[s].size(); ++b)
+      for (char x : DeadCols[s][b]) n += x ? 0 : 1;
+    return n;
+  }
+
+  // eigenvalues of the projected matrix of s (dead columns removed); evecs: Ritz coefficients,
+  // column-major over all m*k Krylov columns, the first Live(s) columns belong to `ev`
+  void Eigen(int s, std::vector<double>& ev, std::vector<T>* evecs = nullptr) const {
+    const int m = (int)A[s].size();
+    std::vector<std::vector<T>> As = A[s];
+    double bound = 0.0;  // Gershgorin bound of the matrix
+    for (int b = 0; b < m; ++b)
+      for (int p = 0; p < K; ++p) {
+        double rs = 0.0;
+        for (int q = 0; q < K; ++q) {
+          rs += std::abs(A[s][b][(size_t)p * K + q]);
+          if (b > 0) rs += std::abs(B[s][b - 1][(size_t)p * K + q]);
+          if (b + 1 < m) rs += std::abs(B[s][b][(size_t)q * K + p]);
+        }
+        bound = std::max(bound, rs);
+      }
+    const double sentinel = 2.0 * bound + 1.0;
+    for (int b = 0; b < m; ++b)
+      for (int p = 0; p < K; ++p)
+        if (DeadCols[s][b][p]) As[b][(size_t)p * K + p] = T(sentinel);
+    BlockTridiagonalEigen(As, B[s], K, m, ev, evecs);
+    ev.resize((size_t)Live(s));
+  }
+
+  // Block Lanczos convergence checks cost a band reduction; past the first ones they are spaced
+  // so that at most ~10% extra blocks run after convergence.
+  bool CheckDue(int s, int nbrEigen) const {
+    const int m = NbrBlocks(s);
+    if (Exhausted(s)) return true;
+    if ((int64_t)m * K < nbrEigen + 2) return false;
+    return K == 1 || LastCheck[s] < 0 || m - LastCheck[s] >= std::max(1, m / 10);
+  }
+
+  // strong test of FullReorthogonalizedLanczosAlgorithm.cc:325: every wanted eigenvalue moved by
+  // less than precision * |value| since the previous test of s (or the Krylov space is complete)
+  bool TestConvergence(int s, int nbrEigen, double precision, std::vector<double>& current) {
+    std::vector<double> ev;
+    Eigen(s, ev);
+    LastCheck[s] = NbrBlocks(s);
+    const int want = std::min<int>(nbrEigen, (int)ev.size());
+    current.assign(ev.begin(), ev.begin() + want);
+    if (Exhausted(s)) {
+      Previous[s] = current;
+      return true;
+    }
+    bool ok = !Previous[s].empty() && (int)ev.size() > nbrEigen && (int)Previous[s].size() == want;
+    if (ok)
+      for (int e = 0; e < want; ++e)
+        if (std::fabs(current[e] - Previous[s][e]) > precision * std::fabs(current[e])) {
+          ok = false;
+          break;
+        }
+    Previous[s] = current;
+    return ok;
+  }
+
+ private:
+  // per segment: W_in = V_{0..nb} coef-part + Q R; coef = coefficients on the newest basis block
+  struct Rounds {
+    std::vector<T> coef, R;
+    std::vector<char> good;
+  };
+
+  static std::vector<char> Dead(const std::vector<char>& good) {
+    std::vector<char> d(good.size());
+    for (size_t p = 0; p < good.size(); ++p) d[p] = good[p] ? 0 : 1;
+    return d;
+  }
+
+  // Semidefinite Cholesky G = R^H R.  Column i is deficient when its pivot is below 1e-13 of its
+  // squared norm gii (dependent on the previous columns) or when gii is below `collapse` times
+  // ref[i], its squared norm before the projection that preceded (only rounding noise inside the
+  // span of the basis survived).  Deficient columns get zero rows in R.
+  static void CholeskySemidefinite(int k, const T* G, const double* ref, double collapse, std::vector<T>& R,
+                                   std::vector<char>& good) {
+    R.assign((size_t)k * k, T(0.0));
+    good.assign(k, 0);
+    for (int i = 0; i < k; ++i) {
+      const double gii = std::real(G[(size_t)i * k + i]);
+      double s = gii;
+      for (int p = 0; p < i; ++p) s -= S::Norm(R[(size_t)p * k + i]);
+      if (!(s > 0.0) || !(s > 1e-13 * gii) || !(gii > collapse * ref[i])) continue;
+      good[i] = 1;
+      const double rii = std::sqrt(s);
+      R[(size_t)i * k + i] = T(rii);
+      for (int j = i + 1; j < k; ++j) {
+        T t = G[(size_t)i * k + j];
+        for (int p = 0; p < i; ++p) t -= S::Conj(R[(size_t)p * k + i]) * R[(size_t)p * k + j];
+        R[(size_t)i * k + j] = t / rii;
+      }
+    }
+  }
+
+  // inverse of R on its non-zero rows/columns (upper triangular), zero elsewhere
+  static std::vector<T> PseudoInverse(int k, const std::vector<T>& R, const std::vector<char>& good) {
+    std::vector<T> X((size_t)k * k, T(0.0));
+    for (int j = 0; j < k; ++j) {
+      if (!good[j]) continue;
+      X[(size_t)j * k + j] = T(1.0) / R[(size_t)j * k + j];
+      for (int i = j - 1; i >= 0; --i) {
+        if (!good[i]) continue;
+        T s(0.0);
+        for (int p = i + 1; p <= j; ++p)
+          if (good[p]) s += R[(size_t)i * k + p] * X[(size_t)p * k + j];
+        X[(size_t)i * k + j] = -s / R[(size_t)i * k + i];
+      }
+    }
+    return X;
+  }
+
+  static std::vector<T> Mul(int k, const std::vector<T>& X, const std::vector<T>& Y) {
+    std::vector<T> Z((size_t)k * k, T(0.0));
+    for (int i = 0; i < k; ++i)
+      for (int p = 0; p < k; ++p)
+        for (int j = 0; j < k; ++j) Z[(size_t)i * k + j] += X[(size_t)i * k + p] * Y[(size_t)p * k + j];
+    return Z;
+  }
+
+  // Two rounds of (projection on the nb basis blocks, Cholesky-QR) of W over the segments in
+  // `act`: W_in = V C + Q R with C = C1 + C2 R1, R = R2 R1.  Deficient columns of Q are zero.
+  void TwoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
+    const int kk = K * K, KK = KR * KR;
+    const size_t na = act.size();
+    std::vector<unsigned char> mask(Dims.size(), 0);
+    for (int s : act) mask[s] = 1;
+    out.assign(na, Rounds());
+    for (auto& o : out) {
+      o.coef.assign((size_t)kk, T(0.0));
+      o.R.assign((size_t)kk, T(0.0));
+      for (int p = 0; p < K; ++p) o.R[(size_t)p * K + p] = T(1.0);
+      o.good.assign(K, 1);
+    }
+    std::vector<T> CT((size_t)kk), GT((size_t)kk);
+    std::vector<double> G(na * KK), M(na * KK);
+    for (int round = 0; round < 2; ++round) {
+      std::vector<std::vector<double>> ref(na, std::vector<double>(K, 0.0));
+      if (nb > 0) {
+        const size_t per = (size_t)nb * KK;
+        std::vector<double> C(na * per);
+        Ops->Dots(Storage, nb, W, mask.data(), C.data());
+        for (size_t a = 0; a < na; ++a)
+          for (int b = 0; b < nb; ++b) {
+            double* blk = &C[a * per + (size_t)b * KK];
+            S::FromGram(K, blk, CT.data());
+            for (int p = 0; p < K; ++p)
+              for (int q = 0; q < K; ++q) ref[a][q] += S::Norm(CT[(size_t)p * K + q]);
+            if (b == nb - 1) {
+              std::vector<T> add = round == 0 ? CT : Mul(K, CT, out[a].R);
+              for (int q = 0; q < kk; ++q) out[a].coef[q] += add[q];
+            }
+            for (auto& v : CT) v = -v;
+            S::ToDevice(K, CT.data(), blk);
+          }
+        Ops->Axpy(Storage, nb, W, mask.data(), C.data());
+      }
+      Ops->Dots(W, 1, W, mask.data(), G.data());
+      for (size_t a = 0; a < na; ++a) {
+        S::FromGram(K, &G[a * KK], GT.data());
+        for (int p = 0; p < K; ++p) ref[a][p] += std::real(GT[(size_t)p * K + p]);
+        std::vector<T> Rr;
+        std::vector<char> good;
+        // noise after the first projection is ~1e-15 of the column; after the second, a column
+        // that kept less than 1e-8 of itself was (normalised) noise inside the span
+        CholeskySemidefinite(K, GT.data(), ref[a].data(), round == 0 ? 1e-24 : 1e-16, Rr, good);
+        S::ToDevice(K, PseudoInverse(K, Rr, good).data(), &M[a * KK]);
+        out[a].R = Mul(K, Rr, out[a].R);
+        out[a].good = good;
+      }
+      Ops->Rmul(W, mask.data(), M.data());
+    }
+  }
+
+  // W (nb basis blocks before it) -> orthonormal Q with W_in = V C + Q R per segment of `act`;
+  // deficient columns are replaced by random directions where the segment has room left
+  void Orthonormalize(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
+    const int kk = K * K, KK = KR * KR;
+    TwoRounds(W, nb, act, out);
+    std::vector<int> need, needSeg;
+    for (size_t a = 0; a < act.size(); ++a)
+      if (std::find(out[a].good.begin(), out[a].good.end(), 0) != out[a].good.end()) {
+        need.push_back((int)a);
+        needSeg.push_back(act[a]);
+      }
+    if (need.empty()) return;
+    // W = Q (deficient columns zero) + random vectors in the deficient columns
+    std::vector<unsigned char> mask(Dims.size(), 0);
+    for (int s : needSeg) mask[s] = 1;
+    double* Rnd = Block(MaxBlocks + 1);
+    Ops->Random(Rnd, NextSeed++);
+    std::vector<double> Sel(need.size() * KK, 0.0);
+    for (size_t i = 0; i < need.size(); ++i) {
+      std::vector<T> E((size_t)kk, T(0.0));
+      for (int p = 0; p < K; ++p)
+        if (!out[need[i]].good[p]) E[(size_t)p * K + p] = T(1.0);
+      S::ToDevice(K, E.data(), &Sel[i * KK]);
+    }
+    Ops->Axpy(Rnd, 1, W, mask.data(), Sel.data());
+    // Q_rep = V C' + Q' R'  =>  W_in = V (C + C' R) + Q' (R' R)  (R has zero rows where Q_rep is random)
+    std::vector<Rounds> rep;
+    TwoRounds(W, nb, needSeg, rep);
+    for (size_t i = 0; i < need.size(); ++i) {
+      Rounds& o = out[need[i]];
+      std::vector<T> add = Mul(K, rep[i].coef, o.R);
+      for (int q = 0; q < kk; ++q) o.coef[q] += add[q];
+      o.R = Mul(K, rep[i].R, o.R);
+      o.good = rep[i].good;
+    }
+  }
+
+  BlockOps* Ops;
+  std::vector<int64_t> Dims;
+  int K, KR, MaxBlocks;
+  double* Storage;
+  int64_t BlockDoubles;
+  int StepCount = 0;
+  uint64_t NextSeed = 1;
+  std::vector<std::vector<std::vector<T>>> A, B;
+  std::vector<std::vector<std::vector<char>>> DeadCols;  // per segment, per block X_b, per column
+  std::vector<std::vector<double>> Previous;
+  std::vector<int> LastCheck;
+};
+
+}  // namespace dhh

@@ -59,6 +59,10 @@ ClebschGordanCoefficients::ClebschGordanCoefficients(int j1, int j2) : J1(j1), J
     }
 }
 
+double ClebschGordanCoefficients::Coefficient(int j1, int j2, int m1, int m2, int j) {
+  return racah(j1, m1, j2, m2, j);
+}
+
 double ClebschGordanCoefficients::GetCoefficient(int m1, int m2, int j) const {
   if (m1 < -J1 || m1 > J1 || m2 < -J2 || m2 > J2 || j < 0 || j > J1 + J2) return 0.0;
   int p1 = (J1 + m1) >> 1, p2 = (J2 + m2) >> 1;

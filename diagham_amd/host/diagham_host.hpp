@@ -3,12 +3,15 @@
 // include/dhgpu.h.  Vectors live in device memory for their whole life.
 #pragma once
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/dhgpu.h"
+#include "block_krylov.hpp"
 
 namespace dhh {
 
@@ -22,6 +25,8 @@ class ClebschGordanCoefficients {
   ClebschGordanCoefficients(int j1, int j2);
   // zero outside the allowed range, like GetCoefficient (:150)
   double GetCoefficient(int m1, int m2, int j) const;
+  // one coefficient without building the table
+  static double Coefficient(int j1, int j2, int m1, int m2, int j);
 
  private:
   int J1, J2;
@@ -210,25 +215,23 @@ class FullReorthogonalizedBlockLanczosAlgorithm {
   // Ritz vectors of the nbr lowest eigenvalues into d_out (nbr * dim doubles, vector-major)
   void GetEigenstates(int nbr, double* d_out);
   LanczosResult Solve();
-  int NbrBlocks() const { return (int)A.size(); }
+  int NbrBlocks() const;
 
  private:
+  struct HamiltonianOps;
   void Diagonalize(std::vector<double>& evals, std::vector<double>* evecs);
-  void Orthonormalize(double* d_w, std::vector<double>& R);
   bool Dense() const;
   void DenseSolve();
-  bool Exhausted = false;
   GPUArchitecture* Arch;
   AbstractHamiltonian* Hamiltonian;
   int NbrEigenvalue, BlockSize, MaxIter;
   double Precision, Residual;
   int64_t Dim;
-  double* d_basis = nullptr;  // MaxBlocks+1 blocks of Dim*BlockSize
+  double* d_basis = nullptr;  // MaxBlocks + 2 blocks of Dim * BlockSize (last: scratch)
   int MaxBlocks;
-  double* d_work = nullptr;
-  std::vector<std::vector<double>> A, B;  // diagonal and sub-diagonal k x k blocks
-  std::vector<double> Current, Previous;
-  bool HavePrevious = false;
+  std::unique_ptr<BlockOps> Ops;
+  std::unique_ptr<SegmentedBlockLanczos<double>> Engine;
+  std::vector<double> Current;
 };
 
 // All Lz sectors of one fermions-on-the-sphere problem solved together: the per-sector loop of
@@ -243,6 +246,69 @@ struct SectorResult {
   int products = 0;
   bool converged = false;
 };
+
+// BlockOps over the rows of a sector set: k vectors per block, kr doubles per row (k for real
+// vectors, 2k for complex ones)
+class SectorSetOps : public BlockOps {
+ public:
+  SectorSetOps(GPUArchitecture* arch, dh_sector_set* set, int k, int kr, int64_t rows);
+  void Product(const double* X, double* W, const unsigned char* active) override;
+  void Dots(const double* V, int nb, const double* W, const unsigned char* active, double* out) override;
+  void Axpy(const double* V, int nb, double* W, const unsigned char* active, const double* C) override;
+  void Rmul(double* W, const unsigned char* active, const double* M) override;
+  void Random(double* W, uint64_t seed) override;
+
+ private:
+  GPUArchitecture* Arch;
+  dh_sector_set* Set;
+  int K, KR;
+  int64_t Rows;
+};
+
+// Krylov blocks each sector may use (maxIter / k + 1, and no more than its space needs); returns
+// the largest
+inline int SectorBlockCaps(const std::vector<int64_t>& dims, int k, int maxIter, std::vector<int>& caps) {
+  int most = 2;
+  caps.resize(dims.size());
+  for (size_t i = 0; i < dims.size(); ++i) {
+    caps[i] = (int)std::min<int64_t>(std::max(3, maxIter / k + 1), (dims[i] + k - 1) / k + 1);
+    most = std::max(most, caps[i]);
+  }
+  return most;
+}
+
+// Steps every unconverged sector of `lanczos` together until each has converged (strong test of
+// FullReorthogonalizedLanczosAlgorithm.cc:325), completed its Krylov space or used its blocks.
+template <class T>
+void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>& blockCap, int nbrEigenvalue,
+                      double precision, uint64_t seed, std::vector<SectorResult*>& results) {
+  const int na = (int)results.size();
+  const int k = lanczos.BlockSize();
+  lanczos.Start(seed);
+  std::vector<unsigned char> active(na, 1);
+  std::vector<char> done(na, 0);
+  while (true) {
+    std::vector<int> act;
+    for (int i = 0; i < na; ++i) {
+      active[i] = done[i] ? 0 : 1;
+      if (!done[i]) act.push_back(i);
+    }
+    if (act.empty()) break;
+    lanczos.Step(active);
+    for (int i : act) results[i]->products += k;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t a = 0; a < act.size(); ++a) {
+      const int i = act[a];
+      const bool last = lanczos.Exhausted(i) || lanczos.NbrBlocks(i) >= blockCap[i];
+      if (!last && !lanczos.CheckDue(i, nbrEigenvalue)) continue;
+      std::vector<double> cur;
+      const bool ok = lanczos.TestConvergence(i, nbrEigenvalue, precision, cur);
+      results[i]->eigenvalues = cur;
+      if (ok) results[i]->converged = true;
+      if (ok || last) done[i] = 1;
+    }
+  }
+}
 
 class MultiSectorBlockLanczos {
  public:
@@ -302,11 +368,6 @@ class LatticeSectorsBlockLanczos {
   std::vector<double> Factors;
 };
 
-// Eigenvalues (ascending) of the block tridiagonal Lanczos matrix with diagonal blocks A[0..m)
-// and sub-diagonal blocks B[0..m-1) (k x k, row-major); Ritz coefficients (column-major n x n)
-// if evecs is given.
-void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& B,
-                           int k, int m, std::vector<double>& evals, std::vector<double>* evecs);
 // Symmetric band matrix (half-bandwidth b, dense row-major n x n) -> tridiagonal by Givens
 // rotations with bulge chasing; returns diagonal and off-diagonal, and accumulates Q if given.
 void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>& d, std::vector<double>& e,

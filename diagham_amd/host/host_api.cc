@@ -300,8 +300,7 @@ extern "C" int dhh_coulomb_pseudopotentials(int nbrFlux, int round, double* out)
 }
 
 extern "C" double dhh_clebsch_gordan(int j1, int j2, int m1, int m2, int j) {
-  ClebschGordanCoefficients cg(j1, j2);
-  return cg.GetCoefficient(m1, m2, j);
+  return ClebschGordanCoefficients::Coefficient(j1, j2, m1, m2, j);
 }
 
 extern "C" int dhh_sphere_factors(int lzMax, const double* pseudo, double* factors, int* pairOffset, int* nbr) {

@@ -139,46 +139,32 @@ void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>
   for (int i = 0; i + 1 < n; ++i) e[i] = a[(size_t)(i + 1) * n + i];
 }
 
-// Cholesky G = R^T R (R upper, row-major k x k); false if G is not positive definite enough
-static bool cholesky_upper(int k, const std::vector<double>& G, std::vector<double>& R) {
-  R.assign((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i) {
-    double s = G[(size_t)i * k + i];
-    for (int p = 0; p < i; ++p) s -= R[(size_t)p * k + i] * R[(size_t)p * k + i];
-    if (!(s > 1e-28 * std::max(1.0, std::fabs(G[(size_t)i * k + i])))) return false;
-    double rii = std::sqrt(s);
-    R[(size_t)i * k + i] = rii;
-    for (int j = i + 1; j < k; ++j) {
-      double t = G[(size_t)i * k + j];
-      for (int p = 0; p < i; ++p) t -= R[(size_t)p * k + i] * R[(size_t)p * k + j];
-      R[(size_t)i * k + j] = t / rii;
-    }
-  }
-  return true;
-}
-
-static std::vector<double> upper_inverse(int k, const std::vector<double>& R) {
-  std::vector<double> X((size_t)k * k, 0.0);
-  for (int j = 0; j < k; ++j) {
-    X[(size_t)j * k + j] = 1.0 / R[(size_t)j * k + j];
-    for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int p = i + 1; p <= j; ++p) s += R[(size_t)i * k + p] * X[(size_t)p * k + j];
-      X[(size_t)i * k + j] = -s / R[(size_t)i * k + i];
-    }
-  }
-  return X;
-}
-
-static std::vector<double> matmul(int k, const std::vector<double>& A, const std::vector<double>& B) {
-  std::vector<double> C((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i)
-    for (int p = 0; p < k; ++p)
-      for (int j = 0; j < k; ++j) C[(size_t)i * k + j] += A[(size_t)i * k + p] * B[(size_t)p * k + j];
-  return C;
-}
-
 // ------------------------------------------------------------------ block Lanczos
+// the block operations of one Hilbert space: its Hamiltonian's product and the block BLAS
+struct FullReorthogonalizedBlockLanczosAlgorithm::HamiltonianOps : BlockOps {
+  GPUArchitecture* Arch;
+  AbstractHamiltonian* Hamiltonian;
+  int64_t Dim;
+  int K;
+  HamiltonianOps(GPUArchitecture* a, AbstractHamiltonian* h, int64_t dim, int k)
+      : Arch(a), Hamiltonian(h), Dim(dim), K(k) {}
+  void Product(const double* X, double* W, const unsigned char*) override {
+    Arch->Check(dh_hv(Hamiltonian->Handle(), X, W, K, 0, 0, -1), "H.v");
+  }
+  void Dots(const double* V, int nb, const double* W, const unsigned char*, double* out) override {
+    Arch->Check(dh_block_dots(Arch->Context(), V, nb, W, Dim, K, out), "block dots");
+  }
+  void Axpy(const double* V, int nb, double* W, const unsigned char*, const double* C) override {
+    Arch->Check(dh_block_axpy(Arch->Context(), V, nb, W, Dim, K, C), "block axpy");
+  }
+  void Rmul(double* W, const unsigned char*, const double* M) override {
+    Arch->Check(dh_block_rmul(Arch->Context(), W, Dim, K, M), "block rmul");
+  }
+  void Random(double* W, uint64_t seed) override {
+    Arch->Check(dh_fill_random(Arch->Context(), W, Dim * K, seed), "random block");
+  }
+};
+
 FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgorithm(
     GPUArchitecture* architecture, AbstractHamiltonian* hamiltonian, int nbrEigenvalue, int blockSize, int maxIter,
     double precision, double residual)
@@ -192,99 +178,39 @@ FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgor
   Dim = hamiltonian->GetHilbertSpaceDimension();
   if (BlockSize != 1 && BlockSize != 2 && BlockSize != 4 && BlockSize != 8 && BlockSize != 16)
     throw std::invalid_argument("block size must be 1, 2, 4, 8 or 16");
-  MaxBlocks = std::max(3, MaxIter / BlockSize + 1);
-  if ((int64_t)MaxBlocks * BlockSize > Dim) MaxBlocks = (int)std::max<int64_t>(2, Dim / BlockSize);
-  d_basis = Arch->AllocateMemory((int64_t)(MaxBlocks + 1) * Dim * BlockSize);
+  // the Krylov space of the sector is complete after at most ceil(Dim / k) blocks
+  MaxBlocks = (int)std::min<int64_t>(std::max(3, MaxIter / BlockSize + 1), (Dim + BlockSize - 1) / BlockSize + 1);
+  if (!Dense()) {
+    d_basis = Arch->AllocateMemory((int64_t)(MaxBlocks + 2) * Dim * BlockSize);
+    Ops.reset(new HamiltonianOps(Arch, Hamiltonian, Dim, BlockSize));
+    Engine.reset(new SegmentedBlockLanczos<double>(Ops.get(), std::vector<int64_t>{Dim}, BlockSize, MaxBlocks,
+                                                   d_basis, Dim * BlockSize));
+  }
 }
 
 FullReorthogonalizedBlockLanczosAlgorithm::~FullReorthogonalizedBlockLanczosAlgorithm() {
-  Arch->FreeMemory(d_basis);
-}
-
-// W <- W R^{-1} with W = (new W) R, by two Cholesky-QR passes; false on rank deficiency
-void FullReorthogonalizedBlockLanczosAlgorithm::Orthonormalize(double* d_w, std::vector<double>& R) {
-  const int k = BlockSize;
-  dh_context* c = Arch->Context();
-  R.assign((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i) R[(size_t)i * k + i] = 1.0;
-  for (int pass = 0; pass < 2; ++pass) {
-    std::vector<double> G((size_t)k * k), Rp;
-    Arch->Check(dh_block_dots(c, d_w, 1, d_w, Dim, k, G.data()), "gram");
-    if (!cholesky_upper(k, G, Rp)) {
-      R.clear();
-      return;
-    }
-    std::vector<double> inv = upper_inverse(k, Rp);
-    Arch->Check(dh_block_rmul(c, d_w, Dim, k, inv.data()), "rmul");
-    R = matmul(k, Rp, R);
-  }
+  if (d_basis) Arch->FreeMemory(d_basis);
 }
 
 void FullReorthogonalizedBlockLanczosAlgorithm::InitializeLanczosAlgorithm(uint64_t seed) {
-  dh_context* c = Arch->Context();
-  A.clear();
-  B.clear();
-  HavePrevious = false;
-  Exhausted = false;
+  Current.clear();
   if (Dense()) return;
-  Arch->Check(dh_fill_random(c, d_basis, Dim * BlockSize, seed), "random start");
-  std::vector<double> R;
-  Orthonormalize(d_basis, R);
-  if (R.empty()) throw std::runtime_error("random start block is rank deficient");
+  Engine->Start(seed);
 }
 
 void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter) {
-  const int k = BlockSize;
-  const int kk = k * k;
-  dh_context* c = Arch->Context();
+  const std::vector<unsigned char> all(1, 1);
   for (int it = 0; it < nbrIter; ++it) {
-    int j = (int)A.size();  // current block X_j
-    if (j + 1 > MaxBlocks) return;
-    double* Xj = d_basis + (int64_t)j * Dim * k;
-    double* W = d_basis + (int64_t)(j + 1) * Dim * k;
-    Arch->Check(dh_hv(Hamiltonian->Handle(), Xj, W, k, 0, 0, -1), "H.v");
-    std::vector<double> Aj((size_t)kk, 0.0);
-    std::vector<double> C((size_t)(j + 1) * kk);
-    for (int pass = 0; pass < 2; ++pass) {
-      Arch->Check(dh_block_dots(c, d_basis, j + 1, W, Dim, k, C.data()), "reorth dots");
-      for (int i = 0; i < kk; ++i) Aj[i] += C[(size_t)j * kk + i];
-      for (auto& v : C) v = -v;
-      Arch->Check(dh_block_axpy(c, d_basis, j + 1, W, Dim, k, C.data()), "reorth axpy");
-    }
-    for (int p = 0; p < k; ++p)
-      for (int q = p + 1; q < k; ++q) {
-        double s = 0.5 * (Aj[(size_t)p * k + q] + Aj[(size_t)q * k + p]);
-        Aj[(size_t)p * k + q] = Aj[(size_t)q * k + p] = s;
-      }
-    std::vector<double> R;
-    Orthonormalize(W, R);
-    if (R.empty() && (int64_t)(j + 1) * k >= Dim) {
-      // the Krylov space is the whole space: the block tridiagonal matrix is exact
-      Exhausted = true;
-      A.push_back(Aj);
-      B.push_back(std::vector<double>((size_t)kk, 0.0));
-      return;
-    }
-    if (R.empty()) {
-      // invariant subspace reached: restart with a random block orthogonal to the basis
-      // (the subspace-collapse branch of FullReorthogonalizedLanczosAlgorithm.cc:270)
-      Arch->Check(dh_fill_random(c, W, Dim * k, 0x5EEDull + 7919ull * (uint64_t)(j + 1)), "restart");
-      for (int pass = 0; pass < 2; ++pass) {
-        Arch->Check(dh_block_dots(c, d_basis, j + 1, W, Dim, k, C.data()), "restart dots");
-        for (auto& v : C) v = -v;
-        Arch->Check(dh_block_axpy(c, d_basis, j + 1, W, Dim, k, C.data()), "restart axpy");
-      }
-      Orthonormalize(W, R);
-      if (R.empty()) throw std::runtime_error("Lanczos restart failed");
-      R.assign((size_t)kk, 0.0);
-    }
-    A.push_back(Aj);
-    B.push_back(R);
+    if (Engine->Steps() > 0 && Engine->Exhausted(0)) return;
+    if (Engine->Steps() + 1 > MaxBlocks) return;
+    Engine->Step(all);
   }
 }
 
+int FullReorthogonalizedBlockLanczosAlgorithm::NbrBlocks() const { return Engine ? Engine->NbrBlocks(0) : 0; }
+
 void FullReorthogonalizedBlockLanczosAlgorithm::Diagonalize(std::vector<double>& evals, std::vector<double>* evecs) {
-  BlockTridiagonalEigen(A, B, BlockSize, (int)A.size(), evals, evecs);
+  Engine->Eigen(0, evals, evecs);
 }
 
 void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& B,
@@ -331,29 +257,17 @@ void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std:
 }
 
 bool FullReorthogonalizedBlockLanczosAlgorithm::TestConvergence() {
-  std::vector<double> ev;
-  Diagonalize(ev, nullptr);
-  int n = (int)ev.size();
-  int want = std::min(NbrEigenvalue, n);
-  std::vector<double> cur(ev.begin(), ev.begin() + want);
-  bool ok = HavePrevious && n > NbrEigenvalue && (int)Previous.size() == want;
-  if (ok)
-    for (int i = 0; i < want; ++i)
-      if (std::fabs(cur[i] - Previous[i]) > Precision * std::fabs(cur[i])) {
-        ok = false;
-        break;
-      }
-  Previous = cur;
-  Current = cur;
-  HavePrevious = true;
-  if (ok && Residual > 0.0) {
+  bool ok = Engine->TestConvergence(0, NbrEigenvalue, Precision, Current);
+  const int want = (int)Current.size();
+  if (ok && Residual > 0.0 && !Engine->Exhausted(0)) {
     // |H v - theta v| = |B_last * (last k coefficients of the Ritz vector)|: the pending
     // block X_m times B[m-1] is what the Krylov space has not yet absorbed
     const int k = BlockSize;
-    const int m = (int)A.size();
+    const int m = Engine->NbrBlocks(0);
+    const int n = m * k;
     std::vector<double> ev2, vecs;
     Diagonalize(ev2, &vecs);
-    const std::vector<double>& Bl = B[m - 1];
+    const std::vector<double>& Bl = Engine->Coupling(0)[m - 1];
     for (int i = 0; i < want && ok; ++i) {
       double r2 = 0.0;
       for (int p = 0; p < k; ++p) {
@@ -368,13 +282,14 @@ bool FullReorthogonalizedBlockLanczosAlgorithm::TestConvergence() {
 }
 
 void FullReorthogonalizedBlockLanczosAlgorithm::GetEigenstates(int nbr, double* d_out) {
+  if (Dense()) throw std::invalid_argument("no Krylov basis: the space was diagonalised densely");
   const int k = BlockSize;
-  const int m = (int)A.size();
+  const int m = Engine->NbrBlocks(0);
   const int n = m * k;
   std::vector<double> ev, vecs;
   Diagonalize(ev, &vecs);
   dh_context* c = Arch->Context();
-  nbr = std::min(nbr, n);
+  nbr = std::min(nbr, (int)ev.size());
   // Ritz vectors in groups of k: block [Dim][k] = V * S, then copied out column by column
   double* tmp = Arch->AllocateMemory(Dim * k);
   for (int g0 = 0; g0 < nbr; g0 += k) {
@@ -420,13 +335,6 @@ void FullReorthogonalizedBlockLanczosAlgorithm::DenseSolve() {
   Current.assign(w.begin(), w.begin() + std::min<int>(NbrEigenvalue, n));
 }
 
-// convergence checks of a block Lanczos cost a band reduction; past the first ones they are
-// spaced so that at most ~10% extra blocks are run after convergence
-static bool check_due(int blocks, int lastCheck, int blockSize) {
-  if (blockSize == 1 || lastCheck < 0) return true;
-  return blocks - lastCheck >= std::max(1, blocks / 10);
-}
-
 LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
   auto t0 = std::chrono::steady_clock::now();
   LanczosResult res;
@@ -440,30 +348,25 @@ LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
   }
   int first = std::max(2, (NbrEigenvalue + 2 + BlockSize - 1) / BlockSize);
   RunLanczosAlgorithm(first);
-  int lastCheck = -1;
   while (true) {
-    if (Exhausted) {
-      std::vector<double> ev;
-      Diagonalize(ev, nullptr);
-      Current.assign(ev.begin(), ev.begin() + std::min<int>(NbrEigenvalue, (int)ev.size()));
+    if (Engine->CheckDue(0, NbrEigenvalue) && TestConvergence()) {
       res.converged = true;
       break;
     }
-    if (check_due((int)A.size(), lastCheck, BlockSize)) {
-      lastCheck = (int)A.size();
-      if (TestConvergence()) {
-        res.converged = true;
-        break;
-      }
+    if (Engine->Exhausted(0) || NbrBlocks() >= MaxBlocks) {
+      // out of iterations: report the current Ritz values unconverged
+      std::vector<double> ev;
+      Diagonalize(ev, nullptr);
+      Current.assign(ev.begin(), ev.begin() + std::min<int>(NbrEigenvalue, (int)ev.size()));
+      break;
     }
-    if ((int)A.size() >= MaxBlocks) break;
     RunLanczosAlgorithm(1);
   }
   Arch->Check(dh_synchronize(Arch->Context()), "sync");
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   res.eigenvalues = Current;
-  res.blockIterations = (int)A.size();
-  res.iterations = (int)A.size() * BlockSize;
+  res.blockIterations = NbrBlocks();
+  res.iterations = NbrBlocks() * BlockSize;
   return res;
 }
 

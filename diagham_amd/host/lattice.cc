@@ -118,72 +118,11 @@ void CheckerboardSingleBandFactors(int nx, int ny, const std::vector<cplx>& u, d
   }
 }
 
-namespace {
-bool cholesky_hermitian(int k, const std::vector<cplx>& G, std::vector<cplx>& R) {
-  R.assign((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i) {
-    double s = G[(size_t)i * k + i].real();
-    for (int p = 0; p < i; ++p) s -= std::norm(R[(size_t)p * k + i]);
-    if (!(s > 1e-28 * std::max(1.0, std::abs(G[(size_t)i * k + i])))) return false;
-    double rii = std::sqrt(s);
-    R[(size_t)i * k + i] = rii;
-    for (int j = i + 1; j < k; ++j) {
-      cplx t = G[(size_t)i * k + j];
-      for (int p = 0; p < i; ++p) t -= std::conj(R[(size_t)p * k + i]) * R[(size_t)p * k + j];
-      R[(size_t)i * k + j] = t / rii;
-    }
-  }
-  return true;
-}
-
-std::vector<cplx> upper_inverse_c(int k, const std::vector<cplx>& R) {
-  std::vector<cplx> X((size_t)k * k, 0.0);
-  for (int j = 0; j < k; ++j) {
-    X[(size_t)j * k + j] = 1.0 / R[(size_t)j * k + j];
-    for (int i = j - 1; i >= 0; --i) {
-      cplx s = 0.0;
-      for (int p = i + 1; p <= j; ++p) s += R[(size_t)i * k + p] * X[(size_t)p * k + j];
-      X[(size_t)i * k + j] = -s / R[(size_t)i * k + i];
-    }
-  }
-  return X;
-}
-
-std::vector<cplx> matmul_c(int k, const std::vector<cplx>& A, const std::vector<cplx>& B) {
-  std::vector<cplx> C((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i)
-    for (int p = 0; p < k; ++p)
-      for (int j = 0; j < k; ++j) C[(size_t)i * k + j] += A[(size_t)i * k + p] * B[(size_t)p * k + j];
-  return C;
-}
-
-// real (2k x 2k) matrix acting on interleaved (re, im) rows: w_q += sum_p v_p c_pq
-void realify(int k, const cplx* C, double* M) {
-  const int K2 = 2 * k;
-  for (int p = 0; p < k; ++p)
-    for (int q = 0; q < k; ++q) {
-      cplx c = C[(size_t)p * k + q];
-      M[(size_t)(2 * p) * K2 + 2 * q] = c.real();
-      M[(size_t)(2 * p) * K2 + 2 * q + 1] = c.imag();
-      M[(size_t)(2 * p + 1) * K2 + 2 * q] = -c.imag();
-      M[(size_t)(2 * p + 1) * K2 + 2 * q + 1] = c.real();
-    }
-}
-
-// complex k x k block of V^H W from the real (2k x 2k) Gram block G of the interleaved rows
-void complexify(int k, const double* G, cplx* C) {
-  const int K2 = 2 * k;
-  for (int p = 0; p < k; ++p)
-    for (int q = 0; q < k; ++q) {
-      double rr = G[(size_t)(2 * p) * K2 + 2 * q], ii = G[(size_t)(2 * p + 1) * K2 + 2 * q + 1];
-      double ri = G[(size_t)(2 * p) * K2 + 2 * q + 1], ir = G[(size_t)(2 * p + 1) * K2 + 2 * q];
-      C[(size_t)p * k + q] = cplx(rr + ii, ri - ir);
-    }
-}
-
 // eigenvalues of the complex Hermitian block tridiagonal matrix, through its real 2n x 2n form
-void complex_block_eigen(const std::vector<std::vector<cplx>>& A, const std::vector<std::vector<cplx>>& B, int k,
-                         int m, std::vector<double>& evals) {
+// (every eigenvalue appears twice there)
+void BlockTridiagonalEigen(const std::vector<std::vector<cplx>>& A, const std::vector<std::vector<cplx>>& B, int k,
+                           int m, std::vector<double>& evals, std::vector<cplx>* evecs) {
+  if (evecs) throw std::invalid_argument("complex Ritz vectors are not implemented");
   const int n = m * k, n2 = 2 * n;
   std::vector<double> T((size_t)n2 * n2, 0.0);
   auto put = [&](int i, int j, cplx v) {  // complex entry (i, j) -> real 2 x 2 block
@@ -206,9 +145,8 @@ void complex_block_eigen(const std::vector<std::vector<cplx>>& A, const std::vec
   BandToTridiagonal(n2, 2 * k + 1, T, d, e, nullptr);
   TridiagonalEigen(n2, d, e, w, nullptr);
   evals.resize(n);
-  for (int i = 0; i < n; ++i) evals[i] = w[2 * i];  // every eigenvalue appears twice
+  for (int i = 0; i < n; ++i) evals[i] = w[2 * i];
 }
-}  // namespace
 
 LatticeSectorsBlockLanczos::LatticeSectorsBlockLanczos(GPUArchitecture* architecture, int nbrFermions, int nx, int ny,
                                                        const std::vector<int>& kx, const std::vector<int>& ky,
@@ -258,7 +196,7 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
   Waves = 0;
   for (int s = 0; s < ns; ++s) {
     out[s].dimension = dims[s];
-    if (dims[s] <= 4 * k) throw std::invalid_argument("momentum sector too small for the block size");
+    if (dims[s] <= 0) throw std::invalid_argument("empty momentum sector");
     size_t need = (size_t)dims[s] * 2 * k * (maxBlocks + 2) * sizeof(double);
     if (!wave.empty() && used + need > MemoryBytes) {
       SolveWave(wave, seed, out);
@@ -280,7 +218,7 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
 void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_t seed,
                                            std::vector<SectorResult>& out) {
   dh_context* c = Arch->Context();
-  const int k = BlockSize, kk = k * k, K2 = 2 * k, KK2 = K2 * K2;
+  const int k = BlockSize;
   const int na = (int)sectors.size();
   std::vector<int> kx, ky;
   for (int s : sectors) {
@@ -297,135 +235,18 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   dh_sector_set_offsets(S, off.data());
   const int64_t total = off.back();
   Arch->Check(dh_sector_set_launch(S, total < 4ll * 256 * 256 ? 4 : 1, 256), "launch");
-  int maxBlocks = 2;
-  std::vector<int> blockCap(na);
-  for (int i = 0; i < na; ++i) {
-    int64_t d = off[i + 1] - off[i];
-    blockCap[i] = (int)std::min<int64_t>(MaxIter / k + 1, (d + k - 1) / k + 1);
-    maxBlocks = std::max(maxBlocks, blockCap[i]);
-  }
-  const int64_t blockDoubles = total * K2;
-  double* basis = Arch->AllocateMemory((int64_t)(maxBlocks + 1) * blockDoubles);
-  auto blockPtr = [&](int j) { return basis + (int64_t)j * blockDoubles; };
-  Arch->Check(dh_fill_random(c, basis, blockDoubles, seed), "random start");
-
-  std::vector<unsigned char> active(na, 1);
-  std::vector<std::vector<std::vector<cplx>>> A(na), B(na);
-  std::vector<std::vector<double>> prev(na);
-  std::vector<int> lastCheck(na, -1);
-  std::vector<char> done(na, 0), exhausted(na, 0);
-
-  auto orthonormalize = [&](double* W, std::vector<std::vector<cplx>>& R) {
-    std::vector<int> act;
-    for (int i = 0; i < na; ++i)
-      if (active[i]) act.push_back(i);
-    R.assign(na, std::vector<cplx>());
-    for (int i : act) {
-      R[i].assign((size_t)kk, 0.0);
-      for (int p = 0; p < k; ++p) R[i][(size_t)p * k + p] = 1.0;
-    }
-    for (int pass = 0; pass < 2; ++pass) {
-      std::vector<double> G((size_t)act.size() * KK2);
-      Arch->Check(dh_sectors_block_dots(S, W, 1, W, K2, active.data(), G.data()), "gram");
-      std::vector<double> M((size_t)act.size() * KK2, 0.0);
-      for (size_t a = 0; a < act.size(); ++a) {
-        int i = act[a];
-        std::vector<cplx> Gc((size_t)kk), Rp;
-        complexify(k, &G[a * KK2], Gc.data());
-        if (!R[i].empty() && cholesky_hermitian(k, Gc, Rp)) {
-          std::vector<cplx> inv = upper_inverse_c(k, Rp);
-          realify(k, inv.data(), &M[a * KK2]);
-          R[i] = matmul_c(k, Rp, R[i]);
-        } else {
-          R[i].clear();
-          for (int p = 0; p < K2; ++p) M[a * KK2 + (size_t)p * K2 + p] = 1.0;
-        }
-      }
-      Arch->Check(dh_sectors_block_rmul(S, W, K2, active.data(), M.data()), "rmul");
-    }
-  };
-
+  std::vector<int64_t> dims(na);
+  for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
+  std::vector<int> blockCap;
+  const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
+  const int64_t blockDoubles = total * 2 * k;
+  double* basis = Arch->AllocateMemory((int64_t)(maxBlocks + 2) * blockDoubles);
   {
-    std::vector<std::vector<cplx>> R;
-    orthonormalize(blockPtr(0), R);
-    for (int i = 0; i < na; ++i)
-      if (R[i].empty()) throw std::runtime_error("random start block is rank deficient");
-  }
-  for (int j = 0;; ++j) {
-    std::vector<int> act;
-    for (int i = 0; i < na; ++i) active[i] = done[i] ? 0 : 1;
-    for (int i = 0; i < na; ++i)
-      if (!done[i]) act.push_back(i);
-    if (act.empty() || j + 1 > maxBlocks) break;
-    double* X = blockPtr(j);
-    double* W = blockPtr(j + 1);
-    Arch->Check(dh_sectors_hv(S, X, W, k, 0, active.data()), "lattice products");
-    const size_t per = (size_t)(j + 1) * KK2;
-    std::vector<double> G(act.size() * per), M(act.size() * per);
-    std::vector<std::vector<cplx>> Aj(na, std::vector<cplx>((size_t)kk, 0.0));
-    for (int pass = 0; pass < 2; ++pass) {
-      Arch->Check(dh_sectors_block_dots(S, basis, j + 1, W, K2, active.data(), G.data()), "reorth dots");
-      for (size_t a = 0; a < act.size(); ++a)
-        for (int b = 0; b <= j; ++b) {
-          std::vector<cplx> C((size_t)kk);
-          complexify(k, &G[a * per + (size_t)b * KK2], C.data());
-          if (b == j)
-            for (int q = 0; q < kk; ++q) Aj[act[a]][q] += C[q];
-          for (auto& v : C) v = -v;
-          realify(k, C.data(), &M[a * per + (size_t)b * KK2]);
-        }
-      Arch->Check(dh_sectors_block_axpy(S, basis, j + 1, W, K2, active.data(), M.data()), "reorth axpy");
-    }
-    std::vector<std::vector<cplx>> R;
-    orthonormalize(W, R);
-    for (int i : act) {
-      auto& a = Aj[i];
-      for (int p = 0; p < k; ++p) {
-        a[(size_t)p * k + p] = a[(size_t)p * k + p].real();
-        for (int q = p + 1; q < k; ++q) {
-          cplx s = 0.5 * (a[(size_t)p * k + q] + std::conj(a[(size_t)q * k + p]));
-          a[(size_t)p * k + q] = s;
-          a[(size_t)q * k + p] = std::conj(s);
-        }
-      }
-      A[i].push_back(a);
-      if (R[i].empty()) {
-        exhausted[i] = 1;
-        B[i].push_back(std::vector<cplx>((size_t)kk, 0.0));
-      } else {
-        B[i].push_back(R[i]);
-      }
-      out[sectors[i]].products += k;
-    }
-#pragma omp parallel for schedule(dynamic, 1)
-    for (size_t a = 0; a < act.size(); ++a) {
-      int i = act[a];
-      int m = (int)A[i].size();
-      bool last = exhausted[i] || m >= blockCap[i];
-      if (!last && !(k == 1 || lastCheck[i] < 0 || m - lastCheck[i] >= std::max(1, m / 10))) continue;
-      if (!last && m * k < NbrEigenvalue + 2) continue;
-      lastCheck[i] = m;
-      std::vector<double> ev;
-      complex_block_eigen(A[i], B[i], k, m, ev);
-      int want = std::min<int>(NbrEigenvalue, (int)ev.size());
-      std::vector<double> cur(ev.begin(), ev.begin() + want);
-      bool ok = !prev[i].empty() && (int)ev.size() > NbrEigenvalue && (int)prev[i].size() == want;
-      if (ok)
-        for (int e = 0; e < want; ++e)
-          if (std::fabs(cur[e] - prev[i][e]) > Precision * std::fabs(cur[e])) {
-            ok = false;
-            break;
-          }
-      prev[i] = cur;
-      SectorResult& r = out[sectors[i]];
-      r.eigenvalues = cur;
-      if (ok || (exhausted[i] && (int64_t)m * k >= off[i + 1] - off[i])) {
-        r.converged = true;
-        done[i] = 1;
-      } else if (last) {
-        done[i] = 1;
-      }
-    }
+    SectorSetOps ops(Arch, S, k, 2 * k, total);
+    SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, basis, blockDoubles);
+    std::vector<SectorResult*> res;
+    for (int s : sectors) res.push_back(&out[s]);
+    RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, res);
   }
   Arch->Check(dh_synchronize(c), "sync");
   Arch->FreeMemory(basis);

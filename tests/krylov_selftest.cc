@@ -1,0 +1,169 @@
+// CPU check of the block Krylov recurrence of the host layer (SegmentedBlockLanczos +
+// RunSectorLanczos) on dense matrices held in host memory: the device operations are replaced by
+// plain loops over the same block layouts.  Reads a problem description from argv[1] and prints,
+// per segment, "converged products e0 e1 ...".
+//
+// input: complex k maxIter nev precision seed nseg, then nseg dimensions, then per segment the
+// dense matrix row-major (real: d*d values; complex: d*d (re, im) pairs)
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "diagham_host.hpp"
+
+using namespace dhh;
+
+namespace {
+
+struct HostOps : BlockOps {
+  std::vector<int64_t> off;
+  std::vector<std::vector<double>> H;  // per segment, row-major (complex: interleaved)
+  bool cplxMode;
+  int k, kr;
+  int64_t rows;
+
+  std::vector<int> Active(const unsigned char* active) const {
+    std::vector<int> a;
+    for (size_t s = 0; s + 1 < off.size(); ++s)
+      if (active[s]) a.push_back((int)s);
+    return a;
+  }
+  void Product(const double* X, double* W, const unsigned char* active) override {
+    for (int s : Active(active)) {
+      const int64_t o = off[s], d = off[s + 1] - off[s];
+      for (int64_t r = 0; r < d; ++r)
+        for (int q = 0; q < k; ++q) {
+          if (!cplxMode) {
+            double acc = 0.0;
+            for (int64_t c = 0; c < d; ++c) acc += H[s][r * d + c] * X[(o + c) * kr + q];
+            W[(o + r) * kr + q] = acc;
+          } else {
+            double re = 0.0, im = 0.0;
+            for (int64_t c = 0; c < d; ++c) {
+              const double hr = H[s][2 * (r * d + c)], hi = H[s][2 * (r * d + c) + 1];
+              const double xr = X[(o + c) * kr + 2 * q], xi = X[(o + c) * kr + 2 * q + 1];
+              re += hr * xr - hi * xi;
+              im += hr * xi + hi * xr;
+            }
+            W[(o + r) * kr + 2 * q] = re;
+            W[(o + r) * kr + 2 * q + 1] = im;
+          }
+        }
+    }
+  }
+  void Dots(const double* V, int nb, const double* W, const unsigned char* active, double* out) override {
+    const int KK = kr * kr;
+    size_t a = 0;
+    for (int s : Active(active)) {
+      for (int b = 0; b < nb; ++b)
+        for (int p = 0; p < kr; ++p)
+          for (int q = 0; q < kr; ++q) {
+            double acc = 0.0;
+            for (int64_t r = off[s]; r < off[s + 1]; ++r) acc += V[(int64_t)b * rows * kr + r * kr + p] * W[r * kr + q];
+            out[a * nb * KK + (size_t)b * KK + p * kr + q] = acc;
+          }
+      ++a;
+    }
+  }
+  void Axpy(const double* V, int nb, double* W, const unsigned char* active, const double* C) override {
+    const int KK = kr * kr;
+    size_t a = 0;
+    for (int s : Active(active)) {
+      for (int64_t r = off[s]; r < off[s + 1]; ++r)
+        for (int q = 0; q < kr; ++q) {
+          double acc = 0.0;
+          for (int b = 0; b < nb; ++b)
+            for (int p = 0; p < kr; ++p)
+              acc += V[(int64_t)b * rows * kr + r * kr + p] * C[a * nb * KK + (size_t)b * KK + p * kr + q];
+          W[r * kr + q] += acc;
+        }
+      ++a;
+    }
+  }
+  void Rmul(double* W, const unsigned char* active, const double* M) override {
+    const int KK = kr * kr;
+    size_t a = 0;
+    std::vector<double> row(kr);
+    for (int s : Active(active)) {
+      for (int64_t r = off[s]; r < off[s + 1]; ++r) {
+        for (int q = 0; q < kr; ++q) {
+          double acc = 0.0;
+          for (int p = 0; p < kr; ++p) acc += W[r * kr + p] * M[a * KK + p * kr + q];
+          row[q] = acc;
+        }
+        for (int q = 0; q < kr; ++q) W[r * kr + q] = row[q];
+      }
+      ++a;
+    }
+  }
+  void Random(double* W, uint64_t seed) override {
+    uint64_t x = seed;
+    for (int64_t i = 0; i < rows * kr; ++i) {
+      uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      W[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+  }
+};
+
+template <class T>
+void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed) {
+  std::vector<int64_t> dims;
+  for (size_t s = 0; s + 1 < ops.off.size(); ++s) dims.push_back(ops.off[s + 1] - ops.off[s]);
+  std::vector<int> caps;
+  const int maxBlocks = SectorBlockCaps(dims, ops.k, maxIter, caps);
+  std::vector<double> storage((size_t)(maxBlocks + 2) * ops.rows * ops.kr, 0.0);
+  SegmentedBlockLanczos<T> lanczos(&ops, dims, ops.k, maxBlocks, storage.data(), ops.rows * ops.kr);
+  std::vector<SectorResult> out(dims.size());
+  std::vector<SectorResult*> res;
+  for (auto& r : out) res.push_back(&r);
+  RunSectorLanczos(lanczos, caps, nev, precision, seed, res);
+  for (auto& r : out) {
+    std::printf("%d %d", r.converged ? 1 : 0, r.products);
+    for (double e : r.eigenvalues) std::printf(" %.17g", e);
+    std::printf("\n");
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) return 2;
+  FILE* f = std::fopen(argv[1], "r");
+  if (!f) return 2;
+  int cplxMode, k, maxIter, nev, nseg;
+  double precision;
+  unsigned long long seed;
+  if (std::fscanf(f, "%d %d %d %d %lf %llu %d", &cplxMode, &k, &maxIter, &nev, &precision, &seed, &nseg) != 7) return 2;
+  HostOps ops;
+  ops.cplxMode = cplxMode != 0;
+  ops.k = k;
+  ops.kr = cplxMode ? 2 * k : k;
+  ops.off.assign(1, 0);
+  for (int s = 0; s < nseg; ++s) {
+    long long d;
+    if (std::fscanf(f, "%lld", &d) != 1) return 2;
+    ops.off.push_back(ops.off.back() + d);
+  }
+  ops.rows = ops.off.back();
+  ops.H.resize(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    const int64_t d = ops.off[s + 1] - ops.off[s];
+    ops.H[s].resize((size_t)(d * d * (cplxMode ? 2 : 1)));
+    for (auto& v : ops.H[s])
+      if (std::fscanf(f, "%lf", &v) != 1) return 2;
+  }
+  std::fclose(f);
+  try {
+    if (cplxMode)
+      Run<cplx>(ops, maxIter, nev, precision, seed);
+    else
+      Run<double>(ops, maxIter, nev, precision, seed);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

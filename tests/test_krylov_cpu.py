@@ -1,0 +1,142 @@
+"""CPU checks of the host layer's block Krylov recurrence (rank deficiency, small and degenerate
+spaces, complex Hermitian blocks): the recurrence is compiled against the host library with a
+host-memory implementation of its block operations and compared with numpy's dense eigensolver."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def selftest(tmp_path_factory):
+    lib = os.path.join(ROOT, "diagham_amd", "libdiagham_host.so")
+    if not os.path.exists(lib):
+        pytest.skip("build the native libraries first")
+    exe = str(tmp_path_factory.mktemp("krylov") / "krylov_selftest")
+    libdir = os.path.join(ROOT, "diagham_amd")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I", os.path.join(ROOT, "diagham_amd", "host"),
+                    os.path.join(ROOT, "tests", "krylov_selftest.cc"), "-o", exe, "-L", libdir, "-ldiagham_host",
+                    "-Wl,-rpath," + libdir], check=True)
+    return exe
+
+
+def run(exe, tmp_path, mats, k, nev, precision=1e-14, max_iter=4000, seed=11):
+    cplx = any(np.iscomplexobj(m) for m in mats)
+    lines = ["%d %d %d %d %.17g %d %d" % (cplx, k, max_iter, nev, precision, seed, len(mats)),
+             " ".join(str(m.shape[0]) for m in mats)]
+    for m in mats:
+        if cplx:
+            m = np.asarray(m, complex)
+            flat = np.empty(2 * m.size)
+            flat[0::2] = m.real.ravel()
+            flat[1::2] = m.imag.ravel()
+        else:
+            flat = m.ravel()
+        lines.append(" ".join(repr(float(v)) for v in flat))
+    path = tmp_path / "problem.txt"
+    path.write_text("\n".join(lines) + "\n")
+    out = subprocess.run([exe, str(path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 0, out.stderr
+    res = []
+    for line in out.stdout.strip().splitlines():
+        v = line.split()
+        res.append((int(v[0]), int(v[1]), np.array([float(x) for x in v[2:]])))
+    return res
+
+
+def random_hermitian(rng, d, cplx=False):
+    a = rng.standard_normal((d, d))
+    if cplx:
+        a = a + 1j * rng.standard_normal((d, d))
+    return (a + a.conj().T) / 2
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("k", [1, 4, 8, 16])
+def test_sectors_of_every_size(selftest, tmp_path, k, cplx):
+    """Sectors from 1 state to well above the block size converge to the dense spectrum, also when
+    the Krylov space fills a sector in the middle of a block."""
+    rng = np.random.default_rng(k + 7 * cplx)
+    dims = [1, 2, 3, 5, 9, 17, 33, 70, 130]
+    mats = [random_hermitian(rng, d, cplx) for d in dims]
+    nev = 5
+    res = run(selftest, tmp_path, mats, k, nev)
+    for m, (conv, prods, ev) in zip(mats, res):
+        ref = np.linalg.eigvalsh(m)[:nev]
+        assert conv == 1, m.shape
+        assert ev.size == ref.size
+        assert np.max(np.abs(ev - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref))), (m.shape, ev, ref)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("k", [2, 4, 8])
+def test_run_to_exhaustion_recovers_degenerate_spectrum(selftest, tmp_path, k, cplx):
+    """precision 0 runs every sector until its Krylov space is complete: the whole spectrum,
+    including an eigenvalue of multiplicity above k (reached only through the restarts that
+    replace the columns of an invariant Krylov space), comes out with its multiplicities."""
+    rng = np.random.default_rng(100 + k + cplx)
+    mats = []
+    for d, mult in [(24, 9), (41, 12), (60, 3)]:
+        lam = np.concatenate([np.full(mult, -3.0), rng.uniform(-2, 5, d - mult)])
+        q, _ = np.linalg.qr(rng.standard_normal((d, d)) + (1j * rng.standard_normal((d, d)) if cplx else 0))
+        mats.append((q * lam) @ q.conj().T)
+    res = run(selftest, tmp_path, mats, k, nev=60, precision=0.0)
+    for m, (conv, prods, ev) in zip(mats, res):
+        d = m.shape[0]
+        ref = np.linalg.eigvalsh(m)
+        assert conv == 1
+        assert ev.size == d
+        assert np.max(np.abs(ev - ref)) < 1e-10, (d, ev[:14], ref[:14])
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_zero_and_scalar_operators(selftest, tmp_path, k):
+    """H = 0 and H = c I make every new block vanish after reorthogonalisation: each step is a
+    restart, and the spectrum is still exact."""
+    mats = [np.zeros((20, 20)), 2.5 * np.eye(37)]
+    res = run(selftest, tmp_path, mats, k, nev=6, precision=0.0)
+    for m, (conv, prods, ev) in zip(mats, res):
+        assert conv == 1
+        assert np.max(np.abs(ev - np.linalg.eigvalsh(m)[:ev.size])) < 1e-12
+        assert ev.size == 6
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_sector_sets(selftest, tmp_path, seed):
+    """Random sector sets (sizes around multiples of k, spectra with clusters) at the default
+    precision: the lowest eigenvalues of every sector agree with the dense ones."""
+    rng = np.random.default_rng(1000 + seed)
+    k = int(rng.choice([1, 2, 4, 8, 16]))
+    cplx = bool(seed % 2)
+    mats = []
+    for _ in range(5):
+        d = int(max(1, k * rng.integers(1, 12) + rng.integers(-1, 2)))
+        lam = np.sort(rng.uniform(-1, 1, d))
+        lam[: min(d, 3)] = lam[0] + 1e-7 * np.arange(min(d, 3))  # a tight cluster at the bottom
+        q, _ = np.linalg.qr(rng.standard_normal((d, d)) + (1j * rng.standard_normal((d, d)) if cplx else 0))
+        mats.append((q * lam) @ q.conj().T)
+    nev = int(rng.integers(1, 8))
+    res = run(selftest, tmp_path, mats, k, nev=nev, seed=seed)
+    for m, (conv, prods, ev) in zip(mats, res):
+        ref = np.linalg.eigvalsh(m)[:nev]
+        assert conv == 1
+        assert ev.size == ref.size
+        assert np.max(np.abs(ev - ref)) < 1e-10, (k, m.shape, ev, ref)
+
+
+def test_invariant_block_structure(selftest, tmp_path):
+    """A block-diagonal operator with a one-dimensional block: early invariant subspaces in the
+    middle of the recurrence."""
+    rng = np.random.default_rng(5)
+    a = random_hermitian(rng, 30)
+    m = np.zeros((61, 61))
+    m[:30, :30] = a
+    m[30:60, 30:60] = a + 0.25
+    m[60, 60] = -9.0
+    res = run(selftest, tmp_path, [m], 4, nev=8)
+    conv, prods, ev = res[0]
+    assert conv == 1
+    assert np.max(np.abs(ev - np.linalg.eigvalsh(m)[:8])) < 1e-10

@@ -5,9 +5,14 @@
 Headline workload: spin-1/2 Heisenberg chain, L = 28 sites, periodic, J = 1, Sz = 0
 (40,116,600 states).  One step = one H.v over the whole basis (the one-vector product).
 The JSON line carries matrix elements / s, the roofline of that kernel (algorithmic HBM
-bytes per launch from HIP events on the kernel's stream), the Lanczos time to the 4 lowest
-eigenvalues of that chain, a block-vs-plain Lanczos comparison, and the oracle's CPU rate on
-a bounded sample of the same product on the host cores.
+bytes per launch from HIP events on the kernel's stream), and beside it:
+  lanczos           time to the 4 lowest eigenvalues of that chain; block vs plain to the 10
+                    lowest of N = 8 fermions at 2S = 21; and "all_sectors": time to the 10 lowest
+                    eigenvalues of every Lz >= 0 sector of N = 10 fermions at 2S = 27, the plain
+                    one-vector Lanczos sector after sector vs the batched block Lanczos
+  batched_roofline  one k = 8 product over every Lz >= 0 sector of N = 12 fermions at 2S = 33
+                    in one persistent launch (the batching workload's kernel)
+  cpu_baseline      the oracle's CPU rate on a bounded sample of the headline product
 """
 import argparse
 import ctypes
@@ -125,6 +130,62 @@ def lanczos_runs(capi, arch):
     return res
 
 
+def sector_runs(capi, arch, n=10, lz=27, nev=10, block=8):
+    """Time to the nev lowest eigenvalues of every Lz >= 0 sector of n Coulomb fermions at flux lz:
+    the plain one-vector Lanczos run sector after sector (the reference's loop) against the batched
+    block Lanczos that advances all sectors together, and the largest eigenvalue difference."""
+    pp = capi.coulomb_pseudopotentials(lz, True)
+    lzs = capi.sphere_sectors(n, lz)
+    dims = [capi.fermion_sphere_dimension(n, s, lz) for s in lzs]
+    runs = {}
+    for name, k, batched in (("plain_sequential_k1", 1, False), ("batched_k1", 1, True),
+                             ("batched_block_k%d" % block, block, True)):
+        r = capi.sectors_solve(arch, n, lz, lzs, pp, nev, k, batched=batched, allow_unconverged=True)
+        runs[name] = r
+    ref = runs["plain_sequential_k1"]["eigenvalues"]
+    out = {"workload": "fermions_sphere_n%d_2s%d_coulomb_all_Lz_ge_0_sectors_%d_eigenvalues" % (n, lz, nev),
+           "sectors": len(lzs), "states": int(sum(dims)), "largest_sector": int(max(dims))}
+    for name, r in runs.items():
+        out[name] = {"seconds": r["seconds"], "products": int(r["products"].sum()),
+                     "converged": int(r["converged"].sum()),
+                     "max_abs_eigenvalue_difference_vs_plain": float(np.max(np.abs(r["eigenvalues"] - ref)))}
+    t0 = runs["plain_sequential_k1"]["seconds"]
+    for name in runs:
+        if name != "plain_sequential_k1":
+            out[name]["speedup_vs_plain"] = t0 / runs[name]["seconds"]
+    return out
+
+
+def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
+    """One persistent product of k vectors over every Lz >= 0 sector of n fermions at flux lz
+    (the batching workload's kernel), timed with HIP events on the bench stream."""
+    pp = capi.coulomb_pseudopotentials(lz, True)
+    lzs = capi.sphere_sectors(n, lz)
+    S = capi.SphereSectorSet(arch, n, lz, lzs, pp)
+    S.set_launch(1, 256)
+    nnz = int(S.count_elements().sum())
+    x = capi.DeviceBuffer(arch, S.total * k)
+    y = capi.DeviceBuffer(arch, S.total * k)
+    capi.gpu().dh_fill_random(arch.ctx, x.ptr, S.total * k, 0x5EED0003)
+    S.hv_device(x.ptr, y.ptr, k)
+    arch.sync()
+    timer = HipTimer(stream)
+    timer.start()
+    for _ in range(reps):
+        S.hv_device(x.ptr, y.ptr, k)
+    ms = timer.stop_ms() / reps
+    x.free()
+    y.free()
+    rows = S.total
+    S.close()
+    alg = rows * (8 + 8 * k) + nnz * 8 * k
+    achieved = alg / (ms * 1e-3) / 1e9
+    return {"workload": "fermions_sphere_n%d_2s%d_all_Lz_ge_0_sectors_one_product_k%d" % (n, lz, k),
+            "kernel": "sectors_hv_kernel<%d, 1, LO_LDS>" % k, "sectors": len(lzs), "rows": rows,
+            "matrix_elements": nnz, "ms": ms, "element_vector_products_per_s": nnz * k / (ms * 1e-3),
+            "algorithmic_bytes": alg, "achieved_GBps": achieved, "frac_of_peak": achieved / (HBM_PEAK / 1e9)}
+
+
 def cpu_baseline(L=28, seconds_target=15.0):
     """Oracle (CPU restatement of SpinChainHamiltonian::LowLevelAddMultiply, reference thread
     split) on the host cores, over a bounded row sample of the same L = 28 product."""
@@ -166,6 +227,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--skip-lanczos", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--length", type=int, default=28)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -209,6 +271,9 @@ def main():
     }
     if not args.skip_lanczos:
         line["lanczos"] = lanczos_runs(capi, arch)
+        line["lanczos"]["all_sectors"] = sector_runs(capi, arch)
+    if not args.skip_batched:
+        line["batched_roofline"] = batched_hv_roofline(capi, arch, stream)
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(args.length)
     arch.sync()

@@ -66,6 +66,7 @@ GPU_SIGNATURES = {
     "dh_sector_set_get_states": (c_int, [c_void_p, P_U64]),
     "dh_sector_set_factors": (c_int, [c_void_p, P_D, P_I, c_int, c_double]),
     "dh_sector_set_launch": (c_int, [c_void_p, c_int, c_int]),
+    "dh_sector_set_count_elements": (c_int, [c_void_p, P_I64]),
     "dh_sectors_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_char_p]),
     "dh_sectors_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
     "dh_sectors_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
@@ -400,3 +401,72 @@ class Problem:
             raise DHError("dhh_lanczos failed (%d): %s" % (rc, host().dhh_last_error().decode()))
         return {"eigenvalues": ev, "products": prods.value, "blocks": blocks.value, "seconds": secs.value,
                 "converged": rc == 0}
+
+
+class SphereSectorSet:
+    """dh_sector_set of fermions on the sphere: the listed total_lz sectors concatenated, with the
+    interaction factors of the pseudopotentials."""
+
+    def __init__(self, arch, n, lz_max, total_lz, pseudo):
+        self.arch = arch
+        lz = np.ascontiguousarray(total_lz, dtype=np.int32)
+        h = ctypes.c_void_p()
+        arch.check(gpu().dh_sector_set_create(arch.ctx, ctypes.byref(h), n, lz_max, lz.ctypes.data_as(P_I), lz.size),
+                   "dh_sector_set_create")
+        self.h = h
+        self.nbr_sectors = lz.size
+        self.offsets = np.zeros(lz.size + 1, np.int64)
+        gpu().dh_sector_set_offsets(h, self.offsets.ctypes.data_as(P_I64))
+        self.total = int(self.offsets[-1])
+        f, po = sphere_factors(lz_max, pseudo)
+        arch.check(gpu().dh_sector_set_factors(h, dptr(f), po.ctypes.data_as(P_I), f.size, 0.0), "factors")
+
+    def set_launch(self, lanes_per_row=1, block_size=256):
+        self.arch.check(gpu().dh_sector_set_launch(self.h, lanes_per_row, block_size), "dh_sector_set_launch")
+
+    def count_elements(self):
+        counts = np.zeros(self.nbr_sectors, np.int64)
+        self.arch.check(gpu().dh_sector_set_count_elements(self.h, counts.ctypes.data_as(P_I64)), "count")
+        return counts
+
+    def hv_device(self, d_x, d_y, k=1, add=False, active=None):
+        am = None if active is None else bytes(np.asarray(active, np.uint8))
+        self.arch.check(gpu().dh_sectors_hv(self.h, d_x, d_y, k, int(add), am), "dh_sectors_hv")
+
+    def close(self):
+        if self.h:
+            gpu().dh_sector_set_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sphere_sectors(n, lz_max, positive_only=True):
+    """total_lz values (twice Lz) of the non-empty sectors of n fermions at lz_max; Lz >= 0 only
+    by default, as in the Lz loop of FQHESphereFermionsTwoBodyGeneric.cc:270."""
+    top = n * (lz_max - n + 1)
+    low = top % 2 if positive_only else -top
+    return list(range(low, top + 1, 2))
+
+
+def sectors_solve(arch, n, lz_max, total_lz, pseudo, nbr_eigen, block_size=1, batched=True, max_iter=3000,
+                  precision=1e-14, seed=0x5EED0000, memory_gb=200.0, allow_unconverged=False):
+    """dhh_sectors_solve: the nbr_eigen lowest eigenvalues of every sector in total_lz."""
+    lz = np.ascontiguousarray(total_lz, dtype=np.int32)
+    pseudo = np.ascontiguousarray(pseudo, dtype=np.float64)
+    ns = lz.size
+    ev = np.zeros(ns * nbr_eigen)
+    prods = np.zeros(ns, np.int32)
+    conv = np.zeros(ns, np.int32)
+    secs = ctypes.c_double()
+    rc = host().dhh_sectors_solve(arch.h, n, lz_max, lz.ctypes.data_as(P_I), ns, dptr(pseudo), nbr_eigen,
+                                  block_size, max_iter, precision, seed, int(bool(batched)), memory_gb, dptr(ev),
+                                  prods.ctypes.data_as(P_I), conv.ctypes.data_as(P_I), ctypes.byref(secs))
+    if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
+        raise DHError("dhh_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+    return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
+            "seconds": secs.value}

@@ -217,6 +217,48 @@ extern "C" int dh_hamiltonian_count_elements(dh_hamiltonian* h, int64_t* nnz) {
   return DH_OK;
 }
 
+extern "C" int dh_sector_set_count_elements(dh_sector_set* S, int64_t* counts) {
+  if (!S || !counts) return DH_E_ARG;
+  dh_context* c = S->ctx;
+  if (S->kind != DH_SECTORS_SPHERE) return dh_fail(c, DH_E_ARG, "element counts are implemented for sphere sector sets");
+  if (!S->d_factors) return dh_fail(c, DH_E_STATE, "factors not set");
+  const int64_t chunk = 1 << 24;
+  int64_t maxDim = 0;
+  for (int s = 0; s < S->nbrSectors; ++s) maxDim = std::max(maxDim, S->offsets[s + 1] - S->offsets[s]);
+  int64_t* d_counts = nullptr;
+  unsigned long long* d_total = nullptr;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, maxDim)))))
+    return rc;
+  if ((rc = dh_malloc(c, (void**)&d_total, sizeof(unsigned long long) * S->nbrSectors))) {
+    hipFree(d_counts);
+    return rc;
+  }
+  hipMemsetAsync(d_total, 0, sizeof(unsigned long long) * S->nbrSectors, c->stream);
+  for (int s = 0; s < S->nbrSectors; ++s) {
+    const int64_t off = S->offsets[s], dim = S->offsets[s + 1] - off;
+    for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
+      const int64_t r1 = std::min(dim, r0 + chunk);
+      const unsigned blocks = (unsigned)((r1 - r0 + 255) / 256);
+      // counting only: the target lookup tables are not read
+      hipLaunchKernelGGL(fs_elements_kernel, dim3(blocks), dim3(256), 0, c->stream, S->d_states + off, dim,
+                         S->nbrOrbitals, S->d_factors, S->d_pairOffset, (const uint32_t*)nullptr,
+                         (const uint32_t*)nullptr, 0, r0, r1, d_counts, (int64_t*)nullptr, (double*)nullptr,
+                         (int64_t)0);
+      hipLaunchKernelGGL(sum_i64_kernel, dim3(256), dim3(256), 0, c->stream, d_counts, r1 - r0, d_total + s);
+    }
+  }
+  std::vector<unsigned long long> total(S->nbrSectors);
+  hipMemcpyAsync(total.data(), d_total, sizeof(unsigned long long) * S->nbrSectors, hipMemcpyDeviceToHost,
+                 c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_counts);
+  hipFree(d_total);
+  if (e != hipSuccess) return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+  for (int s = 0; s < S->nbrSectors; ++s) counts[s] = (int64_t)total[s];
+  return DH_OK;
+}
+
 extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64_t* targets, double* coefs,
                                            int64_t maxOut, int64_t* n) {
   if (!h || !n || (maxOut > 0 && (!targets || !coefs)) || maxOut < 0) return DH_E_ARG;

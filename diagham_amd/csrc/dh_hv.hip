@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
     auto find = [&](uint64_t t) -> int64_t { return sphere_find<MODE>(a, t, sLut, sLo, sHi); };
-    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x, row, acc);
+    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x, acc);
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)

@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const d
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
     auto find = [&](uint64_t t) -> int64_t { return dh_find_rank(t, lo, hi, a.rankBits, dim); };
-    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x + off * K, active ? local : 0, acc);
+    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x + off * K, acc);
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)

@@ -23,58 +23,16 @@ __device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restr
   }
 }
 
-// Gathers issued together per lane: the terms of a row are queued in registers and the queue is
-// flushed G loads at a time, so a lane keeps G independent gathers of x in flight.
-template <int K>
-struct DhGatherDepth {
-  static constexpr int G = K == 1 ? 8 : (K <= 4 ? 4 : 2);
-};
-
 // Two-body terms of row state s: for every occupied pair n1 > n2 (annihilated as a_n1 a_n2,
 // sign = parity of the occupied orbitals at and above each), every created pair m3 > m4 with
 // m3 + m4 = n1 + n2 (created a+_m3 a+_m4 on the intermediate state) whose orbitals are free.
 // Lane `lane` of the row's LPR lanes takes m3 = m3lo + lane, m3lo + lane + LPR, ...
-// Target indices come from `find(target)`; x is indexed relative to `xbase`, and `pad` is any
-// valid index of it (queue padding gathers it with a zero coefficient).
+// Target indices come from `find(target)`; x is indexed relative to `xbase`.  Each term is one
+// gather; latency is hidden by occupancy (few registers per lane) rather than by batching.
 template <int K, int LPR, typename Find>
 __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const double* __restrict__ sF,
                                               const int* __restrict__ sOff, int lane, Find find,
-                                              const double* __restrict__ xbase, int64_t pad, double* acc) {
-  constexpr int G = DhGatherDepth<K>::G;
-  int64_t qIdx[G];
-  double qCoef[G];
-  int nq = 0;
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    qIdx[g] = pad;
-    qCoef[g] = 0.0;
-  }
-  auto flush = [&]() {
-    double v[G][K];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const double* xr = xbase + qIdx[g] * K;
-      if constexpr (K % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < K; q += 2) {
-          double2 p = *reinterpret_cast<const double2*>(xr + q);
-          v[g][q] = p.x;
-          v[g][q + 1] = p.y;
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < K; ++q) v[g][q] = xr[q];
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-#pragma unroll
-      for (int q = 0; q < K; ++q) acc[q] += qCoef[g] * v[g][q];
-      qIdx[g] = pad;
-      qCoef[g] = 0.0;
-    }
-    nq = 0;
-  };
+                                              const double* __restrict__ xbase, double* acc) {
   for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
     const int n1 = __builtin_ctzll(r1);
     const int pc1 = __popcll(s >> n1);
@@ -94,15 +52,8 @@ __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const
         const int64_t idx = find(t | cm);
         double coef = sF[fbase + m3];
         if (sg & 1) coef = -coef;
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-          if (g == nq) {
-            qIdx[g] = idx;
-            qCoef[g] = coef;
-          }
-        if (++nq == G) flush();
+        dh_gather_fma<K>(acc, xbase, idx, coef);
       }
     }
   }
-  if (nq) flush();
 }

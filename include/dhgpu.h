@@ -150,6 +150,10 @@ int dh_sector_set_get_states(dh_sector_set* set, uint64_t* states);
 int dh_sector_set_factors(dh_sector_set* set, const double* factors, const int* pair_offset, int nbr_factors,
                           double shift);
 int dh_sector_set_launch(dh_sector_set* set, int lanes_per_row, int block_size);
+/* Matrix elements per product of each sector of a sphere set (counts[nbr_sectors]), counted as
+ * dh_hamiltonian_count_elements does: the "nbr interaction" the reference prints per sector
+ * (AbstractQHEOnSphereHamiltonian.cc:3721 PartialFastMultiplicationMemory). */
+int dh_sector_set_count_elements(dh_sector_set* set, int64_t* counts);
 /* y = (add ? y : 0) + (H + shift) x over the active sectors (LowLevelMultipleAddMultiply per sector) */
 int dh_sectors_hv(dh_sector_set* set, const double* d_x, double* d_y, int k, int add, const unsigned char* active);
 /* per active sector j (in sector order): c[j][(b*k+p)*k+q] = sum_rows basis_b[row][p] w[row][q];

@@ -79,6 +79,18 @@ def test_sector_set_basis_and_products(arch, lpr):
     S.close()
 
 
+def test_sector_element_counts(arch):
+    """Per-sector matrix-element counts of a set equal the oracle's count of each sector."""
+    V = coulomb(LZ)
+    S = SectorSet(arch, N, LZ, SECTORS, V)
+    counts = np.zeros(len(SECTORS), np.int64)
+    arch.check(S.capi.gpu().dh_sector_set_count_elements(S.h, counts.ctypes.data_as(S.capi.P_I64)), "count")
+    fac = O.Factors(LZ, V)
+    for i, lz in enumerate(SECTORS):
+        assert counts[i] == O.fs_count_elements(O.FermionSphere(N, lz, LZ), fac), lz
+    S.close()
+
+
 def test_segmented_vector_operations(arch):
     c = __import__("diagham_amd.capi", fromlist=["capi"])
     S = SectorSet(arch, N, LZ, SECTORS, coulomb(LZ))

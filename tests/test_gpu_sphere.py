@@ -109,6 +109,28 @@ def test_hv_shift_add_and_range(cfg1, arch):
     cfg1.set_shift(0.0)
 
 
+def test_autotune_keeps_parity(cfg1):
+    """The autotuner sweeps the launch shapes of each block size, reports a consistent table and
+    leaves a launch configured whose products still match the oracle."""
+    rows, best = cfg1.autotune(ks=(1, 4, 8), repeats=2)
+    assert [int(r["k"]) for r in rows] == [1, 4, 8]
+    assert best in (1, 4, 8)
+    for r in rows:
+        assert r["lanes_per_row"] in (1, 2, 4, 8) and r["block_size"] in (128, 256, 512)
+        assert r["index_mode"] in (0, 1) and r["ms"] > 0
+        assert r["elements_per_s"] > 0
+    rates = {int(r["k"]): r["elements_per_s"] for r in rows}
+    assert rates[best] == max(rates.values())
+    ref_space = O.FermionSphere(8, 0, 21)
+    fac = O.Factors(21, coulomb(21))
+    X = np.stack([O.splitmix(300 + v, cfg1.dim) for v in range(best)], axis=1)
+    Y = cfg1.hv(X.reshape(-1), best).reshape(cfg1.dim, best)
+    for v in range(best):
+        y_ref = O.fs_hv(ref_space, fac, X[:, v])
+        assert np.linalg.norm(Y[:, v] - y_ref) <= 1e-12 * np.linalg.norm(y_ref)
+    cfg1.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
+
+
 def test_hermitian_reference_form_agrees(cfg1):
     """The reference's Hermitian-halved scatter (HermitianLowLevelAddMultiply) and the gather agree."""
     ref_space = O.FermionSphere(8, 0, 21)

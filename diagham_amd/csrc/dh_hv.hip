@@ -25,9 +25,9 @@ enum { MODE_BSEARCH = 0, MODE_RANK_LDS = 1, MODE_RANK_GLOBAL = 2 };
 struct SphereArgs {
   const uint64_t* states;
   int64_t dim;
-  int nbrOrbitals, nbrFactors, nbrPairs;
+  int nbrOrbitals, nbrFactors, nbrSums;  // packed factors and their per-sum bases
   const double* factors;
-  const int* pairOffset;
+  const int* sumBase;
   const int* lut;
   int lutShift, lutSize, lutStride;  // staged LUT entry i = lut[i * lutStride]
   const uint32_t* rankLo;
@@ -54,14 +54,14 @@ __device__ __forceinline__ void gather_fma(double* acc, const double* __restrict
 }
 
 template <int MODE>
-__device__ __forceinline__ void sphere_stage(const SphereArgs& a, double*& sF, int*& sOff, int*& sLut,
+__device__ __forceinline__ void sphere_stage(const SphereArgs& a, double*& sF, int*& sBase, int*& sLut,
                                              uint32_t*& sLo, uint32_t*& sHi) {
   extern __shared__ double smem[];
   sF = smem;
-  sOff = reinterpret_cast<int*>(sF + a.nbrFactors);
-  int* tail = sOff + a.nbrPairs + 1;
+  sBase = reinterpret_cast<int*>(sF + a.nbrFactors);
+  int* tail = sBase + a.nbrSums;
   for (int i = threadIdx.x; i < a.nbrFactors; i += blockDim.x) sF[i] = a.factors[i];
-  for (int i = threadIdx.x; i <= a.nbrPairs; i += blockDim.x) sOff[i] = a.pairOffset[i];
+  for (int i = threadIdx.x; i < a.nbrSums; i += blockDim.x) sBase[i] = a.sumBase[i];
   if constexpr (MODE == MODE_BSEARCH) {
     sLut = tail;
     for (int i = threadIdx.x; i < a.lutSize; i += blockDim.x) sLut[i] = a.lut[(int64_t)i * a.lutStride];
@@ -91,9 +91,9 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
                                                          double* __restrict__ y, int add, int64_t first,
                                                          int64_t count) {
   double* sF;
-  int *sOff, *sLut = nullptr;
+  int *sBase, *sLut = nullptr;
   uint32_t *sLo = nullptr, *sHi = nullptr;
-  sphere_stage<MODE>(a, sF, sOff, sLut, sLo, sHi);
+  sphere_stage<MODE>(a, sF, sBase, sLut, sLo, sHi);
   const int lane = threadIdx.x % LPR;
   const int64_t rowsPerBlock = blockDim.x / LPR;
   const int64_t nbrTiles = (count + rowsPerBlock - 1) / rowsPerBlock;
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
     auto find = [&](uint64_t t) -> int64_t { return sphere_find<MODE>(a, t, sLut, sLo, sHi); };
-    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x, acc);
+    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)
@@ -240,10 +240,10 @@ static SphereArgs sphere_args(const dh_hamiltonian* h, int lutBitsUsed) {
   a.states = b->d_states;
   a.dim = b->dim;
   a.nbrOrbitals = b->nbrOrbitals;
-  a.nbrFactors = h->nbrFactors;
-  a.nbrPairs = h->nbrPairs;
-  a.factors = h->d_factors;
-  a.pairOffset = h->d_pairOffset;
+  a.nbrFactors = h->nbrPackedFactors;
+  a.nbrSums = h->nbrPairSums;
+  a.factors = h->d_packedFactors;
+  a.sumBase = h->d_sumBase;
   a.lut = b->d_lut;
   int drop = b->lutBits - lutBitsUsed;
   a.lutStride = 1 << drop;
@@ -262,7 +262,7 @@ static const size_t kLdsLimit = 160 * 1024;
 
 // LDS bytes of the staged tables for a given mode
 static size_t sphere_lds(const dh_hamiltonian* h, int mode, int lutBitsUsed) {
-  size_t bytes = sizeof(double) * h->nbrFactors + sizeof(int) * (h->nbrPairs + 1);
+  size_t bytes = sizeof(double) * h->nbrPackedFactors + sizeof(int) * h->nbrPairSums;
   bytes = (bytes + 7) & ~size_t(7);
   if (mode == MODE_BSEARCH) bytes += sizeof(int) * ((size_t(1) << lutBitsUsed) + 1);
   if (mode == MODE_RANK_LDS) bytes += sizeof(uint32_t) * (h->basis->rankLoSize + h->basis->rankHiSize);
@@ -407,22 +407,34 @@ extern "C" int dh_sphere_hamiltonian_create(dh_context* c, dh_hamiltonian** out,
     }
   if (pairOffset[nPairs] != expect || nbrFactors != expect)
     return dh_fail(c, DH_E_ARG, "nbr_factors does not match the factor layout");
+  std::vector<double> packed;
+  std::vector<int> sumBase;
+  int rc = dh_pack_factors_checked(c, nOrb, factors, pairOffset, packed, sumBase);
+  if (rc) return rc;
   dh_hamiltonian* h = new dh_hamiltonian();
   h->ctx = c;
   h->basis = b;
   h->kind = DH_HAM_SPHERE;
   h->nbrFactors = nbrFactors;
   h->nbrPairs = nPairs;
-  int rc;
+  h->nbrPackedFactors = (int)packed.size();
+  h->nbrPairSums = (int)sumBase.size();
   if ((rc = dh_malloc(c, (void**)&h->d_factors, sizeof(double) * std::max(nbrFactors, 1))) ||
-      (rc = dh_malloc(c, (void**)&h->d_pairOffset, sizeof(int) * (nPairs + 1)))) {
+      (rc = dh_malloc(c, (void**)&h->d_pairOffset, sizeof(int) * (nPairs + 1))) ||
+      (rc = dh_malloc(c, (void**)&h->d_packedFactors, sizeof(double) * std::max<size_t>(packed.size(), 1))) ||
+      (rc = dh_malloc(c, (void**)&h->d_sumBase, sizeof(int) * sumBase.size()))) {
     dh_hamiltonian_destroy(h);
     return rc;
   }
   DH_HIP(c, hipMemcpyAsync(h->d_factors, factors, sizeof(double) * nbrFactors, hipMemcpyHostToDevice, c->stream));
   DH_HIP(c, hipMemcpyAsync(h->d_pairOffset, pairOffset, sizeof(int) * (nPairs + 1), hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipMemcpyAsync(h->d_packedFactors, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(h->d_sumBase, sumBase.data(), sizeof(int) * sumBase.size(), hipMemcpyHostToDevice,
+                           c->stream));
   DH_HIP(c, hipStreamSynchronize(c->stream));
   h->lanesPerRow = (b->dim < (int64_t)c->nbrCU * 256) ? 4 : 1;
+  h->blockSize = 512;
   h->indexMode = DH_INDEX_RANK;
   *out = h;
   return DH_OK;
@@ -473,6 +485,8 @@ extern "C" int dh_hamiltonian_destroy(dh_hamiltonian* h) {
   hipStreamSynchronize(c->stream);
   if (h->d_factors) hipFree(h->d_factors);
   if (h->d_pairOffset) hipFree(h->d_pairOffset);
+  if (h->d_packedFactors) hipFree(h->d_packedFactors);
+  if (h->d_sumBase) hipFree(h->d_sumBase);
   if (h->d_cg) hipFree(h->d_cg);
   delete h;
   return DH_OK;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -59,6 +61,10 @@ struct dh_hamiltonian {
   int nbrFactors = 0, nbrPairs = 0;
   double* d_factors = nullptr;
   int* d_pairOffset = nullptr;  // [nbrPairs + 1]
+  // the same factors packed symmetric per pair sum (what the product kernels stage in LDS)
+  int nbrPackedFactors = 0, nbrPairSums = 0;
+  double* d_packedFactors = nullptr;
+  int* d_sumBase = nullptr;  // [nbrPairSums]
   // adjoint channel split
   int nbrChannels = 0;
   double* d_cg = nullptr;  // [nbrPairs * nbrChannels]
@@ -90,10 +96,13 @@ struct dh_sector_set {
   uint32_t* d_rankLo = nullptr;
   uint32_t* d_rankHi = nullptr;
   int64_t* d_offsets = nullptr;
-  // sphere factors
+  // sphere factors (per-pair layout, and packed symmetric per pair sum)
   double* d_factors = nullptr;
   int* d_pairOffset = nullptr;
   int nbrFactors = 0, nbrPairs = 0;
+  int nbrPackedFactors = 0, nbrPairSums = 0;
+  double* d_packedFactors = nullptr;
+  int* d_sumBase = nullptr;
   // momentum factors: per momentum-sum sector j its pair list and complex factor matrix
   int nbrSums = 0;
   int* d_pairInfo = nullptr;     // [pair id b(b-1)/2 + a] -> (sum << 16) | position
@@ -141,6 +150,52 @@ __device__ __forceinline__ unsigned dh_xcd_block(unsigned b, unsigned n) {
   // XCD x owns blocks [x*per + min(x,rem), ...) of length per + (x < rem)
   unsigned start = x * per + (x < rem ? x : rem);
   return start + i;
+}
+
+// Two-body factors on the sphere are symmetric, F(a -> b) = F(b -> a), between the pairs of one
+// orbital sum S = n1 + n2.  Pair (n1 > n2) of sum S has position n1 - lo(S), lo(S) = S/2 + 1, and
+// the factor between positions i and j (i <= j) is packed at base[S] + j (j + 1) / 2 + i.  This
+// halves the table the product kernels stage in LDS.  Input: the per-pair layout (pair
+// a = n1 (n1 - 1) / 2 + n2, created m3 from lo(S): factors[pairOffset[a] + m3 - lo(S)]).
+inline void dh_pack_sphere_factors(int nbrOrbitals, const double* factors, const int* pairOffset,
+                                   std::vector<double>& packed, std::vector<int>& base) {
+  const int nbrSums = 2 * nbrOrbitals - 1;
+  packed.clear();
+  base.assign(nbrSums, 0);
+  for (int S = 0; S < nbrSums; ++S) {
+    base[S] = (int)packed.size();
+    const int lo = S / 2 + 1, hi = std::min(S, nbrOrbitals - 1);
+    for (int j = 0; lo + j <= hi; ++j)
+      for (int i = 0; i <= j; ++i) {
+        const int n1 = lo + i, n2 = S - n1;
+        packed.push_back(factors[pairOffset[n1 * (n1 - 1) / 2 + n2] + j]);
+      }
+  }
+}
+
+// packs, and refuses factor tables that are not symmetric (the products read one triangle)
+inline int dh_pack_factors_checked(dh_context* c, int nbrOrbitals, const double* factors, const int* pairOffset,
+                                   std::vector<double>& packed, std::vector<int>& base) {
+  dh_pack_sphere_factors(nbrOrbitals, factors, pairOffset, packed, base);
+  double scale = 0.0, asym = 0.0;
+  for (int S = 0; S < 2 * nbrOrbitals - 1; ++S) {
+    const int lo = S / 2 + 1, hi = std::min(S, nbrOrbitals - 1);
+    for (int i = 0; lo + i <= hi; ++i)
+      for (int j = 0; lo + j <= hi; ++j) {
+        const int n1 = lo + i, n2 = S - n1;
+        const double f = factors[pairOffset[n1 * (n1 - 1) / 2 + n2] + j];
+        const int m3 = lo + j, m4 = S - m3;
+        const double g = factors[pairOffset[m3 * (m3 - 1) / 2 + m4] + i];
+        scale = std::max(scale, std::fabs(f));
+        asym = std::max(asym, std::fabs(f - g));
+      }
+  }
+  if (asym > 1e-12 * scale) return dh_fail(c, DH_E_ARG, "factors are not symmetric between annihilated and created pairs");
+  return DH_OK;
+}
+
+__host__ __device__ __forceinline__ int dh_packed_index(int i, int j) {
+  return i <= j ? ((j * (j + 1)) >> 1) + i : ((i * (i + 1)) >> 1) + j;
 }
 
 // binomial C(n, k) for the host, 64-bit

@@ -161,7 +161,7 @@ extern "C" int dh_sector_set_destroy(dh_sector_set* S) {
   for (void* p : {(void*)S->d_states, (void*)S->d_rankLo, (void*)S->d_rankHi, (void*)S->d_offsets,
                   (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef,
                   (void*)S->d_pairInfo, (void*)S->d_sumPairOff, (void*)S->d_sumPairs, (void*)S->d_sumFactorOff,
-                  (void*)S->d_cfactors})
+                  (void*)S->d_cfactors, (void*)S->d_packedFactors, (void*)S->d_sumBase})
     if (p) hipFree(p);
   delete S;
   return DH_OK;
@@ -194,19 +194,31 @@ extern "C" int dh_sector_set_factors(dh_sector_set* S, const double* factors, co
       expect += std::min(sum, nOrb - 1) - (sum / 2 + 1) + 1;
     }
   if (pairOffset[nPairs] != expect || nbrFactors != expect) return dh_fail(c, DH_E_ARG, "nbr_factors layout");
-  if (S->d_factors) hipFree(S->d_factors);
-  if (S->d_pairOffset) hipFree(S->d_pairOffset);
-  S->d_factors = nullptr;
-  S->d_pairOffset = nullptr;
-  int rc;
+  std::vector<double> packed;
+  std::vector<int> sumBase;
+  int rc = dh_pack_factors_checked(c, nOrb, factors, pairOffset, packed, sumBase);
+  if (rc) return rc;
+  hipStreamSynchronize(c->stream);
+  for (void** p : {(void**)&S->d_factors, (void**)&S->d_pairOffset, (void**)&S->d_packedFactors, (void**)&S->d_sumBase}) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+  }
   if ((rc = dh_malloc(c, (void**)&S->d_factors, sizeof(double) * std::max(1, nbrFactors))) ||
-      (rc = dh_malloc(c, (void**)&S->d_pairOffset, sizeof(int) * (nPairs + 1))))
+      (rc = dh_malloc(c, (void**)&S->d_pairOffset, sizeof(int) * (nPairs + 1))) ||
+      (rc = dh_malloc(c, (void**)&S->d_packedFactors, sizeof(double) * std::max<size_t>(1, packed.size()))) ||
+      (rc = dh_malloc(c, (void**)&S->d_sumBase, sizeof(int) * sumBase.size())))
     return rc;
   DH_HIP(c, hipMemcpyAsync(S->d_factors, factors, sizeof(double) * nbrFactors, hipMemcpyHostToDevice, c->stream));
   DH_HIP(c, hipMemcpyAsync(S->d_pairOffset, pairOffset, sizeof(int) * (nPairs + 1), hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_packedFactors, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_sumBase, sumBase.data(), sizeof(int) * sumBase.size(), hipMemcpyHostToDevice,
+                           c->stream));
   DH_HIP(c, hipStreamSynchronize(c->stream));
   S->nbrFactors = nbrFactors;
   S->nbrPairs = nPairs;
+  S->nbrPackedFactors = (int)packed.size();
+  S->nbrPairSums = (int)sumBase.size();
   S->shift = shift;
   return DH_OK;
 }
@@ -261,9 +273,9 @@ int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_
 struct SectorsArgs {
   const uint64_t* states;
   const int64_t* offsets;
-  int nbrOrbitals, nbrFactors, nbrPairs;
+  int nbrOrbitals, nbrFactors, nbrSums;  // packed factors and their per-sum bases
   const double* factors;
-  const int* pairOffset;
+  const int* sumBase;
   const uint32_t* rankLo;
   const uint32_t* rankHi;
   int rankBits;
@@ -292,11 +304,11 @@ __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const d
                                                           double* __restrict__ y, int add) {
   extern __shared__ double smem[];
   double* sF = smem;
-  int* sOff = reinterpret_cast<int*>(sF + a.nbrFactors);
-  int64_t* sWork = reinterpret_cast<int64_t*>(smem) + ((a.nbrFactors * 8 + (a.nbrPairs + 1) * 4 + 7) / 8);
+  int* sBase = reinterpret_cast<int*>(sF + a.nbrFactors);
+  int64_t* sWork = reinterpret_cast<int64_t*>(smem) + ((a.nbrFactors * 8 + a.nbrSums * 4 + 7) / 8);
   uint32_t* sLo = reinterpret_cast<uint32_t*>(sWork + 2 * a.nActive + 1);
   for (int i = threadIdx.x; i < a.nbrFactors; i += blockDim.x) sF[i] = a.factors[i];
-  for (int i = threadIdx.x; i <= a.nbrPairs; i += blockDim.x) sOff[i] = a.pairOffset[i];
+  for (int i = threadIdx.x; i < a.nbrSums; i += blockDim.x) sBase[i] = a.sumBase[i];
   for (int i = threadIdx.x; i < 2 * a.nActive + 1; i += blockDim.x) sWork[i] = a.work[i];
   if constexpr (LO_LDS)
     for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo[i];
@@ -319,7 +331,7 @@ __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const d
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
     auto find = [&](uint64_t t) -> int64_t { return dh_find_rank(t, lo, hi, a.rankBits, dim); };
-    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sOff, lane, find, x + off * K, acc);
+    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)
@@ -339,10 +351,10 @@ static SectorsArgs sectors_args(dh_sector_set* S, int nActive, int64_t totalTile
   a.states = S->d_states;
   a.offsets = S->d_offsets;
   a.nbrOrbitals = S->nbrOrbitals;
-  a.nbrFactors = S->nbrFactors;
-  a.nbrPairs = S->nbrPairs;
-  a.factors = S->d_factors;
-  a.pairOffset = S->d_pairOffset;
+  a.nbrFactors = S->nbrPackedFactors;
+  a.nbrSums = S->nbrPairSums;
+  a.factors = S->d_packedFactors;
+  a.sumBase = S->d_sumBase;
   a.rankLo = S->d_rankLo;
   a.rankHi = S->d_rankHi;
   a.rankBits = S->rankBits;
@@ -364,7 +376,7 @@ static int sectors_hv_launch(dh_sector_set* S, const double* x, double* y, int a
   int rc = dh_sectors_upload_work(S, active, rowsPerTile, &na, &tiles);
   if (rc) return rc;
   if (tiles == 0) return DH_OK;
-  size_t base = ((size_t)S->nbrFactors * 8 + (size_t)(S->nbrPairs + 1) * 4 + 7) / 8 * 8 + 8 * (2 * na + 1);
+  size_t base = ((size_t)S->nbrPackedFactors * 8 + (size_t)S->nbrPairSums * 4 + 7) / 8 * 8 + 8 * (2 * na + 1);
   size_t withLo = base + 4 * S->loSize;
   bool loLds = withLo <= 64 * 1024;
   size_t lds = loLds ? withLo : base;

@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "dh_index.h"
+#include "dh_internal.h"
 
 template <int K>
 __device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restrict__ x, int64_t idx, double coef) {
@@ -27,11 +28,12 @@ __device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restr
 // sign = parity of the occupied orbitals at and above each), every created pair m3 > m4 with
 // m3 + m4 = n1 + n2 (created a+_m3 a+_m4 on the intermediate state) whose orbitals are free.
 // Lane `lane` of the row's LPR lanes takes m3 = m3lo + lane, m3lo + lane + LPR, ...
+// Factors come packed symmetric per pair sum (sF, sBase: see dh_pack_sphere_factors).
 // Target indices come from `find(target)`; x is indexed relative to `xbase`.  Each term is one
 // gather; latency is hidden by occupancy (few registers per lane) rather than by batching.
 template <int K, int LPR, typename Find>
 __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const double* __restrict__ sF,
-                                              const int* __restrict__ sOff, int lane, Find find,
+                                              const int* __restrict__ sBase, int lane, Find find,
                                               const double* __restrict__ xbase, double* acc) {
   for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
     const int n1 = __builtin_ctzll(r1);
@@ -43,14 +45,15 @@ __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const
       const int sum = n1 + n2;
       const int m3lo = (sum >> 1) + 1;
       const int m3hi = min(sum, nbrOrbitals - 1);
-      const int fbase = sOff[(n1 * (n1 - 1)) / 2 + n2] - m3lo;
+      const double* fS = sF + sBase[sum];
+      const int i = n1 - m3lo;
       for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
         const int m4 = sum - m3;
         const uint64_t cm = (1ull << m3) | (1ull << m4);
         if (t & cm) continue;
         const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
         const int64_t idx = find(t | cm);
-        double coef = sF[fbase + m3];
+        double coef = fS[dh_packed_index(i, m3 - m3lo)];
         if (sg & 1) coef = -coef;
         dh_gather_fma<K>(acc, xbase, idx, coef);
       }

@@ -146,7 +146,7 @@ extern "C" int dhh_autotune(dhh_problem* p, const int* ks, int nbrKs, int repeat
       for (int mode : {DH_INDEX_RANK, DH_INDEX_BSEARCH})
         for (int lpr : {1, 2, 4, 8}) {
           if (spin && lpr != 1) continue;
-          for (int bs : {128, 256, 512}) {
+          for (int bs : {128, 256, 512, 1024}) {
             if (dh_hamiltonian_set_launch(h, lpr, bs, mode, 1) != DH_OK) continue;
             arch->Check(dh_hv(h, x, y, k, 0, 0, -1), "warm-up");
             arch->Check(dh_synchronize(c), "sync");

@@ -148,7 +148,7 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   std::vector<int64_t> off(na + 1);
   dh_sector_set_offsets(S, off.data());
   const int64_t total = off.back();
-  Arch->Check(dh_sector_set_launch(S, total < 4ll * 256 * 256 ? 4 : 1, 256), "launch");
+  Arch->Check(dh_sector_set_launch(S, total < 4ll * 256 * 256 ? 4 : 1, 512), "launch");
   std::vector<int64_t> dims(na);
   for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
   std::vector<int> blockCap;

@@ -91,7 +91,9 @@ int dh_basis_find_state_index(dh_basis* basis, const uint64_t* states, int64_t n
  * (:493): for every annihilated pair m1 > m2 (pair index m1(m1-1)/2 + m2) the factors of
  * the created pairs m3 > m4 = m1 + m2 - m3, m3 ascending, are factors[pair_offset[pair] +
  * (m3 - (m1 + m2) / 2 - 1)].  nbr_factors = pair_offset[nbrPairs].  The host computes the
- * factors; this call copies them to the device. */
+ * factors; this call copies them to the device.  The factors of a Hermitian two-body term are
+ * symmetric, F(a -> b) = F(b -> a); the product kernels keep one triangle per pair sum, and a
+ * table that is not symmetric to 1e-12 is refused with DH_E_ARG. */
 int dh_sphere_hamiltonian_create(dh_context* ctx, dh_hamiltonian** ham, dh_basis* basis,
                                  const double* factors, const int* pair_offset, int nbr_factors);
 /* Per-channel factor split used by the adjoint kernel: the two-body factor of (pair a ->

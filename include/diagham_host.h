@@ -54,7 +54,7 @@ int dhh_lanczos(dhh_problem* problem, int nbr_eigen, int block_size, int max_ite
                 double* d_eigenvectors, int nbr_vectors);
 
 /* Autotuner of the product launch: for each block size ks[i] it times dh_hv (after a warm-up)
- * over lanes_per_row in {1,2,4,8} (64/lanes rows per wavefront), workgroup sizes {128,256,512}
+ * over lanes_per_row in {1,2,4,8} (64/lanes rows per wavefront), workgroup sizes {128,256,512,1024}
  * and both index lookups, then leaves the problem configured with the fastest launch of the
  * block size best_k that maximises matrix elements * vectors per second.  Per block size the
  * table row is: k, lanes_per_row, block_size, index_mode, milliseconds per product, element-

@@ -109,6 +109,30 @@ def test_hv_shift_add_and_range(cfg1, arch):
     cfg1.set_shift(0.0)
 
 
+def test_asymmetric_factors_are_refused(arch):
+    """The kernels read one triangle of each pair-sum factor matrix: a table that is not symmetric
+    must be refused, not silently symmetrised."""
+    import ctypes
+    from diagham_amd import capi
+    lz = 9
+    f, po = capi.sphere_factors(lz, coulomb(lz))
+    basis = ctypes.c_void_p()
+    arch.check(capi.gpu().dh_fermion_sphere_create(arch.ctx, ctypes.byref(basis), 4, 0, lz), "basis")
+    ham = ctypes.c_void_p()
+    rc = capi.gpu().dh_sphere_hamiltonian_create(arch.ctx, ctypes.byref(ham), basis, capi.dptr(f),
+                                                 po.ctypes.data_as(capi.P_I), f.size)
+    assert rc == 0
+    capi.gpu().dh_hamiltonian_destroy(ham)
+    bad = f.copy()
+    # factor of pair (m1, m2) = (5, 2) towards the created pair m3 = 6 (sum 7): only one side changes
+    bad[po[5 * 4 // 2 + 2] + (6 - 4)] += 0.1
+    rc = capi.gpu().dh_sphere_hamiltonian_create(arch.ctx, ctypes.byref(ham), basis, capi.dptr(bad),
+                                                 po.ctypes.data_as(capi.P_I), f.size)
+    assert rc == capi.DH_E_ARG
+    assert b"symmetric" in capi.gpu().dh_last_error(arch.ctx)
+    capi.gpu().dh_basis_destroy(basis)
+
+
 def test_autotune_keeps_parity(cfg1):
     """The autotuner sweeps the launch shapes of each block size, reports a consistent table and
     leaves a launch configured whose products still match the oracle."""
@@ -116,7 +140,7 @@ def test_autotune_keeps_parity(cfg1):
     assert [int(r["k"]) for r in rows] == [1, 4, 8]
     assert best in (1, 4, 8)
     for r in rows:
-        assert r["lanes_per_row"] in (1, 2, 4, 8) and r["block_size"] in (128, 256, 512)
+        assert r["lanes_per_row"] in (1, 2, 4, 8) and r["block_size"] in (128, 256, 512, 1024)
         assert r["index_mode"] in (0, 1) and r["ms"] > 0
         assert r["elements_per_s"] > 0
     rates = {int(r["k"]): r["elements_per_s"] for r in rows}

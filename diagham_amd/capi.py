@@ -49,6 +49,7 @@ GPU_SIGNATURES = {
     "dh_basis_get_states": (c_int, [c_void_p, P_U64]),
     "dh_basis_find_state_index": (c_int, [c_void_p, P_U64, c_int64, c_int, P_I64]),
     "dh_sphere_hamiltonian_create": (c_int, [c_void_p, PP, c_void_p, P_D, P_I, c_int]),
+    "dh_sphere_nbody_hamiltonian_create": (c_int, [c_void_p, PP, c_void_p, c_int, P_I, c_int, P_I, P_D]),
     "dh_sphere_hamiltonian_set_channels": (c_int, [c_void_p, P_D, c_int, P_D]),
     "dh_spin_chain_hamiltonian_create": (c_int, [c_void_p, PP, c_void_p, P_D, P_D, c_int]),
     "dh_hamiltonian_destroy": (c_int, [c_void_p]),
@@ -89,6 +90,7 @@ HOST_SIGNATURES = {
     "dhh_arch_destroy": (c_int, [c_void_p]),
     "dhh_arch_context": (c_void_p, [c_void_p]),
     "dhh_sphere_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_D]),
+    "dhh_sphere_nbody_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, c_int, P_I, c_int, P_I, P_D]),
     "dhh_spin_problem_create": (c_int, [c_void_p, PP, c_int, c_int, P_D, P_D, c_int]),
     "dhh_problem_destroy": (c_int, [c_void_p]),
     "dhh_problem_basis": (c_void_p, [c_void_p]),
@@ -296,6 +298,22 @@ class Problem:
         if rc != 0:
             raise DHError("sphere problem: " + host().dhh_last_error().decode())
         p = cls(arch, h, "sphere")
+        p.n, p.total_lz, p.lz_max = n, total_lz, lz_max
+        return p
+
+    @classmethod
+    def sphere_nbody(cls, arch, n, total_lz, lz_max, nbody, sum_count, tuples, factors):
+        """Three-body term with explicit per-sum factor matrices (dhh_sphere_nbody_problem_create)."""
+        sum_count = np.ascontiguousarray(sum_count, dtype=np.int32)
+        tuples = np.ascontiguousarray(tuples, dtype=np.int32)
+        factors = np.ascontiguousarray(factors, dtype=np.float64)
+        h = c_void_p()
+        rc = host().dhh_sphere_nbody_problem_create(arch.h, ctypes.byref(h), n, total_lz, lz_max, nbody,
+                                                    sum_count.ctypes.data_as(P_I), sum_count.size,
+                                                    tuples.ctypes.data_as(P_I), dptr(factors))
+        if rc != 0:
+            raise DHError("n-body problem: " + host().dhh_last_error().decode())
+        p = cls(arch, h, "sphere_nbody")
         p.n, p.total_lz, p.lz_max = n, total_lz, lz_max
         return p
 

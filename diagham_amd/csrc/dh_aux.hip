@@ -174,6 +174,7 @@ static double spin_diag_value(const dh_hamiltonian* h, uint32_t s) {
 
 extern "C" int dh_hamiltonian_count_elements(dh_hamiltonian* h, int64_t* nnz) {
   if (!h || !nnz) return DH_E_ARG;
+  if (h->kind == DH_HAM_SPHERE_NBODY) return dh_nbody_count_elements(h, nnz);
   dh_context* c = h->ctx;
   dh_basis* b = h->basis;
   const int64_t chunk = 1 << 24;
@@ -265,6 +266,8 @@ extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64
   dh_context* c = h->ctx;
   dh_basis* b = h->basis;
   if (row < 0 || row >= b->dim) return dh_fail(c, DH_E_ARG, "row outside the space");
+  if (h->kind == DH_HAM_SPHERE_NBODY)
+    return dh_fail(c, DH_E_ARG, "row listing is implemented for two-body and spin-chain Hamiltonians");
   int64_t cap = std::max<int64_t>(maxOut, 1);
   int64_t *d_t, *d_n;
   double* d_c;

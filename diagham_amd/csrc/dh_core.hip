@@ -212,13 +212,23 @@ __global__ void __launch_bounds__(256) dh_block_dots_kernel(const double* __rest
   }
 }
 
-__global__ void dh_reduce_chunks_kernel(const double* __restrict__ partial, int nbrChunks, int total,
-                                        double* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
+// out[i] = sum over chunks of partial[chunk][i]: a workgroup owns 64 consecutive outputs (one
+// per lane, coalesced rows of the partial table); its waves split the chunks and meet in LDS.
+__global__ void __launch_bounds__(1024) dh_reduce_chunks_kernel(const double* __restrict__ partial, int nbrChunks,
+                                                                int total, double* __restrict__ out) {
+  __shared__ double red[16][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int c = 0; c < nbrChunks; ++c) s += partial[(int64_t)c * total + i];
-  out[i] = s;
+  if (i < total)
+    for (int c = wave; c < nbrChunks; c += nw) s += partial[(int64_t)c * total + i];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && i < total) {
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += red[w][lane];
+    out[i] = t;
+  }
 }
 
 template <int K>
@@ -236,7 +246,7 @@ static int launch_dots(dh_context* c, const double* V, int nb, const double* W, 
   dim3 grid((nb + G - 1) / G, nbrChunks);
   hipLaunchKernelGGL((dh_block_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, W, dim, rowsPerChunk,
                      scratch);
-  hipLaunchKernelGGL(dh_reduce_chunks_kernel, dim3((total + 255) / 256), dim3(256), 0, c->stream, scratch,
+  hipLaunchKernelGGL(dh_reduce_chunks_kernel, dim3((total + 63) / 64), dim3(1024), 0, c->stream, scratch,
                      nbrChunks, total, d_out);
   DH_HIP(c, hipGetLastError());
   double* pin;

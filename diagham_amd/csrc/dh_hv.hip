@@ -385,6 +385,7 @@ extern "C" int dh_hv(dh_hamiltonian* h, const double* x, double* y, int k, int a
       default: return dh_fail(c, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
     }
   }
+  if (h->kind == DH_HAM_SPHERE_NBODY) return dh_nbody_hv(h, x, y, k, add, first, count);
   return dh_fail(c, DH_E_STATE, "unknown Hamiltonian kind");
 }
 
@@ -488,6 +489,7 @@ extern "C" int dh_hamiltonian_destroy(dh_hamiltonian* h) {
   if (h->d_packedFactors) hipFree(h->d_packedFactors);
   if (h->d_sumBase) hipFree(h->d_sumBase);
   if (h->d_cg) hipFree(h->d_cg);
+  dh_nbody_free(h);
   delete h;
   return DH_OK;
 }

@@ -50,7 +50,7 @@ struct dh_basis {
   uint32_t* d_binom = nullptr;
 };
 
-enum dh_ham_kind { DH_HAM_SPHERE = 1, DH_HAM_SPIN = 2 };
+enum dh_ham_kind { DH_HAM_SPHERE = 1, DH_HAM_SPIN = 2, DH_HAM_SPHERE_NBODY = 3 };
 
 struct dh_hamiltonian {
   dh_context* ctx = nullptr;
@@ -69,6 +69,15 @@ struct dh_hamiltonian {
   int nbrChannels = 0;
   double* d_cg = nullptr;  // [nbrPairs * nbrChannels]
   std::vector<double> channelWeights;
+  // n-body term with explicit per-sum factor matrices: tuples as orbital bit masks grouped by
+  // orbital sum, the position of each tuple in its sum (by combinatorial rank of the tuple)
+  int nbody = 0, nbrTuples = 0, nbrTupleSums = 0;
+  int64_t tuplePosSize = 0, nbrNBodyFactors = 0;
+  uint64_t* d_tupleMask = nullptr;
+  int* d_tuplePos = nullptr;         // [C(nbrOrbitals, nbody)], -1 if the tuple is not listed
+  int* d_sumTupleOff = nullptr;      // [nbrTupleSums + 1]
+  int64_t* d_sumFactorOff = nullptr;  // [nbrTupleSums + 1]
+  double* d_nbodyFactors = nullptr;
   // spin chain couplings
   double J[32] = {0}, Jz[32] = {0};
   int periodic = 0;
@@ -125,6 +134,10 @@ int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_
                            int64_t* totalTiles);
 // complex product over a momentum set (dh_lattice.hip)
 int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active);
+// n-body product and element count (dh_nbody.hip)
+int dh_nbody_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count);
+int dh_nbody_count_elements(dh_hamiltonian* h, int64_t* nnz);
+void dh_nbody_free(dh_hamiltonian* h);
 
 // error helpers
 int dh_fail(dh_context* ctx, int code, const std::string& msg);

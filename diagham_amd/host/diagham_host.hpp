@@ -167,6 +167,21 @@ class ParticleOnSphereGenericHamiltonian : public AbstractHamiltonian {
   std::vector<int> PairOffset, ChannelM;
 };
 
+// Three-body term of AbstractQHEOnSphereNBodyInteractionHamiltonian
+// (FQHE/src/Hamiltonian/AbstractQHEOnSphereNBodyInteractionHamiltonian.cc:107) with explicit MN
+// factors: H = sum_S sum_{i,l} F_S[i][l] ProdAd(M_l) ProdA(N_i), the tuples of each orbital sum S
+// given in descending orbital order and F_S symmetric.
+class ParticleOnSphereNBodyHamiltonian : public AbstractHamiltonian {
+ public:
+  ParticleOnSphereNBodyHamiltonian(FermionOnSphere* particles, int nbody, const std::vector<int>& sumCount,
+                                   const std::vector<int>& tuples, const std::vector<double>& factors,
+                                   GPUArchitecture* architecture);
+
+ private:
+  FermionOnSphere* Particles;
+  int NBody;
+};
+
 // SpinChainHamiltonian (Spin/src/Hamiltonian/SpinChainHamiltonian.cc:97)
 class SpinChainHamiltonian : public AbstractHamiltonian {
  public:

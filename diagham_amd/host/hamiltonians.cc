@@ -85,6 +85,25 @@ ParticleOnSphereGenericHamiltonian::ParticleOnSphereGenericHamiltonian(FermionOn
               "set channels");
 }
 
+ParticleOnSphereNBodyHamiltonian::ParticleOnSphereNBodyHamiltonian(FermionOnSphere* particles, int nbody,
+                                                                   const std::vector<int>& sumCount,
+                                                                   const std::vector<int>& tuples,
+                                                                   const std::vector<double>& factors,
+                                                                   GPUArchitecture* architecture)
+    : Particles(particles), NBody(nbody) {
+  arch = architecture;
+  int64_t nt = 0, nf = 0;
+  for (int c : sumCount) {
+    nt += c;
+    nf += (int64_t)c * c;
+  }
+  if ((int64_t)tuples.size() != nt * nbody || (int64_t)factors.size() != nf)
+    throw std::invalid_argument("tuple or factor count does not match the per-sum tuple counts");
+  arch->Check(dh_sphere_nbody_hamiltonian_create(arch->Context(), &ham, particles->Basis(), nbody, sumCount.data(),
+                                                 (int)sumCount.size(), tuples.data(), factors.data()),
+              "ParticleOnSphereNBodyHamiltonian");
+}
+
 void ParticleOnSphereGenericHamiltonian::Bilinears(const double* d_psi, const double* d_phi, double* g,
                                                    double* d_hpsi) {
   arch->Check(dh_sphere_adjoint(ham, d_psi, d_phi, g, d_hpsi), "Bilinears");

@@ -72,6 +72,29 @@ extern "C" int dhh_sphere_problem_create(dhh_arch* a, dhh_problem** out, int n, 
   });
 }
 
+extern "C" int dhh_sphere_nbody_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
+                                               int nbody, const int* sumCount, int nbrSums, const int* tuples,
+                                               const double* factors) {
+  if (!a || !out || !sumCount || !tuples || !factors || nbrSums <= 0 || nbody <= 0) return DH_E_ARG;
+  return guarded([&] {
+    auto p = std::make_unique<dhh_problem>();
+    p->arch = a;
+    auto* space = new FermionOnSphere(a->arch.get(), n, totalLz, lzMax);
+    p->space.reset(space);
+    std::vector<int> counts(sumCount, sumCount + nbrSums);
+    int64_t nt = 0, nf = 0;
+    for (int c : counts) {
+      nt += c;
+      nf += (int64_t)c * c;
+    }
+    std::vector<int> t(tuples, tuples + nt * nbody);
+    std::vector<double> f(factors, factors + nf);
+    p->hamiltonian.reset(new ParticleOnSphereNBodyHamiltonian(space, nbody, counts, t, f, a->arch.get()));
+    *out = p.release();
+    return DH_OK;
+  });
+}
+
 extern "C" int dhh_spin_problem_create(dhh_arch* a, dhh_problem** out, int L, int sz, const double* j,
                                        const double* jz, int periodic) {
   if (!a || !out || !j || !jz) return DH_E_ARG;

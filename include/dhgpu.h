@@ -96,6 +96,17 @@ int dh_basis_find_state_index(dh_basis* basis, const uint64_t* states, int64_t n
  * table that is not symmetric to 1e-12 is refused with DH_E_ARG. */
 int dh_sphere_hamiltonian_create(dh_context* ctx, dh_hamiltonian** ham, dh_basis* basis,
                                  const double* factors, const int* pair_offset, int nbr_factors);
+/* Three-body term of AbstractQHEOnSphereNBodyInteractionHamiltonian with explicit MN factors
+ * (FQHE/src/Hamiltonian/AbstractQHEOnSphereNBodyInteractionHamiltonian.h:259-283, the
+ * MNNBodyInteractionFactors branch; FermionOnSphere::ProdA :716 / ProdAd :875):
+ *   H = sum_S sum_{i,l} F_S[i][l] ProdAd(M_l) ProdA(N_i).
+ * nbody must be 3.  Orbital sum S = 0 .. nbr_sums-1 lists sum_count[S] tuples of three orbitals in
+ * descending order (as GetAllSkewSymmetricIndices :1600 orders them), the sums' tuples one after
+ * the other in `tuples`; F_S (sum_count[S]^2 doubles, row i = annihilated tuple i) follow each
+ * other in `factors`.  F_S must be symmetric (the product is gathered per destination row). */
+int dh_sphere_nbody_hamiltonian_create(dh_context* ctx, dh_hamiltonian** ham, dh_basis* basis, int nbody,
+                                       const int* sum_count, int nbr_sums, const int* tuples,
+                                       const double* factors);
 /* Per-channel factor split used by the adjoint kernel: the two-body factor of (pair a ->
  * pair b) is sum_c V_c * (-4) * cg[a * nbr_channels + c] * cg[b * nbr_channels + c], with
  * V_c = channel_weights[c]. */

@@ -32,6 +32,12 @@ dh_context* dhh_arch_context(dhh_arch* arch);
  * pseudopotentials V_0..V_lzMax (FQHESphereFermionsTwoBodyGeneric.cc:278-306) */
 int dhh_sphere_problem_create(dhh_arch* arch, dhh_problem** problem, int nbr_fermions, int total_lz, int lz_max,
                               const double* pseudopotentials);
+/* FermionOnSphere(nbrFermions, totalLz, lzMax) + a three-body term with explicit MN factors
+ * (AbstractQHEOnSphereNBodyInteractionHamiltonian.h:259-283); tuples and factors in the layout
+ * of dh_sphere_nbody_hamiltonian_create (nbody = 3) */
+int dhh_sphere_nbody_problem_create(dhh_arch* arch, dhh_problem** problem, int nbr_fermions, int total_lz, int lz_max,
+                                    int nbody, const int* sum_count, int nbr_sums, const int* tuples,
+                                    const double* factors);
 /* Spin1_2Chain(L, sz) + SpinChainHamiltonian(j, jz, periodic) (GenericOpenSpinChain.cc:333-355);
  * j, jz: L entries */
 int dhh_spin_problem_create(dhh_arch* arch, dhh_problem** problem, int chain_length, int sz, const double* j,

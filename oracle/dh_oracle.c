@@ -339,6 +339,96 @@ long or_fs_adad(void* p, int m1, int m2, double* coefficient) {
   return fs_adad_local(s, s->prodATmp, s->prodALzMax, m1, m2, coefficient);
 }
 
+/* FermionOnSphere.cc:716 ProdA: prod_i a_{n[i]} with the rightmost operator n[nbr-1] applied
+ * first; the sign of each annihilation counts the occupied orbitals at and above it.  Returns
+ * 0 if an orbital is empty; the intermediate state and its highest orbital go to *tmpOut, *lzOut. */
+static double fs_prod_a_local(const fs_space* s, long index, const int* n, int nbr, uint64_t* tmpOut, int* lzOut) {
+  uint64_t tmp = s->states[index];
+  int lz = s->stateLzMax[index];
+  double c = 1.0;
+  for (int i = nbr - 1; i >= 0; --i) {
+    int k = n[i];
+    if ((tmp & (0x1ul << k)) == 0x0ul) {
+      *tmpOut = tmp;
+      *lzOut = lz;
+      return 0.0;
+    }
+    c *= fs_sign(s, tmp, k);
+    tmp &= ~(0x1ul << k);
+  }
+  if (tmp == 0x0ul)
+    lz = 0;
+  else
+    while (((tmp >> lz) == 0) && (lz > 0)) --lz;
+  *tmpOut = tmp;
+  *lzOut = lz;
+  return c;
+}
+
+/* FermionOnSphere.cc:875 ProdAd on the state kept by ProdA: prod_i a+_{m[i]}, rightmost first;
+ * an orbital above the current highest one gives no sign.  Returns the target index (dim if an
+ * orbital is already occupied). */
+static long fs_prod_ad_local(const fs_space* s, uint64_t tmp, int lz, const int* m, int nbr, double* coefficient) {
+  double c = 1.0;
+  int newLz = lz;
+  for (int i = nbr - 1; i >= 0; --i) {
+    int k = m[i];
+    if ((tmp & (0x1ul << k)) != 0x0ul) {
+      *coefficient = 0.0;
+      return s->dim;
+    }
+    if (k > newLz)
+      newLz = k;
+    else
+      c *= fs_sign(s, tmp, k);
+    tmp |= (0x1ul << k);
+  }
+  *coefficient = c;
+  return or_fs_find_state_index((void*)s, tmp, newLz);
+}
+
+double or_fs_prod_a(void* p, long index, const int* n, int nbr) {
+  fs_space* s = (fs_space*)p;
+  return fs_prod_a_local(s, index, n, nbr, &s->prodATmp, &s->prodALzMax);
+}
+
+long or_fs_prod_ad(void* p, const int* m, int nbr, double* coefficient) {
+  fs_space* s = (fs_space*)p;
+  return fs_prod_ad_local(s, s->prodATmp, s->prodALzMax, m, nbr, coefficient);
+}
+
+/* AbstractQHEOnSphereNBodyInteractionHamiltonian::EvaluateMNNBodyAddMultiplyComponent with
+ * explicit MN factors (AbstractQHEOnSphereNBodyInteractionHamiltonian.h:259-283), scatter form:
+ * y[ProdAd(M_l)] += coef(ProdAd) * F_S[i][l] * ProdA(N_i) * x[index] for every index, every
+ * orbital sum S, every annihilated tuple N_i and created tuple M_l of that sum.  Tuples are
+ * nbody orbitals each, grouped by sum (sumCount[S] tuples of sum S, S = 0..nbrSums-1); the
+ * factor matrices F_S (sumCount[S]^2, row i = annihilated tuple) follow each other. */
+void or_nbody_hv(void* p, int nbody, int nbrSums, const int* sumCount, const int* tuples, const double* factors,
+                 const double* x, double* y) {
+  fs_space* s = (fs_space*)p;
+  for (long index = 0; index < s->dim; ++index) {
+    const int* T = tuples;
+    const double* F = factors;
+    for (int S = 0; S < nbrSums; ++S) {
+      const int cnt = sumCount[S];
+      for (int i = 0; i < cnt; ++i) {
+        uint64_t tmp;
+        int lz;
+        double c3 = fs_prod_a_local(s, index, T + (size_t)i * nbody, nbody, &tmp, &lz);
+        if (c3 == 0.0) continue;
+        double c2 = c3 * x[index];
+        for (int l = 0; l < cnt; ++l) {
+          double c;
+          long t = fs_prod_ad_local(s, tmp, lz, T + (size_t)l * nbody, nbody, &c);
+          if (t < s->dim) y[t] += c * F[(size_t)i * cnt + l] * c2;
+        }
+      }
+      T += (size_t)cnt * nbody;
+      F += (size_t)cnt * cnt;
+    }
+  }
+}
+
 /* ======================================================================
  * Clebsch-Gordan coefficients: src/MathTools/ClebschGordanCoefficients.cc
  * (Schulten-Gordon recursion, :225 EvaluateClebschGordanCoefficients, :346 recursion)

@@ -27,6 +27,12 @@ long or_fs_find_state_index(void* space, uint64_t state, int lzmax);
 long or_fs_adadaa(void* space, long index, int m1, int m2, int n1, int n2, double* coef);
 double or_fs_aa(void* space, long index, int n1, int n2);
 long or_fs_adad(void* space, int m1, int m2, double* coef);
+/* n-body operators (FermionOnSphere.cc:716 ProdA, :875 ProdAd) and the n-body product with
+ * explicit per-sum factor matrices (AbstractQHEOnSphereNBodyInteractionHamiltonian.h:259) */
+double or_fs_prod_a(void* space, long index, const int* n, int nbrIndices);
+long or_fs_prod_ad(void* space, const int* m, int nbrIndices, double* coef);
+void or_nbody_hv(void* space, int nbody, int nbrSums, const int* sumCount, const int* tuples, const double* factors,
+                 const double* x, double* y);
 
 /* ---------------- Clebsch-Gordan / pseudopotentials / factors ---------------- */
 void* or_cg_create(int j1, int j2);

@@ -44,6 +44,9 @@ def lib():
     _sig(L, "or_fs_adadaa", c_long, [c_void_p, c_long, c_int, c_int, c_int, c_int, P_D])
     _sig(L, "or_fs_aa", c_double, [c_void_p, c_long, c_int, c_int])
     _sig(L, "or_fs_adad", c_long, [c_void_p, c_int, c_int, P_D])
+    _sig(L, "or_fs_prod_a", c_double, [c_void_p, c_long, P_I, c_int])
+    _sig(L, "or_fs_prod_ad", c_long, [c_void_p, P_I, c_int, P_D])
+    _sig(L, "or_nbody_hv", None, [c_void_p, c_int, c_int, P_I, P_I, P_D, P_D, P_D])
     _sig(L, "or_cg_create", c_void_p, [c_int, c_int])
     _sig(L, "or_cg_destroy", None, [c_void_p])
     _sig(L, "or_cg_get", c_double, [c_void_p, c_int, c_int, c_int])
@@ -126,6 +129,16 @@ class FermionSphere:
         idx = lib().or_fs_adad(self.h, m1, m2, ctypes.byref(c))
         return idx, c.value
 
+    def prod_a(self, i, n):
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        return lib().or_fs_prod_a(self.h, i, n.ctypes.data_as(P_I), n.size)
+
+    def prod_ad(self, m):
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        c = c_double(0.0)
+        idx = lib().or_fs_prod_ad(self.h, m.ctypes.data_as(P_I), m.size, ctypes.byref(c))
+        return idx, c.value
+
 
 class Factors:
     """Interaction factor tables of ParticleOnSphereGenericHamiltonian (fermions)."""
@@ -184,6 +197,44 @@ def fs_hv(space, factors, x, shift=0.0, hermitian=False, threads=1):
         lib().or_fs_hv_smp(space.h, factors.h, dptr(x), dptr(y), shift, int(hermitian), threads)
     else:
         lib().or_fs_hv(space.h, factors.h, dptr(x), dptr(y), shift, int(hermitian))
+    return y
+
+
+def nbody_tuples(nbr_orbitals, nbody):
+    """Orbital tuples (descending within a tuple) grouped by orbital sum, in the order of
+    AbstractQHEOnSphereNBodyInteractionHamiltonian::GetAllSkewSymmetricIndices (:1600):
+    lexicographically descending.  Returns (per-sum counts, flat tuple array)."""
+    import itertools
+    combos = sorted(itertools.combinations(range(nbr_orbitals), nbody), reverse=True)
+    combos = [tuple(sorted(c, reverse=True)) for c in combos]
+    combos.sort(reverse=True)
+    nsums = nbody * (nbr_orbitals - 1) + 1
+    per = [[] for _ in range(nsums)]
+    for c in combos:
+        per[sum(c)].append(c)
+    counts = np.array([len(p) for p in per], np.int32)
+    flat = np.array([x for p in per for c in p for x in c], np.int32)
+    return counts, flat
+
+
+def random_nbody_factors(counts, seed):
+    """Random symmetric factor matrix per orbital sum, concatenated."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in counts:
+        a = rng.standard_normal((c, c))
+        out.append(((a + a.T) / 2).ravel())
+    return np.concatenate(out) if out else np.zeros(0)
+
+
+def nbody_hv(space, nbody, counts, tuples, factors, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    counts = np.ascontiguousarray(counts, dtype=np.int32)
+    tuples = np.ascontiguousarray(tuples, dtype=np.int32)
+    factors = np.ascontiguousarray(factors, dtype=np.float64)
+    lib().or_nbody_hv(space.h, nbody, counts.size, counts.ctypes.data_as(P_I), tuples.ctypes.data_as(P_I),
+                      dptr(factors), dptr(x), dptr(y))
     return y
 
 

@@ -20,7 +20,13 @@
 #include "dh_internal.h"
 #include "dh_sphere_row.h"
 
-enum { MODE_BSEARCH = 0, MODE_RANK_LDS = 1, MODE_RANK_GLOBAL = 2 };
+// index lookup of the product kernels; the _X4 forms unroll the created-pair loop four-wide
+enum { MODE_BSEARCH = 0, MODE_RANK_LDS = 1, MODE_RANK_GLOBAL = 2, MODE_RANK_LDS_X4 = 3, MODE_RANK_GLOBAL_X4 = 4 };
+
+template <int MODE>
+struct ModeBase {
+  static constexpr int value = MODE >= MODE_RANK_LDS_X4 ? MODE - 2 : MODE;
+};
 
 struct SphereArgs {
   const uint64_t* states;
@@ -90,10 +96,11 @@ template <int K, int LPR, int MODE>
 __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const double* __restrict__ x,
                                                          double* __restrict__ y, int add, int64_t first,
                                                          int64_t count) {
+  constexpr int BASE = ModeBase<MODE>::value;
   double* sF;
   int *sBase, *sLut = nullptr;
   uint32_t *sLo = nullptr, *sHi = nullptr;
-  sphere_stage<MODE>(a, sF, sBase, sLut, sLo, sHi);
+  sphere_stage<BASE>(a, sF, sBase, sLut, sLo, sHi);
   const int lane = threadIdx.x % LPR;
   const int64_t rowsPerBlock = blockDim.x / LPR;
   const int64_t nbrTiles = (count + rowsPerBlock - 1) / rowsPerBlock;
@@ -106,8 +113,11 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
     double acc[K];
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
-    auto find = [&](uint64_t t) -> int64_t { return sphere_find<MODE>(a, t, sLut, sLo, sHi); };
-    dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+    auto find = [&](uint64_t t) -> int64_t { return sphere_find<BASE>(a, t, sLut, sLo, sHi); };
+    if constexpr (MODE >= MODE_RANK_LDS_X4)
+      dh_sphere_row_x4<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+    else
+      dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)
@@ -284,7 +294,7 @@ static int sphere_launch(dh_hamiltonian* h, const double* x, double* y, int add,
   dh_context* c = h->ctx;
   int lutBitsUsed = std::min(h->basis->lutBits, 13);
   SphereArgs a = sphere_args(h, lutBitsUsed);
-  size_t lds = sphere_lds(h, MODE, lutBitsUsed);
+  size_t lds = sphere_lds(h, ModeBase<MODE>::value, lutBitsUsed);
   if (lds > kLdsLimit) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
   auto kern = sphere_hv_kernel<K, LPR, MODE>;
   if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -301,7 +311,12 @@ static int sphere_dispatch_mode(dh_hamiltonian* h, const double* x, double* y, i
                                 int64_t count) {
   if (h->indexMode == DH_INDEX_BSEARCH) return sphere_launch<K, LPR, MODE_BSEARCH>(h, x, y, add, first, count);
   size_t ldsRank = sphere_lds(h, MODE_RANK_LDS, 0);
-  if (h->ldsTables && ldsRank <= 64 * 1024) return sphere_launch<K, LPR, MODE_RANK_LDS>(h, x, y, add, first, count);
+  const bool tablesInLds = h->ldsTables && ldsRank <= 64 * 1024;
+  if (h->indexMode == DH_INDEX_RANK_X4) {
+    if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_X4>(h, x, y, add, first, count);
+    return sphere_launch<K, LPR, MODE_RANK_GLOBAL_X4>(h, x, y, add, first, count);
+  }
+  if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS>(h, x, y, add, first, count);
   return sphere_launch<K, LPR, MODE_RANK_GLOBAL>(h, x, y, add, first, count);
 }
 
@@ -507,7 +522,8 @@ extern "C" int dh_hamiltonian_set_launch(dh_hamiltonian* h, int lpr, int blockSi
   if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8) return dh_fail(h->ctx, DH_E_ARG, "lanes_per_row in {1,2,4,8}");
   if (h->kind == DH_HAM_SPIN && lpr != 1) return dh_fail(h->ctx, DH_E_ARG, "spin kernels use one row per lane");
   if (blockSize < 64 || blockSize > 1024 || (blockSize % 64)) return dh_fail(h->ctx, DH_E_ARG, "block_size");
-  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK) return dh_fail(h->ctx, DH_E_ARG, "index_mode");
+  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_X4)
+    return dh_fail(h->ctx, DH_E_ARG, "index_mode");
   h->lanesPerRow = lpr;
   h->blockSize = blockSize;
   h->indexMode = mode;

@@ -60,3 +60,45 @@ __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const
     }
   }
 }
+
+// The same terms with the created-pair loop unrolled four-wide: a lane computes four targets
+// (a slot past the last pair or on an occupied orbital looks up the row's own state with a zero
+// coefficient), issues their index lookups together and then their four gathers, so four
+// dependent chains are in flight per lane.  `s` must be a state of the space (the padding
+// slots look it up).
+template <int K, int LPR, typename Find>
+__device__ __forceinline__ void dh_sphere_row_x4(uint64_t s, int nbrOrbitals, const double* __restrict__ sF,
+                                                 const int* __restrict__ sBase, int lane, Find find,
+                                                 const double* __restrict__ xbase, double* acc) {
+  for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+    const int n1 = __builtin_ctzll(r1);
+    const int pc1 = __popcll(s >> n1);
+    for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+      const int n2 = __builtin_ctzll(r2);
+      const int sgnA = pc1 + __popcll(s >> n2);
+      const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+      const int sum = n1 + n2;
+      const int m3lo = (sum >> 1) + 1;
+      const int m3hi = min(sum, nbrOrbitals - 1);
+      const double* fS = sF + sBase[sum];
+      const int i = n1 - m3lo;
+      for (int m3b = m3lo + lane; m3b <= m3hi; m3b += 4 * LPR) {
+        int64_t idx[4];
+        double coef[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int m3 = min(m3b + u * LPR, m3hi);  // clamped so that every shift stays in range
+          const int m4 = sum - m3;
+          const uint64_t cm = (1ull << m3) | (1ull << m4);
+          const bool ok = (m3b + u * LPR <= m3hi) && !(t & cm);
+          const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          const double c = ok ? fS[dh_packed_index(i, m3 - m3lo)] : 0.0;
+          coef[u] = (sg & 1) ? -c : c;
+          idx[u] = find(ok ? (t | cm) : s);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dh_gather_fma<K>(acc, xbase, idx[u], coef[u]);
+      }
+    }
+  }
+}

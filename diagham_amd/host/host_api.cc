@@ -166,7 +166,7 @@ extern "C" int dhh_autotune(dhh_problem* p, const int* ks, int nbrKs, int repeat
       arch->Check(dh_fill_random(c, x, dim * k, 11), "random");
       double bestMs = 1e300;
       int cfg[4] = {1, 256, DH_INDEX_RANK, 1};
-      for (int mode : {DH_INDEX_RANK, DH_INDEX_BSEARCH})
+      for (int mode : {DH_INDEX_RANK, DH_INDEX_RANK_X4, DH_INDEX_BSEARCH})
         for (int lpr : {1, 2, 4, 8}) {
           if (spin && lpr != 1) continue;
           for (int bs : {128, 256, 512, 1024}) {

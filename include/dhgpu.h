@@ -42,6 +42,8 @@ typedef struct dh_hamiltonian dh_hamiltonian;
 /* index lookup used by the H.v kernels */
 #define DH_INDEX_BSEARCH 0 /* top-bit LUT staged in LDS, then binary search of the sorted basis */
 #define DH_INDEX_RANK 1    /* combinatorial rank: two table lookups, no basis probes */
+#define DH_INDEX_RANK_X4 2 /* rank, with the two-body created-pair loop unrolled four-wide (sphere only;
+                              other Hamiltonians treat it as DH_INDEX_RANK) */
 
 /* ---------------------------------------------------------------- context
  * Replaces the per-operation thread fork/join of SMPArchitecture::SendJobs

@@ -70,7 +70,7 @@ def test_element_count(cfg1, golden):
 
 
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2)])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2), (2, 1), (2, 8)])
 def test_hv_parity(cfg1, k, mode, lpr):
     """H.X (k vectors, [row][k] layout) vs the oracle, 1e-12 relative norm per vector."""
     ref_space = O.FermionSphere(8, 0, 21)
@@ -141,7 +141,7 @@ def test_autotune_keeps_parity(cfg1):
     assert best in (1, 4, 8)
     for r in rows:
         assert r["lanes_per_row"] in (1, 2, 4, 8) and r["block_size"] in (128, 256, 512, 1024)
-        assert r["index_mode"] in (0, 1) and r["ms"] > 0
+        assert r["index_mode"] in (0, 1, 2) and r["ms"] > 0
         assert r["elements_per_s"] > 0
     rates = {int(r["k"]): r["elements_per_s"] for r in rows}
     assert rates[best] == max(rates.values())

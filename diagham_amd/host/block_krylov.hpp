@@ -2,13 +2,11 @@
 // (FullReorthogonalizedBlockLanczosAlgorithm.cc) and the drivers that advance many sectors
 // together.  The Krylov basis stays on the device; the host keeps the small k x k blocks.
 //
-// Orthogonalisation.  The new block W = H X_j is projected out of the basis and Cholesky-QR'd
-// (block classical Gram-Schmidt).  Like the reference, which orthogonalises once per step, one
-// projection is kept when it was well conditioned (it left more than 1e-4 of every column's
-// squared norm and the block's pivots are above 1e-4): the basis then stays orthogonal to ~1e-12,
-// and one more Cholesky-QR of the block alone tidies its columns.  Otherwise a second round of
-// projection + Cholesky-QR runs for that sector; it removes what the first Cholesky-QR amplified
-// when the columns of W were nearly dependent, and it exposes columns that were only noise.
+// Orthogonalisation.  The new block W = H X_j goes through two rounds of "project out the basis,
+// then Cholesky-QR of the block" (block classical Gram-Schmidt with reorthogonalisation).  The
+// second projection removes what the first Cholesky-QR amplified when the columns of W were
+// nearly dependent, keeps the basis orthonormal to rounding level (which the 1e-14 relative
+// convergence test needs), and exposes columns that were only noise.
 //
 // Rank deficiency.  A column is deficient when its Cholesky pivot is negligible against its own
 // norm (it depends on the previous columns of the block), or when a projection leaves only
@@ -345,17 +343,13 @@ class SegmentedBlockLanczos {
 
   // One Cholesky-QR of W over the segments act[i], i in idx, with the deficiency tests of
   // CholeskySemidefinite (ref[i]: squared norms removed by the projection just before, if any).
-  // out[i].R <- Rr out[i].R.  Returns, per position in idx, whether the pass was well conditioned:
-  // no deficient column, every column kept more than 1e-4 of its squared norm through that
-  // projection and had a pivot above 1e-4 of its squared norm.  Orthogonality to the basis is
-  // then within 1e-12 without another projection.
-  std::vector<char> CholQR(double* W, const std::vector<int>& act, const std::vector<int>& idx,
-                           std::vector<Rounds>& out, std::vector<std::vector<double>>& ref, double collapse) {
+  // out[i].R <- Rr out[i].R.
+  void CholQR(double* W, const std::vector<int>& act, const std::vector<int>& idx, std::vector<Rounds>& out,
+              std::vector<std::vector<double>>& ref, double collapse) {
     const int kk = K * K, KK = KR * KR;
     std::vector<unsigned char> mask = Mask(act, idx);
     std::vector<double> G(idx.size() * KK), M(idx.size() * KK);
     std::vector<T> GT((size_t)kk);
-    std::vector<char> well(idx.size(), 1);
     Ops->Dots(W, 1, W, mask.data(), G.data());
     for (size_t a = 0; a < idx.size(); ++a) {
       Rounds& o = out[idx[a]];
@@ -365,24 +359,18 @@ class SegmentedBlockLanczos {
       std::vector<T> Rr;
       std::vector<char> good;
       CholeskySemidefinite(K, GT.data(), r.data(), collapse, Rr, good);
-      for (int p = 0; p < K; ++p) {
-        const double g = std::real(GT[(size_t)p * K + p]);
-        const double rp = std::real(Rr[(size_t)p * K + p]);
-        if (!good[p] || !(g > 1e-4 * r[p]) || !(rp * rp > 1e-4 * g)) well[a] = 0;
-      }
       S::ToDevice(K, PseudoInverse(K, Rr, good).data(), &M[a * KK]);
       o.R = Mul(K, Rr, o.R);
       o.good = good;
     }
     Ops->Rmul(W, mask.data(), M.data());
-    return well;
   }
 
-  // Projection + Cholesky-QR of W over the segments in `act`: W_in = V C + Q R with
-  // C = C1 + C2 R1 and R = R2 R1 when a second round runs.  The second projection runs for the
-  // segments whose first round was not well conditioned (all of them if !adaptive); the others
-  // get one more Cholesky-QR of the block alone.  Deficient columns of Q are zero.
-  void OrthoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out, bool adaptive) {
+  // Two rounds of projection + Cholesky-QR of W over the segments in `act`: W_in = V C + Q R
+  // with C = C1 + C2 R1 and R = R2 R1.  Deficient columns of Q are zero.  Both rounds always run:
+  // one projection leaves the basis orthogonal to ~1e-13 only, and the Ritz values then move by
+  // more than the 1e-14 relative change that the convergence test asks for.
+  void OrthoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
     const int kk = K * K;
     const size_t na = act.size();
     out.assign(na, Rounds());
@@ -397,24 +385,19 @@ class SegmentedBlockLanczos {
     std::vector<std::vector<double>> ref(na, std::vector<double>(K, 0.0));
     if (nb > 0) Project(W, nb, act, all, out, true, ref);
     // noise left by one projection is ~1e-15 of the column
-    std::vector<char> well = CholQR(W, act, all, out, ref, 1e-24);
-    std::vector<int> again, refine;
-    for (size_t a = 0; a < na; ++a) (adaptive && well[a] ? refine : again).push_back((int)a);
+    CholQR(W, act, all, out, ref, 1e-24);
     for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
-    if (!again.empty()) {
-      if (nb > 0) Project(W, nb, act, again, out, false, ref);
-      // after the second projection, a column that kept less than 1e-8 of itself was
-      // (normalised) noise inside the span of the basis
-      CholQR(W, act, again, out, ref, 1e-16);
-    }
-    if (!refine.empty()) CholQR(W, act, refine, out, ref, 0.0);
+    if (nb > 0) Project(W, nb, act, all, out, false, ref);
+    // after the second projection, a column that kept less than 1e-8 of itself was
+    // (normalised) noise inside the span of the basis
+    CholQR(W, act, all, out, ref, 1e-16);
   }
 
   // W (nb basis blocks before it) -> orthonormal Q with W_in = V C + Q R per segment of `act`;
   // deficient columns are replaced by random directions where the segment has room left
   void Orthonormalize(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
     const int kk = K * K, KK = KR * KR;
-    OrthoRounds(W, nb, act, out, true);
+    OrthoRounds(W, nb, act, out);
     std::vector<int> need, needSeg;
     for (size_t a = 0; a < act.size(); ++a)
       if (std::find(out[a].good.begin(), out[a].good.end(), 0) != out[a].good.end()) {
@@ -437,7 +420,7 @@ class SegmentedBlockLanczos {
     Ops->Axpy(Rnd, 1, W, mask.data(), Sel.data());
     // Q_rep = V C' + Q' R'  =>  W_in = V (C + C' R) + Q' (R' R)  (R has zero rows where Q_rep is random)
     std::vector<Rounds> rep;
-    OrthoRounds(W, nb, needSeg, rep, false);
+    OrthoRounds(W, nb, needSeg, rep);
     for (size_t i = 0; i < need.size(); ++i) {
       Rounds& o = out[need[i]];
       std::vector<T> add = Mul(K, rep[i].coef, o.R);

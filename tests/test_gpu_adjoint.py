@@ -92,7 +92,9 @@ def make_functions(torch, arch, capi, n, total_lz, lz):
             p = capi.Problem.sphere(arch, n, total_lz, lz, V.detach().cpu().numpy())
             vecs = torch.empty((nbr, p.dim), dtype=torch.float64, device="cuda")
             torch.cuda.synchronize()
-            r = p.lanczos(nbr, block_size=1, residual=1e-11, d_vectors=vecs.data_ptr(), nbr_vectors=nbr)
+            r = p.lanczos(nbr, block_size=1, max_iter=800, residual=1e-11, d_vectors=vecs.data_ptr(),
+                          nbr_vectors=nbr)
+            assert r["converged"], "Lanczos did not converge within 800 products"
             G = torch.zeros((nbr, lz + 1), dtype=torch.float64)
             idx = torch.tensor(p.channels(), dtype=torch.long)
             for i in range(nbr):

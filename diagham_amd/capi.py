@@ -91,6 +91,8 @@ HOST_SIGNATURES = {
     "dhh_arch_context": (c_void_p, [c_void_p]),
     "dhh_sphere_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_D]),
     "dhh_sphere_nbody_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, c_int, P_I, c_int, P_I, P_D]),
+    "dhh_sphere_l2_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_D, c_double]),
+    "dhh_sphere_l2_factors": (c_int, [c_int, c_int, c_int, c_double, P_D, P_I, P_I, P_D]),
     "dhh_spin_problem_create": (c_int, [c_void_p, PP, c_int, c_int, P_D, P_D, c_int]),
     "dhh_problem_destroy": (c_int, [c_void_p]),
     "dhh_problem_basis": (c_void_p, [c_void_p]),
@@ -175,6 +177,22 @@ def sphere_factors(lz_max, pseudo):
     if rc != 0:
         raise DHError("dhh_sphere_factors failed: %d" % rc)
     return f[:n.value].copy(), off
+
+
+def sphere_l2_factors(lz_max, n, total_lz, l2_factor=1.0):
+    """factors of l2_factor * L^2 in the sphere factor layout, the pair offsets and the constant
+    l2_factor * Lz (Lz + 1) of the sector (dhh_sphere_l2_factors)."""
+    n_orb = lz_max + 1
+    cap = n_orb * n_orb * n_orb
+    f = np.zeros(cap)
+    off = np.zeros(n_orb * (n_orb - 1) // 2 + 1, np.int32)
+    nf = ctypes.c_int(cap)
+    shift = ctypes.c_double()
+    rc = host().dhh_sphere_l2_factors(lz_max, n, total_lz, l2_factor, dptr(f), off.ctypes.data_as(P_I),
+                                      ctypes.byref(nf), ctypes.byref(shift))
+    if rc != 0:
+        raise DHError("dhh_sphere_l2_factors: " + host().dhh_last_error().decode())
+    return f[:nf.value].copy(), off, shift.value
 
 
 def sphere_channel_table(lz_max):
@@ -298,6 +316,19 @@ class Problem:
         if rc != 0:
             raise DHError("sphere problem: " + host().dhh_last_error().decode())
         p = cls(arch, h, "sphere")
+        p.n, p.total_lz, p.lz_max = n, total_lz, lz_max
+        return p
+
+    @classmethod
+    def sphere_l2(cls, arch, n, total_lz, lz_max, l2_factor=1.0, pseudo=None):
+        """l2_factor * L^2 (plus the pseudopotential interaction if given): dhh_sphere_l2_problem_create."""
+        pp = None if pseudo is None else np.ascontiguousarray(pseudo, dtype=np.float64)
+        h = c_void_p()
+        rc = host().dhh_sphere_l2_problem_create(arch.h, ctypes.byref(h), n, total_lz, lz_max,
+                                                 None if pp is None else dptr(pp), l2_factor)
+        if rc != 0:
+            raise DHError("L^2 problem: " + host().dhh_last_error().decode())
+        p = cls(arch, h, "sphere_l2")
         p.n, p.total_lz, p.lz_max = n, total_lz, lz_max
         return p
 

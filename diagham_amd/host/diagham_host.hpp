@@ -167,6 +167,24 @@ class ParticleOnSphereGenericHamiltonian : public AbstractHamiltonian {
   std::vector<int> PairOffset, ChannelM;
 };
 
+// Total angular momentum of fermions on the sphere as a two-body operator in the factor layout
+// of ParticleOnSphereGenericHamiltonian (the operator of ParticleOnSphereL2Hamiltonian,
+// FQHE/src/Hamiltonian/ParticleOnSphereL2Hamiltonian.cc:240): l2Factor * L^2 on the sector of
+// nbrParticles fermions with twice the total Lz equal to totalLz.  With pseudopotentials it is
+// added to their interaction (the --l2-factor option of the reference programs).
+class ParticleOnSphereL2Hamiltonian : public AbstractHamiltonian {
+ public:
+  ParticleOnSphereL2Hamiltonian(FermionOnSphere* particles, int nbrParticles, int lzmax, double l2Factor,
+                                GPUArchitecture* architecture, const std::vector<double>* pseudoPotential = nullptr);
+  static void EvaluateInteractionFactors(int lzMax, int nbrParticles, int totalLz, double l2Factor,
+                                         std::vector<double>& factors, std::vector<int>& pairOffset, double& shift);
+
+ private:
+  FermionOnSphere* Particles;
+  std::vector<double> Factors;
+  std::vector<int> PairOffset;
+};
+
 // Three-body term of AbstractQHEOnSphereNBodyInteractionHamiltonian
 // (FQHE/src/Hamiltonian/AbstractQHEOnSphereNBodyInteractionHamiltonian.cc:107) with explicit MN
 // factors: H = sum_S sum_{i,l} F_S[i][l] ProdAd(M_l) ProdA(N_i), the tuples of each orbital sum S

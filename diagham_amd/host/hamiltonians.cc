@@ -44,6 +44,73 @@ void ParticleOnSphereGenericHamiltonian::EvaluateInteractionFactors(int lzMax, c
   pairOffset.back() = (int)factors.size();
 }
 
+// Total angular momentum L^2 = L^- L^+ + Lz^2 + Lz of N fermions on the sphere, in the two-body
+// factor layout above (the operator of ParticleOnSphereL2Hamiltonian,
+// FQHE/src/Hamiltonian/ParticleOnSphereL2Hamiltonian.cc:240, derived here from
+// L^+ = sum_p c_p a+_{p+1} a_p with c_p = sqrt((lzMax - p)(p + 1))):
+//   L^- L^+ = sum_p c_p^2 n_p - sum_{p,q} c_p c_q a+_p a+_{q+1} a_{p+1} a_q.
+// The two-body part is reordered to a+_m3 a+_m4 a_n1 a_n2 (n1 > n2, m3 > m4); the one-body part,
+// inside a sector of N particles, is sum_{n1 > n2} (w_n1 + w_n2) n_n1 n_n2 / (N - 1) with
+// n_n1 n_n2 = -a+_n1 a+_n2 a_n1 a_n2.  Everything is scaled by l2Factor; Lz^2 + Lz is a constant
+// of the sector, returned as the shift l2Factor * Lz (Lz + 1), Lz = totalLz / 2.
+void ParticleOnSphereL2Hamiltonian::EvaluateInteractionFactors(int lzMax, int nbrParticles, int totalLz,
+                                                                double l2Factor, std::vector<double>& factors,
+                                                                std::vector<int>& pairOffset, double& shift) {
+  if (nbrParticles < 2) throw std::invalid_argument("L^2 as a two-body term needs at least two particles");
+  const int nOrb = lzMax + 1;
+  // zero table in the layout: pair (m1 > m2), created m3 from (m1 + m2) / 2 + 1 up
+  factors.clear();
+  pairOffset.assign((size_t)nOrb * (nOrb - 1) / 2 + 1, 0);
+  for (int m1 = 0; m1 < nOrb; ++m1)
+    for (int m2 = 0; m2 < m1; ++m2) {
+      pairOffset[(size_t)m1 * (m1 - 1) / 2 + m2] = (int)factors.size();
+      const int sum = m1 + m2;
+      factors.resize(factors.size() + (std::min(sum, lzMax) - (sum / 2 + 1) + 1), 0.0);
+    }
+  pairOffset.back() = (int)factors.size();
+  auto at = [&](int n1, int n2, int m3) -> double& {
+    return factors[pairOffset[(size_t)n1 * (n1 - 1) / 2 + n2] + m3 - ((n1 + n2) / 2 + 1)];
+  };
+  std::vector<double> c(nOrb);
+  for (int p = 0; p < nOrb; ++p) c[p] = std::sqrt((double)(lzMax - p) * (p + 1));
+  for (int p = 0; p < lzMax; ++p)
+    for (int q = 0; q < lzMax; ++q) {
+      if (p + 1 == q || p == q + 1) continue;  // a_{p+1} a_q or a+_p a+_{q+1} vanishes
+      const int a0 = p + 1, a1 = q, b0 = p, b1 = q + 1;
+      const int n1 = std::max(a0, a1), n2 = std::min(a0, a1), m3 = std::max(b0, b1);
+      const double sign = (a0 > a1 ? 1.0 : -1.0) * (b0 > b1 ? 1.0 : -1.0);
+      at(n1, n2, m3) -= l2Factor * sign * c[p] * c[q];
+    }
+  for (int n1 = 0; n1 < nOrb; ++n1)
+    for (int n2 = 0; n2 < n1; ++n2)
+      at(n1, n2, n1) -= l2Factor * (c[n1] * c[n1] + c[n2] * c[n2]) / (nbrParticles - 1);
+  const double lz = 0.5 * totalLz;
+  shift = l2Factor * lz * (lz + 1.0);
+}
+
+ParticleOnSphereL2Hamiltonian::ParticleOnSphereL2Hamiltonian(FermionOnSphere* particles, int nbrParticles,
+                                                             int lzmax, double l2Factor,
+                                                             GPUArchitecture* architecture,
+                                                             const std::vector<double>* pseudoPotential)
+    : Particles(particles) {
+  arch = architecture;
+  if (particles->GetNbrParticles() != nbrParticles || particles->GetLzMax() != lzmax)
+    throw std::invalid_argument("Hilbert space does not match the Hamiltonian parameters");
+  double l2Shift = 0.0;
+  EvaluateInteractionFactors(lzmax, nbrParticles, particles->GetTotalLz(), l2Factor, Factors, PairOffset, l2Shift);
+  if (pseudoPotential) {
+    if ((int)pseudoPotential->size() < lzmax + 1) throw std::invalid_argument("need lzMax + 1 pseudopotentials");
+    std::vector<double> v;
+    std::vector<int> off;
+    ParticleOnSphereGenericHamiltonian::EvaluateInteractionFactors(lzmax, *pseudoPotential, v, off);
+    for (size_t i = 0; i < v.size(); ++i) Factors[i] += v[i];
+  }
+  arch->Check(dh_sphere_hamiltonian_create(arch->Context(), &ham, particles->Basis(), Factors.data(),
+                                           PairOffset.data(), (int)Factors.size()),
+              "ParticleOnSphereL2Hamiltonian");
+  ShiftHamiltonian(l2Shift);
+}
+
 // channels of the adjoint split: one per relative momentum m (odd), the table holds
 // <m1 m2 | L> for every pair and L = lzMax - m
 void ParticleOnSphereGenericHamiltonian::EvaluateChannelTable(int lzMax, std::vector<int>& channelM,

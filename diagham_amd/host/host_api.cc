@@ -72,6 +72,38 @@ extern "C" int dhh_sphere_problem_create(dhh_arch* a, dhh_problem** out, int n, 
   });
 }
 
+extern "C" int dhh_sphere_l2_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
+                                            const double* pseudo, double l2Factor) {
+  if (!a || !out) return DH_E_ARG;
+  return guarded([&] {
+    auto p = std::make_unique<dhh_problem>();
+    p->arch = a;
+    auto* space = new FermionOnSphere(a->arch.get(), n, totalLz, lzMax);
+    p->space.reset(space);
+    std::vector<double> v;
+    if (pseudo) v.assign(pseudo, pseudo + lzMax + 1);
+    p->hamiltonian.reset(
+        new ParticleOnSphereL2Hamiltonian(space, n, lzMax, l2Factor, a->arch.get(), pseudo ? &v : nullptr));
+    *out = p.release();
+    return DH_OK;
+  });
+}
+
+extern "C" int dhh_sphere_l2_factors(int lzMax, int n, int totalLz, double l2Factor, double* factors,
+                                     int* pairOffset, int* nbr, double* shift) {
+  if (!factors || !pairOffset || !nbr || !shift) return DH_E_ARG;
+  return guarded([&] {
+    std::vector<double> f;
+    std::vector<int> off;
+    ParticleOnSphereL2Hamiltonian::EvaluateInteractionFactors(lzMax, n, totalLz, l2Factor, f, off, *shift);
+    if ((int)f.size() > *nbr) return DH_E_RANGE;
+    std::memcpy(factors, f.data(), sizeof(double) * f.size());
+    std::memcpy(pairOffset, off.data(), sizeof(int) * off.size());
+    *nbr = (int)f.size();
+    return DH_OK;
+  });
+}
+
 extern "C" int dhh_sphere_nbody_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
                                                int nbody, const int* sumCount, int nbrSums, const int* tuples,
                                                const double* factors) {

@@ -32,6 +32,16 @@ dh_context* dhh_arch_context(dhh_arch* arch);
  * pseudopotentials V_0..V_lzMax (FQHESphereFermionsTwoBodyGeneric.cc:278-306) */
 int dhh_sphere_problem_create(dhh_arch* arch, dhh_problem** problem, int nbr_fermions, int total_lz, int lz_max,
                               const double* pseudopotentials);
+/* FermionOnSphere(nbrFermions, totalLz, lzMax) + l2_factor * L^2 (ParticleOnSphereL2Hamiltonian,
+ * FQHE/src/Hamiltonian/ParticleOnSphereL2Hamiltonian.cc:240), plus the pseudopotential
+ * interaction if pseudopotentials is not null (the --l2-factor option of
+ * FQHESphereFermionsTwoBodyGeneric.cc).  L^2 is one more two-body term of the product. */
+int dhh_sphere_l2_problem_create(dhh_arch* arch, dhh_problem** problem, int nbr_fermions, int total_lz, int lz_max,
+                                 const double* pseudopotentials, double l2_factor);
+/* factors of l2_factor * L^2 in the dh_sphere_hamiltonian_create layout (*nbr_factors: capacity in,
+ * count out) and the constant l2_factor * Lz (Lz + 1) of the sector (*shift) */
+int dhh_sphere_l2_factors(int lz_max, int nbr_fermions, int total_lz, double l2_factor, double* factors,
+                          int* pair_offset, int* nbr_factors, double* shift);
 /* FermionOnSphere(nbrFermions, totalLz, lzMax) + a three-body term with explicit MN factors
  * (AbstractQHEOnSphereNBodyInteractionHamiltonian.h:259-283); tuples and factors in the layout
  * of dh_sphere_nbody_hamiltonian_create (nbody = 3) */

@@ -181,3 +181,39 @@ def test_lanczos_n10_2s27(arch, golden):
     r = p.lanczos(2, block_size=1)
     np.testing.assert_allclose(r["eigenvalues"], ref, rtol=0, atol=1e-10)
     p.close()
+
+
+def test_l2_operator_on_gpu_labels_coulomb_states(arch):
+    """L^2 through the product kernel: its expectation values on the Coulomb eigenvectors of
+    N=6, 2S=15, Lz=0 are L(L+1) (the interaction conserves L), and with --l2-factor the
+    L = 0 states stay put while the others move by factor * L(L+1)."""
+    from diagham_amd import capi
+    import torch
+    n, tlz, lz = 6, 0, 15
+    V = coulomb(lz)
+    p = capi.Problem.sphere(arch, n, tlz, lz, V)
+    vecs = torch.empty((6, p.dim), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    r = p.lanczos(6, block_size=1, d_vectors=vecs.data_ptr(), nbr_vectors=6)
+    L2 = capi.Problem.sphere_l2(arch, n, tlz, lz, 1.0)
+    Lvals = []
+    for i in range(6):
+        y = torch.empty_like(vecs[i])
+        torch.cuda.synchronize()
+        L2.hv_device(vecs[i].data_ptr(), y.data_ptr(), 1)
+        arch.sync()
+        l2 = torch.dot(vecs[i], y).item()
+        L = (-1 + np.sqrt(1 + 4 * l2)) / 2
+        assert abs(L - round(L)) < 1e-8, (i, l2)
+        Lvals.append(int(round(L)))
+    assert Lvals[0] == 0  # the Laughlin state at 2S = 3(N-1) is a singlet
+    shifted = capi.Problem.sphere_l2(arch, n, tlz, lz, 0.5, pseudo=V)
+    r2 = shifted.lanczos(6, block_size=1)
+    # H + 0.5 L^2 keeps the L = 0 ground state and only raises the others
+    assert abs(r2["eigenvalues"][0] - r["eigenvalues"][0]) < 1e-9
+    for e, L in zip(r["eigenvalues"], Lvals):
+        if L > 0:  # moved up by 0.5 L (L + 1): no longer in the spectrum at e
+            assert np.min(np.abs(np.asarray(r2["eigenvalues"]) - e)) > 1e-6, (e, L)
+    p.close()
+    L2.close()
+    shifted.close()

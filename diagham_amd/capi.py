@@ -93,6 +93,9 @@ HOST_SIGNATURES = {
     "dhh_sphere_nbody_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, c_int, P_I, c_int, P_I, P_D]),
     "dhh_sphere_l2_problem_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_D, c_double]),
     "dhh_sphere_l2_factors": (c_int, [c_int, c_int, c_int, c_double, P_D, P_I, P_I, P_D]),
+    "dhh_write_vector": (c_int, [c_void_p, c_char_p, c_void_p, c_int64, c_int]),
+    "dhh_read_vector": (c_int, [c_void_p, c_char_p, c_void_p, c_int64, c_int]),
+    "dhh_vector_file_dimension": (c_int, [c_char_p, c_int, P_I64]),
     "dhh_spin_problem_create": (c_int, [c_void_p, PP, c_int, c_int, P_D, P_D, c_int]),
     "dhh_problem_destroy": (c_int, [c_void_p]),
     "dhh_problem_basis": (c_void_p, [c_void_p]),

@@ -401,6 +401,13 @@ class LatticeSectorsBlockLanczos {
   std::vector<double> Factors;
 };
 
+// DiagHam binary vector files (RealVector::WriteVector src/Vector/RealVector.cc:3669,
+// ComplexVector::WriteVector src/Vector/ComplexVector.cc:2606, RealVector::ReadVector :3798) from
+// and into device memory; complex vectors are interleaved (re, im).
+void WriteDeviceVector(GPUArchitecture* arch, const char* fileName, const double* d_vec, int64_t dim, bool complex);
+void ReadDeviceVector(GPUArchitecture* arch, const char* fileName, double* d_vec, int64_t dim, bool complex);
+int64_t VectorFileDimension(const char* fileName, bool complex);
+
 // Symmetric band matrix (half-bandwidth b, dense row-major n x n) -> tridiagonal by Givens
 // rotations with bulge chasing; returns diagonal and off-diagonal, and accumulates Q if given.
 void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>& d, std::vector<double>& e,

@@ -72,6 +72,30 @@ extern "C" int dhh_sphere_problem_create(dhh_arch* a, dhh_problem** out, int n, 
   });
 }
 
+extern "C" int dhh_write_vector(dhh_arch* a, const char* fileName, const double* d_vec, int64_t dim, int complex) {
+  if (!a || !fileName || (!d_vec && dim > 0)) return DH_E_ARG;
+  return guarded([&] {
+    WriteDeviceVector(a->arch.get(), fileName, d_vec, dim, complex != 0);
+    return DH_OK;
+  });
+}
+
+extern "C" int dhh_read_vector(dhh_arch* a, const char* fileName, double* d_vec, int64_t dim, int complex) {
+  if (!a || !fileName || (!d_vec && dim > 0)) return DH_E_ARG;
+  return guarded([&] {
+    ReadDeviceVector(a->arch.get(), fileName, d_vec, dim, complex != 0);
+    return DH_OK;
+  });
+}
+
+extern "C" int dhh_vector_file_dimension(const char* fileName, int complex, int64_t* dim) {
+  if (!fileName || !dim) return DH_E_ARG;
+  return guarded([&] {
+    *dim = VectorFileDimension(fileName, complex != 0);
+    return DH_OK;
+  });
+}
+
 extern "C" int dhh_sphere_l2_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
                                             const double* pseudo, double l2Factor) {
   if (!a || !out) return DH_E_ARG;

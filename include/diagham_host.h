@@ -109,6 +109,15 @@ int dhh_checkerboard_factors(int nbr_site_x, int nbr_site_y, double t1, double t
                              double u_potential, double v_potential, int flat_band, int* sum_pair_count,
                              int* sum_pairs, int* nbr_pairs, double* factors, int64_t* nbr_factors);
 
+/* DiagHam binary vector files: RealVector::WriteVector (src/Vector/RealVector.cc:3669),
+ * ComplexVector::WriteVector (src/Vector/ComplexVector.cc:2606) and RealVector::ReadVector (:3798):
+ * int32 dimension (-1 then int64 if it needs more than 31 bits), then little-endian doubles
+ * (complex: interleaved re, im).  d_vec is device memory of dim (complex: 2 dim) doubles; reading
+ * checks the file's dimension and size against dim. */
+int dhh_write_vector(dhh_arch* arch, const char* file_name, const double* d_vec, int64_t dim, int complex);
+int dhh_read_vector(dhh_arch* arch, const char* file_name, double* d_vec, int64_t dim, int complex);
+int dhh_vector_file_dimension(const char* file_name, int complex, int64_t* dim);
+
 /* Host-side tables (no GPU needed) */
 /* LLL Coulomb pseudopotentials V_0..V_nbr_flux in CoulombPseudopotentials units; round != 0
  * rounds to the 14 significant digits written to pseudopotential files */

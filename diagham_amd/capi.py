@@ -58,6 +58,8 @@ GPU_SIGNATURES = {
     "dh_hamiltonian_set_launch": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "dh_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int64, c_int64]),
     "dh_hamiltonian_count_elements": (c_int, [c_void_p, P_I64]),
+    "dh_hamiltonian_precompute": (c_int, [c_void_p, c_int64, P_I64]),
+    "dh_hamiltonian_release_precomputed": (c_int, [c_void_p]),
     "dh_hamiltonian_row_elements": (c_int, [c_void_p, c_int64, P_I64, P_D, c_int64, P_I64]),
     "dh_sphere_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, P_D, c_void_p]),
     "dh_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, P_I, c_int]),
@@ -427,6 +429,19 @@ class Problem:
         g = np.zeros(nc)
         self.arch.check(gpu().dh_sphere_adjoint(self.ham, d_psi, d_phi, dptr(g), d_hpsi), "dh_sphere_adjoint")
         return g
+
+    def precompute(self, max_bytes=1 << 40):
+        """Store every matrix element in device memory (dh_hamiltonian_precompute); returns the
+        bytes used.  Raises if they would exceed max_bytes (the products then stay on the fly)."""
+        n = ctypes.c_int64()
+        rc = gpu().dh_hamiltonian_precompute(self.ham, max_bytes, ctypes.byref(n))
+        if rc != 0:
+            raise DHError("dh_hamiltonian_precompute (%d bytes needed): %s" % (
+                n.value, gpu().dh_last_error(self.arch.ctx).decode()))
+        return n.value
+
+    def release_precomputed(self):
+        self.arch.check(gpu().dh_hamiltonian_release_precomputed(self.ham), "release_precomputed")
 
     def autotune(self, ks=(1, 2, 4, 8, 16), repeats=5):
         """Time every launch shape per block size; returns (table rows, best k) and leaves the

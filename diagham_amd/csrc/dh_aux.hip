@@ -72,16 +72,47 @@ extern "C" int dh_basis_find_state_index(dh_basis* b, const uint64_t* states, in
 }
 
 // ----------------------------------------------------------------------------- element enumeration
-// mode 0: count elements per row (out_count[row]); mode 1: list the elements of one row
+// counts != 0, targets == 0: count the elements of each row (counts[row - rowBegin]);
+// targets != 0, rowOffsets == 0: list the elements of row rowBegin (at most maxOut);
+// rowOffsets != 0: store the elements of every row at rowOffsets[row] (32-bit targets in csrCol)
 __global__ void fs_elements_kernel(const uint64_t* __restrict__ states, int64_t dim, int nbrOrbitals,
                                    const double* __restrict__ factors, const int* __restrict__ pairOffset,
                                    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, int rankBits,
                                    int64_t rowBegin, int64_t rowEnd, int64_t* __restrict__ counts,
-                                   int64_t* __restrict__ targets, double* __restrict__ coefs, int64_t maxOut) {
+                                   int64_t* __restrict__ targets, double* __restrict__ coefs, int64_t maxOut,
+                                   const int64_t* __restrict__ rowOffsets = nullptr,
+                                   int32_t* __restrict__ csrCol = nullptr) {
   int64_t row = rowBegin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= rowEnd) return;
   uint64_t s = states[row];
   int64_t n = 0;
+  if (rowOffsets) {
+    const int64_t base = rowOffsets[row];
+    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+      int n1 = __builtin_ctzll(r1);
+      int pc1 = __popcll(s >> n1);
+      for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+        int n2 = __builtin_ctzll(r2);
+        int sgnA = pc1 + __popcll(s >> n2);
+        uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+        int sum = n1 + n2;
+        int m3lo = (sum >> 1) + 1;
+        int m3hi = min(sum, nbrOrbitals - 1);
+        int fbase = pairOffset[(n1 * (n1 - 1)) / 2 + n2] - m3lo;
+        for (int m3 = m3lo; m3 <= m3hi; ++m3) {
+          int m4 = sum - m3;
+          uint64_t cm = (1ull << m3) | (1ull << m4);
+          if (t & cm) continue;
+          int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          double coef = factors[fbase + m3];
+          csrCol[base + n] = (int32_t)dh_find_rank(t | cm, lo, hi, rankBits, dim);
+          coefs[base + n] = (sg & 1) ? -coef : coef;
+          ++n;
+        }
+      }
+    }
+    return;
+  }
   for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
     int n1 = __builtin_ctzll(r1);
     int pc1 = __popcll(s >> n1);
@@ -443,4 +474,155 @@ extern "C" int dh_sphere_adjoint(dh_hamiltonian* h, const double* psi, const dou
   if (h->nbrChannels <= 8) return adjoint_launch<8>(h, psi, phi, g, hpsi);
   if (h->nbrChannels <= 16) return adjoint_launch<16>(h, psi, phi, g, hpsi);
   return adjoint_launch<32>(h, psi, phi, g, hpsi);
+}
+
+// ----------------------------------------------------------------------------- stored elements
+// AbstractQHEOnSphereHamiltonian::EnableFastMultiplication (FQHE/src/Hamiltonian/
+// AbstractQHEOnSphereHamiltonian.cc:3814): the elements of every row are generated once into
+// CSR arrays (64-bit row offsets, 32-bit columns, FP64 values) and each product then streams them.
+// Gather form again: y[r] = sum_j val[j] x[col[j]] + shift x[r]; LPR lanes share a row and read
+// its elements coalesced, then reduce by shuffles.
+template <int K, int LPR>
+__global__ void __launch_bounds__(512) csr_hv_kernel(const int64_t* __restrict__ rowPtr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val, int64_t first, int64_t count,
+                                                     const double* __restrict__ x, double* __restrict__ y, int add,
+                                                     double shift) {
+  const int lane = threadIdx.x % LPR;
+  const int64_t rowsPerBlock = blockDim.x / LPR;
+  const int64_t nbrTiles = (count + rowsPerBlock - 1) / rowsPerBlock;
+  for (int64_t tile = dh_xcd_block(blockIdx.x, gridDim.x); tile < nbrTiles; tile += gridDim.x) {
+    const int64_t local = tile * rowsPerBlock + threadIdx.x / LPR;
+    const bool active = local < count;
+    const int64_t row = first + (active ? local : 0);
+    double acc[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] = 0.0;
+    if (active) {
+      const int64_t end = rowPtr[row + 1];
+      for (int64_t j = rowPtr[row] + lane; j < end; j += LPR) {
+        const double c = val[j];
+        const double* xr = x + (int64_t)col[j] * K;
+        if constexpr (K % 2 == 0) {
+#pragma unroll
+          for (int v = 0; v < K; v += 2) {
+            double2 p = *reinterpret_cast<const double2*>(xr + v);
+            acc[v] += c * p.x;
+            acc[v + 1] += c * p.y;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < K; ++v) acc[v] += c * xr[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < K; ++v)
+      for (int o = LPR / 2; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o, LPR);
+    if (active && lane == 0) {
+      const double* xr = x + row * K;
+      double* yr = y + row * K;
+#pragma unroll
+      for (int v = 0; v < K; ++v) yr[v] = (add ? yr[v] : 0.0) + acc[v] + shift * xr[v];
+    }
+  }
+}
+
+template <int K>
+static int csr_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+  dh_context* c = h->ctx;
+  constexpr int LPR = K >= 8 ? 16 : 32;
+  auto kern = csr_hv_kernel<K, LPR>;
+  int perCU = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 512, 0) != hipSuccess || perCU < 1) perCU = 1;
+  const int64_t tiles = (count + 512 / LPR - 1) / (512 / LPR);
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(512), 0, c->stream, h->d_csrRow, h->d_csrCol, h->d_csrVal,
+                     first, count, x, y, add, h->shift);
+  DH_HIP(c, hipGetLastError());
+  return DH_OK;
+}
+
+int dh_csr_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count) {
+  switch (k) {
+    case 1: return csr_launch<1>(h, x, y, add, first, count);
+    case 2: return csr_launch<2>(h, x, y, add, first, count);
+    case 4: return csr_launch<4>(h, x, y, add, first, count);
+    case 8: return csr_launch<8>(h, x, y, add, first, count);
+    case 16: return csr_launch<16>(h, x, y, add, first, count);
+    default: return dh_fail(h->ctx, DH_E_ARG, "k must be 1, 2, 4, 8 or 16");
+  }
+}
+
+void dh_csr_free(dh_hamiltonian* h) {
+  for (void* p : {(void*)h->d_csrRow, (void*)h->d_csrCol, (void*)h->d_csrVal})
+    if (p) hipFree(p);
+  h->d_csrRow = nullptr;
+  h->d_csrCol = nullptr;
+  h->d_csrVal = nullptr;
+  h->csrNnz = 0;
+}
+
+extern "C" int dh_hamiltonian_release_precomputed(dh_hamiltonian* h) {
+  if (!h) return DH_E_ARG;
+  hipStreamSynchronize(h->ctx->stream);
+  dh_csr_free(h);
+  return DH_OK;
+}
+
+extern "C" int dh_hamiltonian_precompute(dh_hamiltonian* h, int64_t maxBytes, int64_t* bytes) {
+  if (!h) return DH_E_ARG;
+  dh_context* c = h->ctx;
+  if (h->kind != DH_HAM_SPHERE) return dh_fail(c, DH_E_ARG, "stored elements are implemented for two-body sphere Hamiltonians");
+  const dh_basis* b = h->basis;
+  const int64_t dim = b->dim;
+  if (dim >= (int64_t(1) << 31)) return dh_fail(c, DH_E_RANGE, "stored elements need fewer than 2^31 rows");
+  hipStreamSynchronize(c->stream);
+  dh_csr_free(h);
+  // element count per row, offsets by a host prefix sum
+  std::vector<int64_t> rowPtr(dim + 1, 0);
+  const int64_t chunk = 1 << 24;
+  int64_t* d_counts = nullptr;
+  int rc;
+  if ((rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, dim))))) return rc;
+  for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
+    const int64_t r1 = std::min(dim, r0 + chunk);
+    hipLaunchKernelGGL(fs_elements_kernel, dim3((unsigned)((r1 - r0 + 255) / 256)), dim3(256), 0, c->stream,
+                       b->d_states, dim, b->nbrOrbitals, h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi,
+                       b->rankBits, r0, r1, d_counts, (int64_t*)nullptr, (double*)nullptr, (int64_t)0,
+                       (const int64_t*)nullptr, (int32_t*)nullptr);
+    hipMemcpyAsync(rowPtr.data() + 1 + r0, d_counts, sizeof(int64_t) * (r1 - r0), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      hipFree(d_counts);
+      return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+    }
+  }
+  hipFree(d_counts);
+  for (int64_t r = 0; r < dim; ++r) rowPtr[r + 1] += rowPtr[r];
+  const int64_t nnz = rowPtr[dim];
+  const int64_t need = (int64_t)sizeof(int64_t) * (dim + 1) + nnz * (int64_t)(sizeof(int32_t) + sizeof(double));
+  if (bytes) *bytes = need;
+  if (need > maxBytes) return dh_fail(c, DH_E_NOMEM, "the stored elements need more than max_bytes");
+  if ((rc = dh_malloc(c, (void**)&h->d_csrRow, sizeof(int64_t) * (dim + 1))) ||
+      (rc = dh_malloc(c, (void**)&h->d_csrCol, sizeof(int32_t) * std::max<int64_t>(1, nnz))) ||
+      (rc = dh_malloc(c, (void**)&h->d_csrVal, sizeof(double) * std::max<int64_t>(1, nnz)))) {
+    dh_csr_free(h);
+    return rc;
+  }
+  DH_HIP(c, hipMemcpyAsync(h->d_csrRow, rowPtr.data(), sizeof(int64_t) * (dim + 1), hipMemcpyHostToDevice, c->stream));
+  for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
+    const int64_t r1 = std::min(dim, r0 + chunk);
+    hipLaunchKernelGGL(fs_elements_kernel, dim3((unsigned)((r1 - r0 + 255) / 256)), dim3(256), 0, c->stream,
+                       b->d_states, dim, b->nbrOrbitals, h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi,
+                       b->rankBits, r0, r1, (int64_t*)nullptr, (int64_t*)nullptr, h->d_csrVal, (int64_t)0,
+                       (const int64_t*)h->d_csrRow, h->d_csrCol);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    dh_csr_free(h);
+    return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+  }
+  h->csrNnz = nnz;
+  return DH_OK;
 }

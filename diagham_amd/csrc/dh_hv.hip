@@ -379,6 +379,7 @@ extern "C" int dh_hv(dh_hamiltonian* h, const double* x, double* y, int k, int a
   if (count < 0) count = dim - first;
   if (first < 0 || first + count > dim) return dh_fail(c, DH_E_ARG, "row range outside the space");
   if (count == 0) return DH_OK;
+  if (h->d_csrRow) return dh_csr_hv(h, x, y, k, add, first, count);
   if (h->kind == DH_HAM_SPHERE) {
     if (!h->d_factors) return dh_fail(c, DH_E_STATE, "factors not set");
     switch (k) {
@@ -505,6 +506,7 @@ extern "C" int dh_hamiltonian_destroy(dh_hamiltonian* h) {
   if (h->d_sumBase) hipFree(h->d_sumBase);
   if (h->d_cg) hipFree(h->d_cg);
   dh_nbody_free(h);
+  dh_csr_free(h);
   delete h;
   return DH_OK;
 }

@@ -78,6 +78,11 @@ struct dh_hamiltonian {
   int* d_sumTupleOff = nullptr;      // [nbrTupleSums + 1]
   int64_t* d_sumFactorOff = nullptr;  // [nbrTupleSums + 1]
   double* d_nbodyFactors = nullptr;
+  // stored matrix elements (dh_hamiltonian_precompute): CSR of the operator without the shift
+  int64_t csrNnz = 0;
+  int64_t* d_csrRow = nullptr;  // [dim + 1]
+  int32_t* d_csrCol = nullptr;  // [csrNnz]
+  double* d_csrVal = nullptr;   // [csrNnz]
   // spin chain couplings
   double J[32] = {0}, Jz[32] = {0};
   int periodic = 0;
@@ -134,6 +139,9 @@ int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_
                            int64_t* totalTiles);
 // complex product over a momentum set (dh_lattice.hip)
 int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active);
+// product from stored matrix elements and their release (dh_aux.hip)
+int dh_csr_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count);
+void dh_csr_free(dh_hamiltonian* h);
 // n-body product and element count (dh_nbody.hip)
 int dh_nbody_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count);
 int dh_nbody_count_elements(dh_hamiltonian* h, int64_t* nnz);

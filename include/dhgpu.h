@@ -139,6 +139,15 @@ int dh_hamiltonian_set_launch(dh_hamiltonian* ham, int lanes_per_row, int block_
  * k in {1, 2, 4, 8, 16}.  Gather form: every destination row is written once, no atomics. */
 int dh_hv(dh_hamiltonian* ham, const double* d_x, double* d_y, int k, int add, int64_t first, int64_t count);
 
+/* AbstractQHEOnSphereHamiltonian::EnableFastMultiplication (FQHE/src/Hamiltonian/
+ * AbstractQHEOnSphereHamiltonian.cc:3814): generate every matrix element once into device
+ * memory (CSR: int64 row offsets, int32 columns, FP64 values; the shift stays separate) when that
+ * takes at most max_bytes; later dh_hv calls stream the stored elements.  *bytes (may be null)
+ * receives the size the elements need.  If it exceeds max_bytes the call returns DH_E_NOMEM and
+ * the products stay on the fly.  Two-body sphere Hamiltonians only. */
+int dh_hamiltonian_precompute(dh_hamiltonian* ham, int64_t max_bytes, int64_t* bytes);
+/* free the stored elements: products go back to on the fly */
+int dh_hamiltonian_release_precomputed(dh_hamiltonian* ham);
 /* Number of matrix elements per product (full non-Hermitian-reduced terms with a target in
  * the space, diagonal terms included), counted on the device. */
 int dh_hamiltonian_count_elements(dh_hamiltonian* ham, int64_t* nnz);

@@ -155,6 +155,37 @@ def test_autotune_keeps_parity(cfg1):
     cfg1.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
 
 
+def test_stored_elements_match_on_the_fly(arch, golden):
+    """EnableFastMultiplication form: the CSR of every element gives the same products (k = 1, 4,
+    8, with shift and row ranges) and the same Lanczos eigenvalues; too small a budget is refused
+    and leaves the on-the-fly product in place."""
+    from diagham_amd import capi
+    p = capi.Problem.sphere(arch, 8, 0, 21, coulomb(21))
+    ref_space = O.FermionSphere(8, 0, 21)
+    fac = O.Factors(21, coulomb(21))
+    nnz = p.count_elements()
+    with pytest.raises(capi.DHError):
+        p.precompute(max_bytes=1000)
+    used = p.precompute()
+    assert used == 8 * (p.dim + 1) + 12 * nnz
+    for k in (1, 4, 8):
+        X = np.stack([O.splitmix(500 + v, p.dim) for v in range(k)], axis=1)
+        Y = p.hv(X.reshape(-1), k).reshape(p.dim, k)
+        for v in range(k):
+            y_ref = O.fs_hv(ref_space, fac, X[:, v])
+            assert np.linalg.norm(Y[:, v] - y_ref) <= 1e-12 * np.linalg.norm(y_ref)
+    p.set_shift(-1.5)
+    x = O.splitmix(9, p.dim)
+    assert np.linalg.norm(p.hv(x) - O.fs_hv(ref_space, fac, x, shift=-1.5)) <= 1e-12 * np.linalg.norm(x) * 100
+    p.set_shift(0.0)
+    r = p.lanczos(10, block_size=4)
+    np.testing.assert_allclose(r["eigenvalues"], golden["fermions_sphere_n8_2s21_lz0"]["eigenvalues"], rtol=0,
+                               atol=1e-10)
+    p.release_precomputed()
+    assert np.linalg.norm(p.hv(x) - O.fs_hv(ref_space, fac, x)) <= 1e-12 * np.linalg.norm(O.fs_hv(ref_space, fac, x))
+    p.close()
+
+
 def test_hermitian_reference_form_agrees(cfg1):
     """The reference's Hermitian-halved scatter (HermitianLowLevelAddMultiply) and the gather agree."""
     ref_space = O.FermionSphere(8, 0, 21)

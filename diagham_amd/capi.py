@@ -105,6 +105,8 @@ HOST_SIGNATURES = {
     "dhh_problem_channels": (c_int, [c_void_p, P_I, c_int]),
     "dhh_lanczos": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_uint64, P_D, P_I, P_I, P_D,
                             c_void_p, c_int]),
+    "dhh_lanczos_disk": (c_int, [c_void_p, c_char_p, c_int, c_int, c_int, c_int, c_double, c_double, c_uint64, P_D,
+                                 P_I, P_I, P_D, c_void_p, c_int]),
     "dhh_sectors_solve": (c_int, [c_void_p, c_int, c_int, P_I, c_int, P_D, c_int, c_int, c_int, c_double, c_uint64,
                                   c_int, c_double, P_D, P_I, P_I, P_D]),
     "dhh_autotune": (c_int, [c_void_p, P_I, c_int, c_int, P_D, P_I]),
@@ -458,12 +460,21 @@ class Problem:
 
     # -- Lanczos
     def lanczos(self, nbr_eigen, block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000,
-                d_vectors=None, nbr_vectors=0, allow_unconverged=False, residual=0.0):
+                d_vectors=None, nbr_vectors=0, allow_unconverged=False, residual=0.0, disk=None, resume=False):
+        """disk: directory of the disk storage (dhh_lanczos_disk); resume continues the run stored
+        there."""
         ev = np.zeros(nbr_eigen)
         prods, blocks, secs = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        rc = host().dhh_lanczos(self.h, nbr_eigen, block_size, max_iter, precision, residual, seed, dptr(ev),
-                                ctypes.byref(prods), ctypes.byref(blocks), ctypes.byref(secs), d_vectors,
-                                nbr_vectors)
+        if disk is not None:
+            rc = host().dhh_lanczos_disk(self.h, os.fsencode(disk), int(resume), nbr_eigen, block_size, max_iter,
+                                         precision, residual, seed, dptr(ev), ctypes.byref(prods),
+                                         ctypes.byref(blocks), ctypes.byref(secs), d_vectors, nbr_vectors)
+        else:
+            if resume:
+                raise ValueError("resume needs the disk storage directory")
+            rc = host().dhh_lanczos(self.h, nbr_eigen, block_size, max_iter, precision, residual, seed, dptr(ev),
+                                    ctypes.byref(prods), ctypes.byref(blocks), ctypes.byref(secs), d_vectors,
+                                    nbr_vectors)
         if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
             raise DHError("dhh_lanczos failed (%d): %s" % (rc, host().dhh_last_error().decode()))
         return {"eigenvalues": ev, "products": prods.value, "blocks": blocks.value, "seconds": secs.value,

@@ -26,6 +26,8 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <istream>
+#include <ostream>
 #include <stdexcept>
 #include <vector>
 
@@ -217,6 +219,8 @@ class SegmentedBlockLanczos {
   bool CheckDue(int s, int nbrEigen) const {
     const int m = NbrBlocks(s);
     if (Exhausted(s)) return true;
+    // a second test on the same matrix would compare the Ritz values with themselves
+    if (LastCheck[s] == m) return false;
     if ((int64_t)m * K < nbrEigen + 2) return false;
     return K == 1 || LastCheck[s] < 0 || m - LastCheck[s] >= std::max(1, m / 10);
   }
@@ -242,6 +246,75 @@ class SegmentedBlockLanczos {
         }
     Previous[s] = current;
     return ok;
+  }
+
+  // Host half of a checkpoint: step count, seed of the next random replacement, and per segment
+  // the projected blocks, dead columns and convergence history, in native (little-endian) byte
+  // order.  The basis blocks X_0..X_Steps() are the caller's to store; restoring both continues
+  // the recurrence exactly where it stopped.
+  void SaveState(std::ostream& o) const {
+    auto put = [&](const void* p, size_t n) { o.write(static_cast<const char*>(p), (std::streamsize)n); };
+    const int32_t steps = StepCount, segs = (int32_t)Dims.size(), k = K;
+    put(&steps, 4);
+    put(&NextSeed, 8);
+    put(&segs, 4);
+    put(&k, 4);
+    for (size_t s = 0; s < Dims.size(); ++s) {
+      const int32_t m = (int32_t)A[s].size(), nd = (int32_t)DeadCols[s].size(), np = (int32_t)Previous[s].size();
+      const int32_t last = LastCheck[s];
+      put(&Dims[s], 8);
+      put(&m, 4);
+      put(&nd, 4);
+      put(&np, 4);
+      put(&last, 4);
+      for (int b = 0; b < m; ++b) {
+        put(A[s][b].data(), sizeof(T) * A[s][b].size());
+        put(B[s][b].data(), sizeof(T) * B[s][b].size());
+      }
+      for (int b = 0; b < nd; ++b) put(DeadCols[s][b].data(), (size_t)K);
+      put(Previous[s].data(), sizeof(double) * (size_t)np);
+    }
+    if (!o) throw std::runtime_error("cannot write the Lanczos state");
+  }
+
+  void LoadState(std::istream& in) {
+    auto get = [&](void* p, size_t n) {
+      in.read(static_cast<char*>(p), (std::streamsize)n);
+      if (!in) throw std::invalid_argument("truncated Lanczos state");
+    };
+    int32_t steps, segs, k;
+    uint64_t seed;
+    get(&steps, 4);
+    get(&seed, 8);
+    get(&segs, 4);
+    get(&k, 4);
+    if (segs != (int32_t)Dims.size() || k != K) throw std::invalid_argument("the Lanczos state has another block layout");
+    if (steps < 0 || steps + 1 > MaxBlocks + 1) throw std::invalid_argument("the Lanczos state does not fit the Krylov storage");
+    for (size_t s = 0; s < Dims.size(); ++s) {
+      int64_t dim;
+      int32_t m, nd, np, last;
+      get(&dim, 8);
+      get(&m, 4);
+      get(&nd, 4);
+      get(&np, 4);
+      get(&last, 4);
+      if (dim != Dims[s]) throw std::invalid_argument("Hamiltonian does not match dimension of stored vectors in resuming");
+      if (m < 0 || m > steps || nd != m + 1 || np < 0 || np > (1 << 24) || last > m)
+        throw std::invalid_argument("inconsistent Lanczos state");
+      A[s].assign((size_t)m, std::vector<T>((size_t)K * K));
+      B[s].assign((size_t)m, std::vector<T>((size_t)K * K));
+      for (int b = 0; b < m; ++b) {
+        get(A[s][b].data(), sizeof(T) * (size_t)K * K);
+        get(B[s][b].data(), sizeof(T) * (size_t)K * K);
+      }
+      DeadCols[s].assign((size_t)nd, std::vector<char>((size_t)K));
+      for (int b = 0; b < nd; ++b) get(DeadCols[s][b].data(), (size_t)K);
+      Previous[s].assign((size_t)np, 0.0);
+      get(Previous[s].data(), sizeof(double) * (size_t)np);
+      LastCheck[s] = last;
+    }
+    StepCount = steps;
+    NextSeed = seed;
   }
 
  private:

@@ -242,6 +242,14 @@ class FullReorthogonalizedBlockLanczosAlgorithm {
                                             double residual = 0.0);
   ~FullReorthogonalizedBlockLanczosAlgorithm();
   void InitializeLanczosAlgorithm(uint64_t seed);
+  // FullReorthogonalizedLanczosAlgorithmWithDiskStorage: from now on every Krylov vector is also
+  // written as directory/vector.<i> (DiagHam vector files; column p of block j is vector j*k+p)
+  // and the recurrence state as directory/lanczos.dat, after each step.
+  void SetDiskStorage(const std::string& directory);
+  // ResumeLanczosAlgorithm (FullReorthogonalizedLanczosAlgorithmWithDiskStorage.cc:176): reload
+  // the state and the Krylov vectors that SetDiskStorage wrote and continue from there; the
+  // resumed run repeats the arithmetic of an uninterrupted one.
+  void ResumeLanczosAlgorithm();
   void RunLanczosAlgorithm(int nbrIter);
   bool TestConvergence();
   std::vector<double> GetEigenvalues() const { return Current; }
@@ -255,6 +263,9 @@ class FullReorthogonalizedBlockLanczosAlgorithm {
   void Diagonalize(std::vector<double>& evals, std::vector<double>* evecs);
   bool Dense() const;
   void DenseSolve();
+  void WriteState();
+  void WriteBlock(int j);
+  std::string DiskDirectory;
   GPUArchitecture* Arch;
   AbstractHamiltonian* Hamiltonian;
   int NbrEigenvalue, BlockSize, MaxIter;

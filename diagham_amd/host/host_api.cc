@@ -202,6 +202,28 @@ extern "C" int dhh_lanczos(dhh_problem* p, int nbrEigen, int blockSize, int maxI
   });
 }
 
+extern "C" int dhh_lanczos_disk(dhh_problem* p, const char* directory, int resume, int nbrEigen, int blockSize,
+                                int maxIter, double precision, double residual, uint64_t seed, double* eigenvalues,
+                                int* products, int* blocks, double* seconds, double* d_vectors, int nbrVectors) {
+  if (!p || !directory || !eigenvalues || nbrEigen <= 0) return DH_E_ARG;
+  return guarded([&] {
+    FullReorthogonalizedBlockLanczosAlgorithm lanczos(p->arch->arch.get(), p->hamiltonian.get(), nbrEigen, blockSize,
+                                                      maxIter, precision, residual);
+    lanczos.SetDiskStorage(directory);
+    if (resume)
+      lanczos.ResumeLanczosAlgorithm();
+    else
+      lanczos.InitializeLanczosAlgorithm(seed);
+    LanczosResult r = lanczos.Solve();
+    for (int i = 0; i < nbrEigen; ++i) eigenvalues[i] = (i < (int)r.eigenvalues.size()) ? r.eigenvalues[i] : 0.0;
+    if (products) *products = r.iterations;
+    if (blocks) *blocks = r.blockIterations;
+    if (seconds) *seconds = r.seconds;
+    if (d_vectors && nbrVectors > 0) lanczos.GetEigenstates(nbrVectors, d_vectors);
+    return r.converged ? DH_OK : DH_E_STATE;
+  });
+}
+
 extern "C" int dhh_autotune(dhh_problem* p, const int* ks, int nbrKs, int repeats, double* table, int* bestK) {
   if (!p || !ks || nbrKs <= 0 || !table) return DH_E_ARG;
   return guarded([&] {

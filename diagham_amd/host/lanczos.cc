@@ -2,8 +2,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <stdexcept>
+#include <string>
 
 #include "diagham_host.hpp"
 
@@ -196,6 +199,10 @@ void FullReorthogonalizedBlockLanczosAlgorithm::InitializeLanczosAlgorithm(uint6
   Current.clear();
   if (Dense()) return;
   Engine->Start(seed);
+  if (!DiskDirectory.empty()) {
+    WriteBlock(0);
+    WriteState();
+  }
 }
 
 void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter) {
@@ -204,7 +211,93 @@ void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter)
     if (Engine->Steps() > 0 && Engine->Exhausted(0)) return;
     if (Engine->Steps() + 1 > MaxBlocks) return;
     Engine->Step(all);
+    if (!DiskDirectory.empty()) {
+      WriteBlock(Engine->Steps());
+      WriteState();
+    }
   }
+}
+
+// ------------------------------------------------------------------ disk storage
+// lanczos.dat: 8-byte tag, int64 dimension, int32 wanted eigenvalues, double precision, then the
+// engine's state (SaveState).  The reference's lanczos.dat (WriteState, :472) holds the
+// tridiagonal matrix of its three-vector recurrence; this one holds the block recurrence.
+static const char kStateTag[8] = {'D', 'H', 'G', 'P', 'U', 'L', 'Z', '1'};
+
+static std::string VectorFileName(const std::string& dir, int64_t index) {
+  return dir + "/vector." + std::to_string(index);
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::SetDiskStorage(const std::string& directory) {
+  if (directory.empty()) throw std::invalid_argument("empty directory for the disk storage");
+  DiskDirectory = directory;
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::WriteBlock(int j) {
+  const int k = BlockSize;
+  double* X = Engine->Block(j);
+  if (k == 1) {
+    WriteDeviceVector(Arch, VectorFileName(DiskDirectory, j).c_str(), X, Dim, false);
+    return;
+  }
+  double* column = Engine->Block(MaxBlocks + 1);  // the scratch block is free between steps
+  for (int p = 0; p < k; ++p) {
+    Arch->Check(dh_block_get_column(Arch->Context(), X, Dim, k, p, column), "Krylov column");
+    WriteDeviceVector(Arch, VectorFileName(DiskDirectory, (int64_t)j * k + p).c_str(), column, Dim, false);
+  }
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::WriteState() {
+  const std::string name = DiskDirectory + "/lanczos.dat";
+  const std::string tmp = name + ".tmp";
+  {
+    std::ofstream o(tmp, std::ios::binary | std::ios::trunc);
+    if (!o) throw std::invalid_argument("cannot open the file: " + tmp);
+    const int32_t nev = NbrEigenvalue;
+    o.write(kStateTag, 8);
+    o.write(reinterpret_cast<const char*>(&Dim), 8);
+    o.write(reinterpret_cast<const char*>(&nev), 4);
+    o.write(reinterpret_cast<const char*>(&Precision), 8);
+    Engine->SaveState(o);
+    o.flush();
+    if (!o) throw std::runtime_error("cannot write " + tmp);
+  }
+  // a run stopped while writing keeps the previous complete state
+  if (std::rename(tmp.c_str(), name.c_str()) != 0) throw std::runtime_error("cannot rename " + tmp);
+}
+
+void FullReorthogonalizedBlockLanczosAlgorithm::ResumeLanczosAlgorithm() {
+  if (DiskDirectory.empty()) throw std::invalid_argument("resuming needs the directory of the disk storage");
+  if (Dense()) throw std::invalid_argument("the space is diagonalised densely: there is no Krylov state to resume");
+  const std::string name = DiskDirectory + "/lanczos.dat";
+  std::ifstream in(name, std::ios::binary);
+  if (!in) throw std::invalid_argument("cannot open the file: " + name);
+  char tag[8];
+  int64_t dim;
+  int32_t nev;
+  double precision;
+  in.read(tag, 8);
+  in.read(reinterpret_cast<char*>(&dim), 8);
+  in.read(reinterpret_cast<char*>(&nev), 4);
+  in.read(reinterpret_cast<char*>(&precision), 8);
+  if (!in || std::memcmp(tag, kStateTag, 8) != 0) throw std::invalid_argument("not a Lanczos state file: " + name);
+  if (dim != Dim) throw std::invalid_argument("Hamiltonian does not match dimension of stored vectors in resuming");
+  Engine->LoadState(in);
+  const int k = BlockSize;
+  double* column = Engine->Block(MaxBlocks + 1);
+  for (int j = 0; j <= Engine->Steps(); ++j) {
+    double* X = Engine->Block(j);
+    if (k == 1) {
+      ReadDeviceVector(Arch, VectorFileName(DiskDirectory, j).c_str(), X, Dim, false);
+      continue;
+    }
+    for (int p = 0; p < k; ++p) {
+      ReadDeviceVector(Arch, VectorFileName(DiskDirectory, (int64_t)j * k + p).c_str(), column, Dim, false);
+      Arch->Check(dh_block_set_column(Arch->Context(), X, Dim, k, p, column), "Krylov column");
+    }
+  }
+  Arch->Check(dh_synchronize(Arch->Context()), "sync");
+  Current.clear();
 }
 
 int FullReorthogonalizedBlockLanczosAlgorithm::NbrBlocks() const { return Engine ? Engine->NbrBlocks(0) : 0; }
@@ -346,8 +439,9 @@ LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return res;
   }
+  // a resumed run has already made some of the first steps
   int first = std::max(2, (NbrEigenvalue + 2 + BlockSize - 1) / BlockSize);
-  RunLanczosAlgorithm(first);
+  RunLanczosAlgorithm(std::max(0, first - Engine->Steps()));
   while (true) {
     if (Engine->CheckDue(0, NbrEigenvalue) && TestConvergence()) {
       res.converged = true;
@@ -362,6 +456,7 @@ LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
     }
     RunLanczosAlgorithm(1);
   }
+  if (!DiskDirectory.empty()) WriteState();  // with the history of the last convergence test
   Arch->Check(dh_synchronize(Arch->Context()), "sync");
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   res.eigenvalues = Current;

@@ -69,6 +69,18 @@ int dhh_lanczos(dhh_problem* problem, int nbr_eigen, int block_size, int max_ite
                 double residual, uint64_t seed, double* eigenvalues, int* products, int* blocks, double* seconds,
                 double* d_eigenvectors, int nbr_vectors);
 
+/* dhh_lanczos with disk storage (FullReorthogonalizedLanczosAlgorithmWithDiskStorage, the --disk
+ * and --resume options of GenericRealMainTask.cc:120-121 and :715).  After every step each new
+ * Krylov vector is written to directory/vector.<i> (DiagHam vector files, column p of block j is
+ * vector j*block_size+p) and the recurrence state to directory/lanczos.dat.  resume != 0 reloads
+ * both instead of starting from a random block (seed unused) and continues; the run then repeats
+ * the arithmetic of an uninterrupted one, so max_iter can be raised between the two calls.
+ * Errors: DH_E_ARG with "Hamiltonian does not match dimension of stored vectors in resuming" when
+ * the stored vectors belong to another space, a missing or truncated file, or another block size. */
+int dhh_lanczos_disk(dhh_problem* problem, const char* directory, int resume, int nbr_eigen, int block_size,
+                     int max_iter, double precision, double residual, uint64_t seed, double* eigenvalues,
+                     int* products, int* blocks, double* seconds, double* d_eigenvectors, int nbr_vectors);
+
 /* Autotuner of the product launch: for each block size ks[i] it times dh_hv (after a warm-up)
  * over lanes_per_row in {1,2,4,8} (64/lanes rows per wavefront), workgroup sizes {128,256,512,1024}
  * and both index lookups, then leaves the problem configured with the fastest launch of the

@@ -7,6 +7,8 @@
 // dense matrix row-major (real: d*d values; complex: d*d (re, im) pairs)
 #include <cstdio>
 #include <cstdlib>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "diagham_host.hpp"
@@ -127,6 +129,54 @@ void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed) {
   }
 }
 
+// Checkpoint round trip: one engine runs `steps` steps; a second runs half of them, saves its host
+// state, and a third engine with fresh storage (holding only the saved blocks X_0..X_half) loads
+// it and runs the rest.  Prints, per segment, the Ritz values of both and whether every one of
+// them is bitwise equal.
+template <class T>
+void RunResume(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed) {
+  std::vector<int64_t> dims;
+  for (size_t s = 0; s + 1 < ops.off.size(); ++s) dims.push_back(ops.off[s + 1] - ops.off[s]);
+  std::vector<int> caps;
+  const int maxBlocks = SectorBlockCaps(dims, ops.k, maxIter, caps);
+  const int steps = maxBlocks - 1, half = steps / 2;
+  const int64_t bd = ops.rows * ops.kr;
+  const std::vector<unsigned char> all(dims.size(), 1);
+  auto run = [&](SegmentedBlockLanczos<T>& l, int from, int to) {
+    for (int j = from; j < to; ++j) {
+      l.Step(all);
+      for (size_t s = 0; s < dims.size(); ++s) {
+        std::vector<double> cur;
+        if (l.CheckDue((int)s, nev)) l.TestConvergence((int)s, nev, precision, cur);
+      }
+    }
+  };
+  std::vector<double> st1((size_t)(maxBlocks + 2) * bd), st2(st1.size()), st3(st1.size());
+  SegmentedBlockLanczos<T> a(&ops, dims, ops.k, maxBlocks, st1.data(), bd);
+  a.Start(seed);
+  run(a, 0, steps);
+  SegmentedBlockLanczos<T> b(&ops, dims, ops.k, maxBlocks, st2.data(), bd);
+  b.Start(seed);
+  run(b, 0, half);
+  std::stringstream state;
+  b.SaveState(state);
+  std::copy(st2.begin(), st2.begin() + (size_t)(half + 1) * bd, st3.begin());
+  SegmentedBlockLanczos<T> c(&ops, dims, ops.k, maxBlocks, st3.data(), bd);
+  c.LoadState(state);
+  run(c, half, steps);
+  bool same = true;
+  for (size_t s = 0; s < dims.size(); ++s) {
+    std::vector<double> ea, ec;
+    a.Eigen((int)s, ea);
+    c.Eigen((int)s, ec);
+    same = same && ea.size() == ec.size() && std::equal(ea.begin(), ea.end(), ec.begin());
+    std::printf("%zu %zu", ea.size(), ec.size());
+    for (size_t i = 0; i < std::min<size_t>(ea.size(), 3); ++i) std::printf(" %.17g %.17g", ea[i], ec[i]);
+    std::printf("\n");
+  }
+  std::printf("%s %d %d\n", same ? "same" : "differ", steps, half);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -156,11 +206,12 @@ int main(int argc, char** argv) {
       if (std::fscanf(f, "%lf", &v) != 1) return 2;
   }
   std::fclose(f);
+  const bool resume = argc > 2 && std::string(argv[2]) == "resume";
   try {
     if (cplxMode)
-      Run<cplx>(ops, maxIter, nev, precision, seed);
+      resume ? RunResume<cplx>(ops, maxIter, nev, precision, seed) : Run<cplx>(ops, maxIter, nev, precision, seed);
     else
-      Run<double>(ops, maxIter, nev, precision, seed);
+      resume ? RunResume<double>(ops, maxIter, nev, precision, seed) : Run<double>(ops, maxIter, nev, precision, seed);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

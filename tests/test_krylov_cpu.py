@@ -140,3 +140,40 @@ def test_invariant_block_structure(selftest, tmp_path):
     conv, prods, ev = res[0]
     assert conv == 1
     assert np.max(np.abs(ev - np.linalg.eigvalsh(m)[:8])) < 1e-10
+
+
+def run_resume(exe, tmp_path, mats, k, nev, max_iter, seed=11):
+    cplx = any(np.iscomplexobj(m) for m in mats)
+    lines = ["%d %d %d %d %.17g %d %d" % (cplx, k, max_iter, nev, 1e-14, seed, len(mats)),
+             " ".join(str(m.shape[0]) for m in mats)]
+    for m in mats:
+        m = np.asarray(m, complex if cplx else float)
+        flat = np.stack([m.real.ravel(), m.imag.ravel()], 1).ravel() if cplx else m.ravel()
+        lines.append(" ".join(repr(float(v)) for v in flat))
+    path = tmp_path / "problem.txt"
+    path.write_text("\n".join(lines) + "\n")
+    out = subprocess.run([exe, str(path), "resume"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 0, out.stderr
+    return out.stdout.strip().splitlines()
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("k", [1, 4])
+def test_checkpoint_resume_repeats_the_run(selftest, tmp_path, k, cplx):
+    """SaveState/LoadState (the host half of the disk storage of
+    FullReorthogonalizedLanczosAlgorithmWithDiskStorage.cc:472-530): an engine that loads the
+    state saved halfway, with only the saved Krylov blocks in fresh storage, ends on Ritz values
+    bitwise equal to an uninterrupted run's.  Segments: a generic one, and a degenerate one whose
+    Krylov space becomes invariant early, so random restarts (seeded from the saved state) occur
+    after the checkpoint."""
+    rng = np.random.default_rng(5 + k)
+    generic = random_hermitian(rng, 60, cplx)
+    q, _ = np.linalg.qr(random_hermitian(rng, 40, cplx))
+    degenerate = (q * np.repeat([-1.0, 0.5, 2.0], [10, 20, 10])) @ q.conj().T
+    lines = run_resume(selftest, tmp_path, [generic, degenerate], k, nev=3, max_iter=12 * k)
+    assert lines[-1].split()[0] == "same", lines
+    steps, half = map(int, lines[-1].split()[1:])
+    assert steps >= 8 and 0 < half < steps
+    # the restarts carried the degenerate segment into its multiplicity-10 lowest eigenspace
+    ev = [float(x) for x in lines[1].split()[2::2]]
+    assert np.allclose(ev[:3], [-1.0, -1.0, -1.0], atol=1e-10)

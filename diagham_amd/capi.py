@@ -6,6 +6,7 @@ are missing.  There is no Python or CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -231,6 +232,7 @@ class Arch:
     """GPUArchitecture: one device context."""
 
     def __init__(self, device=0):
+        self._dependents = weakref.WeakSet()
         h = c_void_p()
         rc = host().dhh_arch_create(ctypes.byref(h), device)
         if rc != 0:
@@ -238,8 +240,22 @@ class Arch:
         self.h = h
         self.ctx = host().dhh_arch_context(h)
 
+    def track(self, obj):
+        """Register an object whose native handle lives in this context (it needs `close()`)."""
+        self._dependents.add(obj)
+        return obj
+
     def close(self):
+        """Release every live buffer, problem and sector set of this context, then the context.
+        At interpreter exit the finalizers run in no defined order (the context's may come first
+        even while other objects still reference it), so the context closes its dependents
+        itself rather than relying on reference counts."""
         if getattr(self, "h", None):
+            for obj in list(getattr(self, "_dependents", ())):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
             host().dhh_arch_destroy(self.h)
             self.h = None
 
@@ -270,6 +286,7 @@ class DeviceBuffer:
         p = c_void_p()
         arch.check(gpu().dh_malloc(arch.ctx, ctypes.byref(p), max(self.n, 1) * self.dtype.itemsize), "dh_malloc")
         self.ptr = p.value
+        arch.track(self)
 
     @classmethod
     def from_host(cls, arch, a):
@@ -296,6 +313,8 @@ class DeviceBuffer:
             gpu().dh_free(self.arch.ctx, self.ptr)
             self.ptr = None
 
+    close = free
+
     def __del__(self):
         try:
             self.free()
@@ -309,6 +328,7 @@ class Problem:
     def __init__(self, arch, handle, kind):
         self.arch = arch
         self.h = handle
+        arch.track(self)
         self.kind = kind
         self.ham = host().dhh_problem_hamiltonian(handle)
         self.basis = host().dhh_problem_basis(handle)
@@ -492,6 +512,7 @@ class SphereSectorSet:
         arch.check(gpu().dh_sector_set_create(arch.ctx, ctypes.byref(h), n, lz_max, lz.ctypes.data_as(P_I), lz.size),
                    "dh_sector_set_create")
         self.h = h
+        arch.track(self)
         self.nbr_sectors = lz.size
         self.offsets = np.zeros(lz.size + 1, np.int64)
         gpu().dh_sector_set_offsets(h, self.offsets.ctypes.data_as(P_I64))

@@ -100,6 +100,11 @@ void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std:
                            int k, int m, std::vector<double>& evals, std::vector<double>* evecs);
 void BlockTridiagonalEigen(const std::vector<std::vector<cplx>>& A, const std::vector<std::vector<cplx>>& B, int k,
                            int m, std::vector<double>& evals, std::vector<cplx>* evecs);
+// The `count` lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with diagonal d
+// and off-diagonal e, by Sturm-count bisection to adjacent doubles: O(count * n * 60) instead of
+// the O(n^2) of a full QL sweep, for the per-step convergence tests of the one-vector recurrence.
+void TridiagonalLowest(const std::vector<double>& d, const std::vector<double>& e, int count,
+                       std::vector<double>& evals);
 
 template <class T>
 class SegmentedBlockLanczos {
@@ -214,6 +219,27 @@ class SegmentedBlockLanczos {
     ev.resize((size_t)Live(s));
   }
 
+  // The min(count, Live(s)) lowest eigenvalues of s for block size 1: the projected matrix is
+  // tridiagonal (a Hermitian one has the spectrum of the real one with |B_j| off the diagonal).
+  void LowestEigen(int s, int count, std::vector<double>& ev) const {
+    if (K != 1) {
+      Eigen(s, ev);
+      ev.resize(std::min<size_t>(ev.size(), (size_t)std::max(0, count)));
+      return;
+    }
+    const int m = (int)A[s].size();
+    std::vector<double> d(m), e(m > 0 ? m - 1 : 0);
+    double bound = 0.0;
+    for (int b = 0; b < m; ++b) {
+      d[b] = std::real(A[s][b][0]);
+      if (b + 1 < m) e[b] = std::abs(B[s][b][0]);
+      bound = std::max(bound, std::fabs(d[b]) + (b > 0 ? std::fabs(e[b - 1]) : 0.0) + (b + 1 < m ? std::fabs(e[b]) : 0.0));
+    }
+    for (int b = 0; b < m; ++b)
+      if (DeadCols[s][b][0]) d[b] = 2.0 * bound + 1.0;
+    TridiagonalLowest(d, e, (int)std::min<int64_t>(count, Live(s)), ev);
+  }
+
   // Block Lanczos convergence checks cost a band reduction; past the first ones they are spaced
   // so that at most ~10% extra blocks run after convergence.
   bool CheckDue(int s, int nbrEigen) const {
@@ -229,7 +255,10 @@ class SegmentedBlockLanczos {
   // less than precision * |value| since the previous test of s (or the Krylov space is complete)
   bool TestConvergence(int s, int nbrEigen, double precision, std::vector<double>& current) {
     std::vector<double> ev;
-    Eigen(s, ev);
+    if (K == 1)
+      LowestEigen(s, nbrEigen + 1, ev);
+    else
+      Eigen(s, ev);
     LastCheck[s] = NbrBlocks(s);
     const int want = std::min<int>(nbrEigen, (int)ev.size());
     current.assign(ev.begin(), ev.begin() + want);

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <stdexcept>
 #include <string>
 
@@ -13,6 +14,67 @@
 namespace dhh {
 
 // ------------------------------------------------------------------ small dense eigen tools
+// sqrt(a^2 + b^2): the direct form where neither square can overflow or underflow (every
+// projected matrix here), std::hypot otherwise; hypot's full scaling made it the main cost of tql2.
+static inline double Pythag(double a, double b) {
+  const double big = std::max(std::fabs(a), std::fabs(b));
+  if (big > 1e150 || (big > 0.0 && big < 1e-150)) return std::hypot(a, b);
+  return std::sqrt(a * a + b * b);
+}
+
+// Number of eigenvalues of the symmetric tridiagonal (d, e) below x: negative pivots of the
+// LDL^T factorisation of T - x (Sturm count); a pivot of magnitude below pivmin is replaced by
+// -pivmin, as in LAPACK's dstebz.
+static int SturmCount(const std::vector<double>& d, const std::vector<double>& e2, double x, double pivmin) {
+  int count = 0;
+  double q = 1.0;
+  for (size_t i = 0; i < d.size(); ++i) {
+    q = d[i] - x - (i > 0 ? e2[i - 1] / q : 0.0);
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++count;
+  }
+  return count;
+}
+
+void TridiagonalLowest(const std::vector<double>& d, const std::vector<double>& e, int count,
+                       std::vector<double>& evals) {
+  const int n = (int)d.size();
+  count = std::max(0, std::min(count, n));
+  evals.assign(count, 0.0);
+  if (count == 0) return;
+  std::vector<double> e2(n > 0 ? n - 1 : 0);
+  double lo = d[0], hi = d[0], emax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double left = i > 0 ? std::fabs(e[i - 1]) : 0.0, right = i + 1 < n ? std::fabs(e[i]) : 0.0;
+    lo = std::min(lo, d[i] - left - right);
+    hi = std::max(hi, d[i] + left + right);
+    if (i + 1 < n) {
+      e2[i] = e[i] * e[i];
+      emax = std::max(emax, e2[i]);
+    }
+  }
+  const double norm = std::max(std::fabs(lo), std::fabs(hi));
+  const double pivmin = std::numeric_limits<double>::min() * std::max(1.0, emax);
+  lo -= 2.0 * std::numeric_limits<double>::epsilon() * norm + pivmin;
+  hi += 2.0 * std::numeric_limits<double>::epsilon() * norm + pivmin;
+  // bisection per eigenvalue down to adjacent doubles; each one starts from the bracket the
+  // previous ones left (lambda_i >= lambda_{i-1})
+  double floor = lo;
+  for (int i = 0; i < count; ++i) {
+    double a = floor, b = hi;
+    while (true) {
+      const double mid = 0.5 * (a + b);
+      if (mid <= a || mid >= b) break;
+      if (SturmCount(d, e2, mid, pivmin) > i)
+        b = mid;
+      else
+        a = mid;
+    }
+    evals[i] = b;
+    floor = a;
+  }
+}
+
 // Implicit-shift QL on a symmetric tridiagonal matrix (the textbook tql2 iteration).
 // d: diagonal (n), e: sub-diagonal (e[i] couples i and i+1, n-1 used).  evecs, if given,
 // receives the n x n eigenvector matrix column-major (column j <-> evals[j]).
@@ -36,14 +98,14 @@ void TridiagonalEigen(int n, std::vector<double> d, std::vector<double> e, std::
       if (m == l) break;
       if (++iter > 60) throw std::runtime_error("TridiagonalEigen: no convergence");
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
+      double r = Pythag(g, 1.0);
       g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
       double s = 1.0, c = 1.0, p = 0.0;
       int i = m - 1;
       bool underflow = false;
       for (; i >= l; --i) {
         double f = s * e[i], b = c * e[i];
-        r = std::hypot(f, g);
+        r = Pythag(f, g);
         e[i + 1] = r;
         if (r == 0.0) {
           d[i + 1] -= p;
@@ -113,7 +175,7 @@ void BandToTridiagonal(int n, int b, std::vector<double>& a, std::vector<double>
     // annihilate a[p+1][col] against a[p][col] with a rotation in the plane (p, p+1)
     double x = a[(size_t)p * n + col], y = a[(size_t)(p + 1) * n + col];
     if (y == 0.0) return;
-    double r = std::hypot(x, y);
+    double r = Pythag(x, y);
     double c = x / r, s = y / r;
     int lo = std::max(0, p - b - 1), hi = std::min(n - 1, p + 1 + b + 1);
     rotate_sym(a, n, p, c, s, lo, hi);

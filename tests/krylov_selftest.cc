@@ -179,10 +179,26 @@ void RunResume(HostOps& ops, int maxIter, int nev, double precision, uint64_t se
 
 }  // namespace
 
+// "lowest" mode: n count, then d[0..n) and e[0..n-1); prints TridiagonalLowest's eigenvalues
+static int RunLowest(FILE* f) {
+  int n, count;
+  if (std::fscanf(f, "%d %d", &n, &count) != 2 || n < 1) return 2;
+  std::vector<double> d(n), e(n - 1);
+  for (auto& v : d)
+    if (std::fscanf(f, "%lf", &v) != 1) return 2;
+  for (auto& v : e)
+    if (std::fscanf(f, "%lf", &v) != 1) return 2;
+  std::vector<double> ev;
+  TridiagonalLowest(d, e, count, ev);
+  for (double v : ev) std::printf("%.17g\n", v);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   FILE* f = std::fopen(argv[1], "r");
   if (!f) return 2;
+  if (argc > 2 && std::string(argv[2]) == "lowest") return RunLowest(f);
   int cplxMode, k, maxIter, nev, nseg;
   double precision;
   unsigned long long seed;

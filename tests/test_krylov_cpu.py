@@ -177,3 +177,35 @@ def test_checkpoint_resume_repeats_the_run(selftest, tmp_path, k, cplx):
     # the restarts carried the degenerate segment into its multiplicity-10 lowest eigenspace
     ev = [float(x) for x in lines[1].split()[2::2]]
     assert np.allclose(ev[:3], [-1.0, -1.0, -1.0], atol=1e-10)
+
+
+@pytest.mark.parametrize("case", ["random", "clustered", "graded", "zero_couplings"])
+def test_tridiagonal_lowest_matches_dense(selftest, tmp_path, case):
+    """TridiagonalLowest (Sturm bisection used by the one-vector convergence tests) against
+    numpy's dense eigvalsh on tridiagonal matrices, including multiple eigenvalues, couplings that
+    vanish (a dead Krylov column) and entries spanning many orders of magnitude."""
+    rng = np.random.default_rng(["random", "clustered", "graded", "zero_couplings"].index(case))
+    n = 237
+    d = rng.standard_normal(n) * 10
+    e = rng.standard_normal(n - 1)
+    if case == "clustered":
+        d = np.round(d) / 4
+        e *= 1e-9
+    elif case == "graded":
+        d = d * np.logspace(-6, 3, n)
+        e = e * np.logspace(-6, 3, n - 1)
+    elif case == "zero_couplings":
+        e[::7] = 0.0
+        d[-1] = 1e3
+    path = tmp_path / "tri.txt"
+    count = 11
+    path.write_text("%d %d\n%s\n%s\n" % (n, count, " ".join(repr(float(v)) for v in d),
+                                         " ".join(repr(float(v)) for v in e)))
+    out = subprocess.run([selftest, str(path), "lowest"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 0, out.stderr
+    got = np.array([float(x) for x in out.stdout.split()])
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    ref = np.linalg.eigvalsh(T)[:count]
+    scale = np.abs(np.linalg.eigvalsh(T)).max()
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 4e-15 * scale * np.sqrt(n)

@@ -279,6 +279,7 @@ def main():
     arch.sync()
     print(json.dumps(line))
     sys.stdout.flush()
+    arch.close()
 
 
 if __name__ == "__main__":

@@ -27,7 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-munsafe-f
 GPU_SOURCES = ["dh_core.hip", "dh_basis.hip", "dh_hv.hip", "dh_aux.hip", "dh_multisector.hip", "dh_lattice.hip",
                "dh_nbody.hip"]
 HOST_SOURCES = ["clebsch_gordan.cc", "fermion_on_sphere.cc", "hamiltonians.cc", "lanczos.cc", "sectors.cc",
-                "lattice.cc", "vector_io.cc", "host_api.cc"]
+                "lattice.cc", "torus.cc", "vector_io.cc", "host_api.cc"]
 
 
 def _run(cmd):

@@ -118,6 +118,9 @@ HOST_SIGNATURES = {
                                             c_double, P_D, P_D]),
     "dhh_checkerboard_factors": (c_int, [c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
                                          c_int, P_I, P_I, P_I, P_D, P_I64]),
+    "dhh_torus_factors": (c_int, [c_int, c_double, P_D, c_int, c_int, P_I, P_I, P_I, P_D, P_I64]),
+    "dhh_torus_sectors_solve": (c_int, [c_void_p, c_int, c_int, c_double, P_D, c_int, c_int, P_I, c_int, c_int,
+                                        c_int, c_int, c_double, c_uint64, c_double, P_D, P_I, P_I, P_D]),
     "dhh_coulomb_pseudopotentials": (c_int, [c_int, c_int, P_D]),
     "dhh_clebsch_gordan": (c_double, [c_int, c_int, c_int, c_int, c_int]),
     "dhh_sphere_factors": (c_int, [c_int, P_D, P_D, P_I, P_I]),
@@ -567,5 +570,42 @@ def sectors_solve(arch, n, lz_max, total_lz, pseudo, nbr_eigen, block_size=1, ba
                                   prods.ctypes.data_as(P_I), conv.ctypes.data_as(P_I), ctypes.byref(secs))
     if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
         raise DHError("dhh_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+    return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
+            "seconds": secs.value}
+
+
+def torus_factors(nbr_flux, ratio=1.0, pseudo=(), coulomb_level=-1):
+    """dhh_torus_factors: (sum_pair_count, sum_pairs, complex factors interleaved) of the torus
+    Hamiltonian in the momentum sector set layout (Nx = 1)."""
+    pp = np.ascontiguousarray(pseudo, dtype=np.float64)
+    npairs = nbr_flux * (nbr_flux - 1) // 2
+    cnt = np.zeros(nbr_flux, np.int32)
+    pairs = np.zeros(2 * npairs, np.int32)
+    f = np.zeros(2 * npairs * npairs)
+    npr, nf = ctypes.c_int(npairs), ctypes.c_int64(npairs * npairs)
+    rc = host().dhh_torus_factors(nbr_flux, ratio, dptr(pp) if pp.size else None, pp.size, coulomb_level,
+                                  cnt.ctypes.data_as(P_I), pairs.ctypes.data_as(P_I), ctypes.byref(npr), dptr(f),
+                                  ctypes.byref(nf))
+    if rc != 0:
+        raise DHError("dhh_torus_factors failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+    return cnt, pairs[:2 * npr.value], f[:2 * nf.value]
+
+
+def torus_sectors_solve(arch, n, nbr_flux, ky, nbr_eigen, ratio=1.0, pseudo=(), coulomb_level=-1, block_size=1,
+                        max_iter=3000, precision=1e-14, seed=0x5EED0000, memory_gb=200.0, allow_unconverged=False):
+    """dhh_torus_sectors_solve: the nbr_eigen lowest eigenvalues of every Ky sector in ky."""
+    k = np.ascontiguousarray(ky, dtype=np.int32)
+    pp = np.ascontiguousarray(pseudo, dtype=np.float64)
+    ns = k.size
+    ev = np.zeros(ns * nbr_eigen)
+    prods = np.zeros(ns, np.int32)
+    conv = np.zeros(ns, np.int32)
+    secs = ctypes.c_double()
+    rc = host().dhh_torus_sectors_solve(arch.h, n, nbr_flux, ratio, dptr(pp) if pp.size else None, pp.size,
+                                        coulomb_level, k.ctypes.data_as(P_I), ns, nbr_eigen, block_size, max_iter,
+                                        precision, seed, memory_gb, dptr(ev), prods.ctypes.data_as(P_I),
+                                        conv.ctypes.data_as(P_I), ctypes.byref(secs))
+    if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
+        raise DHError("dhh_torus_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
     return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
             "seconds": secs.value}

@@ -49,6 +49,10 @@ extern "C" int dh_destroy(dh_context* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->d_scratch) hipFree(c->d_scratch);
   if (c->h_pinned) hipHostFree(c->h_pinned);
+  for (auto& s : c->stage) {
+    if (s.host) hipHostFree(s.host);
+    if (s.done) hipEventDestroy(s.done);
+  }
   if (c->ownStream && c->stream) hipStreamDestroy(c->stream);
   delete c;
   return DH_OK;
@@ -139,6 +143,33 @@ int dh_scratch(dh_context* c, size_t bytes, double** p) {
     c->scratchBytes = nb;
   }
   *p = c->d_scratch;
+  return DH_OK;
+}
+
+int dh_upload_async(dh_context* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return DH_OK;
+  dh_context::StageSlot& s = c->stage[c->stageNext];
+  c->stageNext = (c->stageNext + 1) % dh_context::kStageSlots;
+  if (s.pending) {
+    DH_HIP(c, hipEventSynchronize(s.done));
+    s.pending = false;
+  }
+  if (bytes > s.cap) {
+    if (s.host) hipHostFree(s.host);  // its last copy has completed
+    s.host = nullptr;
+    s.cap = 0;
+    const size_t nb = std::max<size_t>(bytes, 64 << 10);
+    if (hipHostMalloc(&s.host, nb, hipHostMallocDefault) != hipSuccess) {
+      s.host = nullptr;
+      return dh_fail(c, DH_E_NOMEM, "pinned staging allocation failed");
+    }
+    s.cap = nb;
+  }
+  if (!s.done) DH_HIP(c, hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+  std::memcpy(s.host, src, bytes);
+  DH_HIP(c, hipMemcpyAsync(dst, s.host, bytes, hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipEventRecord(s.done, c->stream));
+  s.pending = true;
   return DH_OK;
 }
 
@@ -314,16 +345,10 @@ static int launch_axpy(dh_context* c, const double* V, int nb, double* W, int64_
   double* scratch;
   int rc = dh_scratch(c, bytes, &scratch);
   if (rc) return rc;
-  double* pin;
-  rc = dh_pinned(c, bytes, &pin);
-  if (rc) return rc;
-  std::memcpy(pin, h_c, bytes);
-  DH_HIP(c, hipMemcpyAsync(scratch, pin, bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = dh_upload_async(c, scratch, h_c, bytes))) return rc;
   unsigned blocks = (unsigned)((dim + 255) / 256);
   hipLaunchKernelGGL((dh_block_axpy_kernel<K>), dim3(blocks), dim3(256), 0, c->stream, V, nb, W, dim, scratch);
   DH_HIP(c, hipGetLastError());
-  // the pinned staging buffer is reused by the next call: wait for the copy
-  DH_HIP(c, hipStreamSynchronize(c->stream));
   return DH_OK;
 }
 
@@ -369,15 +394,10 @@ static int launch_rmul(dh_context* c, double* W, int64_t dim, const double* h_m)
   double* scratch;
   int rc = dh_scratch(c, bytes, &scratch);
   if (rc) return rc;
-  double* pin;
-  rc = dh_pinned(c, bytes, &pin);
-  if (rc) return rc;
-  std::memcpy(pin, h_m, bytes);
-  DH_HIP(c, hipMemcpyAsync(scratch, pin, bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = dh_upload_async(c, scratch, h_m, bytes))) return rc;
   hipLaunchKernelGGL((dh_block_rmul_kernel<K>), dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, c->stream, W,
                      dim, scratch);
   DH_HIP(c, hipGetLastError());
-  DH_HIP(c, hipStreamSynchronize(c->stream));
   return DH_OK;
 }
 

@@ -23,6 +23,16 @@ struct dh_context {
   size_t scratchBytes = 0;
   double* h_pinned = nullptr;
   size_t pinnedBytes = 0;
+  // ring of pinned slots for host-to-device uploads that do not wait: a slot is reused once the
+  // event recorded after its copy has completed
+  static constexpr int kStageSlots = 16;
+  struct StageSlot {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+  } stage[kStageSlots];
+  int stageNext = 0;
 };
 
 enum dh_basis_kind { DH_BASIS_FERMION_SPHERE = 1, DH_BASIS_SPIN_CHAIN = 2 };
@@ -159,6 +169,9 @@ int dh_fail(dh_context* ctx, int code, const std::string& msg);
 // scratch management (core)
 int dh_scratch(dh_context* ctx, size_t bytes, double** d_ptr);
 int dh_pinned(dh_context* ctx, size_t bytes, double** h_ptr);
+// copy `bytes` of host memory to device memory in stream order without waiting for it: the host
+// data is staged in a pinned slot first, so the caller may reuse it as soon as this returns
+int dh_upload_async(dh_context* ctx, void* d_dst, const void* h_src, size_t bytes);
 
 // XCD-aware block remap: MI355X hands consecutive workgroups to its 8 XCDs round robin;
 // remapping gives every XCD one contiguous range of blocks so neighbouring rows (whose

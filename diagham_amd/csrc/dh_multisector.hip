@@ -259,12 +259,8 @@ int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_
     if (rc) return rc;
     S->workCap = std::max<size_t>(bytes, 4096);
   }
-  double* pin;
-  int rc = dh_pinned(c, bytes, &pin);
+  int rc = dh_upload_async(c, S->d_work, w.data(), bytes);
   if (rc) return rc;
-  std::memcpy(pin, w.data(), bytes);
-  DH_HIP(c, hipMemcpyAsync(S->d_work, pin, bytes, hipMemcpyHostToDevice, c->stream));
-  DH_HIP(c, hipStreamSynchronize(c->stream));  // the pinned buffer is reused by the next call
   *nActive = na;
   *totalTiles = prefix.back();
   return DH_OK;
@@ -545,12 +541,15 @@ __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restr
   constexpr int SLAB = (4096 / KK) > 0 ? (4096 / KK) : 1;
   __shared__ double sC[SLAB * KK];
   const int64_t* prefix = work + nActive;
-  for (int64_t tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {
-    int lo = 0, hi = nActive;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (prefix[mid] <= tile) lo = mid; else hi = mid;
-    }
+  // each workgroup walks a contiguous range of tiles, so it stays in one sector for many tiles:
+  // the sector is found by advancing through the prefix, and a coefficient matrix that fits one
+  // LDS slab is loaded only when the sector changes
+  const int64_t first = totalTiles * blockIdx.x / gridDim.x, last = totalTiles * (blockIdx.x + 1) / gridDim.x;
+  int lo = 0;
+  while (lo + 1 < nActive && prefix[lo + 1] <= first) ++lo;
+  int staged = -1;
+  for (int64_t tile = first; tile < last; ++tile) {
+    while (prefix[lo + 1] <= tile) ++lo;
     const int sec = (int)work[lo];
     const int64_t off = offsets[sec], dim = offsets[sec + 1] - off;
     const int64_t r = (tile - prefix[lo]) * blockDim.x + threadIdx.x;
@@ -562,9 +561,12 @@ __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restr
     for (int q = 0; q < K; ++q) acc[q] = 0.0;
     for (int s0 = 0; s0 < nb; s0 += SLAB) {
       const int ns = min(SLAB, nb - s0);
-      __syncthreads();
-      for (int i = threadIdx.x; i < ns * KK; i += blockDim.x) sC[i] = Cj[(int64_t)s0 * KK + i];
-      __syncthreads();
+      if (nb > SLAB || staged != lo) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < ns * KK; i += blockDim.x) sC[i] = Cj[(int64_t)s0 * KK + i];
+        __syncthreads();
+        staged = lo;
+      }
       if (active) {
         for (int b = 0; b < ns; ++b) {
           const double* vb = (RMUL ? W : V + (int64_t)(s0 + b) * totalDim * K) + row * K;
@@ -608,15 +610,11 @@ static int sectors_axpy(dh_sector_set* S, const double* V, int nb, double* W, co
     if ((rc = dh_malloc(c, (void**)&S->d_coef, bytes))) return rc;
     S->coefCap = bytes;
   }
-  double* pin;
-  if ((rc = dh_pinned(c, bytes, &pin))) return rc;
-  std::memcpy(pin, h_c, bytes);
-  DH_HIP(c, hipMemcpyAsync(S->d_coef, pin, bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = dh_upload_async(c, S->d_coef, h_c, bytes))) return rc;
   int64_t grid = std::min<int64_t>(tiles, (int64_t)c->nbrCU * 8);
   hipLaunchKernelGGL((sectors_axpy_kernel<K, RMUL>), dim3((unsigned)grid), dim3(256), 0, c->stream, V, nb,
                      S->totalDim, W, S->d_offsets, S->d_work, na, tiles, S->d_coef);
   DH_HIP(c, hipGetLastError());
-  DH_HIP(c, hipStreamSynchronize(c->stream));
   return DH_OK;
 }
 

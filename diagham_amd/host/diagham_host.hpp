@@ -389,6 +389,14 @@ void CheckerboardSingleBandFactors(int nx, int ny, const std::vector<std::comple
                                    double vPotential, bool flatBand, std::vector<int>& sumPairCount,
                                    std::vector<int>& sumPairs, std::vector<double>& factors);
 
+// Two-body factors of fermions on a torus of nbrFlux flux quanta and aspect ratio `ratio`
+// (ParticleOnTorusGenericHamiltonian with pseudopotentials when coulombLevel < 0,
+// ParticleOnTorusCoulombHamiltonian in Landau level coulombLevel otherwise), in the layout of
+// dh_sector_set_lattice_factors for a momentum sector set with Nx = 1, Ny = nbrFlux (orbital = ky).
+void TorusInteractionFactors(int nbrFlux, double ratio, const std::vector<double>& pseudo, int coulombLevel,
+                             std::vector<int>& sumPairCount, std::vector<int>& sumPairs,
+                             std::vector<double>& factors);
+
 // FCICheckerboardLatticeModel's (kx, ky) loop (FTI/src/Programs/FCI/FCICheckerboardLatticeModel.cc:412)
 // with FullReorthogonalizedComplex(Block)LanczosAlgorithm per sector, all sectors advanced together.
 class LatticeSectorsBlockLanczos {

@@ -121,6 +121,22 @@ int dhh_checkerboard_factors(int nbr_site_x, int nbr_site_y, double t1, double t
                              double u_potential, double v_potential, int flat_band, int* sum_pair_count,
                              int* sum_pairs, int* nbr_pairs, double* factors, int64_t* nbr_factors);
 
+/* Fermions on the torus, one sector per Ky (FermionOnTorus(N, Nphi, Ky),
+ * FQHE/src/HilbertSpace/FermionOnTorus.cc:106), as in FQHETorusFermionsTwoBodyGeneric /
+ * FQHETorusFermionsCoulomb: ParticleOnTorusGenericHamiltonian (nbr_pseudo pseudopotentials V_0..,
+ * FQHE/src/Hamiltonian/ParticleOnTorusGenericHamiltonian.cc:244) when coulomb_level < 0, else
+ * ParticleOnTorusCoulombHamiltonian in that Landau level (ParticleOnTorusCoulombHamiltonian.cc:138),
+ * aspect ratio `ratio`.  The sectors run as a momentum sector set with Nx = 1 (complex vectors,
+ * real factors) through the same batched complex block Lanczos as dhh_lattice_sectors_solve. */
+int dhh_torus_sectors_solve(dhh_arch* arch, int nbr_fermions, int nbr_flux, double ratio, const double* pseudo,
+                            int nbr_pseudo, int coulomb_level, const int* ky, int nbr_sectors, int nbr_eigen,
+                            int block_size, int max_iter, double precision, uint64_t seed, double memory_gb,
+                            double* eigenvalues, int* products, int* converged, double* seconds);
+/* Its factors in the dh_sector_set_lattice_factors layout (Nx = 1, Ny = nbr_flux; capacities in,
+ * counts out). */
+int dhh_torus_factors(int nbr_flux, double ratio, const double* pseudo, int nbr_pseudo, int coulomb_level,
+                      int* sum_pair_count, int* sum_pairs, int* nbr_pairs, double* factors, int64_t* nbr_factors);
+
 /* DiagHam binary vector files: RealVector::WriteVector (src/Vector/RealVector.cc:3669),
  * ComplexVector::WriteVector (src/Vector/ComplexVector.cc:2606) and RealVector::ReadVector (:3798):
  * int32 dimension (-1 then int64 if it needs more than 31 bits), then little-endian doubles

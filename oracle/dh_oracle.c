@@ -1293,6 +1293,133 @@ long or_fm_row_elements(void* space, void* factors, long row, long* targets, dou
 }
 
 /* ======================================================================
+ * Fermions on the torus (one Ky sector, no magnetic translations):
+ * FQHE/src/HilbertSpace/FermionOnTorus.cc (the momentum-space basis above with Nx = 1: orbital
+ * ky, states of total momentum Ky mod Nphi in descending order, AA / AdAd identical to
+ * FermionOnSphere's) and FQHE/src/Hamiltonian/ParticleOnTorusGenericHamiltonian.cc.
+ * ====================================================================== */
+
+/* L_n(x) by the three-term recurrence (the reference evaluates the monomial form of
+ * LaguerrePolynomial(n)) */
+static double torus_laguerre(int n, double x) {
+  if (n == 0) return 1.0;
+  double l0 = 1.0, l1 = 1.0 - x;
+  for (int k = 1; k < n; ++k) {
+    double l2 = ((2.0 * k + 1.0 - x) * l1 - k * l0) / (k + 1.0);
+    l0 = l1;
+    l1 = l2;
+  }
+  return l1;
+}
+
+typedef struct {
+  int nflux, nbrPseudo, coulombLevel; /* coulombLevel < 0: pseudopotential form */
+  double ratio;
+  const double* pseudo;
+} torus_params;
+
+/* Fourier transform at q^2 * (2 pi / Nphi) = arg (pseudopotential form: sum_i V_i L_i(arg)
+ * e^{-arg/2}), ParticleOnTorusGenericHamiltonian.cc:318 */
+static double torus_vq_generic(const torus_params* p, double piOnM, double q2, int zeroQ) {
+  double t = 0.0;
+  for (int i = 0; i < p->nbrPseudo; ++i)
+    if (p->pseudo[i] != 0.0) t += p->pseudo[i] * torus_laguerre(i, zeroQ ? 0.0 : 2.0 * piOnM * q2);
+  return zeroQ ? t : exp(-piOnM * q2) * t;
+}
+
+/* ParticleOnTorusCoulombHamiltonian.cc:284 GetVofQ: L_n(q^2/2)^2 e^{-q^2/2} / |q| (0 at q = 0) */
+static double torus_vq_coulomb(int level, double q2Half) {
+  if (q2Half == 0.0) return 0.0;
+  double ff = torus_laguerre(level, q2Half);
+  return ff * ff / sqrt(2.0 * q2Half) * exp(-q2Half);
+}
+
+/* EvaluateInteractionCoefficient of ParticleOnTorusGenericHamiltonian.cc:318 (pseudopotentials) and
+ * of ParticleOnTorusCoulombHamiltonian.cc:138: the lattice sum over q = (N1, N2) with
+ * N2 = m1 - m4 mod Nphi, summed outwards in N2 (both directions) and N1 until the terms no longer
+ * change the partial sums, the stopping rules of the reference kept as they are. */
+static double torus_coefficient(const torus_params* p, int m1, int m2, int m3, int m4) {
+  (void)m2;
+  const double nbr = (double)p->nflux;
+  const double piOnM = M_PI / nbr;
+  const double factor = -((double)(m1 - m3)) * piOnM * 2.0;
+  const int coul = p->coulombLevel >= 0;
+  double sum = 0.0;
+  for (int dir = 0; dir < 2; ++dir) {
+    double n2 = dir == 0 ? (double)(m1 - m4) : (double)(m1 - m4 - p->nflux);
+    double coefficient = dir == 0 ? 1.0 : sum;
+    if (coul) coefficient = 1.0;
+    while ((fabs(sum) + fabs(coefficient)) != fabs(sum)) {
+      double n1 = 1.0;
+      double q2 = p->ratio * n2 * n2;
+      double precision;
+      if (coul) {
+        coefficient = torus_vq_coulomb(p->coulombLevel, piOnM * q2);
+        precision = (n2 != 0.0) ? coefficient : 1.0;
+      } else if (n2 != 0.0) {
+        coefficient = torus_vq_generic(p, piOnM, q2, 0);
+        precision = coefficient;
+      } else {
+        precision = 1.0;
+        coefficient = torus_vq_generic(p, piOnM, 0.0, 1);
+      }
+      while ((fabs(coefficient) + precision) != fabs(coefficient)) {
+        q2 = (1.0 / p->ratio) * n1 * n1 + p->ratio * n2 * n2;
+        precision = 2.0 * (coul ? torus_vq_coulomb(p->coulombLevel, piOnM * q2) : torus_vq_generic(p, piOnM, q2, 0));
+        coefficient += precision * cos(n1 * factor);
+        n1 += 1.0;
+      }
+      sum += coefficient;
+      n2 += dir == 0 ? nbr : -nbr;
+    }
+  }
+  return coul ? sum / (2.0 * nbr) : sum / nbr;
+}
+
+/* ParticleOnTorusGenericHamiltonian::EvaluateInteractionFactors (:244, fermions) with the pair
+ * lists of AbstractQHEOnTorusHamiltonian::GetIndices (:80): pairs m1 < m2 grouped by
+ * (m1 + m2) mod Nphi; F[j1][j2] = V(m1,m2,m3,m4) + V(m2,m1,m4,m3) - V(m1,m2,m4,m3) - V(m2,m1,m3,m4)
+ * with (m1, m2) = pair j1 (annihilated) and (m3, m4) = pair j2 (created).  Returned in the
+ * momentum-space factor layout (Nx = 1, imaginary parts zero) for or_fm_hv. */
+void* or_torus_factors_create(int nflux, double ratio, int nbrPseudo, const double* pseudo, int coulombLevel) {
+  torus_params p = {nflux, nbrPseudo, coulombLevel, ratio, pseudo};
+  fm_factors* f = (fm_factors*)calloc(1, sizeof(fm_factors));
+  f->nx = 1;
+  f->ny = nflux;
+  f->nbrSums = nflux;
+  f->nbrPerSum = (int*)calloc(nflux, sizeof(int));
+  for (int m1 = 0; m1 < nflux - 1; ++m1)
+    for (int m2 = m1 + 1; m2 < nflux; ++m2) ++f->nbrPerSum[(m1 + m2) % nflux];
+  f->pairs = (int**)calloc(nflux, sizeof(int*));
+  f->fre = (double**)calloc(nflux, sizeof(double*));
+  f->fim = (double**)calloc(nflux, sizeof(double*));
+  int* fill = (int*)calloc(nflux, sizeof(int));
+  for (int s = 0; s < nflux; ++s) f->pairs[s] = (int*)malloc(sizeof(int) * 2 * (f->nbrPerSum[s] + 1));
+  for (int m1 = 0; m1 < nflux - 1; ++m1)
+    for (int m2 = m1 + 1; m2 < nflux; ++m2) {
+      int s = (m1 + m2) % nflux;
+      f->pairs[s][2 * fill[s]] = m1;
+      f->pairs[s][2 * fill[s] + 1] = m2;
+      ++fill[s];
+    }
+  free(fill);
+  for (int s = 0; s < nflux; ++s) {
+    int n = f->nbrPerSum[s];
+    f->fre[s] = (double*)calloc((size_t)n * n + 1, sizeof(double));
+    f->fim[s] = (double*)calloc((size_t)n * n + 1, sizeof(double));
+    for (int j1 = 0; j1 < n; ++j1) {
+      int m1 = f->pairs[s][2 * j1], m2 = f->pairs[s][2 * j1 + 1];
+      for (int j2 = 0; j2 < n; ++j2) {
+        int m3 = f->pairs[s][2 * j2], m4 = f->pairs[s][2 * j2 + 1];
+        f->fre[s][j1 * n + j2] = torus_coefficient(&p, m1, m2, m3, m4) + torus_coefficient(&p, m2, m1, m4, m3) -
+                                 torus_coefficient(&p, m1, m2, m4, m3) - torus_coefficient(&p, m2, m1, m3, m4);
+      }
+    }
+  }
+  return f;
+}
+
+/* ======================================================================
  * Spin-1/2 chain: Spin/src/HilbertSpace/Spin1_2Chain.cc, Spin/src/Hamiltonian/SpinChainHamiltonian.cc
  * ====================================================================== */
 typedef struct {

@@ -64,6 +64,11 @@ int or_fm_factors_sector_size(void* factors, int sum);
 void or_fm_factors_get(void* factors, int sum, int* pairs, double* fre, double* fim);
 void or_fm_hv(void* space, void* factors, const double* x, double* y, int hermitian);
 long or_fm_row_elements(void* space, void* factors, long row, long* targets, double* cre, double* cim, long maxOut);
+/* torus two-body factors (ParticleOnTorusGenericHamiltonian with pseudopotentials when
+ * coulombLevel < 0, ParticleOnTorusCoulombHamiltonian in Landau level coulombLevel otherwise) in
+ * the momentum-space factor layout with Nx = 1; the torus basis of Ky is or_fm_create(N, 1, Nphi,
+ * 0, Ky).  Free with or_fm_factors_destroy. */
+void* or_torus_factors_create(int nflux, double ratio, int nbrPseudo, const double* pseudo, int coulombLevel);
 
 /* ---------------- spin-1/2 chain ---------------- */
 void* or_spin_create(int chainLength, int sz, int memorySize);

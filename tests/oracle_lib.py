@@ -312,6 +312,7 @@ def _bind_momentum(L):
     _sig(L, "or_fm_factors_get", None, [c_void_p, c_int, P_I, P_D, P_D])
     _sig(L, "or_fm_hv", None, [c_void_p, c_void_p, P_D, P_D, c_int])
     _sig(L, "or_fm_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, P_D, c_long])
+    _sig(L, "or_torus_factors_create", c_void_p, [c_int, c_double, c_int, P_D, c_int])
     L._momentum_bound = True
     return L
 
@@ -388,3 +389,39 @@ def fm_row_elements(space, factors, row, max_out=20000):
     n = lib().or_fm_row_elements(space.h, factors.h, row, t.ctypes.data_as(P_L), dptr(cr), dptr(ci), max_out)
     assert n <= max_out
     return t[:n], cr[:n] + 1j * ci[:n]
+
+
+class TorusFactors(LatticeFactors):
+    """ParticleOnTorusGenericHamiltonian (pseudopotentials) or ParticleOnTorusCoulombHamiltonian
+    (coulomb_level >= 0) factor tables, in the momentum-space layout with Nx = 1."""
+
+    def __init__(self, nflux, ratio=1.0, pseudo=(), coulomb_level=-1):
+        L = _bind_momentum(lib())
+        pp = np.ascontiguousarray(pseudo, dtype=np.float64)
+        if pp.size == 0:
+            pp = np.zeros(1)
+        self.nx, self.ny = 1, nflux
+        self.h = L.or_torus_factors_create(nflux, ratio, int(len(pseudo)), dptr(pp), coulomb_level)
+
+
+def torus_space(n, nflux, ky):
+    """FermionOnTorus(n, nflux, ky): the momentum-space basis with Nx = 1."""
+    return MomentumFermions(n, 1, nflux, 0, ky)
+
+
+def torus_states_brute_force(n, nflux, ky):
+    """Every n-subset of nflux orbitals with total momentum ky mod nflux, descending."""
+    from itertools import combinations
+    out = [sum(1 << o for o in c) for c in combinations(range(nflux), n) if sum(c) % nflux == ky % nflux]
+    return np.array(sorted(out, reverse=True), dtype=np.uint64)
+
+
+def dense_fm(space, factors):
+    """Dense H of a momentum-space (or torus) sector from the oracle product (small spaces)."""
+    d = space.dim
+    H = np.zeros((d, d), complex)
+    for c in range(d):
+        e = np.zeros(d, complex)
+        e[c] = 1.0
+        H[:, c] = fm_hv(space, factors, e)
+    return H

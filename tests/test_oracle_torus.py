@@ -1,0 +1,85 @@
+"""The torus restatement in the oracle (FermionOnTorus with a Ky constraint as the momentum-space
+basis with Nx = 1, ParticleOnTorusGenericHamiltonian / ParticleOnTorusCoulombHamiltonian factors),
+pinned by what the physics fixes: the basis against brute-force enumeration, exact Laughlin zero
+modes of the V1 pseudopotential at filling 1/3 (three of them, the centre-of-mass degeneracy, at
+any aspect ratio), Hermiticity, and the spectra that the magnetic translation (Ky -> Ky + N) and
+the reflection (Ky -> -Ky) must carry over unchanged."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+@pytest.mark.parametrize("n,nflux,ky", [(3, 9, 0), (3, 9, 4), (4, 12, 8), (5, 10, 3), (2, 7, 6)])
+def test_torus_basis_matches_enumeration(n, nflux, ky):
+    s = O.torus_space(n, nflux, ky)
+    assert np.array_equal(s.states(), O.torus_states_brute_force(n, nflux, ky))
+
+
+def sector_spectra(n, nflux, factors):
+    out = {}
+    for ky in range(nflux):
+        s = O.torus_space(n, nflux, ky)
+        H = O.dense_fm(s, factors)
+        assert np.abs(H - H.conj().T).max() < 1e-13
+        assert np.abs(H.imag).max() == 0.0
+        out[ky] = np.linalg.eigvalsh(H.real)
+    return out
+
+
+@pytest.mark.parametrize("n,ratio", [(3, 1.0), (4, 1.0), (4, 0.6)])
+def test_v1_laughlin_zero_modes(n, ratio):
+    nflux = 3 * n
+    f = O.TorusFactors(nflux, ratio, pseudo=[0.0, 1.0])
+    spec = sector_spectra(n, nflux, f)
+    allev = np.concatenate(list(spec.values()))
+    zero = np.sort(allev)[:3]
+    assert np.abs(zero).max() < 1e-11, zero
+    assert np.sort(allev)[3] > 1e-3
+    # the three zero modes sit in the sectors Ky = Ky0 + k N, one each
+    zs = sorted(ky for ky, ev in spec.items() if ev[0] < 1e-10)
+    assert len(zs) == 3 and all((b - a) % n == 0 for a, b in zip(zs, zs[1:]))
+    # a V3 term leaves them at zero too (Laughlin is annihilated by every odd m < 3)
+    f13 = O.TorusFactors(nflux, ratio, pseudo=[0.0, 1.0, 0.0, 0.0])
+    assert np.abs(np.sort(np.concatenate(list(sector_spectra(n, nflux, f13).values())))[:3]).max() < 1e-11
+
+
+@pytest.mark.parametrize("kind", ["pseudo", "coulomb0", "coulomb1"])
+def test_translation_and_reflection_carry_spectra(kind):
+    n, nflux = 4, 10
+    if kind == "pseudo":
+        f = O.TorusFactors(nflux, 0.8, pseudo=[0.0, 1.0, 0.0, 0.4])
+    else:
+        f = O.TorusFactors(nflux, 0.8, coulomb_level=int(kind[-1]))
+    spec = sector_spectra(n, nflux, f)
+    scale = max(np.abs(ev).max() for ev in spec.values())
+    for ky in range(nflux):
+        assert np.allclose(spec[ky], spec[(ky + n) % nflux], atol=1e-12 * scale)
+        assert np.allclose(spec[ky], spec[(-ky) % nflux], atol=1e-12 * scale)
+    # a non-trivial operator: sectors not related by these maps differ
+    assert abs(spec[0][0] - spec[1][0]) > 1e-6 * scale
+
+
+@pytest.mark.parametrize("nflux,ratio,pseudo,level", [(12, 1.0, [0.0, 1.0], -1), (10, 0.8, [0.3, 1.0, 0.0, 0.4], -1),
+                                                      (9, 1.3, [], 0), (10, 0.8, [], 1)])
+def test_host_factors_match_oracle(nflux, ratio, pseudo, level):
+    """The host layer's factors (monomial-form Laguerre polynomials, as the reference evaluates
+    them) against the oracle's (three-term recurrence), pair lists identical."""
+    from diagham_amd import capi
+    cnt, pairs, f = capi.torus_factors(nflux, ratio, pseudo, level)
+    ref = O.TorusFactors(nflux, ratio, pseudo=pseudo, coulomb_level=level)
+    fc = f[0::2] + 1j * f[1::2]
+    off = 0
+    pos = 0
+    scale = 0.0
+    for s in range(nflux):
+        rp, rf = ref.sector(s)
+        n = cnt[s]
+        assert n == rp.shape[0]
+        assert np.array_equal(pairs[2 * pos:2 * (pos + n)].reshape(n, 2), rp)
+        got = fc[off:off + n * n].reshape(n, n)
+        scale = max(scale, np.abs(rf).max())
+        assert np.abs(got - rf).max() <= 1e-13 * max(1.0, np.abs(rf).max())
+        off += n * n
+        pos += n
+    assert scale > 0.0

@@ -77,6 +77,8 @@ GPU_SIGNATURES = {
     "dh_sectors_block_rmul": (c_int, [c_void_p, c_void_p, c_int, c_char_p, P_D]),
     "dh_momentum_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_I, P_I, c_int]),
     "dh_sector_set_lattice_factors": (c_int, [c_void_p, c_int, P_I, P_I, P_D, c_double]),
+    "dh_torus_translation_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, P_I, P_I, c_int]),
+    "dh_sector_set_torus_data": (c_int, [c_void_p, c_void_p, P_U64]),
     "dh_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_rmul": (c_int, [c_void_p, c_void_p, c_int64, c_int, P_D]),
@@ -121,6 +123,9 @@ HOST_SIGNATURES = {
     "dhh_torus_factors": (c_int, [c_int, c_double, P_D, c_int, c_int, P_I, P_I, P_I, P_D, P_I64]),
     "dhh_torus_sectors_solve": (c_int, [c_void_p, c_int, c_int, c_double, P_D, c_int, c_int, P_I, c_int, c_int,
                                         c_int, c_int, c_double, c_uint64, c_double, P_D, P_I, P_I, P_D]),
+    "dhh_torus_translation_sectors_solve": (c_int, [c_void_p, c_int, c_int, c_double, P_D, c_int, c_int, P_I, P_I,
+                                                    c_int, c_int, c_int, c_int, c_double, c_uint64, c_double, P_D,
+                                                    P_I, P_I, P_D]),
     "dhh_coulomb_pseudopotentials": (c_int, [c_int, c_int, P_D]),
     "dhh_clebsch_gordan": (c_double, [c_int, c_int, c_int, c_int, c_int]),
     "dhh_sphere_factors": (c_int, [c_int, P_D, P_D, P_I, P_I]),
@@ -607,5 +612,29 @@ def torus_sectors_solve(arch, n, nbr_flux, ky, nbr_eigen, ratio=1.0, pseudo=(), 
                                         conv.ctypes.data_as(P_I), ctypes.byref(secs))
     if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
         raise DHError("dhh_torus_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+    return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
+            "seconds": secs.value}
+
+
+def torus_translation_sectors_solve(arch, n, nbr_flux, kx, ky, nbr_eigen, ratio=1.0, pseudo=(), coulomb_level=-1,
+                                    block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000, memory_gb=200.0,
+                                    allow_unconverged=False):
+    """dhh_torus_translation_sectors_solve: the nbr_eigen lowest eigenvalues of every (kx[i], ky[i])
+    sector of the torus with magnetic translations."""
+    kxa = np.ascontiguousarray(kx, dtype=np.int32)
+    kya = np.ascontiguousarray(ky, dtype=np.int32)
+    assert kxa.size == kya.size
+    pp = np.ascontiguousarray(pseudo, dtype=np.float64)
+    ns = kxa.size
+    ev = np.zeros(ns * nbr_eigen)
+    prods = np.zeros(ns, np.int32)
+    conv = np.zeros(ns, np.int32)
+    secs = ctypes.c_double()
+    rc = host().dhh_torus_translation_sectors_solve(
+        arch.h, n, nbr_flux, ratio, dptr(pp) if pp.size else None, pp.size, coulomb_level, kxa.ctypes.data_as(P_I),
+        kya.ctypes.data_as(P_I), ns, nbr_eigen, block_size, max_iter, precision, seed, memory_gb, dptr(ev),
+        prods.ctypes.data_as(P_I), conv.ctypes.data_as(P_I), ctypes.byref(secs))
+    if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
+        raise DHError("dhh_torus_translation_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
     return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
             "seconds": secs.value}

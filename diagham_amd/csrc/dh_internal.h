@@ -100,7 +100,7 @@ struct dh_hamiltonian {
   int lanesPerRow = 1, blockSize = 256, indexMode = DH_INDEX_RANK, ldsTables = 1;
 };
 
-enum dh_sector_kind { DH_SECTORS_SPHERE = 1, DH_SECTORS_MOMENTUM = 2 };
+enum dh_sector_kind { DH_SECTORS_SPHERE = 1, DH_SECTORS_MOMENTUM = 2, DH_SECTORS_TORUS_TRANSLATIONS = 3 };
 
 // A set of symmetry sectors of one problem with concatenated bases (sector s owns global
 // rows [offsets[s], offsets[s+1])).  Sphere sets are real (Lz sectors); momentum sets are
@@ -136,6 +136,13 @@ struct dh_sector_set {
   double* d_cfactors = nullptr;  // interleaved (re, im)
   int64_t nbrComplexFactors = 0;
   double shift = 0.0;
+  // torus with magnetic translations: representatives ascending per sector, their orbit sizes and
+  // reordering-sign bits, a top-bits lookup per sector, and the phases exp(2 pi i Kx t / modulo)
+  uint8_t* d_orbit = nullptr;
+  uint64_t* d_reorder = nullptr;
+  int32_t* d_lut = nullptr;    // per sector (1 << lutBits) + 1 first rows of each top-bits prefix
+  double* d_phase = nullptr;   // per sector `modulo` (cos, sin) pairs
+  int lutBits = 0, modulo = 1, stateShift = 0;
   int lanesPerRow = 1, blockSize = 256;
   // per-call work descriptor: [active sector ids (int64)] [tile prefix (int64)]
   int64_t* d_work = nullptr;
@@ -149,6 +156,9 @@ int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_
                            int64_t* totalTiles);
 // complex product over a momentum set (dh_lattice.hip)
 int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active);
+// complex product over a torus set with magnetic translations (dh_torus.hip)
+int dh_torus_translation_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add,
+                                    const unsigned char* active);
 // product from stored matrix elements and their release (dh_aux.hip)
 int dh_csr_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count);
 void dh_csr_free(dh_hamiltonian* h);

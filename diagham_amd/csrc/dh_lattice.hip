@@ -176,7 +176,8 @@ extern "C" int dh_sector_set_lattice_factors(dh_sector_set* S, int nbrSums, cons
                                              const int* sumPairs, const double* factors, double shift) {
   if (!S || !sumPairCount || !sumPairs || !factors) return DH_E_ARG;
   dh_context* c = S->ctx;
-  if (S->kind != DH_SECTORS_MOMENTUM) return dh_fail(c, DH_E_ARG, "not a momentum sector set");
+  if (S->kind != DH_SECTORS_MOMENTUM && S->kind != DH_SECTORS_TORUS_TRANSLATIONS)
+    return dh_fail(c, DH_E_ARG, "not a momentum or torus sector set");
   const int nOrb = S->nbrOrbitals;
   if (nbrSums != S->nbrSiteX * S->nbrSiteY) return dh_fail(c, DH_E_ARG, "nbr_sums must be Nx * Ny");
   const int nPairs = nOrb * (nOrb - 1) / 2;

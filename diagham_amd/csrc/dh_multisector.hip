@@ -161,7 +161,8 @@ extern "C" int dh_sector_set_destroy(dh_sector_set* S) {
   for (void* p : {(void*)S->d_states, (void*)S->d_rankLo, (void*)S->d_rankHi, (void*)S->d_offsets,
                   (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef,
                   (void*)S->d_pairInfo, (void*)S->d_sumPairOff, (void*)S->d_sumPairs, (void*)S->d_sumFactorOff,
-                  (void*)S->d_cfactors, (void*)S->d_packedFactors, (void*)S->d_sumBase})
+                  (void*)S->d_cfactors, (void*)S->d_packedFactors, (void*)S->d_sumBase, (void*)S->d_orbit,
+                  (void*)S->d_reorder, (void*)S->d_lut, (void*)S->d_phase})
     if (p) hipFree(p);
   delete S;
   return DH_OK;
@@ -407,6 +408,7 @@ extern "C" int dh_sectors_hv(dh_sector_set* S, const double* x, double* y, int k
                              const unsigned char* active) {
   if (!S || !x || !y) return DH_E_ARG;
   if (S->kind == DH_SECTORS_MOMENTUM) return dh_lattice_sectors_hv(S, x, y, k, add, active);
+  if (S->kind == DH_SECTORS_TORUS_TRANSLATIONS) return dh_torus_translation_sectors_hv(S, x, y, k, add, active);
   if (!S->d_factors) return dh_fail(S->ctx, DH_E_STATE, "factors not set");
   switch (k) {
     case 1: return sectors_hv_lpr<1>(S, x, y, add, active);

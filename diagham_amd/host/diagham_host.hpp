@@ -406,11 +406,16 @@ class LatticeSectorsBlockLanczos {
                              const std::vector<int>& sumPairCount, const std::vector<int>& sumPairs,
                              const std::vector<double>& factors, int nbrEigenvalue, int blockSize, int maxIter,
                              double precision = 1e-14, size_t memoryBytes = 0);
+  // the sectors are (Kx, Ky) sectors of a torus with magnetic translations
+  // (FermionOnTorusWithMagneticTranslations; nx = 1, ny = Nphi, factors of the torus)
+  void UseTorusTranslations() { TorusTranslations = true; }
   std::vector<SectorResult> Solve(uint64_t seed, double* seconds = nullptr);
   int NbrWaves() const { return Waves; }
 
  private:
   void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
+  dh_sector_set* CreateSet(const std::vector<int>& kx, const std::vector<int>& ky) const;
+  bool TorusTranslations = false;
   GPUArchitecture* Arch;
   int NbrFermions, NbrSiteX, NbrSiteY, NbrEigenvalue, BlockSize, MaxIter;
   double Precision;

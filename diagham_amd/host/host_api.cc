@@ -443,6 +443,38 @@ extern "C" int dhh_torus_sectors_solve(dhh_arch* a, int n, int nbrFlux, double r
   });
 }
 
+extern "C" int dhh_torus_translation_sectors_solve(dhh_arch* a, int n, int nbrFlux, double ratio,
+                                                   const double* pseudo, int nbrPseudo, int coulombLevel,
+                                                   const int* kx, const int* ky, int nbrSectors, int nbrEigen,
+                                                   int blockSize, int maxIter, double precision, uint64_t seed,
+                                                   double memoryGB, double* eigenvalues, int* products,
+                                                   int* converged, double* seconds) {
+  if (!a || !kx || !ky || !eigenvalues || nbrSectors <= 0 || nbrEigen <= 0 || nbrPseudo < 0 || (nbrPseudo && !pseudo))
+    return DH_E_ARG;
+  return guarded([&] {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> cnt, pairs;
+    std::vector<double> f;
+    TorusInteractionFactors(nbrFlux, ratio, std::vector<double>(pseudo, pseudo + nbrPseudo), coulombLevel, cnt,
+                            pairs, f);
+    LatticeSectorsBlockLanczos solver(a->arch.get(), n, 1, nbrFlux, std::vector<int>(kx, kx + nbrSectors),
+                                      std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
+                                      maxIter, precision, (size_t)(memoryGB * 1073741824.0));
+    solver.UseTorusTranslations();
+    std::vector<SectorResult> r = solver.Solve(seed);
+    bool all = true;
+    for (int s = 0; s < nbrSectors; ++s) {
+      for (int q = 0; q < nbrEigen; ++q)
+        eigenvalues[(size_t)s * nbrEigen + q] = q < (int)r[s].eigenvalues.size() ? r[s].eigenvalues[q] : 0.0;
+      if (products) products[s] = r[s].products;
+      if (converged) converged[s] = r[s].converged;
+      all = all && r[s].converged;
+    }
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return all ? (int)DH_OK : (int)DH_E_STATE;
+  });
+}
+
 extern "C" int dhh_coulomb_pseudopotentials(int nbrFlux, int round, double* out) {
   if (!out || nbrFlux < 0) return DH_E_ARG;
   std::vector<double> v = EvaluatePseudopotentials(nbrFlux);

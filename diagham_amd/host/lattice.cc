@@ -183,9 +183,7 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
   dh_context* c = Arch->Context();
   std::vector<int64_t> dims(ns);
   {
-    dh_sector_set* S = nullptr;
-    Arch->Check(dh_momentum_sector_set_create(c, &S, NbrFermions, NbrSiteX, NbrSiteY, Kx.data(), Ky.data(), ns),
-                "momentum sectors");
+    dh_sector_set* S = CreateSet(Kx, Ky);
     std::vector<int64_t> off(ns + 1);
     dh_sector_set_offsets(S, off.data());
     for (int s = 0; s < ns; ++s) dims[s] = off[s + 1] - off[s];
@@ -215,6 +213,20 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
   return out;
 }
 
+dh_sector_set* LatticeSectorsBlockLanczos::CreateSet(const std::vector<int>& kx, const std::vector<int>& ky) const {
+  dh_context* c = Arch->Context();
+  dh_sector_set* S = nullptr;
+  if (TorusTranslations)
+    Arch->Check(dh_torus_translation_sector_set_create(c, &S, NbrFermions, NbrSiteY, kx.data(), ky.data(),
+                                                       (int)kx.size()),
+                "torus sector set");
+  else
+    Arch->Check(dh_momentum_sector_set_create(c, &S, NbrFermions, NbrSiteX, NbrSiteY, kx.data(), ky.data(),
+                                              (int)kx.size()),
+                "momentum sector set");
+  return S;
+}
+
 void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_t seed,
                                            std::vector<SectorResult>& out) {
   dh_context* c = Arch->Context();
@@ -225,9 +237,7 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
     kx.push_back(Kx[s]);
     ky.push_back(Ky[s]);
   }
-  dh_sector_set* S = nullptr;
-  Arch->Check(dh_momentum_sector_set_create(c, &S, NbrFermions, NbrSiteX, NbrSiteY, kx.data(), ky.data(), na),
-              "momentum sector set");
+  dh_sector_set* S = CreateSet(kx, ky);
   Arch->Check(dh_sector_set_lattice_factors(S, NbrSiteX * NbrSiteY, SumPairCount.data(), SumPairs.data(),
                                             Factors.data(), 0.0),
               "lattice factors");

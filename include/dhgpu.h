@@ -206,6 +206,20 @@ int dh_momentum_sector_set_create(dh_context* ctx, dh_sector_set** set, int nbr_
 int dh_sector_set_lattice_factors(dh_sector_set* set, int nbr_sums, const int* sum_pair_count, const int* sum_pairs,
                                   const double* factors, double shift);
 
+/* Fermions on a torus of nbr_flux flux quanta with magnetic translations, sectors (kx[i], ky[i])
+ * with 0 <= kx < gcd(N, Nphi), 0 <= ky < Nphi: FermionOnTorusWithMagneticTranslations(N, Nphi,
+ * Kx, Ky) (FQHE/src/HilbertSpace/FermionOnTorusWithMagneticTranslations.cc:80): per sector the
+ * smallest state of each orbit of the translation by Nphi / gcd(N, Nphi) orbitals, ascending, the
+ * orbits allowed by Kx only.  Complex vectors as for momentum sets.  Its two-body factors go in
+ * with dh_sector_set_lattice_factors (nbr_sums = Nphi, the torus layout of dhh_torus_factors); the
+ * product applies AbstractQHEOnTorusWithMagneticTranslationsHamiltonian.h:160-190 (translation
+ * phases exp(2 pi i Kx t / gcd), orbit rescaling and reordering signs). */
+int dh_torus_translation_sector_set_create(dh_context* ctx, dh_sector_set** set, int nbr_fermions, int nbr_flux,
+                                           const int* kx, const int* ky, int nbr_sectors);
+/* per row of the set: orbit size (NbrStateInOrbit) and reordering-sign bits (ReorderingSign);
+ * either pointer may be null */
+int dh_sector_set_torus_data(dh_sector_set* set, uint8_t* orbits, uint64_t* reorder);
+
 /* ---------------------------------------------------------------- adjoint (no reference counterpart)
  * g[c] = <psi| H_c |phi> for every channel c of dh_sphere_hamiltonian_set_channels, in one
  * pass over the basis; if d_hpsi is not null it also receives H psi (the vector-Jacobian

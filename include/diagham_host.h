@@ -132,6 +132,14 @@ int dhh_torus_sectors_solve(dhh_arch* arch, int nbr_fermions, int nbr_flux, doub
                             int nbr_pseudo, int coulomb_level, const int* ky, int nbr_sectors, int nbr_eigen,
                             int block_size, int max_iter, double precision, uint64_t seed, double memory_gb,
                             double* eigenvalues, int* products, int* converged, double* seconds);
+/* The same Hamiltonians with magnetic translations (FQHETorusFermionsWithTranslations,
+ * FQHE/src/Programs/FQHEOnTorus/FQHETorusFermionsWithTranslations.cc:419): sectors (kx[i], ky[i])
+ * of FermionOnTorusWithMagneticTranslations, 0 <= kx < gcd(N, Nphi), 0 <= ky < Nphi. */
+int dhh_torus_translation_sectors_solve(dhh_arch* arch, int nbr_fermions, int nbr_flux, double ratio,
+                                        const double* pseudo, int nbr_pseudo, int coulomb_level, const int* kx,
+                                        const int* ky, int nbr_sectors, int nbr_eigen, int block_size, int max_iter,
+                                        double precision, uint64_t seed, double memory_gb, double* eigenvalues,
+                                        int* products, int* converged, double* seconds);
 /* Its factors in the dh_sector_set_lattice_factors layout (Nx = 1, Ny = nbr_flux; capacities in,
  * counts out). */
 int dhh_torus_factors(int nbr_flux, double ratio, const double* pseudo, int nbr_pseudo, int coulomb_level,

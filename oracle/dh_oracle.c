@@ -1420,6 +1420,214 @@ void* or_torus_factors_create(int nflux, double ratio, int nbrPseudo, const doub
 }
 
 /* ======================================================================
+ * Fermions on the torus with magnetic translations:
+ * FQHE/src/HilbertSpace/FermionOnTorusWithMagneticTranslations.cc (and the inline helpers of its
+ * header) and the product of AbstractQHEOnTorusWithMagneticTranslationsHamiltonian.h:160-190.
+ * ====================================================================== */
+typedef struct {
+  int N, nflux, kx, ky, modulo, shift, compShift;
+  uint64_t mask;
+  long dim;
+  uint64_t* states;  /* canonical representatives, ascending */
+  int* orbit;        /* NbrStateInOrbit */
+  uint64_t* reorder; /* ReorderingSign: bit k = parity of the particles carried over k translations */
+} ftt_space;
+
+static int ftt_gcd(int a, int b) { return b == 0 ? a : ftt_gcd(b, a % b); }
+
+/* translation by StateShift orbitals: the lowest `shift` orbitals wrap to the top */
+static uint64_t ftt_translate(const ftt_space* s, uint64_t st) {
+  return (st >> s->shift) | ((st & s->mask) << s->compShift);
+}
+
+/* FindCanonicalForm: the smallest state of the orbit, and the number of translations that take
+ * the canonical state back to `st` */
+static uint64_t ftt_canonical(const ftt_space* s, uint64_t st, int* nbrTranslation) {
+  uint64_t best = st, cur = ftt_translate(s, st);
+  int index = 1, at = 0;
+  while (cur != st) {
+    if (cur < best) {
+      best = cur;
+      at = index;
+    }
+    cur = ftt_translate(s, cur);
+    ++index;
+  }
+  *nbrTranslation = at ? index - at : 0;
+  return best;
+}
+
+/* FindNumberXTranslation and TestXMomentumConstraint */
+static int ftt_orbit(const ftt_space* s, uint64_t st, int* signature) {
+  uint64_t cur = st;
+  int index = 0, sig = 0;
+  do {
+    sig += __builtin_popcountll(cur & s->mask) & 1;
+    cur = ftt_translate(s, cur);
+    ++index;
+  } while (cur != st);
+  if (signature) *signature = sig;
+  return index;
+}
+
+static int ftt_kx_allowed(const ftt_space* s, uint64_t st) {
+  int sig;
+  int orbit = ftt_orbit(s, st, &sig);
+  if (s->N & 1) return ((s->kx * orbit) % s->modulo) == 0;
+  return (((s->kx * orbit) - ((s->modulo * sig) >> 1)) % s->modulo) == 0;
+}
+
+static int ftt_cmp(const void* a, const void* b) {
+  uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* all raw states of total momentum ky (any order), then canonical forms, sorted ascending,
+ * duplicates and states outside the Kx constraint removed (GenerateStates, :795) */
+void* or_ftt_create(int nbrFermions, int nflux, int kx, int ky) {
+  if (nbrFermions < 1 || nbrFermions > nflux || nflux > 62) return NULL;
+  ftt_space* s = (ftt_space*)calloc(1, sizeof(ftt_space));
+  s->N = nbrFermions;
+  s->nflux = nflux;
+  s->modulo = ftt_gcd(nbrFermions, nflux);
+  s->kx = kx % s->modulo;
+  s->ky = ky % nflux;
+  s->shift = nflux / s->modulo;
+  s->compShift = nflux - s->shift;
+  s->mask = (1ul << s->shift) - 1ul;
+  long cap = 1024, n = 0;
+  uint64_t* raw = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+  /* subsets in colexicographic order by Gosper's hack */
+  uint64_t st = (1ul << nbrFermions) - 1ul, limit = 1ul << nflux;
+  while (st < limit) {
+    int sum = 0;
+    for (uint64_t t = st; t; t &= t - 1) sum += __builtin_ctzll(t);
+    if (sum % nflux == s->ky) {
+      int nt;
+      if (n == cap) raw = (uint64_t*)realloc(raw, sizeof(uint64_t) * (cap *= 2));
+      raw[n++] = ftt_canonical(s, st, &nt);
+    }
+    uint64_t c = st & (~st + 1), r = st + c;
+    st = (((r ^ st) >> 2) / c) | r;
+  }
+  qsort(raw, n, sizeof(uint64_t), ftt_cmp);
+  s->states = (uint64_t*)malloc(sizeof(uint64_t) * (n + 1));
+  long d = 0;
+  for (long i = 0; i < n; ++i) {
+    if (i > 0 && raw[i] == raw[i - 1]) continue;
+    if (!ftt_kx_allowed(s, raw[i])) continue;
+    s->states[d++] = raw[i];
+  }
+  free(raw);
+  s->dim = d;
+  s->orbit = (int*)malloc(sizeof(int) * (d + 1));
+  s->reorder = (uint64_t*)calloc(d + 1, sizeof(uint64_t));
+  for (long i = 0; i < d; ++i) {
+    s->orbit[i] = ftt_orbit(s, s->states[i], NULL);
+    if ((s->N & 1) == 0) {
+      uint64_t cur = s->states[i], bits = 0;
+      for (int k = 1; k <= s->orbit[i]; ++k) {
+        int moved = __builtin_popcountll(cur & s->mask) & 1;
+        cur = ftt_translate(s, cur);
+        uint64_t prev = (bits >> (k - 1)) & 1ul;
+        bits |= (prev ^ (uint64_t)moved) << k;
+      }
+      s->reorder[i] = bits;
+    }
+  }
+  return s;
+}
+
+void or_ftt_destroy(void* p) {
+  ftt_space* s = (ftt_space*)p;
+  if (!s) return;
+  free(s->states);
+  free(s->orbit);
+  free(s->reorder);
+  free(s);
+}
+
+long or_ftt_dim(void* p) { return ((ftt_space*)p)->dim; }
+uint64_t* or_ftt_states(void* p) { return ((ftt_space*)p)->states; }
+int* or_ftt_orbits(void* p) { return ((ftt_space*)p)->orbit; }
+uint64_t* or_ftt_reorder(void* p) { return ((ftt_space*)p)->reorder; }
+int or_ftt_modulo(void* p) { return ((ftt_space*)p)->modulo; }
+
+static long ftt_find(const ftt_space* s, uint64_t st) {
+  long lo = 0, hi = s->dim;
+  while (lo < hi) {
+    long mid = (lo + hi) >> 1;
+    if (s->states[mid] < st)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < s->dim && s->states[lo] == st) ? lo : s->dim;
+}
+
+/* parity of the occupied orbitals at and above n, as the sign tables of the reference read it */
+static double ftt_sign(uint64_t st, int n) { return (__builtin_popcountll(st >> n) & 1) ? -1.0 : 1.0; }
+
+/* AdAdAA through AA (annihilate n2, then n1) and AdAd (create m2, then m1), :473 and :561 */
+static long ftt_term(const ftt_space* s, long index, int n1, int n2, int m1, int m2, double* coefficient,
+                     int* nbrTranslation) {
+  uint64_t st = s->states[index];
+  *coefficient = 0.0;
+  if (!((st >> n1) & 1) || !((st >> n2) & 1) || n1 == n2 || m1 == m2) return s->dim;
+  double c = ftt_sign(st, n2);
+  st &= ~(1ul << n2);
+  c *= ftt_sign(st, n1);
+  st &= ~(1ul << n1);
+  if ((st >> m2) & 1) return s->dim;
+  c *= ftt_sign(st, m2); /* zero above m2 gives +1, as the reference's shortcut */
+  st |= 1ul << m2;
+  if ((st >> m1) & 1) return s->dim;
+  c *= ftt_sign(st, m1);
+  st |= 1ul << m1;
+  int nt;
+  uint64_t canon = ftt_canonical(s, st, &nt);
+  if (!ftt_kx_allowed(s, canon)) return s->dim;
+  long t = ftt_find(s, canon);
+  if (t >= s->dim) return s->dim;
+  c *= sqrt((double)s->orbit[index] / (double)s->orbit[t]);
+  if ((s->reorder[t] >> nt) & 1ul) c = -c;
+  *coefficient = c;
+  *nbrTranslation = nt;
+  return t;
+}
+
+/* y = H x over one (Kx, Ky) sector, complex interleaved, with the torus factor tables (or_torus_factors_create)
+ * and the phases exp(2 pi i Kx t / modulo) of t translations (EvaluateExponentialFactors, :67) */
+void or_ftt_hv(void* space, void* factors, const double* x, double* y) {
+  const ftt_space* s = (const ftt_space*)space;
+  const fm_factors* f = (const fm_factors*)factors;
+  memset(y, 0, sizeof(double) * 2 * s->dim);
+  for (long index = 0; index < s->dim; ++index) {
+    const double xr = x[2 * index], xi = x[2 * index + 1];
+    for (int j = 0; j < f->nbrSums; ++j) {
+      int n = f->nbrPerSum[j];
+      const int* idx = f->pairs[j];
+      for (int i1 = 0; i1 < n; ++i1)
+        for (int i2 = 0; i2 < n; ++i2) {
+          double fr = f->fre[j][i1 * n + i2], fi = f->fim[j][i1 * n + i2];
+          if (fr == 0.0 && fi == 0.0) continue;
+          double c;
+          int nt = 0;
+          long t = ftt_term(s, index, idx[2 * i1], idx[2 * i1 + 1], idx[2 * i2], idx[2 * i2 + 1], &c, &nt);
+          if (t >= s->dim) continue;
+          double ph = 2.0 * M_PI * s->kx * (double)nt / (double)s->modulo;
+          double er = cos(ph), ei = sin(ph);
+          double ar, ai, br, bi;
+          cmul(c * fr, c * fi, er, ei, &ar, &ai);
+          cmul(ar, ai, xr, xi, &br, &bi);
+          y[2 * t] += br;
+          y[2 * t + 1] += bi;
+        }
+    }
+  }
+}
+
+/* ======================================================================
  * Spin-1/2 chain: Spin/src/HilbertSpace/Spin1_2Chain.cc, Spin/src/Hamiltonian/SpinChainHamiltonian.cc
  * ====================================================================== */
 typedef struct {

@@ -69,6 +69,17 @@ long or_fm_row_elements(void* space, void* factors, long row, long* targets, dou
  * the momentum-space factor layout with Nx = 1; the torus basis of Ky is or_fm_create(N, 1, Nphi,
  * 0, Ky).  Free with or_fm_factors_destroy. */
 void* or_torus_factors_create(int nflux, double ratio, int nbrPseudo, const double* pseudo, int coulombLevel);
+/* FermionOnTorusWithMagneticTranslations(N, Nphi, Kx, Ky): canonical representatives ascending,
+ * orbit sizes, reordering-sign bits; or_ftt_hv applies the torus factors with translation phases
+ * (complex interleaved vectors). */
+void* or_ftt_create(int nbrFermions, int nflux, int kx, int ky);
+void or_ftt_destroy(void* space);
+long or_ftt_dim(void* space);
+uint64_t* or_ftt_states(void* space);
+int* or_ftt_orbits(void* space);
+uint64_t* or_ftt_reorder(void* space);
+int or_ftt_modulo(void* space);
+void or_ftt_hv(void* space, void* factors, const double* x, double* y);
 
 /* ---------------- spin-1/2 chain ---------------- */
 void* or_spin_create(int chainLength, int sz, int memorySize);

@@ -313,6 +313,14 @@ def _bind_momentum(L):
     _sig(L, "or_fm_hv", None, [c_void_p, c_void_p, P_D, P_D, c_int])
     _sig(L, "or_fm_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, P_D, c_long])
     _sig(L, "or_torus_factors_create", c_void_p, [c_int, c_double, c_int, P_D, c_int])
+    _sig(L, "or_ftt_create", c_void_p, [c_int, c_int, c_int, c_int])
+    _sig(L, "or_ftt_destroy", None, [c_void_p])
+    _sig(L, "or_ftt_dim", c_long, [c_void_p])
+    _sig(L, "or_ftt_states", P_U64, [c_void_p])
+    _sig(L, "or_ftt_orbits", P_I, [c_void_p])
+    _sig(L, "or_ftt_reorder", P_U64, [c_void_p])
+    _sig(L, "or_ftt_modulo", c_int, [c_void_p])
+    _sig(L, "or_ftt_hv", None, [c_void_p, c_void_p, P_D, P_D])
     L._momentum_bound = True
     return L
 
@@ -425,3 +433,45 @@ def dense_fm(space, factors):
         e[c] = 1.0
         H[:, c] = fm_hv(space, factors, e)
     return H
+
+
+class TorusTranslations:
+    """FermionOnTorusWithMagneticTranslations(n, nflux, kx, ky) restatement."""
+
+    def __init__(self, n, nflux, kx, ky):
+        L = _bind_momentum(lib())
+        self.n, self.nflux, self.kx, self.ky = n, nflux, kx, ky
+        self.h = L.or_ftt_create(n, nflux, kx, ky)
+        if not self.h:
+            raise ValueError("bad torus space")
+        self.dim = L.or_ftt_dim(self.h)
+        self.modulo = L.or_ftt_modulo(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_ftt_destroy(self.h)
+            self.h = None
+
+    def states(self):
+        return np.ctypeslib.as_array(lib().or_ftt_states(self.h), shape=(max(self.dim, 1),))[:self.dim].copy()
+
+    def orbits(self):
+        return np.ctypeslib.as_array(lib().or_ftt_orbits(self.h), shape=(max(self.dim, 1),))[:self.dim].copy()
+
+    def reorder(self):
+        return np.ctypeslib.as_array(lib().or_ftt_reorder(self.h), shape=(max(self.dim, 1),))[:self.dim].copy()
+
+    def hv(self, factors, x):
+        xr = np.empty(2 * self.dim)
+        xr[0::2], xr[1::2] = np.real(x), np.imag(x)
+        y = np.zeros(2 * self.dim)
+        lib().or_ftt_hv(self.h, factors.h, dptr(xr), dptr(y))
+        return y[0::2] + 1j * y[1::2]
+
+    def dense(self, factors):
+        H = np.zeros((self.dim, self.dim), complex)
+        for c in range(self.dim):
+            e = np.zeros(self.dim, complex)
+            e[c] = 1.0
+            H[:, c] = self.hv(factors, e)
+        return H

@@ -83,3 +83,34 @@ def test_host_factors_match_oracle(nflux, ratio, pseudo, level):
         off += n * n
         pos += n
     assert scale > 0.0
+
+
+@pytest.mark.parametrize("n,nflux", [(4, 8), (3, 9), (4, 10), (5, 10), (6, 12), (4, 12)])
+def test_magnetic_translation_sectors_split_the_ky_sector(n, nflux):
+    """FermionOnTorusWithMagneticTranslations: for every Ky, the (Kx, Ky) sectors over all Kx
+    together hold exactly the spectrum of the Ky sector without translations (the translation by
+    Nphi / gcd(N, Nphi) orbitals commutes with H and the Kx sectors are its eigenspaces), and each
+    sector's matrix is Hermitian.  Pins canonical forms, orbit sizes, the Kx constraint with its
+    fermionic signature, the rescaling factors, the reordering signs and the phases."""
+    f = O.TorusFactors(nflux, 0.9, pseudo=[0.2, 1.0, 0.0, 0.3])
+    for ky in range(nflux):
+        ref = np.linalg.eigvalsh(O.dense_fm(O.torus_space(n, nflux, ky), f).real)
+        parts = []
+        dims = 0
+        mod = None
+        for kx in range(64):
+            sp = O.TorusTranslations(n, nflux, kx, ky)
+            mod = sp.modulo
+            if kx >= mod:
+                break
+            dims += sp.dim
+            if sp.dim == 0:
+                continue
+            H = sp.dense(f)
+            assert np.abs(H - H.conj().T).max() < 1e-13
+            parts.append(np.linalg.eigvalsh(H))
+            st = sp.states()
+            assert np.all(np.diff(st.astype(np.int64)) > 0)
+        got = np.sort(np.concatenate(parts))
+        assert dims == ref.size
+        assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max()), (ky, got[:4], ref[:4])

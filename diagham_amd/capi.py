@@ -82,6 +82,8 @@ GPU_SIGNATURES = {
     "dh_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_rmul": (c_int, [c_void_p, c_void_p, c_int64, c_int, P_D]),
+    "dh_block_update": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D, P_D, P_D, P_D]),
+    "dh_sectors_block_update": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D, P_D, P_D, P_D]),
     "dh_block_get_column": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
     "dh_block_set_column": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
     "dh_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),

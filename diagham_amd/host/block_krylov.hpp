@@ -49,6 +49,16 @@ struct BlockOps {
   // W <- W M[a] over segment a
   virtual void Rmul(double* W, const unsigned char* active, const double* M) = 0;
   virtual void Random(double* W, uint64_t seed) = 0;  // the whole block
+  // W <- W M[a] + sum_b V_b C[a][b] (M null: W M = W), then dots[a][b] = V_b^T W (dots may be null)
+  // and gram[a] = W^T W over the updated W.  Implementations may do it in one pass over V; this
+  // default composes the operations above.
+  virtual void Update(const double* V, int nb, double* W, const unsigned char* active, const double* C,
+                      const double* M, double* dots, double* gram) {
+    if (M) Rmul(W, active, M);
+    Axpy(V, nb, W, active, C);
+    if (dots) Dots(V, nb, W, active, dots);
+    Dots(W, 1, W, active, gram);
+  }
 };
 
 template <class T>
@@ -414,17 +424,15 @@ class SegmentedBlockLanczos {
     return mask;
   }
 
-  // One projection of W on the nb basis blocks over the segments act[i], i in idx: W <- W - V C.
-  // Adds the coefficient on the newest block to out[i].coef (times out[i].R after the first
-  // projection, see OrthoRounds) and, per column, the squared norm removed to ref[i].
-  void Project(double* W, int nb, const std::vector<int>& act, const std::vector<int>& idx, std::vector<Rounds>& out,
-               bool first, std::vector<std::vector<double>>& ref) {
+  // Coefficients of one projection from the device-layout dots C (per segment of idx, nb blocks of
+  // KR x KR): the coefficient on the newest block goes to out[i].coef (times out[i].R after the
+  // first projection, see OrthoRounds) and the squared norms removed to ref[i]; C is negated in
+  // place, ready to be added to W.
+  void Coefficients(double* C, int nb, const std::vector<int>& idx, std::vector<Rounds>& out, bool first,
+                    std::vector<std::vector<double>>& ref) const {
     const int kk = K * K, KK = KR * KR;
     const size_t per = (size_t)nb * KK;
-    std::vector<unsigned char> mask = Mask(act, idx);
-    std::vector<double> C(idx.size() * per);
     std::vector<T> CT((size_t)kk);
-    Ops->Dots(Storage, nb, W, mask.data(), C.data());
     for (size_t a = 0; a < idx.size(); ++a) {
       Rounds& o = out[idx[a]];
       for (int b = 0; b < nb; ++b) {
@@ -440,19 +448,18 @@ class SegmentedBlockLanczos {
         S::ToDevice(K, CT.data(), blk);
       }
     }
-    Ops->Axpy(Storage, nb, W, mask.data(), C.data());
   }
 
-  // One Cholesky-QR of W over the segments act[i], i in idx, with the deficiency tests of
-  // CholeskySemidefinite (ref[i]: squared norms removed by the projection just before, if any).
-  // out[i].R <- Rr out[i].R.
-  void CholQR(double* W, const std::vector<int>& act, const std::vector<int>& idx, std::vector<Rounds>& out,
-              std::vector<std::vector<double>>& ref, double collapse) {
+  // Cholesky-QR factor of the Gram blocks G of the segments of idx, with the deficiency tests of
+  // CholeskySemidefinite (ref[i]: squared norms removed by the projection just before, if any):
+  // out[i].R <- Rr out[i].R, out[i].good, the device-layout inverse of Rr into M and, if asked,
+  // its T form into inv.
+  void Factor(const double* G, const std::vector<int>& idx, std::vector<Rounds>& out,
+              std::vector<std::vector<double>>& ref, double collapse, double* M,
+              std::vector<std::vector<T>>* inv) const {
     const int kk = K * K, KK = KR * KR;
-    std::vector<unsigned char> mask = Mask(act, idx);
-    std::vector<double> G(idx.size() * KK), M(idx.size() * KK);
     std::vector<T> GT((size_t)kk);
-    Ops->Dots(W, 1, W, mask.data(), G.data());
+    if (inv) inv->assign(idx.size(), std::vector<T>());
     for (size_t a = 0; a < idx.size(); ++a) {
       Rounds& o = out[idx[a]];
       std::vector<double>& r = ref[idx[a]];
@@ -461,10 +468,22 @@ class SegmentedBlockLanczos {
       std::vector<T> Rr;
       std::vector<char> good;
       CholeskySemidefinite(K, GT.data(), r.data(), collapse, Rr, good);
-      S::ToDevice(K, PseudoInverse(K, Rr, good).data(), &M[a * KK]);
+      std::vector<T> X = PseudoInverse(K, Rr, good);
+      S::ToDevice(K, X.data(), &M[a * KK]);
+      if (inv) (*inv)[a] = X;
       o.R = Mul(K, Rr, o.R);
       o.good = good;
     }
+  }
+
+  // One Cholesky-QR of W over the segments act[i], i in idx (the start block, before any basis).
+  void CholQR(double* W, const std::vector<int>& act, const std::vector<int>& idx, std::vector<Rounds>& out,
+              std::vector<std::vector<double>>& ref, double collapse) {
+    const int KK = KR * KR;
+    std::vector<unsigned char> mask = Mask(act, idx);
+    std::vector<double> G(idx.size() * KK), M(idx.size() * KK);
+    Ops->Dots(W, 1, W, mask.data(), G.data());
+    Factor(G.data(), idx, out, ref, collapse, M.data(), nullptr);
     Ops->Rmul(W, mask.data(), M.data());
   }
 
@@ -472,8 +491,16 @@ class SegmentedBlockLanczos {
   // with C = C1 + C2 R1 and R = R2 R1.  Deficient columns of Q are zero.  Both rounds always run:
   // one projection leaves the basis orthogonal to ~1e-13 only, and the Ritz values then move by
   // more than the 1e-14 relative change that the convergence test asks for.
+  //
+  // Passes over the basis V per round: the dots C = V^T W, then one update W - V C that also
+  // returns the Gram matrix of the result.  For one vector per block (k = 1) the first update also
+  // returns D = V^T W1, and the second round's dots are D / r1 (r1 the norm of W1): dividing by a
+  // scalar keeps every component to relative rounding, so V^T (W1 / r1) needs no pass of its own
+  // (three passes over V instead of four).  For k > 1 the second dots are taken from the stored
+  // W1 R1^-1: the triangular solve mixes nearly dependent columns, and the second round exists to
+  // remove the basis components of exactly that rounding.
   void OrthoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
-    const int kk = K * K;
+    const int kk = K * K, KK = KR * KR;
     const size_t na = act.size();
     out.assign(na, Rounds());
     for (auto& o : out) {
@@ -485,14 +512,42 @@ class SegmentedBlockLanczos {
     std::vector<int> all(na);
     for (size_t a = 0; a < na; ++a) all[a] = (int)a;
     std::vector<std::vector<double>> ref(na, std::vector<double>(K, 0.0));
-    if (nb > 0) Project(W, nb, act, all, out, true, ref);
+    if (nb == 0) {
+      // noise left by one Cholesky-QR is ~1e-15 of the column
+      CholQR(W, act, all, out, ref, 1e-24);
+      for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
+      CholQR(W, act, all, out, ref, 1e-16);
+      return;
+    }
+    const size_t per = (size_t)nb * KK;
+    const bool single = K == 1;
+    std::vector<unsigned char> mask = Mask(act, all);
+    std::vector<double> C(na * per), D(single ? na * per : 0), G(na * KK), M1(na * KK), M2(na * KK);
+    Ops->Dots(Storage, nb, W, mask.data(), C.data());
+    Coefficients(C.data(), nb, all, out, true, ref);
+    Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, single ? D.data() : nullptr, G.data());
     // noise left by one projection is ~1e-15 of the column
-    CholQR(W, act, all, out, ref, 1e-24);
+    std::vector<std::vector<T>> inv;
+    Factor(G.data(), all, out, ref, 1e-24, M1.data(), &inv);
     for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
-    if (nb > 0) Project(W, nb, act, all, out, false, ref);
+    if (single) {
+      std::vector<T> DT((size_t)kk);
+      for (size_t a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) {
+          S::FromGram(K, &D[a * per + (size_t)b * KK], DT.data());
+          std::vector<T> CT = Mul(K, DT, inv[a]);
+          S::ToDevice(K, CT.data(), &C[a * per + (size_t)b * KK]);
+        }
+    } else {
+      Ops->Rmul(W, mask.data(), M1.data());
+      Ops->Dots(Storage, nb, W, mask.data(), C.data());
+    }
+    Coefficients(C.data(), nb, all, out, false, ref);
+    Ops->Update(Storage, nb, W, mask.data(), C.data(), single ? M1.data() : nullptr, nullptr, G.data());
     // after the second projection, a column that kept less than 1e-8 of itself was
     // (normalised) noise inside the span of the basis
-    CholQR(W, act, all, out, ref, 1e-16);
+    Factor(G.data(), all, out, ref, 1e-16, M2.data(), nullptr);
+    Ops->Rmul(W, mask.data(), M2.data());
   }
 
   // W (nb basis blocks before it) -> orthonormal Q with W_in = V C + Q R per segment of `act`;

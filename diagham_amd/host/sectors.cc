@@ -28,6 +28,15 @@ void SectorSetOps::Rmul(double* W, const unsigned char* active, const double* M)
 void SectorSetOps::Random(double* W, uint64_t seed) {
   Arch->Check(dh_fill_random(Arch->Context(), W, Rows * KR, seed), "random block");
 }
+void SectorSetOps::Update(const double* V, int nb, double* W, const unsigned char* active, const double* C,
+                          const double* M, double* dots, double* gram) {
+  if (KR <= 2) {
+    const int rc = dh_sectors_block_update(Set, V, nb, W, KR, active, C, M, dots, gram);
+    if (rc == DH_OK) return;
+    if (rc != DH_E_RANGE) Arch->Check(rc, "sector update");
+  }
+  BlockOps::Update(V, nb, W, active, C, M, dots, gram);
+}
 
 MultiSectorBlockLanczos::MultiSectorBlockLanczos(GPUArchitecture* architecture, int nbrFermions, int lzMax,
                                                  const std::vector<int>& totalLz,
@@ -148,7 +157,12 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   std::vector<int64_t> off(na + 1);
   dh_sector_set_offsets(S, off.data());
   const int64_t total = off.back();
-  Arch->Check(dh_sector_set_launch(S, total < 4ll * 256 * 256 ? 4 : 1, 512), "launch");
+  // small sets: more lanes per row for enough wavefronts; large ones: two lanes per row and
+  // 1024-thread workgroups (the fastest shape of the all-sector product on MI355X, k = 1 and 8)
+  if (total < 4ll * 256 * 256)
+    Arch->Check(dh_sector_set_launch(S, 4, 512), "launch");
+  else
+    Arch->Check(dh_sector_set_launch(S, 2, BlockSize >= 4 ? 1024 : 512), "launch");
   std::vector<int64_t> dims(na);
   for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
   std::vector<int> blockCap;

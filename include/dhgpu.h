@@ -189,6 +189,12 @@ int dh_sectors_block_axpy(dh_sector_set* set, const double* d_basis, int nbr_blo
                           const unsigned char* active, const double* c);
 /* w[row] <- w[row] * m[j] (k x k per active sector j) */
 int dh_sectors_block_rmul(dh_sector_set* set, double* d_w, int k, const unsigned char* active, const double* m);
+/* w <- w m[j] + sum_b basis_b c[j][b] over the rows of active sector j (m may be null: identity),
+ * then dots[j][b] = basis_b^T w (dots may be null) and gram[j] = w^T w over the new w, in one pass
+ * over the basis (k = 1 or 2 doubles per row).  Returns DH_E_RANGE when nbr_blocks is too large for
+ * the staged tile; the separate axpy / dots / rmul calls do the same work. */
+int dh_sectors_block_update(dh_sector_set* set, const double* d_basis, int nbr_blocks, double* d_w, int k,
+                            const unsigned char* active, const double* c, const double* m, double* dots, double* gram);
 
 /* Momentum sectors (kx[i], ky[i]) of nbr_fermions spinless fermions in one band of an Nx x Ny
  * lattice: FermionOnSquareLatticeMomentumSpace (FQHE/src/HilbertSpace/FermionOnSquareLatticeMomentumSpace.cc:76),
@@ -240,6 +246,13 @@ int dh_block_axpy(dh_context* ctx, const double* d_basis, int nbr_blocks, double
                   const double* c);
 /* w[row][q] = sum_p w[row][p] * m[p * k + q]  (in-place right multiplication by a k x k host matrix) */
 int dh_block_rmul(dh_context* ctx, double* d_w, int64_t dim, int k, const double* m);
+/* w <- w m + sum_b basis_b c[b] (m may be null: identity), then dots[b] = basis_b^T w (dots may be
+ * null) and gram = w^T w over the new w, in one pass over the basis (k = 1 or 2): the middle step of
+ * the two-round Gram-Schmidt of FullReorthogonalizedLanczosAlgorithm::RunLanczosAlgorithm
+ * (src/LanczosAlgorithm/FullReorthogonalizedLanczosAlgorithm.cc:240-250).  DH_E_RANGE when
+ * nbr_blocks is too large for the staged tile. */
+int dh_block_update(dh_context* ctx, const double* d_basis, int nbr_blocks, double* d_w, int64_t dim, int k,
+                    const double* c, const double* m, double* dots, double* gram);
 /* copy column `col` of a [row][k] block into / out of a plain vector of dim doubles */
 int dh_block_get_column(dh_context* ctx, const double* d_w, int64_t dim, int k, int col, double* d_out);
 int dh_block_set_column(dh_context* ctx, double* d_w, int64_t dim, int k, int col, const double* d_in);

@@ -108,7 +108,15 @@ def copy_calibration(capi, arch, stream, n=1 << 27, reps=10):
     return {"doubles": n, "ms": ms, "GBps": 16.0 * n / (ms * 1e-3) / 1e9}
 
 
-def lanczos_runs(capi, arch):
+def progress(msg):
+    """One progress line per finished leg, on stderr (stdout carries only the JSON line)."""
+    print("[bench %.1f s] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
+
+
+_T0 = time.perf_counter()
+
+
+def lanczos_runs(capi, arch, report=None):
     res = {}
     # time to the 4 lowest eigenvalues of the L = 28 chain (plain, full reorthogonalisation)
     p = capi.Problem.spin_chain(arch, 28, 0)
@@ -116,6 +124,8 @@ def lanczos_runs(capi, arch):
     res["heisenberg_L28_4_eigenvalues"] = {"seconds": r["seconds"], "products": r["products"],
                                            "eigenvalues": r["eigenvalues"].tolist(), "converged": r["converged"]}
     p.close()
+    if report:
+        report("L=28 Lanczos: %.2f s" % r["seconds"])
     # block vs plain to the 10 lowest eigenvalues of N = 8 fermions at 2S = 21, Lz = 0
     pp = capi.coulomb_pseudopotentials(21, True)
     p = capi.Problem.sphere(arch, 8, 0, 21, pp)
@@ -130,7 +140,7 @@ def lanczos_runs(capi, arch):
     return res
 
 
-def sector_runs(capi, arch, n=10, lz=27, nev=10, block=8):
+def sector_runs(capi, arch, report=None, n=10, lz=27, nev=10, block=8):
     """Time to the nev lowest eigenvalues of every Lz >= 0 sector of n Coulomb fermions at flux lz:
     the plain one-vector Lanczos run sector after sector (the reference's loop) against the batched
     block Lanczos that advances all sectors together, and the largest eigenvalue difference."""
@@ -142,6 +152,8 @@ def sector_runs(capi, arch, n=10, lz=27, nev=10, block=8):
                              ("batched_block_k%d" % block, block, True)):
         r = capi.sectors_solve(arch, n, lz, lzs, pp, nev, k, batched=batched, allow_unconverged=True)
         runs[name] = r
+        if report:
+            report("all sectors, %s: %.2f s" % (name, r["seconds"]))
     ref = runs["plain_sequential_k1"]["eigenvalues"]
     out = {"workload": "fermions_sphere_n%d_2s%d_coulomb_all_Lz_ge_0_sectors_%d_eigenvalues" % (n, lz, nev),
            "sectors": len(lzs), "states": int(sum(dims)), "largest_sector": int(max(dims))}
@@ -269,13 +281,16 @@ def main():
                      "kernel": "spin_hv_kernel<1, RANK>", "algorithmic_bytes_per_launch": alg_bytes},
         "copy_calibration": calib,
     }
+    progress("headline product: %.3f ms" % ms)
     if not args.skip_lanczos:
-        line["lanczos"] = lanczos_runs(capi, arch)
-        line["lanczos"]["all_sectors"] = sector_runs(capi, arch)
+        line["lanczos"] = lanczos_runs(capi, arch, progress)
+        line["lanczos"]["all_sectors"] = sector_runs(capi, arch, progress)
     if not args.skip_batched:
         line["batched_roofline"] = batched_hv_roofline(capi, arch, stream)
+        progress("batched product: %.1f ms" % line["batched_roofline"]["ms"])
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(args.length)
+        progress("cpu baseline done")
     arch.sync()
     print(json.dumps(line))
     sys.stdout.flush()

@@ -135,7 +135,8 @@ int dh_scratch(dh_context* c, size_t bytes, double** p) {
       hipFree(c->d_scratch);
       c->d_scratch = nullptr;
     }
-    size_t nb = std::max<size_t>(bytes, 1 << 20);
+    // grow geometrically: the Krylov basis adds a block per step and would otherwise reallocate often
+    size_t nb = std::max<size_t>(std::max<size_t>(bytes, 2 * c->scratchBytes), 1 << 20);
     if (hipMalloc(&c->d_scratch, nb) != hipSuccess) {
       c->scratchBytes = 0;
       return dh_fail(c, DH_E_NOMEM, "scratch allocation failed");
@@ -156,9 +157,9 @@ int dh_upload_async(dh_context* c, void* dst, const void* src, size_t bytes) {
   }
   if (bytes > s.cap) {
     if (s.host) hipHostFree(s.host);  // its last copy has completed
+    const size_t nb = std::max<size_t>(std::max<size_t>(bytes, 2 * s.cap), 64 << 10);
     s.host = nullptr;
     s.cap = 0;
-    const size_t nb = std::max<size_t>(bytes, 64 << 10);
     if (hipHostMalloc(&s.host, nb, hipHostMallocDefault) != hipSuccess) {
       s.host = nullptr;
       return dh_fail(c, DH_E_NOMEM, "pinned staging allocation failed");
@@ -180,7 +181,7 @@ int dh_pinned(dh_context* c, size_t bytes, double** p) {
       hipHostFree(c->h_pinned);
       c->h_pinned = nullptr;
     }
-    size_t nb = std::max<size_t>(bytes, 1 << 20);
+    size_t nb = std::max<size_t>(std::max<size_t>(bytes, 2 * c->pinnedBytes), 1 << 20);
     if (hipHostMalloc((void**)&c->h_pinned, nb, hipHostMallocDefault) != hipSuccess) {
       c->pinnedBytes = 0;
       return dh_fail(c, DH_E_NOMEM, "pinned allocation failed");

@@ -428,8 +428,10 @@ class SegmentedBlockLanczos {
   // KR x KR): the coefficient on the newest block goes to out[i].coef (times out[i].R after the
   // first projection, see OrthoRounds) and the squared norms removed to ref[i]; C is negated in
   // place, ready to be added to W.
-  void Coefficients(double* C, int nb, const std::vector<int>& idx, std::vector<Rounds>& out, bool first,
-                    std::vector<std::vector<double>>& ref) const {
+  // CT: the coefficients as k x k scalar blocks (segment a, block b at (a * nb + b) * k * k); C
+  // receives their negation in the device layout of the axpy.
+  void Coefficients(const std::vector<T>& CTall, double* C, int nb, const std::vector<int>& idx,
+                    std::vector<Rounds>& out, bool first, std::vector<std::vector<double>>& ref) const {
     const int kk = K * K, KK = KR * KR;
     const size_t per = (size_t)nb * KK;
     std::vector<T> CT((size_t)kk);
@@ -437,7 +439,7 @@ class SegmentedBlockLanczos {
       Rounds& o = out[idx[a]];
       for (int b = 0; b < nb; ++b) {
         double* blk = &C[a * per + (size_t)b * KK];
-        S::FromGram(K, blk, CT.data());
+        std::copy(CTall.begin() + (a * nb + b) * kk, CTall.begin() + (a * nb + b + 1) * kk, CT.begin());
         for (int p = 0; p < K; ++p)
           for (int q = 0; q < K; ++q) ref[idx[a]][q] += S::Norm(CT[(size_t)p * K + q]);
         if (b == nb - 1) {
@@ -492,13 +494,11 @@ class SegmentedBlockLanczos {
   // one projection leaves the basis orthogonal to ~1e-13 only, and the Ritz values then move by
   // more than the 1e-14 relative change that the convergence test asks for.
   //
-  // Passes over the basis V per round: the dots C = V^T W, then one update W - V C that also
-  // returns the Gram matrix of the result.  For one vector per block (k = 1) the first update also
-  // returns D = V^T W1, and the second round's dots are D / r1 (r1 the norm of W1): dividing by a
-  // scalar keeps every component to relative rounding, so V^T (W1 / r1) needs no pass of its own
-  // (three passes over V instead of four).  For k > 1 the second dots are taken from the stored
-  // W1 R1^-1: the triangular solve mixes nearly dependent columns, and the second round exists to
-  // remove the basis components of exactly that rounding.
+  // Passes over the basis V per round: the dots C = V^T W, then the update W - V C (with the Gram
+  // matrix of the result), then W R^-1.  Four passes over V per step.  Taking the second round's
+  // dots from the first update instead (D R1^-1 with D = V^T W1) would save one, but only for one
+  // vector per block: for k > 1 the triangular solve mixes nearly dependent columns, and the
+  // second round exists to remove the basis components of exactly that rounding.
   void OrthoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
     const int kk = K * K, KK = KR * KR;
     const size_t na = act.size();
@@ -520,30 +520,26 @@ class SegmentedBlockLanczos {
       return;
     }
     const size_t per = (size_t)nb * KK;
-    const bool single = K == 1;
     std::vector<unsigned char> mask = Mask(act, all);
-    std::vector<double> C(na * per), D(single ? na * per : 0), G(na * KK), M1(na * KK), M2(na * KK);
-    Ops->Dots(Storage, nb, W, mask.data(), C.data());
-    Coefficients(C.data(), nb, all, out, true, ref);
-    Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, single ? D.data() : nullptr, G.data());
-    // noise left by one projection is ~1e-15 of the column
-    std::vector<std::vector<T>> inv;
-    Factor(G.data(), all, out, ref, 1e-24, M1.data(), &inv);
-    for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
-    if (single) {
-      std::vector<T> DT((size_t)kk);
+    std::vector<double> C(na * per), G(na * KK), M1(na * KK), M2(na * KK);
+    std::vector<T> CT(na * (size_t)nb * kk);
+    // scalar blocks of the dots in the device layout
+    auto fromDevice = [&](const double* src) {
       for (size_t a = 0; a < na; ++a)
-        for (int b = 0; b < nb; ++b) {
-          S::FromGram(K, &D[a * per + (size_t)b * KK], DT.data());
-          std::vector<T> CT = Mul(K, DT, inv[a]);
-          S::ToDevice(K, CT.data(), &C[a * per + (size_t)b * KK]);
-        }
-    } else {
-      Ops->Rmul(W, mask.data(), M1.data());
-      Ops->Dots(Storage, nb, W, mask.data(), C.data());
-    }
-    Coefficients(C.data(), nb, all, out, false, ref);
-    Ops->Update(Storage, nb, W, mask.data(), C.data(), single ? M1.data() : nullptr, nullptr, G.data());
+        for (int b = 0; b < nb; ++b) S::FromGram(K, &src[a * per + (size_t)b * KK], &CT[(a * nb + b) * kk]);
+    };
+    Ops->Dots(Storage, nb, W, mask.data(), C.data());
+    fromDevice(C.data());
+    Coefficients(CT, C.data(), nb, all, out, true, ref);
+    Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, nullptr, G.data());
+    // noise left by one projection is ~1e-15 of the column
+    Factor(G.data(), all, out, ref, 1e-24, M1.data(), nullptr);
+    for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
+    Ops->Rmul(W, mask.data(), M1.data());
+    Ops->Dots(Storage, nb, W, mask.data(), C.data());
+    fromDevice(C.data());
+    Coefficients(CT, C.data(), nb, all, out, false, ref);
+    Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, nullptr, G.data());
     // after the second projection, a column that kept less than 1e-8 of itself was
     // (normalised) noise inside the span of the basis
     Factor(G.data(), all, out, ref, 1e-16, M2.data(), nullptr);

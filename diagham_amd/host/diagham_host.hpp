@@ -301,8 +301,6 @@ class SectorSetOps : public BlockOps {
   void Axpy(const double* V, int nb, double* W, const unsigned char* active, const double* C) override;
   void Rmul(double* W, const unsigned char* active, const double* M) override;
   void Random(double* W, uint64_t seed) override;
-  void Update(const double* V, int nb, double* W, const unsigned char* active, const double* C, const double* M,
-              double* dots, double* gram) override;
 
  private:
   GPUArchitecture* Arch;

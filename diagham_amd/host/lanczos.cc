@@ -228,16 +228,6 @@ struct FullReorthogonalizedBlockLanczosAlgorithm::HamiltonianOps : BlockOps {
   void Random(double* W, uint64_t seed) override {
     Arch->Check(dh_fill_random(Arch->Context(), W, Dim * K, seed), "random block");
   }
-  // one pass over V for blocks of one or two vectors (too many basis blocks: separate kernels)
-  void Update(const double* V, int nb, double* W, const unsigned char* active, const double* C, const double* M,
-              double* dots, double* gram) override {
-    if (K <= 2) {
-      const int rc = dh_block_update(Arch->Context(), V, nb, W, Dim, K, C, M, dots, gram);
-      if (rc == DH_OK) return;
-      if (rc != DH_E_RANGE) Arch->Check(rc, "block update");
-    }
-    BlockOps::Update(V, nb, W, active, C, M, dots, gram);
-  }
 };
 
 FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgorithm(

@@ -28,15 +28,6 @@ void SectorSetOps::Rmul(double* W, const unsigned char* active, const double* M)
 void SectorSetOps::Random(double* W, uint64_t seed) {
   Arch->Check(dh_fill_random(Arch->Context(), W, Rows * KR, seed), "random block");
 }
-void SectorSetOps::Update(const double* V, int nb, double* W, const unsigned char* active, const double* C,
-                          const double* M, double* dots, double* gram) {
-  if (KR <= 2) {
-    const int rc = dh_sectors_block_update(Set, V, nb, W, KR, active, C, M, dots, gram);
-    if (rc == DH_OK) return;
-    if (rc != DH_E_RANGE) Arch->Check(rc, "sector update");
-  }
-  BlockOps::Update(V, nb, W, active, C, M, dots, gram);
-}
 
 MultiSectorBlockLanczos::MultiSectorBlockLanczos(GPUArchitecture* architecture, int nbrFermions, int lzMax,
                                                  const std::vector<int>& totalLz,

@@ -177,6 +177,48 @@ void RunResume(HostOps& ops, int maxIter, int nev, double precision, uint64_t se
   std::printf("%s %d %d\n", same ? "same" : "differ", steps, half);
 }
 
+// Orthonormality of the stored Krylov basis after maxIter / k steps: prints the largest
+// |<x_i, x_j> - delta_ij| over all stored vectors of every segment (live columns only).
+template <class T>
+void RunOrtho(HostOps& ops, int maxIter, uint64_t seed) {
+  std::vector<int64_t> dims;
+  for (size_t s = 0; s + 1 < ops.off.size(); ++s) dims.push_back(ops.off[s + 1] - ops.off[s]);
+  std::vector<int> caps;
+  const int maxBlocks = SectorBlockCaps(dims, ops.k, maxIter, caps);
+  const int steps = maxBlocks - 1;
+  const int64_t bd = ops.rows * ops.kr;
+  std::vector<double> st((size_t)(maxBlocks + 2) * bd);
+  SegmentedBlockLanczos<T> l(&ops, dims, ops.k, maxBlocks, st.data(), bd);
+  l.Start(seed);
+  const std::vector<unsigned char> all(dims.size(), 1);
+  for (int j = 0; j < steps; ++j) l.Step(all);
+  double worst = 0.0;
+  const int cols = (steps + 1) * ops.k;
+  for (size_t s = 0; s < dims.size(); ++s)
+    for (int a = 0; a < cols; ++a)
+      for (int b = a; b < cols; ++b) {
+        double re = 0.0, im = 0.0, na = 0.0;
+        const double* va = st.data() + (size_t)(a / ops.k) * bd;
+        const double* vb = st.data() + (size_t)(b / ops.k) * bd;
+        for (int64_t r = ops.off[s]; r < ops.off[s + 1]; ++r) {
+          if (!ops.cplxMode) {
+            const double x = va[r * ops.kr + a % ops.k], y = vb[r * ops.kr + b % ops.k];
+            re += x * y;
+            na += x * x;
+          } else {
+            const double xr = va[r * ops.kr + 2 * (a % ops.k)], xi = va[r * ops.kr + 2 * (a % ops.k) + 1];
+            const double yr = vb[r * ops.kr + 2 * (b % ops.k)], yi = vb[r * ops.kr + 2 * (b % ops.k) + 1];
+            re += xr * yr + xi * yi;
+            im += xr * yi - xi * yr;
+            na += xr * xr + xi * xi;
+          }
+        }
+        if (na == 0.0) continue;  // dead column
+        worst = std::max(worst, std::hypot(re - (a == b ? 1.0 : 0.0), im));
+      }
+  std::printf("%.3e\n", worst);
+}
+
 }  // namespace
 
 // "lowest" mode: n count, then d[0..n) and e[0..n-1); prints TridiagonalLowest's eigenvalues
@@ -223,7 +265,15 @@ int main(int argc, char** argv) {
   }
   std::fclose(f);
   const bool resume = argc > 2 && std::string(argv[2]) == "resume";
+  const bool ortho = argc > 2 && std::string(argv[2]) == "ortho";
   try {
+    if (ortho) {
+      if (cplxMode)
+        RunOrtho<cplx>(ops, maxIter, seed);
+      else
+        RunOrtho<double>(ops, maxIter, seed);
+      return 0;
+    }
     if (cplxMode)
       resume ? RunResume<cplx>(ops, maxIter, nev, precision, seed) : Run<cplx>(ops, maxIter, nev, precision, seed);
     else

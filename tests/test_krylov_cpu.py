@@ -209,3 +209,25 @@ def test_tridiagonal_lowest_matches_dense(selftest, tmp_path, case):
     scale = np.abs(np.linalg.eigvalsh(T)).max()
     assert got.shape == ref.shape
     assert np.abs(got - ref).max() <= 4e-15 * scale * np.sqrt(n)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("k", [1, 2, 4])
+def test_krylov_basis_stays_orthonormal(selftest, tmp_path, k, cplx):
+    """After 60 vectors of a clustered spectrum the stored Krylov basis is orthonormal to rounding
+    level, for real and complex blocks (the second Gram-Schmidt round of k = 1 reuses the first
+    update's dots, divided by the norm: a wrong scale there leaves basis components behind)."""
+    rng = np.random.default_rng(31 + k + 5 * cplx)
+    d = 220
+    lam = np.concatenate([rng.uniform(-1, -0.99, 20), rng.uniform(0, 3, d - 20)])
+    q, _ = np.linalg.qr(rng.standard_normal((d, d)) + (1j * rng.standard_normal((d, d)) if cplx else 0))
+    m = (q * lam) @ q.conj().T
+    lines = [("%d %d %d %d %.17g %d %d" % (cplx, k, 60, 4, 1e-14, 3, 1)), str(d)]
+    mm = np.asarray(m, complex if cplx else float)
+    flat = np.stack([mm.real.ravel(), mm.imag.ravel()], 1).ravel() if cplx else mm.ravel()
+    lines.append(" ".join(repr(float(v)) for v in flat))
+    path = tmp_path / "p.txt"
+    path.write_text("\n".join(lines) + "\n")
+    out = subprocess.run([selftest, str(path), "ortho"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 0, out.stderr
+    assert float(out.stdout.split()[0]) < 1e-12

@@ -70,6 +70,7 @@ GPU_SIGNATURES = {
     "dh_sector_set_get_states": (c_int, [c_void_p, P_U64]),
     "dh_sector_set_factors": (c_int, [c_void_p, P_D, P_I, c_int, c_double]),
     "dh_sector_set_launch": (c_int, [c_void_p, c_int, c_int]),
+    "dh_sector_set_blas_launch": (c_int, [c_void_p, c_int, c_int64]),
     "dh_sector_set_count_elements": (c_int, [c_void_p, P_I64]),
     "dh_sectors_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_char_p]),
     "dh_sectors_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
@@ -532,6 +533,10 @@ class SphereSectorSet:
 
     def set_launch(self, lanes_per_row=1, block_size=256):
         self.arch.check(gpu().dh_sector_set_launch(self.h, lanes_per_row, block_size), "dh_sector_set_launch")
+
+    def set_blas_launch(self, axpy_groups_per_cu=16, dots_rows_per_tile=0):
+        self.arch.check(gpu().dh_sector_set_blas_launch(self.h, axpy_groups_per_cu, dots_rows_per_tile),
+                        "dh_sector_set_blas_launch")
 
     def count_elements(self):
         counts = np.zeros(self.nbr_sectors, np.int64)

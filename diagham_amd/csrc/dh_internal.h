@@ -144,6 +144,9 @@ struct dh_sector_set {
   double* d_phase = nullptr;   // per sector `modulo` (cos, sin) pairs
   int lutBits = 0, modulo = 1, stateShift = 0;
   int lanesPerRow = 1, blockSize = 256;
+  // segmented BLAS launch shape: persistent axpy workgroups per CU, dots rows per tile (0: auto)
+  int axpyGroupsPerCU = 16;
+  int64_t dotsRowsPerTile = 0;
   // per-call work descriptor: [active sector ids (int64)] [tile prefix (int64)]
   int64_t* d_work = nullptr;
   size_t workCap = 0;

@@ -234,6 +234,16 @@ extern "C" int dh_sector_set_launch(dh_sector_set* S, int lanesPerRow, int block
   return DH_OK;
 }
 
+extern "C" int dh_sector_set_blas_launch(dh_sector_set* S, int axpyGroupsPerCU, int64_t dotsRowsPerTile) {
+  if (!S) return DH_E_ARG;
+  if (axpyGroupsPerCU < 1 || axpyGroupsPerCU > 64) return dh_fail(S->ctx, DH_E_ARG, "axpy groups per CU in [1, 64]");
+  if (dotsRowsPerTile < 0 || (dotsRowsPerTile && dotsRowsPerTile < 256))
+    return dh_fail(S->ctx, DH_E_ARG, "dots rows per tile: 0 (automatic) or >= 256");
+  S->axpyGroupsPerCU = axpyGroupsPerCU;
+  S->dotsRowsPerTile = dotsRowsPerTile;
+  return DH_OK;
+}
+
 // work descriptor of the active sectors for tiles of `rowsPerTile` rows:
 // d_work[0..na) = sector ids, d_work[na..2na] = tile prefix
 int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_t rowsPerTile, int* nActive,
@@ -495,7 +505,8 @@ static int sectors_dots(dh_sector_set* S, const double* V, int nb, const double*
                         double* h_out) {
   dh_context* c = S->ctx;
   constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
-  const int64_t rowsPerTile = std::max<int64_t>(8192, (S->totalDim + 32767) / 32768);
+  const int64_t rowsPerTile =
+      S->dotsRowsPerTile ? S->dotsRowsPerTile : std::max<int64_t>(8192, (S->totalDim + 32767) / 32768);
   int na;
   int64_t tiles;
   int rc = dh_sectors_upload_work(S, active, rowsPerTile, &na, &tiles);
@@ -532,17 +543,21 @@ extern "C" int dh_sectors_block_dots(dh_sector_set* S, const double* V, int nb, 
   }
 }
 
-// axpy / rmul: W[row][q] (+)= sum over (b, p) of V_b[row][p] C_j[(b*K+p)*K+q]; rmul uses W itself
+// axpy / rmul: W[row][q] (+)= sum over (b, p) of V_b[row][p] C_j[(b*K+p)*K+q]; rmul uses W itself.
+// The coefficients sit in LDS in slabs of `slab` basis blocks (dynamic LDS of slab * K * K
+// doubles, so short bases leave room for more workgroups per CU). Basis rows are read U blocks at
+// a time, so one lane keeps U independent loads in flight.
 template <int K, bool RMUL>
 __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restrict__ V, int nb, int64_t totalDim,
                                                            double* __restrict__ W,
                                                            const int64_t* __restrict__ offsets,
                                                            const int64_t* __restrict__ work, int nActive,
-                                                           int64_t totalTiles, const double* __restrict__ C) {
+                                                           int64_t totalTiles, int slab, const double* __restrict__ C) {
   constexpr int KK = K * K;
-  constexpr int SLAB = (4096 / KK) > 0 ? (4096 / KK) : 1;
-  __shared__ double sC[SLAB * KK];
+  constexpr int U = K == 1 ? 8 : (K == 2 ? 4 : (K == 4 ? 2 : 1));
+  extern __shared__ double sC[];
   const int64_t* prefix = work + nActive;
+  const int64_t stride = totalDim * K;
   // each workgroup walks a contiguous range of tiles, so it stays in one sector for many tiles:
   // the sector is found by advancing through the prefix, and a coefficient matrix that fits one
   // LDS slab is loaded only when the sector changes
@@ -561,20 +576,34 @@ __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restr
     double acc[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) acc[q] = 0.0;
-    for (int s0 = 0; s0 < nb; s0 += SLAB) {
-      const int ns = min(SLAB, nb - s0);
-      if (nb > SLAB || staged != lo) {
+    for (int s0 = 0; s0 < nb; s0 += slab) {
+      const int ns = min(slab, nb - s0);
+      if (nb > slab || staged != lo) {
         __syncthreads();
         for (int i = threadIdx.x; i < ns * KK; i += blockDim.x) sC[i] = Cj[(int64_t)s0 * KK + i];
         __syncthreads();
         staged = lo;
       }
       if (active) {
-        for (int b = 0; b < ns; ++b) {
-          const double* vb = (RMUL ? W : V + (int64_t)(s0 + b) * totalDim * K) + row * K;
+        const double* vb = (RMUL ? W : V + (int64_t)s0 * stride) + row * K;
+        int b = 0;
+        for (; b + U <= ns; b += U) {
+          double v[U][K];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int p = 0; p < K; ++p) v[u][p] = vb[(int64_t)(b + u) * stride + p];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int p = 0; p < K; ++p)
+#pragma unroll
+              for (int q = 0; q < K; ++q) acc[q] += v[u][p] * sC[(b + u) * KK + p * K + q];
+        }
+        for (; b < ns; ++b) {
 #pragma unroll
           for (int p = 0; p < K; ++p) {
-            double v = vb[p];
+            double v = vb[(int64_t)b * stride + p];
 #pragma unroll
             for (int q = 0; q < K; ++q) acc[q] += v * sC[b * KK + p * K + q];
           }
@@ -613,9 +642,12 @@ static int sectors_axpy(dh_sector_set* S, const double* V, int nb, double* W, co
     S->coefCap = bytes;
   }
   if ((rc = dh_upload_async(c, S->d_coef, h_c, bytes))) return rc;
-  int64_t grid = std::min<int64_t>(tiles, (int64_t)c->nbrCU * 8);
-  hipLaunchKernelGGL((sectors_axpy_kernel<K, RMUL>), dim3((unsigned)grid), dim3(256), 0, c->stream, V, nb,
-                     S->totalDim, W, S->d_offsets, S->d_work, na, tiles, S->d_coef);
+  // LDS slab: the whole coefficient block when it fits 32 KiB, else 32 KiB pieces
+  const int slab = std::min(nb, std::max(1, 4096 / (K * K)));
+  const size_t lds = sizeof(double) * (size_t)slab * K * K;
+  int64_t grid = std::min<int64_t>(tiles, (int64_t)c->nbrCU * S->axpyGroupsPerCU);
+  hipLaunchKernelGGL((sectors_axpy_kernel<K, RMUL>), dim3((unsigned)grid), dim3(256), lds, c->stream, V, nb,
+                     S->totalDim, W, S->d_offsets, S->d_work, na, tiles, slab, S->d_coef);
   DH_HIP(c, hipGetLastError());
   return DH_OK;
 }

@@ -160,17 +160,39 @@ class SegmentedBlockLanczos {
       LastCheck[s] = -1;
     }
     StepCount = 0;
+    LaunchedStep = -1;
   }
 
   // one Lanczos step X_j -> (A_j, B_j, X_{j+1}) in every segment flagged in `active`
   void Step(const std::vector<unsigned char>& active) {
+    LaunchProduct(active);
+    FinishStep(active);
+  }
+
+  // first half of a step: enqueue W = H X_j for the segments flagged in `active` (nothing waits,
+  // so the host can test convergence while the device multiplies)
+  void LaunchProduct(const std::vector<unsigned char>& active) {
     const int j = StepCount;
     if (j + 1 > MaxBlocks) throw std::runtime_error("Krylov storage exhausted");
+    Ops->Product(Block(j), Block(j + 1), active.data());
+    Launched = active;
+    LaunchedStep = j;
+  }
+
+  // second half: orthogonalise the product of the segments flagged in `active`, which must be
+  // among those of the pending LaunchProduct
+  void FinishStep(const std::vector<unsigned char>& active) {
+    const int j = StepCount;
+    if (LaunchedStep != j || Launched.size() != active.size())
+      throw std::logic_error("FinishStep without the product of this step");
     std::vector<int> act;
     for (size_t s = 0; s < active.size(); ++s)
-      if (active[s]) act.push_back((int)s);
+      if (active[s]) {
+        if (!Launched[s]) throw std::logic_error("FinishStep on a segment whose product was not launched");
+        act.push_back((int)s);
+      }
+    LaunchedStep = -1;
     double* W = Block(j + 1);
-    Ops->Product(Block(j), W, active.data());
     std::vector<Rounds> r;
     Orthonormalize(W, j + 1, act, r);
     for (size_t a = 0; a < act.size(); ++a) {
@@ -353,6 +375,7 @@ class SegmentedBlockLanczos {
       LastCheck[s] = last;
     }
     StepCount = steps;
+    LaunchedStep = -1;
     NextSeed = seed;
   }
 
@@ -589,6 +612,8 @@ class SegmentedBlockLanczos {
   double* Storage;
   int64_t BlockDoubles;
   int StepCount = 0;
+  int LaunchedStep = -1;               // step whose product LaunchProduct enqueued (-1: none)
+  std::vector<unsigned char> Launched;  // segments of that product
   uint64_t NextSeed = 1;
   std::vector<std::vector<std::vector<T>>> A, B;
   std::vector<std::vector<std::vector<char>>> DeadCols;  // per segment, per block X_b, per column

@@ -323,27 +323,35 @@ inline int SectorBlockCaps(const std::vector<int64_t>& dims, int k, int maxIter,
 
 // Steps every unconverged sector of `lanczos` together until each has converged (strong test of
 // FullReorthogonalizedLanczosAlgorithm.cc:325), completed its Krylov space or used its blocks.
+// The product of the next step is enqueued before the convergence tests of this one run on the
+// host, for every sector that can still step; a sector that converges in those tests drops out of
+// the orthogonalisation, and its one extra product is discarded.
 template <class T>
 void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>& blockCap, int nbrEigenvalue,
                       double precision, uint64_t seed, std::vector<SectorResult*>& results) {
   const int na = (int)results.size();
   const int k = lanczos.BlockSize();
   lanczos.Start(seed);
-  std::vector<unsigned char> active(na, 1);
+  std::vector<unsigned char> active(na, 1), next(na, 0);
   std::vector<char> done(na, 0);
+  lanczos.LaunchProduct(active);
   while (true) {
     std::vector<int> act;
-    for (int i = 0; i < na; ++i) {
-      active[i] = done[i] ? 0 : 1;
-      if (!done[i]) act.push_back(i);
-    }
+    for (int i = 0; i < na; ++i)
+      if (active[i]) act.push_back(i);
     if (act.empty()) break;
-    lanczos.Step(active);
+    lanczos.FinishStep(active);
+    bool any = false;
+    for (int i = 0; i < na; ++i) {
+      next[i] = active[i] && !(lanczos.Exhausted(i) || lanczos.NbrBlocks(i) >= blockCap[i]);
+      any = any || next[i];
+    }
+    if (any) lanczos.LaunchProduct(next);
     for (int i : act) results[i]->products += k;
 #pragma omp parallel for schedule(dynamic, 1)
     for (size_t a = 0; a < act.size(); ++a) {
       const int i = act[a];
-      const bool last = lanczos.Exhausted(i) || lanczos.NbrBlocks(i) >= blockCap[i];
+      const bool last = !next[i];
       if (!last && !lanczos.CheckDue(i, nbrEigenvalue)) continue;
       std::vector<double> cur;
       const bool ok = lanczos.TestConvergence(i, nbrEigenvalue, precision, cur);
@@ -351,6 +359,7 @@ void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>&
       if (ok) results[i]->converged = true;
       if (ok || last) done[i] = 1;
     }
+    for (int i = 0; i < na; ++i) active[i] = next[i] && !done[i];
   }
 }
 

@@ -148,12 +148,15 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   std::vector<int64_t> off(na + 1);
   dh_sector_set_offsets(S, off.data());
   const int64_t total = off.back();
-  // small sets: more lanes per row for enough wavefronts; large ones: two lanes per row and
-  // 1024-thread workgroups (the fastest shape of the all-sector product on MI355X, k = 1 and 8)
+  // small sets: more lanes per row for enough wavefronts; large ones: the fastest shapes of the
+  // all-sector product measured on MI355X (k >= 4: two lanes per row, 1024-thread workgroups;
+  // k < 4: one lane per row, 256-thread workgroups)
   if (total < 4ll * 256 * 256)
     Arch->Check(dh_sector_set_launch(S, 4, 512), "launch");
+  else if (BlockSize >= 4)
+    Arch->Check(dh_sector_set_launch(S, 2, 1024), "launch");
   else
-    Arch->Check(dh_sector_set_launch(S, 2, BlockSize >= 4 ? 1024 : 512), "launch");
+    Arch->Check(dh_sector_set_launch(S, 1, 256), "launch");
   std::vector<int64_t> dims(na);
   for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
   std::vector<int> blockCap;

@@ -174,6 +174,10 @@ int dh_sector_set_get_states(dh_sector_set* set, uint64_t* states);
 int dh_sector_set_factors(dh_sector_set* set, const double* factors, const int* pair_offset, int nbr_factors,
                           double shift);
 int dh_sector_set_launch(dh_sector_set* set, int lanes_per_row, int block_size);
+/* Launch shape of the segmented vector operations of a set (no counterpart in the reference, a
+ * tuning knob like dh_sector_set_launch): persistent axpy / rmul workgroups per CU (default 16) and
+ * rows per dots tile (0: automatic). */
+int dh_sector_set_blas_launch(dh_sector_set* set, int axpy_groups_per_cu, int64_t dots_rows_per_tile);
 /* Matrix elements per product of each sector of a sphere set (counts[nbr_sectors]), counted as
  * dh_hamiltonian_count_elements does: the "nbr interaction" the reference prints per sector
  * (AbstractQHEOnSphereHamiltonian.cc:3721 PartialFastMultiplicationMemory). */

@@ -91,10 +91,14 @@ def test_sector_element_counts(arch):
     S.close()
 
 
-def test_segmented_vector_operations(arch):
+# (k, basis blocks, axpy workgroups per CU, dots rows per tile): the unrolled and remainder block
+# loops of the axpy, one and several LDS slabs (k = 16: 16 blocks per slab), short and long tiles
+@pytest.mark.parametrize("k,nb,groups,rows", [(4, 3, 8, 0), (1, 11, 8, 0), (1, 11, 2, 256), (2, 5, 32, 0),
+                                              (8, 2, 8, 512), (16, 17, 8, 0)])
+def test_segmented_vector_operations(arch, k, nb, groups, rows):
     c = __import__("diagham_amd.capi", fromlist=["capi"])
     S = SectorSet(arch, N, LZ, SECTORS, coulomb(LZ))
-    k, nb = 4, 3
+    arch.check(c.gpu().dh_sector_set_blas_launch(S.h, groups, rows), "blas launch")
     T = S.total
     rng = np.random.default_rng(2)
     Vb = rng.standard_normal(nb * T * k)

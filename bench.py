@@ -68,7 +68,7 @@ def load_traffic(name):
 
 def spin_hv_bench(capi, arch, stream, L, steps, warmup, k=1):
     p = capi.Problem.spin_chain(arch, L, 0, 1.0, 1.0, True)
-    p.set_launch(1, 256, capi.INDEX_RANK, 1)
+    p.set_launch(1, 512, capi.INDEX_RANK, 1)  # 512-thread workgroups: the fastest measured shape
     nnz = p.count_elements()
     x = capi.DeviceBuffer(arch, p.dim * k)
     y = capi.DeviceBuffer(arch, p.dim * k)
@@ -216,20 +216,26 @@ def cpu_baseline(L=28, seconds_target=15.0):
     t0 = time.perf_counter()
     O.lib().or_spin_hv_rows(s.h, O.dptr(J), O.dptr(J), 1, O.dptr(x), O.dptr(y), 0, rows, cores)
     dt = time.perf_counter() - t0
-    # grow the sample to about seconds_target of CPU work
-    if dt < seconds_target / 4:
-        rows = int(min(s.dim, rows * max(1.0, seconds_target / max(dt, 1e-3))))
-        y[:] = 0.0
+    # grow the sample to about seconds_target of CPU work: more rows, then whole products repeated
+    passes = 1
+    while dt < seconds_target / 2 and (rows < s.dim or passes == 1):
+        scale = seconds_target / max(dt, 1e-3)
+        if rows < s.dim:
+            rows = int(min(s.dim, rows * scale))
+        else:
+            passes = max(2, int(round(scale)))
         t0 = time.perf_counter()
-        O.lib().or_spin_hv_rows(s.h, O.dptr(J), O.dptr(J), 1, O.dptr(x), O.dptr(y), 0, rows, cores)
+        for _ in range(passes):
+            y[:] = 0.0
+            O.lib().or_spin_hv_rows(s.h, O.dptr(J), O.dptr(J), 1, O.dptr(x), O.dptr(y), 0, rows, cores)
         dt = time.perf_counter() - t0
     states = s.states()[:rows] & ((1 << L) - 1)
     rot = ((states >> 1) | ((states & 1) << (L - 1)))
     anti = np.unpackbits((states ^ rot).astype(">u4").view(np.uint8)).sum()
-    elements = int(anti) + rows  # off-diagonal flips + diagonal
+    elements = (int(anti) + rows) * passes  # off-diagonal flips + diagonal, per pass
     return {"value": elements / dt, "unit": "matrix_elements/s", "cores": cores, "kind": "port",
-            "sample": "oracle H.v over rows [0, %d) of the L=28 basis (%d matrix elements, %.1f s)" %
-                      (rows, elements, dt)}
+            "sample": "oracle H.v over rows [0, %d) of the L=28 basis, %d pass(es) (%d matrix elements, %.1f s)" %
+                      (rows, passes, elements, dt)}
 
 
 def main():
@@ -278,7 +284,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9),
                      "traffic": (traffic or {}).get("bytes_per_launch"),
-                     "kernel": "spin_hv_kernel<1, RANK>", "algorithmic_bytes_per_launch": alg_bytes},
+                     "kernel": "spin_rank_kernel<1, UNIFORM>", "algorithmic_bytes_per_launch": alg_bytes},
         "copy_calibration": calib,
     }
     progress("headline product: %.3f ms" % ms)

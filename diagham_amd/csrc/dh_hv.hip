@@ -243,6 +243,116 @@ __global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double*
   }
 }
 
+// Rank-lookup spin product with 32-bit index arithmetic (a spin-1/2 chain of L <= 32 sites has
+// at most C(32, 16) < 2^31 states in one Sz sector).  The binomials are staged in LDS in the two
+// forms the product reads, with a power-of-two row stride so an entry is one shift-or away:
+//   sStep[i * 64 + c] = C(L-2-i, c-1): index distance of the flip of bond (i, i+1) when c spins up
+//                       sit at bits >= i;
+//   sRank[i * 64 + c] = C(L-1-i, c): contribution of an up spin at bit i with c ups at bits >= i
+//                       to the number of states ordered after the state (the wrap bond's rank).
+// UNIFORM: every bond has the same J and Jz (the Heisenberg and XXZ chains): the diagonal is one
+// popcount and the coefficient of every flip is J/2.
+template <int K, bool UNIFORM>
+__global__ void __launch_bounds__(1024) spin_rank_kernel(SpinArgs a, const double* __restrict__ x,
+                                                         double* __restrict__ y, int add, int64_t first,
+                                                         int64_t count) {
+  constexpr int G = K == 1 ? 8 : (K == 2 ? 4 : 2);
+  __shared__ uint32_t sStep[32 * 64];
+  __shared__ uint32_t sRank[32 * 64];
+  __shared__ double sHalfJ[32], sJz[32];
+  const int L = a.L;
+  for (int e = threadIdx.x; e < 32 * 64; e += blockDim.x) {
+    const int i = e >> 6, c = e & 63;
+    uint32_t st = 0, rk = 0;
+    if (i <= L - 2 && c >= 1 && c - 1 <= L - 2 - i) st = a.binom[(L - 2 - i) * 33 + (c - 1)];
+    if (i <= L - 1 && c <= L - 1 - i) rk = a.binom[(L - 1 - i) * 33 + c];
+    sStep[e] = st;
+    sRank[e] = rk;
+  }
+  if (threadIdx.x < 32) {
+    sHalfJ[threadIdx.x] = a.halfJ[threadIdx.x];
+    sJz[threadIdx.x] = a.Jz[threadIdx.x];
+  }
+  __syncthreads();
+  const uint32_t mask = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
+  const uint32_t openBonds = (1u << (L - 1)) - 1u;  // bonds (i, i+1), i < L-1
+  const uint32_t last = (uint32_t)a.dim - 1u;
+  const double halfJ0 = a.halfJ[0], base = a.shift + 0.25 * a.jzSum, halfJz0 = 0.5 * a.Jz[0];
+  const int64_t nbrTiles = (count + blockDim.x - 1) / blockDim.x;
+  const unsigned vb = dh_xcd_block(blockIdx.x, gridDim.x);
+  for (int64_t tile = vb; tile < nbrTiles; tile += gridDim.x) {
+    const int64_t local = tile * blockDim.x + threadIdx.x;
+    if (local >= count) continue;
+    const uint32_t row = (uint32_t)(first + local);
+    const uint32_t s = a.words[row] & mask;
+    uint32_t anti = (s ^ (s >> 1)) & openBonds;
+    const bool wrap = a.periodic && (((s >> (L - 1)) ^ s) & 1u);
+    double diag;
+    if constexpr (UNIFORM) {
+      diag = base - halfJz0 * (double)(__popc(anti) + (wrap ? 1 : 0));
+    } else {
+      diag = base;
+      for (uint32_t r = anti; r; r &= r - 1) diag -= 0.5 * sJz[__builtin_ctz(r)];
+      if (wrap) diag -= 0.5 * sJz[L - 1];
+    }
+    double acc[K];
+    {
+      const double* xr = x + (size_t)row * K;
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v] = diag * xr[v];
+    }
+    while (anti) {
+      uint32_t idx[G];
+      double coef[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        idx[g] = row;  // padding entries gather the row itself with a zero coefficient
+        coef[g] = 0.0;
+        if (anti) {
+          const int i = __builtin_ctz(anti);
+          anti &= anti - 1;
+          const uint32_t d = sStep[(i << 6) | __popc(s >> i)];
+          const uint32_t t = ((s >> i) & 1u) ? row + d : row - d;
+          idx[g] = min(t, last);  // a state of the space never leaves [0, dim): malformed words stay in bounds
+          coef[g] = UNIFORM ? halfJ0 : sHalfJ[i];
+        }
+      }
+      double v[G][K];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const double* xr = x + (size_t)idx[g] * K;
+        if constexpr (K % 2 == 0) {
+#pragma unroll
+          for (int q = 0; q < K; q += 2) {
+            double2 p = *reinterpret_cast<const double2*>(xr + q);
+            v[g][q] = p.x;
+            v[g][q + 1] = p.y;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < K; ++q) v[g][q] = xr[q];
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] += coef[g] * v[g][q];
+    }
+    if (wrap) {
+      const uint32_t t = s ^ (1u | (1u << (L - 1)));
+      uint32_t after = 0;
+      for (uint32_t r = t; r; r &= r - 1) {
+        const int i = __builtin_ctz(r);
+        after += sRank[(i << 6) | __popc(t >> i)];
+      }
+      gather_fma<K>(acc, x, (int64_t)min(last - after, last), UNIFORM ? halfJ0 : sHalfJ[L - 1]);
+    }
+    double* yr = y + (size_t)row * K;
+#pragma unroll
+    for (int v = 0; v < K; ++v) yr[v] = (add ? yr[v] : 0.0) + acc[v];
+  }
+}
+
 // ----------------------------------------------------------------------------- host plumbing
 static SphereArgs sphere_args(const dh_hamiltonian* h, int lutBitsUsed) {
   const dh_basis* b = h->basis;
@@ -366,10 +476,32 @@ static int spin_launch(dh_hamiltonian* h, const double* x, double* y, int add, i
   return DH_OK;
 }
 
+template <int K, bool UNIFORM>
+static int spin_rank_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+  dh_context* c = h->ctx;
+  SpinArgs a = spin_args(h);
+  auto kern = spin_rank_kernel<K, UNIFORM>;
+  dim3 grid;
+  launch_persistent(c, kern, h->blockSize, 0, (count + h->blockSize - 1) / h->blockSize, grid);
+  hipLaunchKernelGGL(kern, grid, dim3(h->blockSize), 0, c->stream, a, x, y, add, first, count);
+  DH_HIP(c, hipGetLastError());
+  return DH_OK;
+}
+
+// every bond of the chain (the L-1 open bonds, and the wrap bond when periodic) has the same J and Jz
+static bool spin_uniform(const dh_hamiltonian* h) {
+  const int L = h->basis->chainLength, nb = h->periodic ? L : L - 1;
+  for (int i = 1; i < nb; ++i)
+    if (h->J[i] != h->J[0] || h->Jz[i] != h->Jz[0]) return false;
+  return true;
+}
+
 template <int K>
 static int spin_dispatch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
   if (h->indexMode == DH_INDEX_BSEARCH) return spin_launch<K, MODE_BSEARCH>(h, x, y, add, first, count);
-  return spin_launch<K, MODE_RANK_GLOBAL>(h, x, y, add, first, count);
+  if (h->basis->dim >= (1ll << 31)) return dh_fail(h->ctx, DH_E_RANGE, "spin sector of 2^31 states or more");
+  if (spin_uniform(h)) return spin_rank_launch<K, true>(h, x, y, add, first, count);
+  return spin_rank_launch<K, false>(h, x, y, add, first, count);
 }
 
 extern "C" int dh_hv(dh_hamiltonian* h, const double* x, double* y, int k, int add, int64_t first, int64_t count) {

@@ -41,6 +41,27 @@ def test_hv_parity(arch, L, periodic, mode, k):
     p.close()
 
 
+@pytest.mark.parametrize("L,sz,periodic,k", [(16, 0, True, 1), (17, -1, True, 2), (18, 2, False, 1),
+                                             (20, 0, True, 8), (13, 3, True, 4)])
+def test_hv_parity_uniform_couplings(arch, L, sz, periodic, k):
+    """Equal J and Jz on every bond (the kernel's one-coefficient branch), odd and even L, Sz != 0."""
+    from diagham_amd import capi
+    J = np.full(L, 0.7)
+    Jz = np.full(L, 1.3)
+    p = capi.Problem.spin_chain(arch, L, sz, J, Jz, periodic)
+    p.set_launch(1, 256, 1, 1)
+    ref = O.SpinChain(L, sz)
+    X = np.stack([O.splitmix(300 + v, p.dim) for v in range(k)], axis=1)
+    Y = p.hv(X.reshape(-1), k).reshape(p.dim, k)
+    for v in range(k):
+        x = np.ascontiguousarray(X[:, v])
+        y_ref = np.zeros_like(x)
+        O.lib().or_spin_hv(ref.h, O.dptr(J), O.dptr(Jz), int(periodic), O.dptr(x), O.dptr(y_ref))
+        err = np.linalg.norm(Y[:, v] - y_ref) / np.linalg.norm(y_ref)
+        assert err < 1e-12, (L, sz, periodic, k, v, err)
+    p.close()
+
+
 def test_row_targets_bit_exact(arch):
     from diagham_amd import capi
     L = 18

@@ -92,7 +92,9 @@ __device__ __forceinline__ int64_t sphere_find(const SphereArgs& a, uint64_t tgt
     return dh_find_rank(tgt, sLo, sHi, a.rankBits, a.dim);
 }
 
-template <int K, int LPR, int MODE>
+// S32 (rank modes, k <= 2): the orbitals fit a 32-bit word and the space has fewer than 2^31
+// states, so the row routine runs on 32-bit state words and 32-bit indices.
+template <int K, int LPR, int MODE, bool S32>
 __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const double* __restrict__ x,
                                                          double* __restrict__ y, int add, int64_t first,
                                                          int64_t count) {
@@ -113,11 +115,22 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
     double acc[K];
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
-    auto find = [&](uint64_t t) -> int64_t { return sphere_find<BASE>(a, t, sLut, sLo, sHi); };
-    if constexpr (MODE >= MODE_RANK_LDS_X4)
-      dh_sphere_row_x4<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
-    else
-      dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+    if constexpr (S32 && BASE != MODE_BSEARCH) {
+      const uint32_t last = (uint32_t)a.dim - 1u, mask = (1u << a.rankBits) - 1u;
+      const int bits = a.rankBits;
+      // clamped so that a target outside the space can never address outside the vectors
+      auto find = [&](uint32_t t) -> uint32_t { return min(last - sLo[t & mask] - sHi[t >> bits], last); };
+      if constexpr (MODE >= MODE_RANK_LDS_X4)
+        dh_sphere_row_x4<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+      else
+        dh_sphere_row<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+    } else {
+      auto find = [&](uint64_t t) -> int64_t { return sphere_find<BASE>(a, t, sLut, sLo, sHi); };
+      if constexpr (MODE >= MODE_RANK_LDS_X4)
+        dh_sphere_row_x4<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+      else
+        dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+    }
     if constexpr (LPR > 1) {
 #pragma unroll
       for (int v = 0; v < K; ++v)
@@ -406,7 +419,9 @@ static int sphere_launch(dh_hamiltonian* h, const double* x, double* y, int add,
   SphereArgs a = sphere_args(h, lutBitsUsed);
   size_t lds = sphere_lds(h, ModeBase<MODE>::value, lutBitsUsed);
   if (lds > kLdsLimit) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
-  auto kern = sphere_hv_kernel<K, LPR, MODE>;
+  // 32-bit words for one or two vectors only, as for the sector sets
+  const bool s32 = K <= 2 && MODE != MODE_BSEARCH && h->basis->nbrOrbitals <= 32 && h->basis->dim < (1ll << 31);
+  auto kern = s32 ? sphere_hv_kernel<K, LPR, MODE, (K <= 2 && MODE != MODE_BSEARCH)> : sphere_hv_kernel<K, LPR, MODE, false>;
   if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int64_t rowsPerBlock = h->blockSize / LPR;
   dim3 grid;

@@ -8,9 +8,16 @@
 #include "dh_index.h"
 #include "dh_internal.h"
 
-template <int K>
-__device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restrict__ x, int64_t idx, double coef) {
-  const double* xr = x + idx * K;
+// Bit helpers over the state word: 64-bit words in general, 32-bit words when the orbitals fit
+// (half the instructions for every shift, mask, ctz and popcount of the row routine).
+__device__ __forceinline__ int dh_ctz(uint64_t v) { return __builtin_ctzll(v); }
+__device__ __forceinline__ int dh_ctz(uint32_t v) { return __builtin_ctz(v); }
+__device__ __forceinline__ int dh_popc(uint64_t v) { return __popcll(v); }
+__device__ __forceinline__ int dh_popc(uint32_t v) { return __popc(v); }
+
+template <int K, typename I>
+__device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restrict__ x, I idx, double coef) {
+  const double* xr = x + (size_t)idx * K;
   if constexpr (K % 2 == 0) {
 #pragma unroll
     for (int v = 0; v < K; v += 2) {
@@ -31,17 +38,17 @@ __device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restr
 // Factors come packed symmetric per pair sum (sF, sBase: see dh_pack_sphere_factors).
 // Target indices come from `find(target)`; x is indexed relative to `xbase`.  Each term is one
 // gather; latency is hidden by occupancy (few registers per lane) rather than by batching.
-template <int K, int LPR, typename Find>
-__device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const double* __restrict__ sF,
+template <int K, int LPR, typename U, typename Find>
+__device__ __forceinline__ void dh_sphere_row(U s, int nbrOrbitals, const double* __restrict__ sF,
                                               const int* __restrict__ sBase, int lane, Find find,
                                               const double* __restrict__ xbase, double* acc) {
-  for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
-    const int n1 = __builtin_ctzll(r1);
-    const int pc1 = __popcll(s >> n1);
-    for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
-      const int n2 = __builtin_ctzll(r2);
-      const int sgnA = pc1 + __popcll(s >> n2);
-      const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+  for (U r1 = s; r1; r1 &= r1 - 1) {
+    const int n1 = dh_ctz(r1);
+    const int pc1 = dh_popc(s >> n1);
+    for (U r2 = s & ((U(1) << n1) - U(1)); r2; r2 &= r2 - 1) {
+      const int n2 = dh_ctz(r2);
+      const int sgnA = pc1 + dh_popc(s >> n2);
+      const U t = s ^ (U(1) << n1) ^ (U(1) << n2);
       const int sum = n1 + n2;
       const int m3lo = (sum >> 1) + 1;
       const int m3hi = min(sum, nbrOrbitals - 1);
@@ -49,10 +56,10 @@ __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const
       const int i = n1 - m3lo;
       for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
         const int m4 = sum - m3;
-        const uint64_t cm = (1ull << m3) | (1ull << m4);
+        const U cm = (U(1) << m3) | (U(1) << m4);
         if (t & cm) continue;
-        const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
-        const int64_t idx = find(t | cm);
+        const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
+        const auto idx = find(t | cm);
         double coef = fS[dh_packed_index(i, m3 - m3lo)];
         if (sg & 1) coef = -coef;
         dh_gather_fma<K>(acc, xbase, idx, coef);
@@ -66,32 +73,32 @@ __device__ __forceinline__ void dh_sphere_row(uint64_t s, int nbrOrbitals, const
 // coefficient), issues their index lookups together and then their four gathers, so four
 // dependent chains are in flight per lane.  `s` must be a state of the space (the padding
 // slots look it up).
-template <int K, int LPR, typename Find>
-__device__ __forceinline__ void dh_sphere_row_x4(uint64_t s, int nbrOrbitals, const double* __restrict__ sF,
+template <int K, int LPR, typename U, typename Find>
+__device__ __forceinline__ void dh_sphere_row_x4(U s, int nbrOrbitals, const double* __restrict__ sF,
                                                  const int* __restrict__ sBase, int lane, Find find,
                                                  const double* __restrict__ xbase, double* acc) {
-  for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
-    const int n1 = __builtin_ctzll(r1);
-    const int pc1 = __popcll(s >> n1);
-    for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
-      const int n2 = __builtin_ctzll(r2);
-      const int sgnA = pc1 + __popcll(s >> n2);
-      const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+  for (U r1 = s; r1; r1 &= r1 - 1) {
+    const int n1 = dh_ctz(r1);
+    const int pc1 = dh_popc(s >> n1);
+    for (U r2 = s & ((U(1) << n1) - U(1)); r2; r2 &= r2 - 1) {
+      const int n2 = dh_ctz(r2);
+      const int sgnA = pc1 + dh_popc(s >> n2);
+      const U t = s ^ (U(1) << n1) ^ (U(1) << n2);
       const int sum = n1 + n2;
       const int m3lo = (sum >> 1) + 1;
       const int m3hi = min(sum, nbrOrbitals - 1);
       const double* fS = sF + sBase[sum];
       const int i = n1 - m3lo;
       for (int m3b = m3lo + lane; m3b <= m3hi; m3b += 4 * LPR) {
-        int64_t idx[4];
+        decltype(find(s)) idx[4];
         double coef[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int m3 = min(m3b + u * LPR, m3hi);  // clamped so that every shift stays in range
           const int m4 = sum - m3;
-          const uint64_t cm = (1ull << m3) | (1ull << m4);
+          const U cm = (U(1) << m3) | (U(1) << m4);
           const bool ok = (m3b + u * LPR <= m3hi) && !(t & cm);
-          const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
           const double c = ok ? fS[dh_packed_index(i, m3 - m3lo)] : 0.0;
           coef[u] = (sg & 1) ? -c : c;
           idx[u] = find(ok ? (t | cm) : s);

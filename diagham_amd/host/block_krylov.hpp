@@ -26,6 +26,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <istream>
 #include <ostream>
 #include <stdexcept>
@@ -121,7 +122,8 @@ class SegmentedBlockLanczos {
  public:
   using S = KrylovScalar<T>;
 
-  // storage: (maxBlocks + 2) device blocks of blockDoubles (the last one is scratch)
+  // storage: (maxBlocks + 2) device blocks of blockDoubles (the last one is scratch), or fewer with
+  // SetGrowth
   SegmentedBlockLanczos(BlockOps* ops, const std::vector<int64_t>& dims, int k, int maxBlocks, double* storage,
                         int64_t blockDoubles)
       : Ops(ops),
@@ -129,6 +131,7 @@ class SegmentedBlockLanczos {
         K(k),
         KR(k * S::Real),
         MaxBlocks(maxBlocks),
+        Capacity(maxBlocks),
         Storage(storage),
         BlockDoubles(blockDoubles),
         A(dims.size()),
@@ -138,6 +141,29 @@ class SegmentedBlockLanczos {
         LastCheck(dims.size(), -1) {}
 
   double* Block(int j) const { return Storage + (int64_t)j * BlockDoubles; }
+  // the scratch block after the last Krylov block (free between steps)
+  double* Scratch() const { return Block(Capacity + 1); }
+  double* StorageBase() const { return Storage; }
+  int BlockCapacity() const { return Capacity; }
+
+  // Krylov storage that grows on demand instead of holding maxBlocks + 2 blocks from the start:
+  // the storage passed to the constructor holds `capacity` + 2 blocks, and grow(blocks, used, old)
+  // must return storage for blocks + 2 blocks whose first `used` blocks are those of `old` (and
+  // release `old`).  Capacity grows by half each time.
+  using GrowFn = std::function<double*(int blocks, int used, double* old)>;
+  void SetGrowth(int capacity, GrowFn grow) {
+    Capacity = std::max(1, std::min(capacity, MaxBlocks));
+    Grow = std::move(grow);
+  }
+  // room for the Krylov blocks X_0 .. X_blocks
+  void Reserve(int blocks) {
+    if (blocks <= Capacity) return;
+    if (blocks > MaxBlocks || !Grow) throw std::runtime_error("Krylov storage exhausted");
+    const int cap = std::min(MaxBlocks, std::max(blocks, Capacity + Capacity / 2));
+    Storage = Grow(cap, std::min(StepCount + 1, Capacity + 1), Storage);
+    Capacity = cap;
+  }
+
   int BlockSize() const { return K; }
   int Steps() const { return StepCount; }
   int NbrBlocks(int s) const { return (int)A[s].size(); }
@@ -174,6 +200,7 @@ class SegmentedBlockLanczos {
   void LaunchProduct(const std::vector<unsigned char>& active) {
     const int j = StepCount;
     if (j + 1 > MaxBlocks) throw std::runtime_error("Krylov storage exhausted");
+    Reserve(j + 1);
     Ops->Product(Block(j), Block(j + 1), active.data());
     Launched = active;
     LaunchedStep = j;
@@ -351,6 +378,7 @@ class SegmentedBlockLanczos {
     get(&k, 4);
     if (segs != (int32_t)Dims.size() || k != K) throw std::invalid_argument("the Lanczos state has another block layout");
     if (steps < 0 || steps + 1 > MaxBlocks + 1) throw std::invalid_argument("the Lanczos state does not fit the Krylov storage");
+    Reserve(steps);
     for (size_t s = 0; s < Dims.size(); ++s) {
       int64_t dim;
       int32_t m, nd, np, last;
@@ -584,7 +612,7 @@ class SegmentedBlockLanczos {
     // W = Q (deficient columns zero) + random vectors in the deficient columns
     std::vector<unsigned char> mask(Dims.size(), 0);
     for (int s : needSeg) mask[s] = 1;
-    double* Rnd = Block(MaxBlocks + 1);
+    double* Rnd = Scratch();
     Ops->Random(Rnd, NextSeed++);
     std::vector<double> Sel(need.size() * KK, 0.0);
     for (size_t i = 0; i < need.size(); ++i) {
@@ -609,6 +637,8 @@ class SegmentedBlockLanczos {
   BlockOps* Ops;
   std::vector<int64_t> Dims;
   int K, KR, MaxBlocks;
+  int Capacity;  // Krylov blocks the storage holds (blocks 0 .. Capacity, then the scratch block)
+  GrowFn Grow;
   double* Storage;
   int64_t BlockDoubles;
   int StepCount = 0;

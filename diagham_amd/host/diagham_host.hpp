@@ -309,6 +309,15 @@ class SectorSetOps : public BlockOps {
   int64_t Rows;
 };
 
+// Krylov blocks a basis starts with (about 256 vectors, never more than maxBlocks); the engine
+// grows it by half when it is full.  Holding all maxBlocks + 2 blocks from the start (160 GB for
+// 3000 vectors over 6.7 M rows) made every later solve of the same process run its
+// reorthogonalisation at about half speed on MI355X; sized to what the run uses, it does not.
+inline int InitialKrylovBlocks(int k, int maxBlocks) { return std::min(maxBlocks, std::max(32, 256 / k)); }
+// device storage for blocks + 2 blocks of blockDoubles that holds the first `used` blocks of `old`;
+// `old` is released
+double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int blocks, int64_t blockDoubles);
+
 // Krylov blocks each sector may use (maxIter / k + 1, and no more than its space needs); returns
 // the largest
 inline int SectorBlockCaps(const std::vector<int64_t>& dims, int k, int maxIter, std::vector<int>& caps) {

@@ -246,11 +246,25 @@ FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgor
   // the Krylov space of the sector is complete after at most ceil(Dim / k) blocks
   MaxBlocks = (int)std::min<int64_t>(std::max(3, MaxIter / BlockSize + 1), (Dim + BlockSize - 1) / BlockSize + 1);
   if (!Dense()) {
-    d_basis = Arch->AllocateMemory((int64_t)(MaxBlocks + 2) * Dim * BlockSize);
+    const int capacity = InitialKrylovBlocks(BlockSize, MaxBlocks);
+    d_basis = Arch->AllocateMemory((int64_t)(capacity + 2) * Dim * BlockSize);
     Ops.reset(new HamiltonianOps(Arch, Hamiltonian, Dim, BlockSize));
     Engine.reset(new SegmentedBlockLanczos<double>(Ops.get(), std::vector<int64_t>{Dim}, BlockSize, MaxBlocks,
                                                    d_basis, Dim * BlockSize));
+    Engine->SetGrowth(capacity, [this](int blocks, int used, double* old) {
+      d_basis = GrowKrylovStorage(Arch, old, used, blocks, Dim * BlockSize);
+      return d_basis;
+    });
   }
+}
+
+double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int blocks, int64_t blockDoubles) {
+  double* p = arch->AllocateMemory((int64_t)(blocks + 2) * blockDoubles);
+  dh_context* c = arch->Context();
+  if (used > 0) arch->Check(dh_copy(c, p, old, (int64_t)used * blockDoubles), "grow Krylov storage");
+  arch->Check(dh_synchronize(c), "sync");
+  arch->FreeMemory(old);
+  return p;
 }
 
 FullReorthogonalizedBlockLanczosAlgorithm::~FullReorthogonalizedBlockLanczosAlgorithm() {
@@ -302,7 +316,7 @@ void FullReorthogonalizedBlockLanczosAlgorithm::WriteBlock(int j) {
     WriteDeviceVector(Arch, VectorFileName(DiskDirectory, j).c_str(), X, Dim, false);
     return;
   }
-  double* column = Engine->Block(MaxBlocks + 1);  // the scratch block is free between steps
+  double* column = Engine->Scratch();  // the scratch block is free between steps
   for (int p = 0; p < k; ++p) {
     Arch->Check(dh_block_get_column(Arch->Context(), X, Dim, k, p, column), "Krylov column");
     WriteDeviceVector(Arch, VectorFileName(DiskDirectory, (int64_t)j * k + p).c_str(), column, Dim, false);
@@ -346,7 +360,7 @@ void FullReorthogonalizedBlockLanczosAlgorithm::ResumeLanczosAlgorithm() {
   if (dim != Dim) throw std::invalid_argument("Hamiltonian does not match dimension of stored vectors in resuming");
   Engine->LoadState(in);
   const int k = BlockSize;
-  double* column = Engine->Block(MaxBlocks + 1);
+  double* column = Engine->Scratch();
   for (int j = 0; j <= Engine->Steps(); ++j) {
     double* X = Engine->Block(j);
     if (k == 1) {

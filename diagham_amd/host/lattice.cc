@@ -250,10 +250,14 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   std::vector<int> blockCap;
   const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
   const int64_t blockDoubles = total * 2 * k;
-  double* basis = Arch->AllocateMemory((int64_t)(maxBlocks + 2) * blockDoubles);
+  const int capacity = InitialKrylovBlocks(k, maxBlocks);
+  double* basis = Arch->AllocateMemory((int64_t)(capacity + 2) * blockDoubles);
   {
     SectorSetOps ops(Arch, S, k, 2 * k, total);
     SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, basis, blockDoubles);
+    lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
+      return basis = GrowKrylovStorage(Arch, old, used, blocks, blockDoubles);
+    });
     std::vector<SectorResult*> res;
     for (int s : sectors) res.push_back(&out[s]);
     RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, res);

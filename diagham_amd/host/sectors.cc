@@ -161,10 +161,14 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
   std::vector<int> blockCap;
   const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
-  double* basis = Arch->AllocateMemory((int64_t)(maxBlocks + 2) * total * k);
+  const int capacity = InitialKrylovBlocks(k, maxBlocks);
+  double* basis = Arch->AllocateMemory((int64_t)(capacity + 2) * total * k);
   {
     SectorSetOps ops(Arch, S, k, k, total);
     SegmentedBlockLanczos<double> lanczos(&ops, dims, k, maxBlocks, basis, total * k);
+    lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
+      return basis = GrowKrylovStorage(Arch, old, used, blocks, total * k);
+    });
     std::vector<SectorResult*> res;
     for (int s : sectors) res.push_back(&out[s]);
     RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, res);

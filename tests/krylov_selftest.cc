@@ -110,18 +110,33 @@ struct HostOps : BlockOps {
   }
 };
 
+// grow: the Krylov storage starts with two blocks and grows on demand (SetGrowth), each time into
+// a fresh host buffer; the output must not change.
 template <class T>
-void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed) {
+void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed, bool grow) {
   std::vector<int64_t> dims;
   for (size_t s = 0; s + 1 < ops.off.size(); ++s) dims.push_back(ops.off[s + 1] - ops.off[s]);
   std::vector<int> caps;
   const int maxBlocks = SectorBlockCaps(dims, ops.k, maxIter, caps);
-  std::vector<double> storage((size_t)(maxBlocks + 2) * ops.rows * ops.kr, 0.0);
-  SegmentedBlockLanczos<T> lanczos(&ops, dims, ops.k, maxBlocks, storage.data(), ops.rows * ops.kr);
+  const int64_t bd = ops.rows * ops.kr;
+  const int first = grow ? std::min(2, maxBlocks) : maxBlocks;
+  std::vector<double> storage((size_t)(first + 2) * bd, 0.0);
+  SegmentedBlockLanczos<T> lanczos(&ops, dims, ops.k, maxBlocks, storage.data(), bd);
+  int growths = 0;
+  if (grow)
+    lanczos.SetGrowth(first, [&](int blocks, int used, double* old) {
+      if (old != storage.data()) throw std::logic_error("growth from storage the engine does not own");
+      std::vector<double> next((size_t)(blocks + 2) * bd, -7.0);  // poison: only the used blocks are kept
+      std::copy(old, old + (size_t)used * bd, next.begin());
+      storage.swap(next);
+      ++growths;
+      return storage.data();
+    });
   std::vector<SectorResult> out(dims.size());
   std::vector<SectorResult*> res;
   for (auto& r : out) res.push_back(&r);
   RunSectorLanczos(lanczos, caps, nev, precision, seed, res);
+  if (grow) std::fprintf(stderr, "growths %d capacity %d\n", growths, lanczos.BlockCapacity());
   for (auto& r : out) {
     std::printf("%d %d", r.converged ? 1 : 0, r.products);
     for (double e : r.eigenvalues) std::printf(" %.17g", e);
@@ -266,6 +281,7 @@ int main(int argc, char** argv) {
   std::fclose(f);
   const bool resume = argc > 2 && std::string(argv[2]) == "resume";
   const bool ortho = argc > 2 && std::string(argv[2]) == "ortho";
+  const bool grow = argc > 2 && std::string(argv[2]) == "grow";
   try {
     if (ortho) {
       if (cplxMode)
@@ -275,9 +291,10 @@ int main(int argc, char** argv) {
       return 0;
     }
     if (cplxMode)
-      resume ? RunResume<cplx>(ops, maxIter, nev, precision, seed) : Run<cplx>(ops, maxIter, nev, precision, seed);
+      resume ? RunResume<cplx>(ops, maxIter, nev, precision, seed) : Run<cplx>(ops, maxIter, nev, precision, seed, grow);
     else
-      resume ? RunResume<double>(ops, maxIter, nev, precision, seed) : Run<double>(ops, maxIter, nev, precision, seed);
+      resume ? RunResume<double>(ops, maxIter, nev, precision, seed)
+             : Run<double>(ops, maxIter, nev, precision, seed, grow);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -23,7 +23,7 @@ def selftest(tmp_path_factory):
     return exe
 
 
-def run(exe, tmp_path, mats, k, nev, precision=1e-14, max_iter=4000, seed=11):
+def run(exe, tmp_path, mats, k, nev, precision=1e-14, max_iter=4000, seed=11, mode=None, raw=False):
     cplx = any(np.iscomplexobj(m) for m in mats)
     lines = ["%d %d %d %d %.17g %d %d" % (cplx, k, max_iter, nev, precision, seed, len(mats)),
              " ".join(str(m.shape[0]) for m in mats)]
@@ -38,8 +38,11 @@ def run(exe, tmp_path, mats, k, nev, precision=1e-14, max_iter=4000, seed=11):
         lines.append(" ".join(repr(float(v)) for v in flat))
     path = tmp_path / "problem.txt"
     path.write_text("\n".join(lines) + "\n")
-    out = subprocess.run([exe, str(path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    out = subprocess.run([exe, str(path)] + ([mode] if mode else []), stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
     assert out.returncode == 0, out.stderr
+    if raw:
+        return out
     res = []
     for line in out.stdout.strip().splitlines():
         v = line.split()
@@ -69,6 +72,26 @@ def test_sectors_of_every_size(selftest, tmp_path, k, cplx):
         assert conv == 1, m.shape
         assert ev.size == ref.size
         assert np.max(np.abs(ev - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref))), (m.shape, ev, ref)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("k", [1, 2, 8])
+def test_growing_krylov_storage_repeats_the_run(selftest, tmp_path, k, cplx):
+    """Storage that starts with two Krylov blocks and grows on demand (each time into a new
+    buffer whose unused part is poisoned) gives bitwise the same eigenvalues and products as
+    storage for every block from the start, through convergence, exhaustion and the random restarts
+    of a degenerate sector."""
+    rng = np.random.default_rng(300 + k + cplx)
+    mats = [random_hermitian(rng, d, cplx) for d in (3, 17, 70, 130)]
+    lam = np.concatenate([np.full(9, -3.0), rng.uniform(-2, 5, 31)])
+    q, _ = np.linalg.qr(rng.standard_normal((40, 40)) + (1j * rng.standard_normal((40, 40)) if cplx else 0))
+    mats.append((q * lam) @ q.conj().T)
+    for nev, precision in ((5, 1e-14), (40, 0.0)):
+        fixed = run(selftest, tmp_path, mats, k, nev, precision=precision, raw=True)
+        grown = run(selftest, tmp_path, mats, k, nev, precision=precision, mode="grow", raw=True)
+        assert fixed.stdout == grown.stdout
+        growths = int(grown.stderr.split()[1])
+        assert growths >= 2, grown.stderr
 
 
 @pytest.mark.parametrize("cplx", [False, True])

@@ -518,19 +518,28 @@ LanczosResult FullReorthogonalizedBlockLanczosAlgorithm::Solve() {
   // a resumed run has already made some of the first steps
   int first = std::max(2, (NbrEigenvalue + 2 + BlockSize - 1) / BlockSize);
   RunLanczosAlgorithm(std::max(0, first - Engine->Steps()));
+  const std::vector<unsigned char> all(1, 1);
   while (true) {
+    const bool room = !Engine->Exhausted(0) && NbrBlocks() < MaxBlocks;
+    // without disk storage the product of the next step runs while the host tests convergence;
+    // if the test succeeds that product is discarded
+    const bool ahead = room && DiskDirectory.empty();
+    if (ahead) Engine->LaunchProduct(all);
     if (Engine->CheckDue(0, NbrEigenvalue) && TestConvergence()) {
       res.converged = true;
       break;
     }
-    if (Engine->Exhausted(0) || NbrBlocks() >= MaxBlocks) {
+    if (!room) {
       // out of iterations: report the current Ritz values unconverged
       std::vector<double> ev;
       Diagonalize(ev, nullptr);
       Current.assign(ev.begin(), ev.begin() + std::min<int>(NbrEigenvalue, (int)ev.size()));
       break;
     }
-    RunLanczosAlgorithm(1);
+    if (ahead)
+      Engine->FinishStep(all);
+    else
+      RunLanczosAlgorithm(1);
   }
   if (!DiskDirectory.empty()) WriteState();  // with the history of the last convergence test
   Arch->Check(dh_synchronize(Arch->Context()), "sync");

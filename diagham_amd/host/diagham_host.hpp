@@ -309,6 +309,22 @@ class SectorSetOps : public BlockOps {
   int64_t Rows;
 };
 
+// A sector set and its Krylov storage, released on every way out of a solve (errors leave as
+// exceptions, and the storage pointer changes when it grows).
+struct SectorSolveResources {
+  explicit SectorSolveResources(GPUArchitecture* a) : arch(a) {}
+  ~SectorSolveResources() {
+    dh_synchronize(arch->Context());
+    if (basis) arch->FreeMemory(basis);
+    if (set) dh_sector_set_destroy(set);
+  }
+  SectorSolveResources(const SectorSolveResources&) = delete;
+  SectorSolveResources& operator=(const SectorSolveResources&) = delete;
+  GPUArchitecture* arch;
+  double* basis = nullptr;
+  dh_sector_set* set = nullptr;
+};
+
 // Krylov blocks a basis starts with (about 256 vectors, never more than maxBlocks); the engine
 // grows it by half when it is full.  Holding all maxBlocks + 2 blocks from the start (160 GB for
 // 3000 vectors over 6.7 M rows) made every later solve of the same process run its

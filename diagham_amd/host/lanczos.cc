@@ -252,6 +252,7 @@ FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgor
     Engine.reset(new SegmentedBlockLanczos<double>(Ops.get(), std::vector<int64_t>{Dim}, BlockSize, MaxBlocks,
                                                    d_basis, Dim * BlockSize));
     Engine->SetGrowth(capacity, [this](int blocks, int used, double* old) {
+      d_basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
       d_basis = GrowKrylovStorage(Arch, old, used, blocks, Dim * BlockSize);
       return d_basis;
     });
@@ -259,10 +260,18 @@ FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgor
 }
 
 double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int blocks, int64_t blockDoubles) {
-  double* p = arch->AllocateMemory((int64_t)(blocks + 2) * blockDoubles);
+  double* p = nullptr;
   dh_context* c = arch->Context();
-  if (used > 0) arch->Check(dh_copy(c, p, old, (int64_t)used * blockDoubles), "grow Krylov storage");
-  arch->Check(dh_synchronize(c), "sync");
+  try {
+    p = arch->AllocateMemory((int64_t)(blocks + 2) * blockDoubles);
+    if (used > 0) arch->Check(dh_copy(c, p, old, (int64_t)used * blockDoubles), "grow Krylov storage");
+    arch->Check(dh_synchronize(c), "sync");
+  } catch (...) {
+    dh_synchronize(c);
+    if (p) arch->FreeMemory(p);
+    arch->FreeMemory(old);
+    throw;
+  }
   arch->FreeMemory(old);
   return p;
 }

@@ -237,7 +237,9 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
     kx.push_back(Kx[s]);
     ky.push_back(Ky[s]);
   }
-  dh_sector_set* S = CreateSet(kx, ky);
+  SectorSolveResources guard(Arch);
+  guard.set = CreateSet(kx, ky);
+  dh_sector_set* S = guard.set;
   Arch->Check(dh_sector_set_lattice_factors(S, NbrSiteX * NbrSiteY, SumPairCount.data(), SumPairs.data(),
                                             Factors.data(), 0.0),
               "lattice factors");
@@ -251,20 +253,17 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
   const int64_t blockDoubles = total * 2 * k;
   const int capacity = InitialKrylovBlocks(k, maxBlocks);
-  double* basis = Arch->AllocateMemory((int64_t)(capacity + 2) * blockDoubles);
-  {
-    SectorSetOps ops(Arch, S, k, 2 * k, total);
-    SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, basis, blockDoubles);
-    lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
-      return basis = GrowKrylovStorage(Arch, old, used, blocks, blockDoubles);
-    });
-    std::vector<SectorResult*> res;
-    for (int s : sectors) res.push_back(&out[s]);
-    RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, res);
-  }
+  guard.basis = Arch->AllocateMemory((int64_t)(capacity + 2) * blockDoubles);
+  SectorSetOps ops(Arch, S, k, 2 * k, total);
+  SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, guard.basis, blockDoubles);
+  lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
+    guard.basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
+    return guard.basis = GrowKrylovStorage(Arch, old, used, blocks, blockDoubles);
+  });
+  std::vector<SectorResult*> res;
+  for (int s : sectors) res.push_back(&out[s]);
+  RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, res);
   Arch->Check(dh_synchronize(c), "sync");
-  Arch->FreeMemory(basis);
-  dh_sector_set_destroy(S);
 }
 
 }  // namespace dhh

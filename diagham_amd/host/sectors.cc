@@ -142,8 +142,9 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   const int na = (int)sectors.size();
   std::vector<int> lz;
   for (int s : sectors) lz.push_back(TotalLz[s]);
-  dh_sector_set* S = nullptr;
-  Arch->Check(dh_sector_set_create(c, &S, NbrFermions, LzMax, lz.data(), na), "sector set");
+  SectorSolveResources res(Arch);
+  Arch->Check(dh_sector_set_create(c, &res.set, NbrFermions, LzMax, lz.data(), na), "sector set");
+  dh_sector_set* S = res.set;
   Arch->Check(dh_sector_set_factors(S, Factors.data(), PairOffset.data(), (int)Factors.size(), 0.0), "factors");
   std::vector<int64_t> off(na + 1);
   dh_sector_set_offsets(S, off.data());
@@ -162,20 +163,17 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   std::vector<int> blockCap;
   const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
   const int capacity = InitialKrylovBlocks(k, maxBlocks);
-  double* basis = Arch->AllocateMemory((int64_t)(capacity + 2) * total * k);
-  {
-    SectorSetOps ops(Arch, S, k, k, total);
-    SegmentedBlockLanczos<double> lanczos(&ops, dims, k, maxBlocks, basis, total * k);
-    lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
-      return basis = GrowKrylovStorage(Arch, old, used, blocks, total * k);
-    });
-    std::vector<SectorResult*> res;
-    for (int s : sectors) res.push_back(&out[s]);
-    RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, res);
-  }
+  res.basis = Arch->AllocateMemory((int64_t)(capacity + 2) * total * k);
+  SectorSetOps ops(Arch, S, k, k, total);
+  SegmentedBlockLanczos<double> lanczos(&ops, dims, k, maxBlocks, res.basis, total * k);
+  lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
+    res.basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
+    return res.basis = GrowKrylovStorage(Arch, old, used, blocks, total * k);
+  });
+  std::vector<SectorResult*> results;
+  for (int s : sectors) results.push_back(&out[s]);
+  RunSectorLanczos(lanczos, blockCap, NbrEigenvalue, Precision, seed, results);
   Arch->Check(dh_synchronize(c), "sync");
-  Arch->FreeMemory(basis);
-  dh_sector_set_destroy(S);
 }
 
 }  // namespace dhh

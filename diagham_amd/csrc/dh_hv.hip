@@ -161,22 +161,20 @@ struct SpinArgs {
 // Only antiparallel bonds contribute off-diagonal terms (and lower the diagonal by Jz/2 each):
 // they are the set bits of s ^ (s >> 1).  Their target indices are computed G at a time before
 // the G gathers of x are issued, so each lane keeps G independent loads in flight instead of
-// waiting on one gather per bond.
-template <int K, int MODE>
-__global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double* __restrict__ x,
-                                                       double* __restrict__ y, int add, int64_t first,
-                                                       int64_t count) {
+// waiting on one gather per bond.  This kernel finds targets by the LUT-seeded binary search of
+// Spin1_2Chain::FindStateIndex; spin_rank_kernel below computes them from binomials.
+template <int K>
+__global__ void __launch_bounds__(1024) spin_bsearch_kernel(SpinArgs a, const double* __restrict__ x,
+                                                            double* __restrict__ y, int add, int64_t first,
+                                                            int64_t count) {
   constexpr int G = K == 1 ? 8 : (K == 2 ? 4 : 2);
-  __shared__ uint32_t sBinom[33 * 33];
   __shared__ double sHalfJ[32], sJz[32];
   extern __shared__ int sLut[];
-  for (int i = threadIdx.x; i < 33 * 33; i += blockDim.x) sBinom[i] = a.binom[i];
   if (threadIdx.x < 32) {
     sHalfJ[threadIdx.x] = a.halfJ[threadIdx.x];
     sJz[threadIdx.x] = a.Jz[threadIdx.x];
   }
-  if constexpr (MODE == MODE_BSEARCH)
-    for (int i = threadIdx.x; i < a.lutSize; i += blockDim.x) sLut[i] = a.lut[i];
+  for (int i = threadIdx.x; i < a.lutSize; i += blockDim.x) sLut[i] = a.lut[i];
   __syncthreads();
   const int L = a.L;
   const uint32_t mask = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
@@ -209,14 +207,7 @@ __global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double*
         if (anti) {
           const int i = __builtin_ctz(anti);
           anti &= anti - 1;
-          if constexpr (MODE == MODE_BSEARCH) {
-            idx[g] = dh_spin_find_bsearch(s ^ (3u << i), L, sLut, a.lutShift, a.words);
-          } else {
-            const int kc = __popc(s >> i);
-            const int64_t d = sBinom[(L - 2 - i) * 33 + (kc - 1)];
-            int64_t t = ((s >> i) & 1u) ? row + d : row - d;
-            idx[g] = t < 0 ? 0 : (t >= a.dim ? a.dim - 1 : t);
-          }
+          idx[g] = dh_spin_find_bsearch(s ^ (3u << i), L, sLut, a.lutShift, a.words);
           coef[g] = sHalfJ[i];
         }
       }
@@ -243,12 +234,7 @@ __global__ void __launch_bounds__(1024) spin_hv_kernel(SpinArgs a, const double*
     }
     if (wrap) {
       const uint32_t t = s ^ (1u | (1u << (L - 1)));
-      int64_t idx;
-      if constexpr (MODE == MODE_BSEARCH)
-        idx = dh_spin_find_bsearch(t, L, sLut, a.lutShift, a.words);
-      else
-        idx = dh_spin_find_rank(t, L, sBinom, a.dim);
-      gather_fma<K>(acc, x, idx, sHalfJ[L - 1]);
+      gather_fma<K>(acc, x, dh_spin_find_bsearch(t, L, sLut, a.lutShift, a.words), sHalfJ[L - 1]);
     }
     double* yr = y + row * K;
 #pragma unroll
@@ -481,12 +467,13 @@ static SpinArgs spin_args(const dh_hamiltonian* h) {
   return a;
 }
 
-template <int K, int MODE>
-static int spin_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+template <int K>
+static int spin_bsearch_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first,
+                               int64_t count) {
   dh_context* c = h->ctx;
   SpinArgs a = spin_args(h);
-  size_t lds = (MODE == MODE_BSEARCH) ? sizeof(int) * (size_t)a.lutSize : 0;
-  auto kern = spin_hv_kernel<K, MODE>;
+  size_t lds = sizeof(int) * (size_t)a.lutSize;
+  auto kern = spin_bsearch_kernel<K>;
   dim3 grid;
   launch_persistent(c, kern, h->blockSize, lds, (count + h->blockSize - 1) / h->blockSize, grid);
   hipLaunchKernelGGL(kern, grid, dim3(h->blockSize), lds, c->stream, a, x, y, add, first, count);
@@ -516,7 +503,7 @@ static bool spin_uniform(const dh_hamiltonian* h) {
 
 template <int K>
 static int spin_dispatch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
-  if (h->indexMode == DH_INDEX_BSEARCH) return spin_launch<K, MODE_BSEARCH>(h, x, y, add, first, count);
+  if (h->indexMode == DH_INDEX_BSEARCH) return spin_bsearch_launch<K>(h, x, y, add, first, count);
   if (h->basis->dim >= (1ll << 31)) return dh_fail(h->ctx, DH_E_RANGE, "spin sector of 2^31 states or more");
   if (spin_uniform(h)) return spin_rank_launch<K, true>(h, x, y, add, first, count);
   return spin_rank_launch<K, false>(h, x, y, add, first, count);

@@ -519,8 +519,12 @@ static int sectors_dots(dh_sector_set* S, const double* V, int nb, const double*
                         double* h_out) {
   dh_context* c = S->ctx;
   constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
-  const int64_t rowsPerTile =
-      S->dotsRowsPerTile ? S->dotsRowsPerTile : std::max<int64_t>(8192, (S->totalDim + 32767) / 32768);
+  // automatic tile: at least 8192 rows (4096 for k >= 4, whose one-block workgroups are few:
+  // 3.9-4.0 against 2.5-3.6 TB/s at k = 8 over the 91 sectors of N=10 2S=27), and no more
+  // than about 32768 tiles
+  const int64_t rowsPerTile = S->dotsRowsPerTile
+                                  ? S->dotsRowsPerTile
+                                  : std::max<int64_t>(K >= 4 ? 4096 : 8192, (S->totalDim + 32767) / 32768);
   int na;
   int64_t tiles;
   int rc = dh_sectors_upload_work(S, active, rowsPerTile, &na, &tiles);

@@ -79,6 +79,28 @@ def test_sector_set_basis_and_products(arch, lpr):
     S.close()
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_sector_products_beyond_32_orbitals(arch, k):
+    """36 orbitals: one- and two-vector products keep 64-bit state words (the 32-bit form is for
+    at most 32 orbitals) and match the oracle in every sector."""
+    n, lz_max, sectors = 4, 35, [0, 2, 10, 60]
+    V = coulomb(lz_max)
+    S = SectorSet(arch, n, lz_max, sectors, V)
+    fac = O.Factors(lz_max, V)
+    X = O.splitmix(40 + k, S.total * k)
+    Y = S.hv(X, k, [1] * len(sectors), np.zeros(S.total * k))
+    for i, lz in enumerate(sectors):
+        ref = O.FermionSphere(n, lz, lz_max)
+        a, b = S.off[i], S.off[i + 1]
+        assert b - a == ref.dim
+        Xs = X[a * k:b * k].reshape(-1, k)
+        Ys = Y[a * k:b * k].reshape(-1, k)
+        for v in range(k):
+            y_ref = O.fs_hv(ref, fac, np.ascontiguousarray(Xs[:, v]))
+            assert np.linalg.norm(Ys[:, v] - y_ref) <= 1e-12 * max(np.linalg.norm(y_ref), 1e-300), (lz, k, v)
+    S.close()
+
+
 def test_sector_element_counts(arch):
     """Per-sector matrix-element counts of a set equal the oracle's count of each sector."""
     V = coulomb(LZ)

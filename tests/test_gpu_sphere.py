@@ -85,6 +85,27 @@ def test_hv_parity(cfg1, k, mode, lpr):
     cfg1.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
 
 
+@pytest.mark.parametrize("k", [1, 2, 4])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (2, 2), (0, 1)])
+def test_hv_parity_beyond_32_orbitals(arch, k, mode, lpr):
+    """36 orbitals: the rank kernels keep 64-bit state words for every k (the 32-bit form is for
+    at most 32 orbitals)."""
+    from diagham_amd import capi
+    n, lz, lz_max = 4, 2, 35
+    p = capi.Problem.sphere(arch, n, lz, lz_max, coulomb(lz_max))
+    ref_space = O.FermionSphere(n, lz, lz_max)
+    assert p.dim == ref_space.dim
+    fac = O.Factors(lz_max, coulomb(lz_max))
+    X = np.stack([O.splitmix(500 + v, p.dim) for v in range(k)], axis=1)
+    p.set_launch(lanes_per_row=lpr, block_size=256, index_mode=mode)
+    Y = p.hv(X.reshape(-1), k).reshape(p.dim, k)
+    for v in range(k):
+        y_ref = O.fs_hv(ref_space, fac, X[:, v])
+        err = np.linalg.norm(Y[:, v] - y_ref) / np.linalg.norm(y_ref)
+        assert err < 1e-12, (k, mode, lpr, v, err)
+    p.close()
+
+
 def test_hv_shift_add_and_range(cfg1, arch):
     from diagham_amd import capi
     ref_space = O.FermionSphere(8, 0, 21)

@@ -267,7 +267,11 @@ template <int K>
 static int launch_dots(dh_context* c, const double* V, int nb, const double* W, int64_t dim, double* h_out) {
   constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
   const int KK = K * K;
-  int64_t target = std::max<int64_t>(1, std::min<int64_t>((int64_t)c->nbrCU * 4, (dim + 1023) / 1024));
+  // row chunks: enough workgroups in all (about 16 per CU) also when there are few basis blocks to
+  // spread over, at least 4 chunks per CU, and no chunk shorter than 1024 rows
+  const int64_t cols = (nb + G - 1) / G;
+  const int64_t want = std::max<int64_t>((int64_t)c->nbrCU * 4, ((int64_t)c->nbrCU * 16 + cols - 1) / cols);
+  int64_t target = std::max<int64_t>(1, std::min<int64_t>(want, (dim + 1023) / 1024));
   int64_t rowsPerChunk = (dim + target - 1) / target;
   int nbrChunks = (int)((dim + rowsPerChunk - 1) / rowsPerChunk);
   int total = nb * KK;

@@ -193,7 +193,7 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
     alg = rows * (8 + 8 * k) + nnz * 8 * k
     achieved = alg / (ms * 1e-3) / 1e9
     return {"workload": "fermions_sphere_n%d_2s%d_all_Lz_ge_0_sectors_one_product_k%d" % (n, lz, k),
-            "kernel": "sectors_hv_kernel<%d, 2, LO_LDS>" % k, "sectors": len(lzs), "rows": rows,
+            "kernel": "sectors_hv_kernel<%d, 2, LO_LDS, false>" % k, "sectors": len(lzs), "rows": rows,
             "matrix_elements": nnz, "ms": ms, "element_vector_products_per_s": nnz * k / (ms * 1e-3),
             "algorithmic_bytes": alg, "achieved_GBps": achieved, "frac_of_peak": achieved / (HBM_PEAK / 1e9)}
 

@@ -234,6 +234,37 @@ void RunOrtho(HostOps& ops, int maxIter, uint64_t seed) {
   std::printf("%.3e\n", worst);
 }
 
+// The split step refuses misuse: finishing a step whose product was not launched, or
+// orthogonalising a segment the launched product skipped.  Prints one word per check.
+template <class T>
+void RunMisuse(HostOps& ops, int maxIter, uint64_t seed) {
+  std::vector<int64_t> dims;
+  for (size_t s = 0; s + 1 < ops.off.size(); ++s) dims.push_back(ops.off[s + 1] - ops.off[s]);
+  std::vector<int> caps;
+  const int maxBlocks = SectorBlockCaps(dims, ops.k, maxIter, caps);
+  const int64_t bd = ops.rows * ops.kr;
+  std::vector<double> st((size_t)(maxBlocks + 2) * bd);
+  SegmentedBlockLanczos<T> l(&ops, dims, ops.k, maxBlocks, st.data(), bd);
+  l.Start(seed);
+  std::vector<unsigned char> all(dims.size(), 1), some(dims.size(), 1);
+  some[0] = 0;
+  auto refused = [&](auto f) {
+    try {
+      f();
+    } catch (const std::logic_error&) {
+      return true;
+    }
+    return false;
+  };
+  std::printf("%s\n", refused([&] { l.FinishStep(all); }) ? "refused" : "accepted");
+  l.LaunchProduct(some);
+  std::printf("%s\n", refused([&] { l.FinishStep(all); }) ? "refused" : "accepted");
+  l.LaunchProduct(some);
+  std::printf("%s\n", refused([&] { l.FinishStep(some); }) ? "refused" : "accepted");
+  std::printf("%s\n", refused([&] { l.FinishStep(some); }) ? "refused" : "accepted");
+  std::printf("steps %d\n", l.Steps());
+}
+
 }  // namespace
 
 // "lowest" mode: n count, then d[0..n) and e[0..n-1); prints TridiagonalLowest's eigenvalues
@@ -282,7 +313,15 @@ int main(int argc, char** argv) {
   const bool resume = argc > 2 && std::string(argv[2]) == "resume";
   const bool ortho = argc > 2 && std::string(argv[2]) == "ortho";
   const bool grow = argc > 2 && std::string(argv[2]) == "grow";
+  const bool misuse = argc > 2 && std::string(argv[2]) == "misuse";
   try {
+    if (misuse) {
+      if (cplxMode)
+        RunMisuse<cplx>(ops, maxIter, seed);
+      else
+        RunMisuse<double>(ops, maxIter, seed);
+      return 0;
+    }
     if (ortho) {
       if (cplxMode)
         RunOrtho<cplx>(ops, maxIter, seed);

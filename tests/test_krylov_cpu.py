@@ -95,6 +95,16 @@ def test_growing_krylov_storage_repeats_the_run(selftest, tmp_path, k, cplx):
 
 
 @pytest.mark.parametrize("cplx", [False, True])
+def test_split_step_refuses_misuse(selftest, tmp_path, cplx):
+    """FinishStep needs the product of its own step, launched for every segment it
+    orthogonalises; a refused call changes nothing, and a proper one makes exactly one step."""
+    rng = np.random.default_rng(5 + cplx)
+    mats = [random_hermitian(rng, d, cplx) for d in (12, 30)]
+    out = run(selftest, tmp_path, mats, 2, 4, mode="misuse", raw=True)
+    assert out.stdout.split() == ["refused", "refused", "accepted", "refused", "steps", "1"], out.stdout
+
+
+@pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("k", [2, 4, 8])
 def test_run_to_exhaustion_recovers_degenerate_spectrum(selftest, tmp_path, k, cplx):
     """precision 0 runs every sector until its Krylov space is complete: the whole spectrum,

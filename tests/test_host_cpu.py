@@ -40,6 +40,23 @@ def test_libraries_load_without_gpu():
     capi.host()
 
 
+def test_null_handles_are_refused_without_gpu():
+    """Entry points given a null handle or buffer return DH_E_ARG before touching a device."""
+    from diagham_amd import capi
+    g = capi.gpu()
+    E = -1  # DH_E_ARG
+    assert g.dh_sector_set_launch(None, 1, 256) == E
+    assert g.dh_sector_set_blas_launch(None, 8, 0) == E
+    assert g.dh_sectors_hv(None, None, None, 1, 0, None) == E
+    assert g.dh_sectors_block_dots(None, None, 4, None, 1, None, None) == E
+    assert g.dh_sectors_block_axpy(None, None, 4, None, 1, None, None) == E
+    assert g.dh_sectors_block_rmul(None, None, 1, None, None) == E
+    assert g.dh_sector_set_destroy(None) == E
+    assert g.dh_hv(None, None, None, 1, 0, 0, -1) == E
+    assert g.dh_block_dots(None, None, 4, None, 16, 1, None) == E
+    assert g.dh_block_axpy(None, None, 4, None, 16, 1, None) == E
+
+
 def test_dimension_without_gpu(golden):
     from diagham_amd import capi
     assert capi.fermion_sphere_dimension(8, 0, 21) == golden["fermions_sphere_n8_2s21_lz0"]["dimension"]

@@ -192,10 +192,13 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
     S.close()
     alg = rows * (8 + 8 * k) + nnz * 8 * k
     achieved = alg / (ms * 1e-3) / 1e9
+    traffic = load_traffic("traffic_sectors_hv_n%d_2s%d_k%d.json" % (n, lz, k))
     return {"workload": "fermions_sphere_n%d_2s%d_all_Lz_ge_0_sectors_one_product_k%d" % (n, lz, k),
             "kernel": "sectors_hv_kernel<%d, 2, LO_LDS, false>" % k, "sectors": len(lzs), "rows": rows,
             "matrix_elements": nnz, "ms": ms, "element_vector_products_per_s": nnz * k / (ms * 1e-3),
-            "algorithmic_bytes": alg, "achieved_GBps": achieved, "frac_of_peak": achieved / (HBM_PEAK / 1e9)}
+            "algorithmic_bytes": alg, "achieved_GBps": achieved, "frac_of_peak": achieved / (HBM_PEAK / 1e9),
+            "traffic": traffic and traffic["bytes_per_launch"],
+            "measured_traffic_GBps": traffic and traffic["bytes_per_launch"] / (ms * 1e-3) / 1e9}
 
 
 def cpu_baseline(L=28, seconds_target=15.0):

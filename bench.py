@@ -5,7 +5,8 @@
 Headline workload: spin-1/2 Heisenberg chain, L = 28 sites, periodic, J = 1, Sz = 0
 (40,116,600 states).  One step = one H.v over the whole basis (the one-vector product).
 The JSON line carries matrix elements / s, the roofline of that kernel (algorithmic HBM
-bytes per launch from HIP events on the kernel's stream), and beside it:
+bytes per launch from HIP events on the kernel's stream), the measured HBM traffic of that kernel
+(rocprofv3 counters kept under profiles/) as a rate, and beside it:
   lanczos           time to the 4 lowest eigenvalues of that chain; block vs plain to the 10
                     lowest of N = 8 fermions at 2S = 21; and "all_sectors": time to the 10 lowest
                     eigenvalues of every Lz >= 0 sector of N = 10 fermions at 2S = 27, the plain

@@ -200,48 +200,14 @@ __global__ void __launch_bounds__(256) dh_block_dots_kernel(const double* __rest
                                                             const double* __restrict__ W, int64_t dim,
                                                             int64_t rowsPerChunk, double* __restrict__ partial) {
   constexpr int KK = K * K;
-  __shared__ double red[256 / 64][G * KK];
+  __shared__ double red[256 / 64][G * KK];  // dh_dots_tile's per-wavefront sums
   const int chunk = blockIdx.y;  // blocks of one chunk are adjacent: its W rows stay in L2
   const int b0 = blockIdx.x * G;
   const int64_t rowBegin = (int64_t)chunk * rowsPerChunk;
   const int64_t rowEnd = min(dim, rowBegin + rowsPerChunk);
-  double acc[G * KK];
-#pragma unroll
-  for (int i = 0; i < G * KK; ++i) acc[i] = 0.0;
-  for (int64_t row = rowBegin + threadIdx.x; row < rowEnd; row += blockDim.x) {
-    double w[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) w[q] = W[row * K + q];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      if (b0 + g < nbrBlocks) {
-        const double* vb = V + (int64_t)(b0 + g) * dim * K + row * K;
-#pragma unroll
-        for (int p = 0; p < K; ++p) {
-          double v = vb[p];
-#pragma unroll
-          for (int q = 0; q < K; ++q) acc[g * KK + p * K + q] += v * w[q];
-        }
-      }
-    }
-  }
-  // wave reduction then across waves
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < G * KK; ++i) {
-    double v = acc[i];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wave][i] = v;
-  }
-  __syncthreads();
-  const int total = nbrBlocks * KK;
-  for (int i = threadIdx.x; i < G * KK; i += blockDim.x) {
-    if (b0 + i / KK < nbrBlocks) {
-      double s = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][i];
-      partial[(int64_t)chunk * total + (int64_t)b0 * KK + i] = s;
-    }
-  }
+  const int64_t total = (int64_t)nbrBlocks * KK;
+  dh_dots_tile<K, G>(V, dim * K, nbrBlocks, b0, W, rowBegin, rowEnd, &red[0][0],
+                     partial + (int64_t)chunk * total + (int64_t)b0 * KK);
 }
 
 // out[i] = sum over chunks of partial[chunk][i]: a workgroup owns 64 consecutive outputs (one

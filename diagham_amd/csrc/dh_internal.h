@@ -186,6 +186,67 @@ int dh_pinned(dh_context* ctx, size_t bytes, double** h_ptr);
 // data is staged in a pinned slot first, so the caller may reuse it as soon as this returns
 int dh_upload_async(dh_context* ctx, void* d_dst, const void* h_src, size_t bytes);
 
+// Lanes that share one row in the block dot products: the K x K products of a row are split over
+// QP x QQ lanes (rows of the basis block x columns of W), so a lane holds at most 16 accumulators
+// per basis block.  One lane per row would hold K*K: 64 doubles at K = 8 (166 VGPRs, three
+// wavefronts per SIMD) and 256 at K = 16 (spilled to scratch, one wavefront per SIMD).
+template <int K>
+struct DotsLanes {
+  static constexpr int QP = K >= 16 ? 4 : (K >= 8 ? 2 : 1);
+  static constexpr int QQ = QP;
+  static constexpr int QS = QP * QQ;
+};
+
+// Sums over rows [rowBegin, rowEnd) of V_b[row][p] W[row][q] for the basis blocks b0 .. b0+G-1
+// (block b at V + b * blockStride, rows of K doubles), written b-major then p then q to out[],
+// for the blocks below nbrBlocks.  Every thread of the workgroup must call it (one barrier);
+// red is LDS for (blockDim.x / 64) * G * K * K doubles.
+template <int K, int G>
+__device__ __forceinline__ void dh_dots_tile(const double* __restrict__ V, int64_t blockStride, int nbrBlocks,
+                                             int b0, const double* __restrict__ W, int64_t rowBegin,
+                                             int64_t rowEnd, double* red, double* __restrict__ out) {
+  constexpr int QP = DotsLanes<K>::QP, QQ = DotsLanes<K>::QQ, QS = DotsLanes<K>::QS;
+  constexpr int KP = K / QP, KQ = K / QQ, NA = G * KP * KQ, KK = K * K;
+  const int sub = threadIdx.x % QS, sp = sub / QQ, sq = sub % QQ;
+  double acc[NA];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) acc[i] = 0.0;
+  for (int64_t row = rowBegin + threadIdx.x / QS; row < rowEnd; row += blockDim.x / QS) {
+    double w[KQ];
+#pragma unroll
+    for (int qq = 0; qq < KQ; ++qq) w[qq] = W[row * K + sq * KQ + qq];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (b0 + g < nbrBlocks) {
+        const double* vb = V + (int64_t)(b0 + g) * blockStride + row * K + sp * KP;
+#pragma unroll
+        for (int pp = 0; pp < KP; ++pp) {
+          const double v = vb[pp];
+#pragma unroll
+          for (int qq = 0; qq < KQ; ++qq) acc[(g * KP + pp) * KQ + qq] += v * w[qq];
+        }
+      }
+  }
+  // sum over the lanes of a wavefront with the same share (lane mod QS), then over wavefronts
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    double v = acc[i];
+    for (int o = 32; o >= QS; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane < QS) red[(wave * QS + lane) * NA + i] = v;
+  }
+  __syncthreads();
+  const int nw = (int)(blockDim.x >> 6);
+  for (int idx = threadIdx.x; idx < G * KK; idx += blockDim.x) {
+    const int g = idx / KK, p = (idx % KK) / K, q = idx % K;
+    if (b0 + g >= nbrBlocks) continue;
+    const int s = (p / KP) * QQ + q / KQ, i = (g * KP + p % KP) * KQ + q % KQ;
+    double t = 0.0;
+    for (int w2 = 0; w2 < nw; ++w2) t += red[(w2 * QS + s) * NA + i];
+    out[idx] = t;
+  }
+}
+
 // XCD-aware block remap: MI355X hands consecutive workgroups to its 8 XCDs round robin;
 // remapping gives every XCD one contiguous range of blocks so neighbouring rows (whose
 // gather targets are close) share that XCD's L2.

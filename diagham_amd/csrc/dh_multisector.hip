@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(256) sectors_dots_kernel(const double* __restr
                                                            const int64_t* __restrict__ work, int nActive,
                                                            int64_t rowsPerTile, double* __restrict__ partial) {
   constexpr int KK = K * K;
-  __shared__ double red[256 / 64][G * KK];
+  __shared__ double red[256 / 64][G * KK];  // dh_dots_tile's per-wavefront sums
   const int64_t tile = blockIdx.y;  // blocks of one tile are adjacent: its W rows stay in L2
   const int64_t* prefix = work + nActive;
   int lo = 0, hi = nActive;
@@ -466,41 +466,9 @@ __global__ void __launch_bounds__(256) sectors_dots_kernel(const double* __restr
   const int64_t r0 = (tile - prefix[lo]) * rowsPerTile;
   const int64_t r1 = min(dim, r0 + rowsPerTile);
   const int b0 = blockIdx.x * G;
-  double acc[G * KK];
-#pragma unroll
-  for (int i = 0; i < G * KK; ++i) acc[i] = 0.0;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    const int64_t row = off + r;
-    double w[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) w[q] = W[row * K + q];
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (b0 + g < nbrBlocks) {
-        const double* vb = V + (int64_t)(b0 + g) * totalDim * K + row * K;
-#pragma unroll
-        for (int p = 0; p < K; ++p) {
-          double v = vb[p];
-#pragma unroll
-          for (int q = 0; q < K; ++q) acc[g * KK + p * K + q] += v * w[q];
-        }
-      }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < G * KK; ++i) {
-    double v = acc[i];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wave][i] = v;
-  }
-  __syncthreads();
   const int64_t total = (int64_t)nbrBlocks * KK;
-  for (int i = threadIdx.x; i < G * KK; i += blockDim.x)
-    if (b0 + i / KK < nbrBlocks) {
-      double s = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][i];
-      partial[tile * total + (int64_t)b0 * KK + i] = s;
-    }
+  dh_dots_tile<K, G>(V, totalDim * K, nbrBlocks, b0, W, off + r0, off + r1, &red[0][0],
+                     partial + tile * total + (int64_t)b0 * KK);
 }
 
 __global__ void sectors_reduce_kernel(const double* __restrict__ partial, const int64_t* __restrict__ work,

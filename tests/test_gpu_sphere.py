@@ -15,7 +15,7 @@ def coulomb(lz_max):
 
 
 @pytest.fixture(scope="module")
-def cfg1(arch):
+def n8_2s21(arch):
     from diagham_amd import capi
     p = capi.Problem.sphere(arch, 8, 0, 21, coulomb(21))
     yield p
@@ -42,22 +42,22 @@ def test_doc_fixture_basis(arch, golden):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-def test_find_state_index(cfg1, mode):
-    states = cfg1.states()
-    idx = cfg1.find_state_index(states, mode)
-    assert np.array_equal(idx, np.arange(cfg1.dim))
+def test_find_state_index(n8_2s21, mode):
+    states = n8_2s21.states()
+    idx = n8_2s21.find_state_index(states, mode)
+    assert np.array_equal(idx, np.arange(n8_2s21.dim))
     # states outside the sector (wrong particle number / wrong Lz / beyond lzMax) give dim
     s0 = int(states[0])
     bad = np.array([s0 | (1 << 30), s0 & (s0 - 1), 0], dtype=np.uint64)
-    assert np.all(cfg1.find_state_index(bad, mode) == cfg1.dim)
+    assert np.all(n8_2s21.find_state_index(bad, mode) == n8_2s21.dim)
 
 
-def test_element_targets_bit_exact(cfg1):
+def test_element_targets_bit_exact(n8_2s21):
     ref_space = O.FermionSphere(8, 0, 21)
     fac = O.Factors(21, coulomb(21))
-    rows = list(range(0, cfg1.dim, 97)) + [cfg1.dim - 1]
+    rows = list(range(0, n8_2s21.dim, 97)) + [n8_2s21.dim - 1]
     for r in rows:
-        t_gpu, c_gpu = cfg1.row_elements(r)
+        t_gpu, c_gpu = n8_2s21.row_elements(r)
         t_ref, c_ref = O.fs_row_elements(ref_space, fac, r)
         o_gpu = np.lexsort((c_gpu, t_gpu))
         o_ref = np.lexsort((c_ref, t_ref))
@@ -65,24 +65,24 @@ def test_element_targets_bit_exact(cfg1):
         np.testing.assert_allclose(c_gpu[o_gpu], c_ref[o_ref], rtol=0, atol=1e-13)
 
 
-def test_element_count(cfg1, golden):
-    assert cfg1.count_elements() == golden["fermions_sphere_n8_2s21_lz0"]["full_matrix_elements"]
+def test_element_count(n8_2s21, golden):
+    assert n8_2s21.count_elements() == golden["fermions_sphere_n8_2s21_lz0"]["full_matrix_elements"]
 
 
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
 @pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2), (2, 1), (2, 8)])
-def test_hv_parity(cfg1, k, mode, lpr):
+def test_hv_parity(n8_2s21, k, mode, lpr):
     """H.X (k vectors, [row][k] layout) vs the oracle, 1e-12 relative norm per vector."""
     ref_space = O.FermionSphere(8, 0, 21)
     fac = O.Factors(21, coulomb(21))
-    X = np.stack([O.splitmix(100 + v, cfg1.dim) for v in range(k)], axis=1)
-    cfg1.set_launch(lanes_per_row=lpr, block_size=256, index_mode=mode)
-    Y = cfg1.hv(X.reshape(-1), k).reshape(cfg1.dim, k)
+    X = np.stack([O.splitmix(100 + v, n8_2s21.dim) for v in range(k)], axis=1)
+    n8_2s21.set_launch(lanes_per_row=lpr, block_size=256, index_mode=mode)
+    Y = n8_2s21.hv(X.reshape(-1), k).reshape(n8_2s21.dim, k)
     for v in range(k):
         y_ref = O.fs_hv(ref_space, fac, X[:, v])
         err = np.linalg.norm(Y[:, v] - y_ref) / np.linalg.norm(y_ref)
         assert err < 1e-12, (k, mode, lpr, v, err)
-    cfg1.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
+    n8_2s21.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
 
 
 @pytest.mark.parametrize("k", [1, 2, 4])
@@ -106,28 +106,28 @@ def test_hv_parity_beyond_32_orbitals(arch, k, mode, lpr):
     p.close()
 
 
-def test_hv_shift_add_and_range(cfg1, arch):
+def test_hv_shift_add_and_range(n8_2s21, arch):
     from diagham_amd import capi
     ref_space = O.FermionSphere(8, 0, 21)
     fac = O.Factors(21, coulomb(21))
-    x = O.splitmix(7, cfg1.dim)
-    y0 = O.splitmix(8, cfg1.dim)
+    x = O.splitmix(7, n8_2s21.dim)
+    y0 = O.splitmix(8, n8_2s21.dim)
     shift = -0.5 * 64 / (0.5 * 21)  # FQHESphereFermionsTwoBodyGeneric.cc:320
-    cfg1.set_shift(shift)
+    n8_2s21.set_shift(shift)
     dx = capi.DeviceBuffer.from_host(arch, x)
     dy = capi.DeviceBuffer.from_host(arch, y0)
-    cfg1.hv_device(dx.ptr, dy.ptr, 1, add=True)
+    n8_2s21.hv_device(dx.ptr, dy.ptr, 1, add=True)
     got = dy.download()
     ref = y0 + O.fs_hv(ref_space, fac, x, shift=shift)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
     # a row range writes only those rows
     dy.upload(y0)
-    cfg1.hv_device(dx.ptr, dy.ptr, 1, add=False, first=100, count=1000)
+    n8_2s21.hv_device(dx.ptr, dy.ptr, 1, add=False, first=100, count=1000)
     got = dy.download()
     full = O.fs_hv(ref_space, fac, x, shift=shift)
     assert np.array_equal(got[:100], y0[:100]) and np.array_equal(got[1100:], y0[1100:])
     assert np.linalg.norm(got[100:1100] - full[100:1100]) / np.linalg.norm(full[100:1100]) < 1e-12
-    cfg1.set_shift(0.0)
+    n8_2s21.set_shift(0.0)
 
 
 def test_asymmetric_factors_are_refused(arch):
@@ -154,10 +154,10 @@ def test_asymmetric_factors_are_refused(arch):
     capi.gpu().dh_basis_destroy(basis)
 
 
-def test_autotune_keeps_parity(cfg1):
+def test_autotune_keeps_parity(n8_2s21):
     """The autotuner sweeps the launch shapes of each block size, reports a consistent table and
     leaves a launch configured whose products still match the oracle."""
-    rows, best = cfg1.autotune(ks=(1, 4, 8), repeats=2)
+    rows, best = n8_2s21.autotune(ks=(1, 4, 8), repeats=2)
     assert [int(r["k"]) for r in rows] == [1, 4, 8]
     assert best in (1, 4, 8)
     for r in rows:
@@ -168,12 +168,12 @@ def test_autotune_keeps_parity(cfg1):
     assert rates[best] == max(rates.values())
     ref_space = O.FermionSphere(8, 0, 21)
     fac = O.Factors(21, coulomb(21))
-    X = np.stack([O.splitmix(300 + v, cfg1.dim) for v in range(best)], axis=1)
-    Y = cfg1.hv(X.reshape(-1), best).reshape(cfg1.dim, best)
+    X = np.stack([O.splitmix(300 + v, n8_2s21.dim) for v in range(best)], axis=1)
+    Y = n8_2s21.hv(X.reshape(-1), best).reshape(n8_2s21.dim, best)
     for v in range(best):
         y_ref = O.fs_hv(ref_space, fac, X[:, v])
         assert np.linalg.norm(Y[:, v] - y_ref) <= 1e-12 * np.linalg.norm(y_ref)
-    cfg1.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
+    n8_2s21.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
 
 
 def test_stored_elements_match_on_the_fly(arch, golden):
@@ -207,20 +207,20 @@ def test_stored_elements_match_on_the_fly(arch, golden):
     p.close()
 
 
-def test_hermitian_reference_form_agrees(cfg1):
+def test_hermitian_reference_form_agrees(n8_2s21):
     """The reference's Hermitian-halved scatter (HermitianLowLevelAddMultiply) and the gather agree."""
     ref_space = O.FermionSphere(8, 0, 21)
     fac_h = O.Factors(21, coulomb(21), hermitian=True)
-    x = O.splitmix(11, cfg1.dim)
+    x = O.splitmix(11, n8_2s21.dim)
     y_ref = O.fs_hv(ref_space, fac_h, x, hermitian=True)
-    y = cfg1.hv(x)
+    y = n8_2s21.hv(x)
     assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-12
 
 
 @pytest.mark.parametrize("block", [1, 4, 8])
-def test_lanczos_reference_eigenvalues(cfg1, golden, block):
+def test_lanczos_reference_eigenvalues(n8_2s21, golden, block):
     ref = np.array(golden["fermions_sphere_n8_2s21_lz0"]["eigenvalues"])
-    r = cfg1.lanczos(10, block_size=block)
+    r = n8_2s21.lanczos(10, block_size=block)
     assert r["converged"]
     np.testing.assert_allclose(r["eigenvalues"], ref, rtol=0, atol=1e-10)
 

@@ -299,7 +299,7 @@ def test_lanczos_matches_dense_small():
 
 
 @pytest.mark.slow
-def test_lanczos_reference_eigenvalues_cfg1(golden):
+def test_lanczos_reference_eigenvalues_n8_2s21(golden):
     ref = np.array(golden["fermions_sphere_n8_2s21_lz0"]["eigenvalues"])
     s = O.FermionSphere(8, 0, 21)
     f = O.Factors(21, coulomb(21))

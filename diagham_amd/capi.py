@@ -37,6 +37,7 @@ GPU_SIGNATURES = {
     "dh_set_stream": (c_int, [c_void_p, c_void_p]),
     "dh_synchronize": (c_int, [c_void_p]),
     "dh_device_info": (c_int, [c_void_p, P_I, c_char_p]),
+    "dh_memory_info": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "dh_malloc": (c_int, [c_void_p, PP, c_size_t]),
     "dh_free": (c_int, [c_void_p, c_void_p]),
     "dh_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

@@ -84,6 +84,13 @@ extern "C" int dh_device_info(dh_context* c, int* nbrCU, char* name) {
   return DH_OK;
 }
 
+extern "C" int dh_memory_info(dh_context* c, size_t* freeBytes, size_t* totalBytes) {
+  if (!c || !freeBytes || !totalBytes) return DH_E_ARG;
+  hipSetDevice(c->device);
+  DH_HIP(c, hipMemGetInfo(freeBytes, totalBytes));
+  return DH_OK;
+}
+
 extern "C" int dh_malloc(dh_context* c, void** p, size_t bytes) {
   if (!c || !p) return DH_E_ARG;
   hipSetDevice(c->device);

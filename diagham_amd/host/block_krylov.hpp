@@ -147,10 +147,11 @@ class SegmentedBlockLanczos {
   int BlockCapacity() const { return Capacity; }
 
   // Krylov storage that grows on demand instead of holding maxBlocks + 2 blocks from the start:
-  // the storage passed to the constructor holds `capacity` + 2 blocks, and grow(blocks, used, old)
-  // must return storage for blocks + 2 blocks whose first `used` blocks are those of `old` (and
-  // release `old`).  Capacity grows by half each time.
-  using GrowFn = std::function<double*(int blocks, int used, double* old)>;
+  // the storage passed to the constructor holds `capacity` + 2 blocks, and grow(blocks, need, used,
+  // old) must return storage for blocks + 2 blocks whose first `used` blocks are those of `old`
+  // (and release `old`); it may lower `blocks` to what fits, but not below `need`.  Capacity grows
+  // by half each time.
+  using GrowFn = std::function<double*(int& blocks, int need, int used, double* old)>;
   void SetGrowth(int capacity, GrowFn grow) {
     Capacity = std::max(1, std::min(capacity, MaxBlocks));
     Grow = std::move(grow);
@@ -159,8 +160,9 @@ class SegmentedBlockLanczos {
   void Reserve(int blocks) {
     if (blocks <= Capacity) return;
     if (blocks > MaxBlocks || !Grow) throw std::runtime_error("Krylov storage exhausted");
-    const int cap = std::min(MaxBlocks, std::max(blocks, Capacity + Capacity / 2));
-    Storage = Grow(cap, std::min(StepCount + 1, Capacity + 1), Storage);
+    int cap = std::min(MaxBlocks, std::max(blocks, Capacity + Capacity / 2));
+    Storage = Grow(cap, blocks, std::min(StepCount + 1, Capacity + 1), Storage);
+    if (cap < blocks || cap > MaxBlocks) throw std::logic_error("Krylov storage grown outside its bounds");
     Capacity = cap;
   }
 

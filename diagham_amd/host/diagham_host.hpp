@@ -331,8 +331,9 @@ struct SectorSolveResources {
 // reorthogonalisation at about half speed on MI355X; sized to what the run uses, it does not.
 inline int InitialKrylovBlocks(int k, int maxBlocks) { return std::min(maxBlocks, std::max(32, 256 / k)); }
 // device storage for blocks + 2 blocks of blockDoubles that holds the first `used` blocks of `old`;
-// `old` is released
-double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int blocks, int64_t blockDoubles);
+// `old` is released.  While old is still held, `blocks` is lowered (not below `need`) to what the
+// device's free memory holds beside it, keeping 1 GiB spare.
+double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int& blocks, int need, int64_t blockDoubles);
 
 // Krylov blocks each sector may use (maxIter / k + 1, and no more than its space needs); returns
 // the largest

@@ -251,17 +251,23 @@ FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgor
     Ops.reset(new HamiltonianOps(Arch, Hamiltonian, Dim, BlockSize));
     Engine.reset(new SegmentedBlockLanczos<double>(Ops.get(), std::vector<int64_t>{Dim}, BlockSize, MaxBlocks,
                                                    d_basis, Dim * BlockSize));
-    Engine->SetGrowth(capacity, [this](int blocks, int used, double* old) {
+    Engine->SetGrowth(capacity, [this](int& blocks, int need, int used, double* old) {
       d_basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
-      d_basis = GrowKrylovStorage(Arch, old, used, blocks, Dim * BlockSize);
+      d_basis = GrowKrylovStorage(Arch, old, used, blocks, need, Dim * BlockSize);
       return d_basis;
     });
   }
 }
 
-double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int blocks, int64_t blockDoubles) {
+double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int& blocks, int need, int64_t blockDoubles) {
   double* p = nullptr;
   dh_context* c = arch->Context();
+  size_t freeBytes = 0, totalBytes = 0;
+  if (dh_memory_info(c, &freeBytes, &totalBytes) == DH_OK) {
+    const size_t spare = (size_t)1 << 30, blockBytes = sizeof(double) * (size_t)blockDoubles;
+    const int64_t fit = freeBytes > spare ? (int64_t)((freeBytes - spare) / blockBytes) - 2 : 0;
+    if (blocks > fit) blocks = (int)std::max<int64_t>(need, fit);
+  }
   try {
     p = arch->AllocateMemory((int64_t)(blocks + 2) * blockDoubles);
     if (used > 0) arch->Check(dh_copy(c, p, old, (int64_t)used * blockDoubles), "grow Krylov storage");

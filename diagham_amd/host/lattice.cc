@@ -256,9 +256,9 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   guard.basis = Arch->AllocateMemory((int64_t)(capacity + 2) * blockDoubles);
   SectorSetOps ops(Arch, S, k, 2 * k, total);
   SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, guard.basis, blockDoubles);
-  lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
+  lanczos.SetGrowth(capacity, [&](int& blocks, int need, int used, double* old) {
     guard.basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
-    return guard.basis = GrowKrylovStorage(Arch, old, used, blocks, blockDoubles);
+    return guard.basis = GrowKrylovStorage(Arch, old, used, blocks, need, blockDoubles);
   });
   std::vector<SectorResult*> res;
   for (int s : sectors) res.push_back(&out[s]);

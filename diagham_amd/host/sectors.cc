@@ -166,9 +166,9 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   res.basis = Arch->AllocateMemory((int64_t)(capacity + 2) * total * k);
   SectorSetOps ops(Arch, S, k, k, total);
   SegmentedBlockLanczos<double> lanczos(&ops, dims, k, maxBlocks, res.basis, total * k);
-  lanczos.SetGrowth(capacity, [&](int blocks, int used, double* old) {
+  lanczos.SetGrowth(capacity, [&](int& blocks, int need, int used, double* old) {
     res.basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
-    return res.basis = GrowKrylovStorage(Arch, old, used, blocks, total * k);
+    return res.basis = GrowKrylovStorage(Arch, old, used, blocks, need, total * k);
   });
   std::vector<SectorResult*> results;
   for (int s : sectors) results.push_back(&out[s]);

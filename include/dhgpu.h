@@ -55,6 +55,9 @@ int dh_set_stream(dh_context* ctx, void* hip_stream);
 int dh_synchronize(dh_context* ctx);
 /* number of compute units and gfx name of the context's device (name >= 64 bytes) */
 int dh_device_info(dh_context* ctx, int* nbr_cu, char* name);
+/* free and total device memory of the context's device (hipMemGetInfo); no counterpart in the
+ * reference: the host sizes the Krylov storage it grows on demand by it */
+int dh_memory_info(dh_context* ctx, size_t* free_bytes, size_t* total_bytes);
 
 /* Device vector storage; AbstractArchitecture::GetNewRealVector / AllocateMemory hooks
  * (src/Architecture/AbstractArchitecture.h:150-246). */

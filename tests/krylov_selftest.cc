@@ -124,7 +124,8 @@ void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed, bo
   SegmentedBlockLanczos<T> lanczos(&ops, dims, ops.k, maxBlocks, storage.data(), bd);
   int growths = 0;
   if (grow)
-    lanczos.SetGrowth(first, [&](int blocks, int used, double* old) {
+    lanczos.SetGrowth(first, [&](int& blocks, int need, int used, double* old) {
+      if (blocks > need && growths % 2) blocks = need;  // every other growth takes only what is needed
       if (old != storage.data()) throw std::logic_error("growth from storage the engine does not own");
       std::vector<double> next((size_t)(blocks + 2) * bd, -7.0);  // poison: only the used blocks are kept
       std::copy(old, old + (size_t)used * bd, next.begin());

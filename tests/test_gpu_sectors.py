@@ -202,3 +202,20 @@ def test_batched_reference_sector(arch, golden):
     ev, prods, conv, secs = solve(arch, 8, 21, [0, 2, 4, 6], 10, 8, True)
     assert conv.all()
     np.testing.assert_allclose(ev[0], ref, rtol=0, atol=1e-10)
+
+
+def test_batched_solve_grows_krylov_storage(arch, golden):
+    """The batched run for the 40 lowest eigenvalues of the Lz = 0 and 1 sectors of N=10 2S=27 takes
+    more one-vector steps than the initial Krylov storage holds (256 vectors); the storage grows
+    within the device's free memory and the Lz = 0 sector still gives the reference eigenvalues."""
+    import ctypes
+    from diagham_amd import capi
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    arch.check(capi.gpu().dh_memory_info(arch.ctx, ctypes.byref(free), ctypes.byref(total)), "memory info")
+    assert 0 < free.value <= total.value
+    pp = capi.coulomb_pseudopotentials(27, True)
+    r = capi.sectors_solve(arch, 10, 27, [0, 2], pp, 40, 1, batched=True)
+    assert r["converged"].all()
+    assert r["products"].max() > 256, r["products"]
+    ref = np.array(golden["fermions_sphere_n10_2s27_lz0"]["eigenvalues"])
+    np.testing.assert_allclose(r["eigenvalues"][0][:ref.size], ref, rtol=0, atol=1e-10)

@@ -66,7 +66,7 @@ class MomentumSet:
         self.capi.gpu().dh_sector_set_destroy(self.h)
 
 
-@pytest.mark.parametrize("lpr", [1, 4])
+@pytest.mark.parametrize("lpr", [1, 2, 4])
 def test_momentum_basis_and_complex_products(arch, lpr):
     n, nx, ny = 4, 4, 3
     sectors = [(0, 0), (1, 2), (3, 1)]

@@ -4,6 +4,7 @@ The product path is native: importing this module loads ``libdhgpu.so`` (HIP ker
 gfx950) and ``libdiagham_host.so`` (C++ host layer) from this directory and raises if they
 are missing.  There is no Python or CPU fallback.
 """
+import atexit
 import ctypes
 import os
 import weakref
@@ -240,8 +241,27 @@ def fermion_sphere_dimension(n, total_lz, lz_max):
 
 
 # ---------------------------------------------------------------------------- GPU objects
+_live_archs = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_archs():
+    """Close every open Arch (and what it tracks) while everything is still reachable, before the
+    interpreter's final collections run finalizers in no defined order."""
+    for a in list(_live_archs):
+        try:
+            a.close()
+        except Exception:
+            pass
+
+
 class Arch:
-    """GPUArchitecture: one device context."""
+    """GPUArchitecture: one device context.
+
+    The native layers count references: every problem, sector set and buffer keeps its context
+    alive until it is released, so objects may be closed or collected in any order, before or
+    after the Arch.  close() releases the tracked objects first and drops the handles it held;
+    calls on a closed Arch then pass null handles, which the C ABI refuses."""
 
     def __init__(self, device=0):
         self._dependents = weakref.WeakSet()
@@ -251,6 +271,7 @@ class Arch:
             raise DHError("dhh_arch_create failed (%d): %s" % (rc, host().dhh_last_error().decode()))
         self.h = h
         self.ctx = host().dhh_arch_context(h)
+        _live_archs.add(self)
 
     def track(self, obj):
         """Register an object whose native handle lives in this context (it needs `close()`)."""
@@ -258,10 +279,7 @@ class Arch:
         return obj
 
     def close(self):
-        """Release every live buffer, problem and sector set of this context, then the context.
-        At interpreter exit the finalizers run in no defined order (the context's may come first
-        even while other objects still reference it), so the context closes its dependents
-        itself rather than relying on reference counts."""
+        """Release every live buffer, problem and sector set of this context, then the context."""
         if getattr(self, "h", None):
             for obj in list(getattr(self, "_dependents", ())):
                 try:
@@ -270,6 +288,7 @@ class Arch:
                     pass
             host().dhh_arch_destroy(self.h)
             self.h = None
+            self.ctx = None
 
     def __del__(self):
         self.close()
@@ -298,6 +317,8 @@ class DeviceBuffer:
         p = c_void_p()
         arch.check(gpu().dh_malloc(arch.ctx, ctypes.byref(p), max(self.n, 1) * self.dtype.itemsize), "dh_malloc")
         self.ptr = p.value
+        # the context this buffer was allocated in: the buffer keeps it alive until dh_free
+        self._ctx = arch.ctx
         arch.track(self)
 
     @classmethod
@@ -322,7 +343,7 @@ class DeviceBuffer:
 
     def free(self):
         if getattr(self, "ptr", None):
-            gpu().dh_free(self.arch.ctx, self.ptr)
+            gpu().dh_free(self._ctx, self.ptr)
             self.ptr = None
 
     close = free
@@ -403,6 +424,8 @@ class Problem:
         if getattr(self, "h", None):
             host().dhh_problem_destroy(self.h)
             self.h = None
+            self.ham = None
+            self.basis = None
 
     def __del__(self):
         self.close()
@@ -549,7 +572,7 @@ class SphereSectorSet:
         self.arch.check(gpu().dh_sectors_hv(self.h, d_x, d_y, k, int(add), am), "dh_sectors_hv")
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             gpu().dh_sector_set_destroy(self.h)
             self.h = None
 

@@ -49,8 +49,8 @@ extern "C" int dh_basis_find_state_index(dh_basis* b, const uint64_t* states, in
   uint64_t* d_in;
   int64_t* d_out;
   int rc;
-  if ((rc = dh_malloc(c, (void**)&d_in, sizeof(uint64_t) * n))) return rc;
-  if ((rc = dh_malloc(c, (void**)&d_out, sizeof(int64_t) * n))) {
+  if ((rc = dh_alloc(c, (void**)&d_in, sizeof(uint64_t) * n))) return rc;
+  if ((rc = dh_alloc(c, (void**)&d_out, sizeof(int64_t) * n))) {
     hipFree(d_in);
     return rc;
   }
@@ -212,8 +212,8 @@ extern "C" int dh_hamiltonian_count_elements(dh_hamiltonian* h, int64_t* nnz) {
   int64_t* d_counts;
   unsigned long long* d_total;
   int rc;
-  if ((rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::min<int64_t>(chunk, b->dim)))) return rc;
-  if ((rc = dh_malloc(c, (void**)&d_total, sizeof(unsigned long long)))) {
+  if ((rc = dh_alloc(c, (void**)&d_counts, sizeof(int64_t) * std::min<int64_t>(chunk, b->dim)))) return rc;
+  if ((rc = dh_alloc(c, (void**)&d_total, sizeof(unsigned long long)))) {
     hipFree(d_counts);
     return rc;
   }
@@ -222,7 +222,7 @@ extern "C" int dh_hamiltonian_count_elements(dh_hamiltonian* h, int64_t* nnz) {
   if (h->kind == DH_HAM_SPIN) {
     double hj[32];
     for (int i = 0; i < 32; ++i) hj[i] = 0.5 * h->J[i];
-    dh_malloc(c, (void**)&d_halfJ, sizeof(hj));
+    dh_alloc(c, (void**)&d_halfJ, sizeof(hj));
     hipMemcpyAsync(d_halfJ, hj, sizeof(hj), hipMemcpyHostToDevice, c->stream);
   }
   for (int64_t r0 = 0; r0 < b->dim; r0 += chunk) {
@@ -260,9 +260,9 @@ extern "C" int dh_sector_set_count_elements(dh_sector_set* S, int64_t* counts) {
   int64_t* d_counts = nullptr;
   unsigned long long* d_total = nullptr;
   int rc;
-  if ((rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, maxDim)))))
+  if ((rc = dh_alloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, maxDim)))))
     return rc;
-  if ((rc = dh_malloc(c, (void**)&d_total, sizeof(unsigned long long) * S->nbrSectors))) {
+  if ((rc = dh_alloc(c, (void**)&d_total, sizeof(unsigned long long) * S->nbrSectors))) {
     hipFree(d_counts);
     return rc;
   }
@@ -303,9 +303,9 @@ extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64
   int64_t *d_t, *d_n;
   double* d_c;
   double* d_halfJ = nullptr;
-  dh_malloc(c, (void**)&d_t, sizeof(int64_t) * cap);
-  dh_malloc(c, (void**)&d_c, sizeof(double) * cap);
-  dh_malloc(c, (void**)&d_n, sizeof(int64_t));
+  dh_alloc(c, (void**)&d_t, sizeof(int64_t) * cap);
+  dh_alloc(c, (void**)&d_c, sizeof(double) * cap);
+  dh_alloc(c, (void**)&d_n, sizeof(int64_t));
   if (h->kind == DH_HAM_SPHERE) {
     hipLaunchKernelGGL(fs_elements_kernel, dim3(1), dim3(64), 0, c->stream, b->d_states, b->dim, b->nbrOrbitals,
                        h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi, b->rankBits, row, row + 1, d_n, d_t,
@@ -313,7 +313,7 @@ extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64
   } else {
     double hj[32];
     for (int i = 0; i < 32; ++i) hj[i] = 0.5 * h->J[i];
-    dh_malloc(c, (void**)&d_halfJ, sizeof(hj));
+    dh_alloc(c, (void**)&d_halfJ, sizeof(hj));
     hipMemcpyAsync(d_halfJ, hj, sizeof(hj), hipMemcpyHostToDevice, c->stream);
     uint64_t w = 0;
     hipMemcpyAsync(&w, b->d_words + row, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
@@ -584,7 +584,7 @@ extern "C" int dh_hamiltonian_precompute(dh_hamiltonian* h, int64_t maxBytes, in
   const int64_t chunk = 1 << 24;
   int64_t* d_counts = nullptr;
   int rc;
-  if ((rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, dim))))) return rc;
+  if ((rc = dh_alloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, dim))))) return rc;
   for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
     const int64_t r1 = std::min(dim, r0 + chunk);
     hipLaunchKernelGGL(fs_elements_kernel, dim3((unsigned)((r1 - r0 + 255) / 256)), dim3(256), 0, c->stream,
@@ -604,9 +604,9 @@ extern "C" int dh_hamiltonian_precompute(dh_hamiltonian* h, int64_t maxBytes, in
   const int64_t need = (int64_t)sizeof(int64_t) * (dim + 1) + nnz * (int64_t)(sizeof(int32_t) + sizeof(double));
   if (bytes) *bytes = need;
   if (need > maxBytes) return dh_fail(c, DH_E_NOMEM, "the stored elements need more than max_bytes");
-  if ((rc = dh_malloc(c, (void**)&h->d_csrRow, sizeof(int64_t) * (dim + 1))) ||
-      (rc = dh_malloc(c, (void**)&h->d_csrCol, sizeof(int32_t) * std::max<int64_t>(1, nnz))) ||
-      (rc = dh_malloc(c, (void**)&h->d_csrVal, sizeof(double) * std::max<int64_t>(1, nnz)))) {
+  if ((rc = dh_alloc(c, (void**)&h->d_csrRow, sizeof(int64_t) * (dim + 1))) ||
+      (rc = dh_alloc(c, (void**)&h->d_csrCol, sizeof(int32_t) * std::max<int64_t>(1, nnz))) ||
+      (rc = dh_alloc(c, (void**)&h->d_csrVal, sizeof(double) * std::max<int64_t>(1, nnz)))) {
     dh_csr_free(h);
     return rc;
   }

@@ -143,7 +143,7 @@ __global__ void spin_lut_kernel(const uint32_t* __restrict__ words, int64_t dim,
 
 // ----------------------------------------------------------------------------- basis creation
 static int upload(dh_context* c, void** d, const void* h, size_t bytes) {
-  int rc = dh_malloc(c, d, bytes);
+  int rc = dh_alloc(c, d, bytes);
   if (rc) return rc;
   DH_HIP(c, hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, c->stream));
   DH_HIP(c, hipStreamSynchronize(c->stream));
@@ -187,6 +187,7 @@ static void fermion_rank_tables(const SubsetCountTable& T, int nbrOrbitals, int 
 extern "C" int dh_fermion_sphere_create(dh_context* c, dh_basis** out, int N, int totalLz, int lzMax) {
   if (!c || !out) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (N <= 0 || lzMax < 0 || N > lzMax + 1) return dh_fail(c, DH_E_ARG, "need 0 < nbr_fermions <= lz_max + 1");
   if (lzMax > 62) return dh_fail(c, DH_E_RANGE, "lz_max must be <= 62 (64-bit states)");
   int twiceSum = totalLz + N * lzMax;
@@ -213,7 +214,7 @@ extern "C" int dh_fermion_sphere_create(dh_context* c, dh_basis** out, int N, in
     delete b;
     return rc;
   }
-  if ((rc = dh_malloc(c, (void**)&b->d_states, sizeof(uint64_t) * dim))) {
+  if ((rc = dh_alloc(c, (void**)&b->d_states, sizeof(uint64_t) * dim))) {
     hipFree(d_T);
     delete b;
     return rc;
@@ -223,7 +224,7 @@ extern "C" int dh_fermion_sphere_create(dh_context* c, dh_basis** out, int N, in
   // LUT over the top bits (binary-search mode)
   b->lutBits = std::min(nbrOrbitals, 14);
   b->lutShift = nbrOrbitals - b->lutBits;
-  if ((rc = dh_malloc(c, (void**)&b->d_lut, sizeof(int) * (((size_t)1 << b->lutBits) + 1)))) {
+  if ((rc = dh_alloc(c, (void**)&b->d_lut, sizeof(int) * (((size_t)1 << b->lutBits) + 1)))) {
     hipFree(d_T);
     dh_basis_destroy(b);
     return rc;
@@ -255,6 +256,7 @@ extern "C" int dh_fermion_sphere_create(dh_context* c, dh_basis** out, int N, in
 extern "C" int dh_spin_chain_create(dh_context* c, dh_basis** out, int L, int sz) {
   if (!c || !out) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (L < 2 || L > 32) return dh_fail(c, DH_E_RANGE, "chain_length must be in [2, 32]");
   if (((L + sz) & 1) || sz > L || sz < -L) return dh_fail(c, DH_E_ARG, "sz inconsistent with chain_length");
   hipSetDevice(c->device);
@@ -273,7 +275,7 @@ extern "C" int dh_spin_chain_create(dh_context* c, dh_basis** out, int L, int sz
     for (int k = 0; k <= 32; ++k) binom[n * 33 + k] = (uint32_t)std::min<uint64_t>(dh_binom_host(n, k), 0xffffffffu);
   int rc;
   if ((rc = upload(c, (void**)&b->d_binom, binom.data(), sizeof(uint32_t) * binom.size())) ||
-      (rc = dh_malloc(c, (void**)&b->d_words, sizeof(uint32_t) * dim))) {
+      (rc = dh_alloc(c, (void**)&b->d_words, sizeof(uint32_t) * dim))) {
     dh_basis_destroy(b);
     return rc;
   }
@@ -281,7 +283,7 @@ extern "C" int dh_spin_chain_create(dh_context* c, dh_basis** out, int L, int sz
                      dim, L, nbrUp, b->d_binom);
   b->lutBits = std::min(L, 14);
   b->lutShift = L - b->lutBits;
-  if ((rc = dh_malloc(c, (void**)&b->d_lut, sizeof(int) * (((size_t)1 << b->lutBits) + 1)))) {
+  if ((rc = dh_alloc(c, (void**)&b->d_lut, sizeof(int) * (((size_t)1 << b->lutBits) + 1)))) {
     dh_basis_destroy(b);
     return rc;
   }
@@ -325,7 +327,7 @@ extern "C" int dh_basis_get_states(dh_basis* b, uint64_t* states) {
     DH_HIP(c, hipMemcpyAsync(states, b->d_states, sizeof(uint64_t) * b->dim, hipMemcpyDeviceToHost, c->stream));
   } else {
     uint64_t* tmp;
-    int rc = dh_malloc(c, (void**)&tmp, sizeof(uint64_t) * b->dim);
+    int rc = dh_alloc(c, (void**)&tmp, sizeof(uint64_t) * b->dim);
     if (rc) return rc;
     hipLaunchKernelGGL(widen_words_kernel, dim3((unsigned)((b->dim + 255) / 256)), dim3(256), 0, c->stream,
                        b->d_words, tmp, b->dim);

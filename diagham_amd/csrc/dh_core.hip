@@ -43,8 +43,11 @@ extern "C" int dh_create(dh_context** out, int device) {
   return DH_OK;
 }
 
-extern "C" int dh_destroy(dh_context* c) {
-  if (!c) return DH_E_ARG;
+void dh_ctx_retain(dh_context* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+// the last reference frees the context: by then no object of it is left to use its stream
+void dh_ctx_release(dh_context* c) {
+  if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->d_scratch) hipFree(c->d_scratch);
@@ -55,6 +58,15 @@ extern "C" int dh_destroy(dh_context* c) {
   }
   if (c->ownStream && c->stream) hipStreamDestroy(c->stream);
   delete c;
+}
+
+extern "C" int dh_destroy(dh_context* c) {
+  if (!c) return DH_E_ARG;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context already destroyed");
+  c->closed = true;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  dh_ctx_release(c);
   return DH_OK;
 }
 
@@ -91,16 +103,38 @@ extern "C" int dh_memory_info(dh_context* c, size_t* freeBytes, size_t* totalByt
   return DH_OK;
 }
 
-extern "C" int dh_malloc(dh_context* c, void** p, size_t bytes) {
-  if (!c || !p) return DH_E_ARG;
+int dh_alloc(dh_context* c, void** p, size_t bytes) {
   hipSetDevice(c->device);
   if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess) return dh_fail(c, DH_E_NOMEM, "hipMalloc failed");
   return DH_OK;
 }
 
+extern "C" int dh_malloc(dh_context* c, void** p, size_t bytes) {
+  if (!c || !p) return DH_E_ARG;
+  int rc = dh_alloc(c, p, bytes);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> g(c->allocMutex);
+    c->allocs.insert(*p);
+  }
+  dh_ctx_retain(c);
+  return DH_OK;
+}
+
+// a buffer keeps its context alive until it is freed, so this may follow dh_destroy
 extern "C" int dh_free(dh_context* c, void* p) {
   if (!c) return DH_E_ARG;
-  if (p) DH_HIP(c, hipFree(p));
+  if (!p) return DH_OK;
+  {
+    std::lock_guard<std::mutex> g(c->allocMutex);
+    auto it = c->allocs.find(p);
+    if (it == c->allocs.end()) return dh_fail(c, DH_E_ARG, "dh_free: not a live dh_malloc buffer of this context");
+    c->allocs.erase(it);
+  }
+  hipSetDevice(c->device);
+  hipError_t e = hipFree(p);
+  dh_ctx_release(c);
+  if (e != hipSuccess) return DH_E_HIP;
   return DH_OK;
 }
 

@@ -547,6 +547,7 @@ extern "C" int dh_sphere_hamiltonian_create(dh_context* c, dh_hamiltonian** out,
                                             const int* pairOffset, int nbrFactors) {
   if (!c || !out || !b || !factors || !pairOffset) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (b->kind != DH_BASIS_FERMION_SPHERE) return dh_fail(c, DH_E_ARG, "basis is not a fermion-on-sphere basis");
   const int nOrb = b->nbrOrbitals;
   const int nPairs = nOrb * (nOrb - 1) / 2;
@@ -573,10 +574,10 @@ extern "C" int dh_sphere_hamiltonian_create(dh_context* c, dh_hamiltonian** out,
   h->nbrPairs = nPairs;
   h->nbrPackedFactors = (int)packed.size();
   h->nbrPairSums = (int)sumBase.size();
-  if ((rc = dh_malloc(c, (void**)&h->d_factors, sizeof(double) * std::max(nbrFactors, 1))) ||
-      (rc = dh_malloc(c, (void**)&h->d_pairOffset, sizeof(int) * (nPairs + 1))) ||
-      (rc = dh_malloc(c, (void**)&h->d_packedFactors, sizeof(double) * std::max<size_t>(packed.size(), 1))) ||
-      (rc = dh_malloc(c, (void**)&h->d_sumBase, sizeof(int) * sumBase.size()))) {
+  if ((rc = dh_alloc(c, (void**)&h->d_factors, sizeof(double) * std::max(nbrFactors, 1))) ||
+      (rc = dh_alloc(c, (void**)&h->d_pairOffset, sizeof(int) * (nPairs + 1))) ||
+      (rc = dh_alloc(c, (void**)&h->d_packedFactors, sizeof(double) * std::max<size_t>(packed.size(), 1))) ||
+      (rc = dh_alloc(c, (void**)&h->d_sumBase, sizeof(int) * sumBase.size()))) {
     dh_hamiltonian_destroy(h);
     return rc;
   }
@@ -604,7 +605,7 @@ extern "C" int dh_sphere_hamiltonian_set_channels(dh_hamiltonian* h, const doubl
     hipFree(h->d_cg);
     h->d_cg = nullptr;
   }
-  int rc = dh_malloc(c, (void**)&h->d_cg, sizeof(double) * h->nbrPairs * nbrChannels);
+  int rc = dh_alloc(c, (void**)&h->d_cg, sizeof(double) * h->nbrPairs * nbrChannels);
   if (rc) return rc;
   DH_HIP(c, hipMemcpyAsync(h->d_cg, cg, sizeof(double) * h->nbrPairs * nbrChannels, hipMemcpyHostToDevice, c->stream));
   DH_HIP(c, hipStreamSynchronize(c->stream));
@@ -617,6 +618,7 @@ extern "C" int dh_spin_chain_hamiltonian_create(dh_context* c, dh_hamiltonian** 
                                                 const double* jz, int periodic) {
   if (!c || !out || !b || !j || !jz) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (b->kind != DH_BASIS_SPIN_CHAIN) return dh_fail(c, DH_E_ARG, "basis is not a spin chain");
   dh_hamiltonian* h = new dh_hamiltonian();
   h->ctx = c;

@@ -4,14 +4,25 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/dhgpu.h"
 
+// A context is reference counted.  dh_create hands its caller one reference, and every basis,
+// Hamiltonian, sector set and dh_malloc buffer of the context holds one more, so the context (its
+// stream, scratch and staging memory) stays valid until the last of them is released, whatever
+// order their owners release them in.  dh_destroy gives up the caller's reference.
 struct dh_context {
+  std::atomic<int> refs{1};
+  bool closed = false;  // dh_destroy was called
+  std::mutex allocMutex;
+  std::unordered_set<void*> allocs;  // live dh_malloc buffers
   int device = 0;
   hipStream_t stream = nullptr;
   bool ownStream = false;
@@ -35,10 +46,37 @@ struct dh_context {
   int stageNext = 0;
 };
 
+void dh_ctx_retain(dh_context* c);
+void dh_ctx_release(dh_context* c);
+
+// Counted reference to a context, held by every object that uses one.  Assigning a context takes
+// a reference and destroying the holder releases it, so every path that deletes the object
+// (its destroy call or a failed creation) releases the context too.
+class dh_ctx_ref {
+ public:
+  dh_ctx_ref() = default;
+  dh_ctx_ref(const dh_ctx_ref&) = delete;
+  dh_ctx_ref& operator=(const dh_ctx_ref&) = delete;
+  ~dh_ctx_ref() {
+    if (p) dh_ctx_release(p);
+  }
+  dh_ctx_ref& operator=(dh_context* c) {
+    if (c) dh_ctx_retain(c);
+    if (p) dh_ctx_release(p);
+    p = c;
+    return *this;
+  }
+  operator dh_context*() const { return p; }
+  dh_context* operator->() const { return p; }
+
+ private:
+  dh_context* p = nullptr;
+};
+
 enum dh_basis_kind { DH_BASIS_FERMION_SPHERE = 1, DH_BASIS_SPIN_CHAIN = 2 };
 
 struct dh_basis {
-  dh_context* ctx = nullptr;
+  dh_ctx_ref ctx;
   int kind = 0;
   int64_t dim = 0;
   // fermions on the sphere
@@ -63,7 +101,7 @@ struct dh_basis {
 enum dh_ham_kind { DH_HAM_SPHERE = 1, DH_HAM_SPIN = 2, DH_HAM_SPHERE_NBODY = 3 };
 
 struct dh_hamiltonian {
-  dh_context* ctx = nullptr;
+  dh_ctx_ref ctx;
   dh_basis* basis = nullptr;
   int kind = 0;
   double shift = 0.0;
@@ -106,7 +144,7 @@ enum dh_sector_kind { DH_SECTORS_SPHERE = 1, DH_SECTORS_MOMENTUM = 2, DH_SECTORS
 // rows [offsets[s], offsets[s+1])).  Sphere sets are real (Lz sectors); momentum sets are
 // complex ((kx, ky) sectors, vectors stored as interleaved (re, im) pairs).
 struct dh_sector_set {
-  dh_context* ctx = nullptr;
+  dh_ctx_ref ctx;
   int kind = DH_SECTORS_SPHERE;
   int nbrFermions = 0, lzMax = 0, nbrOrbitals = 0, nbrSectors = 0;
   std::vector<int> totalLz, orbitalSum;  // sphere: 2 Lz and orbital-index sum per sector
@@ -179,6 +217,8 @@ int dh_fail(dh_context* ctx, int code, const std::string& msg);
       return dh_fail((ctx), DH_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// device memory owned by the library's own objects (not counted as dh_malloc buffers)
+int dh_alloc(dh_context* ctx, void** d_ptr, size_t bytes);
 // scratch management (core)
 int dh_scratch(dh_context* ctx, size_t bytes, double** d_ptr);
 int dh_pinned(dh_context* ctx, size_t bytes, double** h_ptr);

@@ -72,6 +72,7 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
                                              const int* kx, const int* ky, int nbrSectors) {
   if (!c || !out || !kx || !ky || nbrSectors <= 0) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   const int nOrb = nx * ny;
   if (nx <= 0 || ny <= 0 || N <= 0 || N > nOrb) return dh_fail(c, DH_E_ARG, "need 0 < nbr_fermions <= Nx * Ny");
   if (nOrb > 62) return dh_fail(c, DH_E_RANGE, "Nx * Ny must be <= 62 (64-bit states)");
@@ -143,11 +144,11 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
     }
   int rc;
   int64_t* d_T = nullptr;
-  if ((rc = dh_malloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
-      (rc = dh_malloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
+  if ((rc = dh_alloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
+      (rc = dh_alloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
     if (d_T) hipFree(d_T);
     dh_sector_set_destroy(S);
     return rc;
@@ -210,11 +211,11 @@ extern "C" int dh_sector_set_lattice_factors(dh_sector_set* S, int nbrSums, cons
   S->d_sumFactorOff = nullptr;
   S->d_cfactors = nullptr;
   int rc;
-  if ((rc = dh_malloc(c, (void**)&S->d_pairInfo, sizeof(int) * nPairs)) ||
-      (rc = dh_malloc(c, (void**)&S->d_sumPairOff, sizeof(int) * (nbrSums + 1))) ||
-      (rc = dh_malloc(c, (void**)&S->d_sumPairs, sizeof(int) * packed.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_sumFactorOff, sizeof(int64_t) * (nbrSums + 1))) ||
-      (rc = dh_malloc(c, (void**)&S->d_cfactors, sizeof(double) * 2 * facOff.back())))
+  if ((rc = dh_alloc(c, (void**)&S->d_pairInfo, sizeof(int) * nPairs)) ||
+      (rc = dh_alloc(c, (void**)&S->d_sumPairOff, sizeof(int) * (nbrSums + 1))) ||
+      (rc = dh_alloc(c, (void**)&S->d_sumPairs, sizeof(int) * packed.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_sumFactorOff, sizeof(int64_t) * (nbrSums + 1))) ||
+      (rc = dh_alloc(c, (void**)&S->d_cfactors, sizeof(double) * 2 * facOff.back())))
     return rc;
   DH_HIP(c, hipMemcpyAsync(S->d_pairInfo, info.data(), sizeof(int) * nPairs, hipMemcpyHostToDevice, c->stream));
   DH_HIP(c, hipMemcpyAsync(S->d_sumPairOff, pairOff.data(), sizeof(int) * (nbrSums + 1), hipMemcpyHostToDevice,

@@ -58,6 +58,7 @@ extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, i
                                     int nbrSectors) {
   if (!c || !out || !totalLz || nbrSectors <= 0) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (N <= 0 || lzMax < 0 || N > lzMax + 1) return dh_fail(c, DH_E_ARG, "need 0 < nbr_fermions <= lz_max + 1");
   if (lzMax > 62) return dh_fail(c, DH_E_RANGE, "lz_max must be <= 62");
   hipSetDevice(c->device);
@@ -125,11 +126,11 @@ extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, i
     }
   int rc;
   int64_t* d_T = nullptr;
-  if ((rc = dh_malloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
-      (rc = dh_malloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
+  if ((rc = dh_alloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
+      (rc = dh_alloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
     if (d_T) hipFree(d_T);
     dh_sector_set_destroy(S);
     return rc;
@@ -204,10 +205,10 @@ extern "C" int dh_sector_set_factors(dh_sector_set* S, const double* factors, co
     if (*p) hipFree(*p);
     *p = nullptr;
   }
-  if ((rc = dh_malloc(c, (void**)&S->d_factors, sizeof(double) * std::max(1, nbrFactors))) ||
-      (rc = dh_malloc(c, (void**)&S->d_pairOffset, sizeof(int) * (nPairs + 1))) ||
-      (rc = dh_malloc(c, (void**)&S->d_packedFactors, sizeof(double) * std::max<size_t>(1, packed.size()))) ||
-      (rc = dh_malloc(c, (void**)&S->d_sumBase, sizeof(int) * sumBase.size())))
+  if ((rc = dh_alloc(c, (void**)&S->d_factors, sizeof(double) * std::max(1, nbrFactors))) ||
+      (rc = dh_alloc(c, (void**)&S->d_pairOffset, sizeof(int) * (nPairs + 1))) ||
+      (rc = dh_alloc(c, (void**)&S->d_packedFactors, sizeof(double) * std::max<size_t>(1, packed.size()))) ||
+      (rc = dh_alloc(c, (void**)&S->d_sumBase, sizeof(int) * sumBase.size())))
     return rc;
   DH_HIP(c, hipMemcpyAsync(S->d_factors, factors, sizeof(double) * nbrFactors, hipMemcpyHostToDevice, c->stream));
   DH_HIP(c, hipMemcpyAsync(S->d_pairOffset, pairOffset, sizeof(int) * (nPairs + 1), hipMemcpyHostToDevice, c->stream));
@@ -266,7 +267,7 @@ int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_
       hipFree(S->d_work);
     }
     S->d_work = nullptr;
-    int rc = dh_malloc(c, (void**)&S->d_work, std::max<size_t>(bytes, 4096));
+    int rc = dh_alloc(c, (void**)&S->d_work, std::max<size_t>(bytes, 4096));
     if (rc) return rc;
     S->workCap = std::max<size_t>(bytes, 4096);
   }
@@ -624,7 +625,7 @@ static int sectors_axpy(dh_sector_set* S, const double* V, int nb, double* W, co
       hipFree(S->d_coef);
     }
     S->d_coef = nullptr;
-    if ((rc = dh_malloc(c, (void**)&S->d_coef, bytes))) return rc;
+    if ((rc = dh_alloc(c, (void**)&S->d_coef, bytes))) return rc;
     S->coefCap = bytes;
   }
   if ((rc = dh_upload_async(c, S->d_coef, h_c, bytes))) return rc;

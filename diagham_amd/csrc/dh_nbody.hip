@@ -217,7 +217,7 @@ int dh_nbody_count_elements(dh_hamiltonian* h, int64_t* nnz) {
   const dh_basis* b = h->basis;
   const int64_t chunk = 1 << 22;
   int64_t* d_counts = nullptr;
-  int rc = dh_malloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, b->dim)));
+  int rc = dh_alloc(c, (void**)&d_counts, sizeof(int64_t) * std::max<int64_t>(1, std::min(chunk, b->dim)));
   if (rc) return rc;
   const size_t lds = nbody_lds(h);
   if (lds > 64 * 1024)
@@ -254,6 +254,7 @@ extern "C" int dh_sphere_nbody_hamiltonian_create(dh_context* c, dh_hamiltonian*
                                                   const double* factors) {
   if (!c || !out || !b || !sumCount || !tuples || !factors || nbrSums <= 0) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (b->kind != DH_BASIS_FERMION_SPHERE) return dh_fail(c, DH_E_ARG, "basis is not a fermion-on-sphere basis");
   if (nbody != 3) return dh_fail(c, DH_E_ARG, "the n-body kernel handles three-body terms (nbody = 3)");
   const int nOrb = b->nbrOrbitals;
@@ -305,11 +306,11 @@ extern "C" int dh_sphere_nbody_hamiltonian_create(dh_context* c, dh_hamiltonian*
     return dh_fail(c, DH_E_RANGE, "the tuple tables exceed LDS");
   }
   int rc;
-  if ((rc = dh_malloc(c, (void**)&h->d_tupleMask, sizeof(uint64_t) * std::max<size_t>(1, masks.size()))) ||
-      (rc = dh_malloc(c, (void**)&h->d_tuplePos, sizeof(int) * pos.size())) ||
-      (rc = dh_malloc(c, (void**)&h->d_sumTupleOff, sizeof(int) * tupleOff.size())) ||
-      (rc = dh_malloc(c, (void**)&h->d_sumFactorOff, sizeof(int64_t) * factorOff.size())) ||
-      (rc = dh_malloc(c, (void**)&h->d_nbodyFactors, sizeof(double) * std::max<int64_t>(1, factorOff[nbrSums])))) {
+  if ((rc = dh_alloc(c, (void**)&h->d_tupleMask, sizeof(uint64_t) * std::max<size_t>(1, masks.size()))) ||
+      (rc = dh_alloc(c, (void**)&h->d_tuplePos, sizeof(int) * pos.size())) ||
+      (rc = dh_alloc(c, (void**)&h->d_sumTupleOff, sizeof(int) * tupleOff.size())) ||
+      (rc = dh_alloc(c, (void**)&h->d_sumFactorOff, sizeof(int64_t) * factorOff.size())) ||
+      (rc = dh_alloc(c, (void**)&h->d_nbodyFactors, sizeof(double) * std::max<int64_t>(1, factorOff[nbrSums])))) {
     dh_hamiltonian_destroy(h);
     return rc;
   }

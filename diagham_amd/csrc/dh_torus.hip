@@ -110,6 +110,7 @@ extern "C" int dh_torus_translation_sector_set_create(dh_context* c, dh_sector_s
                                                       const int* kx, const int* ky, int nbrSectors) {
   if (!c || !out || !kx || !ky || nbrSectors <= 0) return DH_E_ARG;
   *out = nullptr;
+  if (c->closed) return dh_fail(c, DH_E_STATE, "context destroyed");
   if (nflux < 2 || nflux > 62) return dh_fail(c, DH_E_RANGE, "the torus needs 2 to 62 flux quanta");
   if (N <= 0 || N > nflux) return dh_fail(c, DH_E_ARG, "need 0 < nbr_fermions <= nbr_flux");
   hipSetDevice(c->device);
@@ -175,12 +176,12 @@ extern "C" int dh_torus_translation_sector_set_create(dh_context* c, dh_sector_s
     }
   }
   int rc;
-  if ((rc = dh_malloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
-      (rc = dh_malloc(c, (void**)&S->d_orbit, S->totalDim)) ||
-      (rc = dh_malloc(c, (void**)&S->d_reorder, sizeof(uint64_t) * S->totalDim)) ||
-      (rc = dh_malloc(c, (void**)&S->d_lut, sizeof(int32_t) * lut.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_phase, sizeof(double) * phase.size())) ||
-      (rc = dh_malloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
+  if ((rc = dh_alloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
+      (rc = dh_alloc(c, (void**)&S->d_orbit, S->totalDim)) ||
+      (rc = dh_alloc(c, (void**)&S->d_reorder, sizeof(uint64_t) * S->totalDim)) ||
+      (rc = dh_alloc(c, (void**)&S->d_lut, sizeof(int32_t) * lut.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_phase, sizeof(double) * phase.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
     dh_sector_set_destroy(S);
     return rc;
   }

@@ -313,7 +313,10 @@ class SegmentedBlockLanczos {
   }
 
   // strong test of FullReorthogonalizedLanczosAlgorithm.cc:325: every wanted eigenvalue moved by
-  // less than precision * |value| since the previous test of s (or the Krylov space is complete)
+  // less than precision * |value| since the previous test of s (or the Krylov space is complete).
+  // A value that is zero to within 3 MACHINE_PRECISION now and before counts as converged
+  // (:335-338): exact zero modes otherwise move by rounding noise relative to themselves forever.
+  static constexpr double kMachinePrecision = 1e-14;  // src/config.h:49
   bool TestConvergence(int s, int nbrEigen, double precision, std::vector<double>& current) {
     std::vector<double> ev;
     if (K == 1)
@@ -330,7 +333,8 @@ class SegmentedBlockLanczos {
     bool ok = !Previous[s].empty() && (int)ev.size() > nbrEigen && (int)Previous[s].size() == want;
     if (ok)
       for (int e = 0; e < want; ++e)
-        if (std::fabs(current[e] - Previous[s][e]) > precision * std::fabs(current[e])) {
+        if (std::fabs(current[e] - Previous[s][e]) > precision * std::fabs(current[e]) &&
+            (std::fabs(current[e]) > 3 * kMachinePrecision || std::fabs(Previous[s][e]) > 3 * kMachinePrecision)) {
           ok = false;
           break;
         }

@@ -1,4 +1,5 @@
 // C facade of the host layer (include/diagham_host.h).
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -10,12 +11,43 @@
 
 using namespace dhh;
 
+// An architecture is reference counted: dhh_arch_create hands its caller one reference and every
+// problem holds one more, so the device context outlives the problems built on it whatever order
+// their owners release them in.  dhh_arch_destroy gives up the caller's reference.
 struct dhh_arch {
   std::unique_ptr<GPUArchitecture> arch;
+  std::atomic<int> refs{1};
+  bool closed = false;
+};
+
+static void arch_release(dhh_arch* a) {
+  if (a->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete a;
+}
+
+// counted reference of a problem to its architecture (released after the problem's space and
+// Hamiltonian, which are declared after it)
+class ArchRef {
+ public:
+  ArchRef() = default;
+  ArchRef(const ArchRef&) = delete;
+  ArchRef& operator=(const ArchRef&) = delete;
+  ~ArchRef() {
+    if (p) arch_release(p);
+  }
+  ArchRef& operator=(dhh_arch* a) {
+    if (a) a->refs.fetch_add(1, std::memory_order_relaxed);
+    if (p) arch_release(p);
+    p = a;
+    return *this;
+  }
+  dhh_arch* operator->() const { return p; }
+
+ private:
+  dhh_arch* p = nullptr;
 };
 
 struct dhh_problem {
-  dhh_arch* arch = nullptr;
+  ArchRef arch;
   std::unique_ptr<AbstractHilbertSpace> space;
   std::unique_ptr<AbstractHamiltonian> hamiltonian;
   std::vector<int> channels;
@@ -49,7 +81,13 @@ extern "C" int dhh_arch_create(dhh_arch** out, int device) {
 }
 
 extern "C" int dhh_arch_destroy(dhh_arch* a) {
-  delete a;
+  if (!a) return DH_E_ARG;
+  if (a->closed) {
+    g_error = "architecture already destroyed";
+    return DH_E_STATE;
+  }
+  a->closed = true;
+  arch_release(a);
   return DH_OK;
 }
 
@@ -58,6 +96,10 @@ extern "C" dh_context* dhh_arch_context(dhh_arch* a) { return a ? a->arch->Conte
 extern "C" int dhh_sphere_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
                                          const double* pseudo) {
   if (!a || !out || !pseudo) return DH_E_ARG;
+  if (a->closed) {
+    g_error = "architecture destroyed";
+    return DH_E_STATE;
+  }
   return guarded([&] {
     auto p = std::make_unique<dhh_problem>();
     p->arch = a;
@@ -99,6 +141,10 @@ extern "C" int dhh_vector_file_dimension(const char* fileName, int complex, int6
 extern "C" int dhh_sphere_l2_problem_create(dhh_arch* a, dhh_problem** out, int n, int totalLz, int lzMax,
                                             const double* pseudo, double l2Factor) {
   if (!a || !out) return DH_E_ARG;
+  if (a->closed) {
+    g_error = "architecture destroyed";
+    return DH_E_STATE;
+  }
   return guarded([&] {
     auto p = std::make_unique<dhh_problem>();
     p->arch = a;
@@ -132,6 +178,10 @@ extern "C" int dhh_sphere_nbody_problem_create(dhh_arch* a, dhh_problem** out, i
                                                int nbody, const int* sumCount, int nbrSums, const int* tuples,
                                                const double* factors) {
   if (!a || !out || !sumCount || !tuples || !factors || nbrSums <= 0 || nbody <= 0) return DH_E_ARG;
+  if (a->closed) {
+    g_error = "architecture destroyed";
+    return DH_E_STATE;
+  }
   return guarded([&] {
     auto p = std::make_unique<dhh_problem>();
     p->arch = a;
@@ -154,6 +204,10 @@ extern "C" int dhh_sphere_nbody_problem_create(dhh_arch* a, dhh_problem** out, i
 extern "C" int dhh_spin_problem_create(dhh_arch* a, dhh_problem** out, int L, int sz, const double* j,
                                        const double* jz, int periodic) {
   if (!a || !out || !j || !jz) return DH_E_ARG;
+  if (a->closed) {
+    g_error = "architecture destroyed";
+    return DH_E_STATE;
+  }
   return guarded([&] {
     auto p = std::make_unique<dhh_problem>();
     p->arch = a;

@@ -269,3 +269,23 @@ def test_l2_operator_on_gpu_labels_coulomb_states(arch):
     p.close()
     L2.close()
     shifted.close()
+
+
+@pytest.mark.parametrize("block", [1, 4])
+def test_laughlin_zero_mode_converges(arch, block):
+    """V1 alone at 2S = 3(N - 1) has the Laughlin state at exactly zero energy.  Its Ritz value
+    moves by rounding noise only, so the run stops by the near-zero rule of the strong convergence
+    test (FullReorthogonalizedLanczosAlgorithm.cc:335-338), long before max_iter, here in a sector
+    (8,512 states) larger than max_iter."""
+    from diagham_amd import capi
+    V = np.zeros(22)
+    V[1] = 1.0
+    p = capi.Problem.sphere(arch, 8, 0, 21, V)
+    max_iter = 300
+    r = p.lanczos(4, block_size=block, max_iter=max_iter)
+    assert r["converged"] and r["products"] < max_iter, r
+    ref_space, fac = O.FermionSphere(8, 0, 21), O.Factors(21, V)
+    ev_ref, _ = O.fs_lanczos(ref_space, fac, 4, max_iter=max_iter, threads=8)
+    assert abs(r["eigenvalues"][0]) < 1e-12
+    np.testing.assert_allclose(r["eigenvalues"], ev_ref, rtol=0, atol=1e-10)
+    p.close()

@@ -264,3 +264,21 @@ def test_krylov_basis_stays_orthonormal(selftest, tmp_path, k, cplx):
     out = subprocess.run([selftest, str(path), "ortho"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     assert out.returncode == 0, out.stderr
     assert float(out.stdout.split()[0]) < 1e-12
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_exact_zero_mode_converges(selftest, tmp_path, k):
+    """A wanted eigenvalue that is exactly zero converges (FullReorthogonalizedLanczosAlgorithm.cc:
+    335-338): its Ritz value only moves by rounding noise, which is never below 1e-14 of itself.
+    The space is larger than max_iter, so only the convergence test can stop the run."""
+    rng = np.random.default_rng(41 + k)
+    d, max_iter = 300, 250
+    lam = np.concatenate([[0.0, 1.0, 1.5, 2.0], rng.uniform(3.0, 5.0, d - 4)])
+    q, _ = np.linalg.qr(rng.standard_normal((d, d)))
+    m = (q * lam) @ q.T
+    m = (m + m.T) / 2
+    nev = 3
+    (conv, prods, ev), = run(selftest, tmp_path, [m], k, nev, max_iter=max_iter)
+    assert conv == 1 and prods < max_iter, (conv, prods)
+    assert abs(ev[0]) < 1e-13
+    assert np.max(np.abs(ev - lam[:nev])) < 1e-10

@@ -39,6 +39,7 @@ GPU_SIGNATURES = {
     "dh_synchronize": (c_int, [c_void_p]),
     "dh_device_info": (c_int, [c_void_p, P_I, c_char_p]),
     "dh_memory_info": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "dh_memory_peak": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_size_t), c_int]),
     "dh_malloc": (c_int, [c_void_p, PP, c_size_t]),
     "dh_free": (c_int, [c_void_p, c_void_p]),
     "dh_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
@@ -60,6 +61,7 @@ GPU_SIGNATURES = {
     "dh_hamiltonian_dimension": (c_int64, [c_void_p]),
     "dh_hamiltonian_set_launch": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "dh_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int64, c_int64]),
+    "dh_hv_vectors": (c_int, [c_void_p, PP, PP, c_int, c_int, c_int64, c_int64]),
     "dh_hamiltonian_count_elements": (c_int, [c_void_p, P_I64]),
     "dh_hamiltonian_precompute": (c_int, [c_void_p, c_int64, P_I64]),
     "dh_hamiltonian_release_precomputed": (c_int, [c_void_p]),
@@ -96,6 +98,7 @@ GPU_SIGNATURES = {
 
 HOST_SIGNATURES = {
     "dhh_last_error": (c_char_p, []),
+    "dhh_last_solve_waves": (c_int, []),
     "dhh_arch_create": (c_int, [PP, c_int]),
     "dhh_arch_destroy": (c_int, [c_void_p]),
     "dhh_arch_context": (c_void_p, [c_void_p]),
@@ -111,6 +114,7 @@ HOST_SIGNATURES = {
     "dhh_problem_basis": (c_void_p, [c_void_p]),
     "dhh_problem_hamiltonian": (c_void_p, [c_void_p]),
     "dhh_problem_channels": (c_int, [c_void_p, P_I, c_int]),
+    "dhh_problem_multiple_multiply": (c_int, [c_void_p, PP, PP, c_int, c_int, c_int64, c_int64]),
     "dhh_lanczos": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_uint64, P_D, P_I, P_I, P_D,
                             c_void_p, c_int]),
     "dhh_lanczos_disk": (c_int, [c_void_p, c_char_p, c_int, c_int, c_int, c_int, c_double, c_double, c_uint64, P_D,
@@ -306,6 +310,12 @@ class Arch:
     def sync(self):
         self.check(gpu().dh_synchronize(self.ctx), "dh_synchronize")
 
+    def memory_peak(self, reset=False):
+        """highest device memory in use since creation or the last reset (dh_memory_peak)"""
+        n = ctypes.c_size_t()
+        self.check(gpu().dh_memory_peak(self.ctx, ctypes.byref(n), int(reset)), "dh_memory_peak")
+        return n.value
+
 
 class DeviceBuffer:
     """Device memory from dh_malloc holding float64 (or raw bytes)."""
@@ -454,6 +464,20 @@ class Problem:
     def hv_device(self, d_x, d_y, k=1, add=False, first=0, count=-1):
         self.arch.check(gpu().dh_hv(self.ham, d_x, d_y, k, int(add), first, count), "dh_hv")
 
+    def hv_vectors_device(self, d_xs, d_ys, add=False, first=0, count=-1, via_host=False):
+        """H applied to len(d_xs) separate device vectors (dh_hv_vectors; via_host: through the
+        host layer's MultipleVectorHamiltonianMultiplyOperation, dhh_problem_multiple_multiply)"""
+        n = len(d_xs)
+        assert len(d_ys) == n
+        xs = (c_void_p * max(n, 1))(*d_xs)
+        ys = (c_void_p * max(n, 1))(*d_ys)
+        if via_host:
+            rc = host().dhh_problem_multiple_multiply(self.h, xs, ys, n, int(add), first, count)
+            if rc != 0:
+                raise DHError("dhh_problem_multiple_multiply: " + host().dhh_last_error().decode())
+        else:
+            self.arch.check(gpu().dh_hv_vectors(self.ham, xs, ys, n, int(add), first, count), "dh_hv_vectors")
+
     def hv(self, x, k=1):
         """H x for host arrays (x: dim*k, [row][k] layout); convenience for tests."""
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -592,7 +616,7 @@ def sphere_sectors(n, lz_max, positive_only=True):
 
 
 def sectors_solve(arch, n, lz_max, total_lz, pseudo, nbr_eigen, block_size=1, batched=True, max_iter=3000,
-                  precision=1e-14, seed=0x5EED0000, memory_gb=200.0, allow_unconverged=False):
+                  precision=1e-14, seed=0x5EED0000, memory_gb=0.0, allow_unconverged=False):
     """dhh_sectors_solve: the nbr_eigen lowest eigenvalues of every sector in total_lz."""
     lz = np.ascontiguousarray(total_lz, dtype=np.int32)
     pseudo = np.ascontiguousarray(pseudo, dtype=np.float64)
@@ -607,7 +631,30 @@ def sectors_solve(arch, n, lz_max, total_lz, pseudo, nbr_eigen, block_size=1, ba
     if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
         raise DHError("dhh_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
     return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
-            "seconds": secs.value}
+            "seconds": secs.value, "waves": host().dhh_last_solve_waves()}
+
+
+def lattice_sectors_solve(arch, n, nx, ny, kx, ky, nbr_eigen, block_size=1, t1=1.0, t2=0.5, tpp=-0.5, mus=0.0,
+                          u_potential=1.0, v_potential=0.0, flat_band=True, max_iter=3000, precision=1e-14,
+                          seed=0x5EED0000, memory_gb=0.0, allow_unconverged=False):
+    """dhh_lattice_sectors_solve: the nbr_eigen lowest eigenvalues of every (kx[i], ky[i]) sector of
+    the projected checkerboard model, all sectors advanced together"""
+    kxa = np.ascontiguousarray(kx, dtype=np.int32)
+    kya = np.ascontiguousarray(ky, dtype=np.int32)
+    assert kxa.size == kya.size
+    ns = kxa.size
+    ev = np.zeros(ns * nbr_eigen)
+    prods = np.zeros(ns, np.int32)
+    conv = np.zeros(ns, np.int32)
+    secs = ctypes.c_double()
+    rc = host().dhh_lattice_sectors_solve(arch.h, n, nx, ny, t1, t2, tpp, mus, u_potential, v_potential,
+                                          int(flat_band), kxa.ctypes.data_as(P_I), kya.ctypes.data_as(P_I), ns,
+                                          nbr_eigen, block_size, max_iter, precision, seed, memory_gb, dptr(ev),
+                                          prods.ctypes.data_as(P_I), conv.ctypes.data_as(P_I), ctypes.byref(secs))
+    if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
+        raise DHError("dhh_lattice_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
+    return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
+            "seconds": secs.value, "waves": host().dhh_last_solve_waves()}
 
 
 def torus_factors(nbr_flux, ratio=1.0, pseudo=(), coulomb_level=-1):
@@ -628,7 +675,7 @@ def torus_factors(nbr_flux, ratio=1.0, pseudo=(), coulomb_level=-1):
 
 
 def torus_sectors_solve(arch, n, nbr_flux, ky, nbr_eigen, ratio=1.0, pseudo=(), coulomb_level=-1, block_size=1,
-                        max_iter=3000, precision=1e-14, seed=0x5EED0000, memory_gb=200.0, allow_unconverged=False):
+                        max_iter=3000, precision=1e-14, seed=0x5EED0000, memory_gb=0.0, allow_unconverged=False):
     """dhh_torus_sectors_solve: the nbr_eigen lowest eigenvalues of every Ky sector in ky."""
     k = np.ascontiguousarray(ky, dtype=np.int32)
     pp = np.ascontiguousarray(pseudo, dtype=np.float64)
@@ -644,11 +691,11 @@ def torus_sectors_solve(arch, n, nbr_flux, ky, nbr_eigen, ratio=1.0, pseudo=(), 
     if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
         raise DHError("dhh_torus_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
     return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
-            "seconds": secs.value}
+            "seconds": secs.value, "waves": host().dhh_last_solve_waves()}
 
 
 def torus_translation_sectors_solve(arch, n, nbr_flux, kx, ky, nbr_eigen, ratio=1.0, pseudo=(), coulomb_level=-1,
-                                    block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000, memory_gb=200.0,
+                                    block_size=1, max_iter=3000, precision=1e-14, seed=0x5EED0000, memory_gb=0.0,
                                     allow_unconverged=False):
     """dhh_torus_translation_sectors_solve: the nbr_eigen lowest eigenvalues of every (kx[i], ky[i])
     sector of the torus with magnetic translations."""
@@ -668,4 +715,4 @@ def torus_translation_sectors_solve(arch, n, nbr_flux, kx, ky, nbr_eigen, ratio=
     if rc != 0 and not (allow_unconverged and rc == DH_E_STATE):
         raise DHError("dhh_torus_translation_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
     return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
-            "seconds": secs.value}
+            "seconds": secs.value, "waves": host().dhh_last_solve_waves()}

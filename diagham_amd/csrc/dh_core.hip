@@ -51,6 +51,7 @@ void dh_ctx_release(dh_context* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->d_pack) hipFree(c->d_pack);
   if (c->h_pinned) hipHostFree(c->h_pinned);
   for (auto& s : c->stage) {
     if (s.host) hipHostFree(s.host);
@@ -103,9 +104,28 @@ extern "C" int dh_memory_info(dh_context* c, size_t* freeBytes, size_t* totalByt
   return DH_OK;
 }
 
+static void sample_used(dh_context* c) {
+  size_t freeBytes = 0, totalBytes = 0;
+  if (hipMemGetInfo(&freeBytes, &totalBytes) == hipSuccess && totalBytes >= freeBytes)
+    c->usedPeak = std::max(c->usedPeak, totalBytes - freeBytes);
+}
+
 int dh_alloc(dh_context* c, void** p, size_t bytes) {
   hipSetDevice(c->device);
   if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess) return dh_fail(c, DH_E_NOMEM, "hipMalloc failed");
+  sample_used(c);
+  return DH_OK;
+}
+
+extern "C" int dh_memory_peak(dh_context* c, size_t* peakBytes, int reset) {
+  if (!c || !peakBytes) return DH_E_ARG;
+  hipSetDevice(c->device);
+  sample_used(c);
+  *peakBytes = c->usedPeak;
+  if (reset) {
+    c->usedPeak = 0;
+    sample_used(c);
+  }
   return DH_OK;
 }
 

@@ -542,6 +542,109 @@ extern "C" int dh_hv(dh_hamiltonian* h, const double* x, double* y, int k, int a
   return dh_fail(c, DH_E_STATE, "unknown Hamiltonian kind");
 }
 
+// ----------------------------------------------------------------------------- k separate vectors
+// LowLevelMultipleAddMultiply over an array of vectors (each its own allocation, as DiagHam's
+// RealVector* vSources): up to 16 of them are interleaved into one [row][k] block, multiplied by the
+// block kernels (one index lookup per element for the whole block) and written back.
+struct VectorPointers {
+  const double* x[16];
+  double* y[16];
+};
+
+template <int K>
+__global__ void interleave_kernel(VectorPointers v, double* __restrict__ X, int64_t dim) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride) {
+    double t[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) t[q] = v.x[q][r];
+#pragma unroll
+    for (int q = 0; q < K; ++q) X[r * K + q] = t[q];
+  }
+}
+
+template <int K>
+__global__ void deinterleave_kernel(VectorPointers v, const double* __restrict__ Y, int64_t first, int64_t count,
+                                    int add) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const int64_t r = first + i;
+#pragma unroll
+    for (int q = 0; q < K; ++q) v.y[q][r] = (add ? v.y[q][r] : 0.0) + Y[r * K + q];
+  }
+}
+
+template <int K>
+static int hv_group(dh_hamiltonian* h, const double* const* x, double* const* y, int add, int64_t first,
+                    int64_t count) {
+  dh_context* c = h->ctx;
+  const int64_t dim = h->basis->dim;
+  const size_t need = sizeof(double) * (size_t)dim * K * 2;
+  if (need > c->packBytes) {
+    if (c->d_pack) {
+      DH_HIP(c, hipStreamSynchronize(c->stream));
+      hipFree(c->d_pack);
+      c->d_pack = nullptr;
+      c->packBytes = 0;
+    }
+    int rc = dh_alloc(c, (void**)&c->d_pack, need);
+    if (rc) return rc;
+    c->packBytes = need;
+  }
+  double* X = c->d_pack;
+  double* Y = c->d_pack + (size_t)dim * K;
+  VectorPointers v{};
+  for (int q = 0; q < K; ++q) {
+    v.x[q] = x[q];
+    v.y[q] = y[q];
+  }
+  const unsigned grid = (unsigned)std::min<int64_t>((dim + 255) / 256, (int64_t)c->nbrCU * 8);
+  hipLaunchKernelGGL(interleave_kernel<K>, dim3(std::max(grid, 1u)), dim3(256), 0, c->stream, v, X, dim);
+  DH_HIP(c, hipGetLastError());
+  int rc = dh_hv(h, X, Y, K, 0, first, count);
+  if (rc) return rc;
+  const unsigned g2 = (unsigned)std::min<int64_t>((count + 255) / 256, (int64_t)c->nbrCU * 8);
+  hipLaunchKernelGGL(deinterleave_kernel<K>, dim3(std::max(g2, 1u)), dim3(256), 0, c->stream, v, Y, first, count,
+                     add);
+  DH_HIP(c, hipGetLastError());
+  return DH_OK;
+}
+
+extern "C" int dh_hv_vectors(dh_hamiltonian* h, const double* const* x, double* const* y, int nbrVectors, int add,
+                             int64_t first, int64_t count) {
+  if (!h || !x || !y || nbrVectors < 0) return DH_E_ARG;
+  dh_context* c = h->ctx;
+  for (int q = 0; q < nbrVectors; ++q)
+    if (!x[q] || !y[q]) return dh_fail(c, DH_E_ARG, "null vector in dh_hv_vectors");
+  const int64_t dim = h->basis->dim;
+  if (count < 0) count = dim - first;
+  if (first < 0 || first + count > dim) return dh_fail(c, DH_E_ARG, "row range outside the space");
+  if (count == 0) return DH_OK;
+  // groups of 16, 8, 4, 2 vectors share their lookups; a last single vector goes straight through
+  for (int q = 0; q < nbrVectors;) {
+    const int left = nbrVectors - q;
+    int rc;
+    if (left >= 16) {
+      rc = hv_group<16>(h, x + q, y + q, add, first, count);
+      q += 16;
+    } else if (left >= 8) {
+      rc = hv_group<8>(h, x + q, y + q, add, first, count);
+      q += 8;
+    } else if (left >= 4) {
+      rc = hv_group<4>(h, x + q, y + q, add, first, count);
+      q += 4;
+    } else if (left >= 2) {
+      rc = hv_group<2>(h, x + q, y + q, add, first, count);
+      q += 2;
+    } else {
+      rc = dh_hv(h, x[q], y[q], 1, add, first, count);
+      q += 1;
+    }
+    if (rc) return rc;
+  }
+  return DH_OK;
+}
+
 // ----------------------------------------------------------------------------- Hamiltonian objects
 extern "C" int dh_sphere_hamiltonian_create(dh_context* c, dh_hamiltonian** out, dh_basis* b, const double* factors,
                                             const int* pairOffset, int nbrFactors) {

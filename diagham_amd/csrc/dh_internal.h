@@ -34,6 +34,12 @@ struct dh_context {
   size_t scratchBytes = 0;
   double* h_pinned = nullptr;
   size_t pinnedBytes = 0;
+  // interleaved staging block of dh_hv_vectors (device), grown on demand
+  double* d_pack = nullptr;
+  size_t packBytes = 0;
+  // device memory in use (all of the device's, sampled after each allocation of this context) at
+  // its highest since creation or the last dh_memory_peak reset
+  size_t usedPeak = 0;
   // ring of pinned slots for host-to-device uploads that do not wait: a slot is reused once the
   // event recorded after its copy has completed
   static constexpr int kStageSlots = 16;

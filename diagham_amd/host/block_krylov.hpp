@@ -223,7 +223,7 @@ class SegmentedBlockLanczos {
     LaunchedStep = -1;
     double* W = Block(j + 1);
     std::vector<Rounds> r;
-    Orthonormalize(W, j + 1, act, r);
+    Orthonormalize(W, j + 1, act, r, true);
     for (size_t a = 0; a < act.size(); ++a) {
       const int s = act[a];
       std::vector<T>& h = r[a].coef;
@@ -547,16 +547,22 @@ class SegmentedBlockLanczos {
   }
 
   // Two rounds of projection + Cholesky-QR of W over the segments in `act`: W_in = V C + Q R
-  // with C = C1 + C2 R1 and R = R2 R1.  Deficient columns of Q are zero.  Both rounds always run:
-  // one projection leaves the basis orthogonal to ~1e-13 only, and the Ritz values then move by
-  // more than the 1e-14 relative change that the convergence test asks for.
+  // with C = C1 + C2 R1 and R = R2 R1.  Deficient columns of Q are zero.
   //
-  // Passes over the basis V per round: the dots C = V^T W, then the update W - V C (with the Gram
-  // matrix of the result), then W R^-1.  Four passes over V per step.  Taking the second round's
-  // dots from the first update instead (D R1^-1 with D = V^T W1) would save one, but only for one
-  // vector per block: for k > 1 the triangular solve mixes nearly dependent columns, and the
-  // second round exists to remove the basis components of exactly that rounding.
-  void OrthoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
+  // For the new block of a step (W = H X_j, `local`), the first round projects on the two newest
+  // blocks only: in exact arithmetic W has no component on older blocks (the block three-term
+  // recurrence), so this round removes the large coefficients A_j and B_j^H, as the local
+  // recurrence of FullReorthogonalizedLanczosAlgorithm.cc:225-239 does.  The second round projects
+  // on the whole basis (the full reorthogonalisation of :240-252) and removes the rounding left on
+  // every block.  Both rounds always run: one projection leaves the basis orthogonal to ~1e-13
+  // only, and the Ritz values then move by more than the 1e-14 relative change that the
+  // convergence test asks for.  Random replacement vectors have components on every block, so both
+  // of their rounds are full (`local` false).
+  //
+  // Passes over the basis V: the dots C = V^T W, then the update W - V C (with the Gram matrix of
+  // the result), then W R^-1.  With the local first round a step reads the whole basis twice and
+  // its two newest blocks twice more.
+  void OrthoRounds(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out, bool local = false) {
     const int kk = K * K, KK = KR * KR;
     const size_t na = act.size();
     out.assign(na, Rounds());
@@ -576,25 +582,28 @@ class SegmentedBlockLanczos {
       CholQR(W, act, all, out, ref, 1e-16);
       return;
     }
-    const size_t per = (size_t)nb * KK;
     std::vector<unsigned char> mask = Mask(act, all);
-    std::vector<double> C(na * per), G(na * KK), M1(na * KK), M2(na * KK);
+    std::vector<double> C(na * (size_t)nb * KK), G(na * KK), M1(na * KK), M2(na * KK);
     std::vector<T> CT(na * (size_t)nb * kk);
-    // scalar blocks of the dots in the device layout
-    auto fromDevice = [&](const double* src) {
+    // scalar blocks of the dots over nv blocks, in the device layout
+    auto fromDevice = [&](const double* src, int nv) {
+      const size_t per = (size_t)nv * KK;
       for (size_t a = 0; a < na; ++a)
-        for (int b = 0; b < nb; ++b) S::FromGram(K, &src[a * per + (size_t)b * KK], &CT[(a * nb + b) * kk]);
+        for (int b = 0; b < nv; ++b) S::FromGram(K, &src[a * per + (size_t)b * KK], &CT[(a * nv + b) * kk]);
     };
-    Ops->Dots(Storage, nb, W, mask.data(), C.data());
-    fromDevice(C.data());
-    Coefficients(CT, C.data(), nb, all, out, true, ref);
-    Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, nullptr, G.data());
+    // first round: the two newest blocks (local) or all of them; the newest is the last either way
+    const int nb1 = local ? std::min(nb, 2) : nb;
+    const double* V1 = Storage + (size_t)(nb - nb1) * BlockDoubles;
+    Ops->Dots(V1, nb1, W, mask.data(), C.data());
+    fromDevice(C.data(), nb1);
+    Coefficients(CT, C.data(), nb1, all, out, true, ref);
+    Ops->Update(V1, nb1, W, mask.data(), C.data(), nullptr, nullptr, G.data());
     // noise left by one projection is ~1e-15 of the column
     Factor(G.data(), all, out, ref, 1e-24, M1.data(), nullptr);
     for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
     Ops->Rmul(W, mask.data(), M1.data());
     Ops->Dots(Storage, nb, W, mask.data(), C.data());
-    fromDevice(C.data());
+    fromDevice(C.data(), nb);
     Coefficients(CT, C.data(), nb, all, out, false, ref);
     Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, nullptr, G.data());
     // after the second projection, a column that kept less than 1e-8 of itself was
@@ -605,9 +614,9 @@ class SegmentedBlockLanczos {
 
   // W (nb basis blocks before it) -> orthonormal Q with W_in = V C + Q R per segment of `act`;
   // deficient columns are replaced by random directions where the segment has room left
-  void Orthonormalize(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out) {
+  void Orthonormalize(double* W, int nb, const std::vector<int>& act, std::vector<Rounds>& out, bool local = false) {
     const int kk = K * K, KK = KR * KR;
-    OrthoRounds(W, nb, act, out);
+    OrthoRounds(W, nb, act, out, local);
     std::vector<int> need, needSeg;
     for (size_t a = 0; a < act.size(); ++a)
       if (std::find(out[a].good.begin(), out[a].good.end(), 0) != out[a].good.end()) {

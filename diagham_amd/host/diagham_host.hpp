@@ -70,6 +70,8 @@ class DeviceVector {
   DeviceVector& operator=(const DeviceVector&) = delete;
   DeviceVector(DeviceVector&& o) noexcept;
   DeviceVector& operator=(DeviceVector&& o) noexcept;
+  // a vector over device memory owned by the caller (released by the caller, not by this object)
+  static DeviceVector View(GPUArchitecture* arch, double* d, int64_t dim, int k = 1);
   double* Data() const { return d; }
   int64_t GetVectorDimension() const { return dim; }
   int BlockSize() const { return k; }
@@ -79,6 +81,7 @@ class DeviceVector {
   double* d = nullptr;
   int64_t dim = 0;
   int k = 1;
+  bool owns = true;
 };
 
 // ------------------------------------------------------------------ Hilbert spaces
@@ -132,6 +135,13 @@ class AbstractHamiltonian {
   // block products: the k vectors of a DeviceVector block share every index lookup
   DeviceVector& LowLevelMultipleAddMultiply(DeviceVector& vSources, DeviceVector& vDestinations);
   DeviceVector& LowLevelMultipleMultiply(DeviceVector& vSources, DeviceVector& vDestinations);
+  // LowLevelMultipleAddMultiply(RealVector* vSources, RealVector* vDestinations, int nbrVectors,
+  // int firstComponent, int nbrComponent) (src/Hamiltonian/AbstractHamiltonian.h:344): nbrVectors
+  // separate one-vector DeviceVectors; groups of them share the index lookups (dh_hv_vectors)
+  DeviceVector* LowLevelMultipleAddMultiply(DeviceVector* vSources, DeviceVector* vDestinations, int nbrVectors,
+                                            int64_t firstComponent, int64_t nbrComponent);
+  DeviceVector* LowLevelMultipleMultiply(DeviceVector* vSources, DeviceVector* vDestinations, int nbrVectors,
+                                         int64_t firstComponent, int64_t nbrComponent);
   dh_hamiltonian* Handle() const { return ham; }
   GPUArchitecture* Architecture() const { return arch; }
 
@@ -220,6 +230,24 @@ class VectorHamiltonianMultiplyOperation {
   DeviceVector *Source, *Destination;
 };
 
+// MultipleVectorHamiltonianMultiplyOperation
+// (src/Architecture/ArchitectureOperation/MultipleVectorHamiltonianMultiplyOperation.cc): H applied
+// to nbrVectors separate vectors; on the GPU architecture the whole row range is one call of
+// LowLevelMultipleMultiply (RawApplyOperation :427 with the full range).
+class MultipleVectorHamiltonianMultiplyOperation {
+ public:
+  MultipleVectorHamiltonianMultiplyOperation(AbstractHamiltonian* hamiltonian, DeviceVector* sources,
+                                             DeviceVector* destinations, int nbrVectors);
+  void SetIndicesRange(int64_t firstComponent, int64_t nbrComponent);
+  bool ApplyOperation(GPUArchitecture* architecture);
+
+ private:
+  AbstractHamiltonian* Hamiltonian;
+  DeviceVector *Sources, *Destinations;
+  int NbrVectors;
+  int64_t FirstComponent = 0, NbrComponent = -1;
+};
+
 // ------------------------------------------------------------------ Lanczos
 struct LanczosResult {
   std::vector<double> eigenvalues;
@@ -287,9 +315,22 @@ struct SectorResult {
   int totalLz = 0;
   int64_t dimension = 0;
   std::vector<double> eigenvalues;
-  int products = 0;
+  int products = 0;   // vector products over every attempt
   bool converged = false;
+  int blocks = 0;     // Krylov blocks of the last attempt
+  bool deferred = false;  // the last attempt stopped because the device could not hold its storage
 };
+
+// A Krylov storage growth that the device memory cannot hold (GrowKrylovStorage).
+struct KrylovStorageFull : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// free device memory of an architecture (0 if it cannot be read)
+inline size_t DeviceFreeBytes(GPUArchitecture* arch) {
+  size_t freeBytes = 0, totalBytes = 0;
+  return dh_memory_info(arch->Context(), &freeBytes, &totalBytes) == DH_OK ? freeBytes : 0;
+}
 
 // BlockOps over the rows of a sector set: k vectors per block, kr doubles per row (k for real
 // vectors, 2k for complex ones)
@@ -332,7 +373,8 @@ struct SectorSolveResources {
 inline int InitialKrylovBlocks(int k, int maxBlocks) { return std::min(maxBlocks, std::max(32, 256 / k)); }
 // device storage for blocks + 2 blocks of blockDoubles that holds the first `used` blocks of `old`;
 // `old` is released.  While old is still held, `blocks` is lowered (not below `need`) to what the
-// device's free memory holds beside it, keeping 1 GiB spare.
+// device's free memory holds beside it, keeping 1 GiB spare; when not even `need` blocks fit, old
+// is released and KrylovStorageFull is thrown.
 double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int& blocks, int need, int64_t blockDoubles);
 
 // Krylov blocks each sector may use (maxIter / k + 1, and no more than its space needs); returns
@@ -352,16 +394,26 @@ inline int SectorBlockCaps(const std::vector<int64_t>& dims, int k, int maxIter,
 // The product of the next step is enqueued before the convergence tests of this one run on the
 // host, for every sector that can still step; a sector that converges in those tests drops out of
 // the orthogonalisation, and its one extra product is discarded.
+//
+// When the Krylov storage cannot grow any more (KrylovStorageFull), the sectors still running after
+// this step's convergence tests stop with `deferred` set, for the caller to solve again with more
+// room; the others keep their results.
 template <class T>
 void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>& blockCap, int nbrEigenvalue,
                       double precision, uint64_t seed, std::vector<SectorResult*>& results) {
   const int na = (int)results.size();
   const int k = lanczos.BlockSize();
+  for (SectorResult* r : results) {
+    r->converged = false;
+    r->deferred = false;
+    r->blocks = 0;
+  }
   lanczos.Start(seed);
   std::vector<unsigned char> active(na, 1), next(na, 0);
   std::vector<char> done(na, 0);
   lanczos.LaunchProduct(active);
-  while (true) {
+  bool full = false;
+  while (!full) {
     std::vector<int> act;
     for (int i = 0; i < na; ++i)
       if (active[i]) act.push_back(i);
@@ -372,13 +424,22 @@ void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>&
       next[i] = active[i] && !(lanczos.Exhausted(i) || lanczos.NbrBlocks(i) >= blockCap[i]);
       any = any || next[i];
     }
-    if (any) lanczos.LaunchProduct(next);
-    for (int i : act) results[i]->products += k;
+    if (any) {
+      try {
+        lanczos.LaunchProduct(next);
+      } catch (const KrylovStorageFull&) {
+        full = true;
+      }
+    }
+    for (int i : act) {
+      results[i]->products += k;
+      results[i]->blocks = lanczos.NbrBlocks(i);
+    }
 #pragma omp parallel for schedule(dynamic, 1)
     for (size_t a = 0; a < act.size(); ++a) {
       const int i = act[a];
       const bool last = !next[i];
-      if (!last && !lanczos.CheckDue(i, nbrEigenvalue)) continue;
+      if (!last && !full && !lanczos.CheckDue(i, nbrEigenvalue)) continue;
       std::vector<double> cur;
       const bool ok = lanczos.TestConvergence(i, nbrEigenvalue, precision, cur);
       results[i]->eigenvalues = cur;
@@ -387,6 +448,74 @@ void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>&
     }
     for (int i = 0; i < na; ++i) active[i] = next[i] && !done[i];
   }
+  if (full)
+    for (int i = 0; i < na; ++i)
+      if (active[i]) results[i]->deferred = true;
+}
+
+// Krylov blocks a sector of `dim` rows is expected to need for nbrEigenvalue eigenvalues at block
+// size k (measured on MI355X runs of Coulomb sectors: about 25 vectors per wanted eigenvalue plus
+// 50 for one vector, each further vector of a block adding 30 %), within the sector's own space
+// and maxIter
+inline int ExpectedKrylovBlocks(int64_t dim, int k, int nbrEigenvalue, int maxIter) {
+  const double vectors = (25.0 * nbrEigenvalue + 50.0) * (1.0 + 0.3 * (k - 1));
+  const int64_t blocks = (int64_t)std::ceil(vectors / k);
+  return (int)std::max<int64_t>(3, std::min<int64_t>({blocks, (dim + k - 1) / k + 1, maxIter / k + 1}));
+}
+
+// Waves of a batched sector solve.  Sectors are taken largest first and admitted to a wave while its
+// expected Krylov storage, (expected blocks + 2) x rows x kr doubles plus the basis words, fits the
+// smaller of memoryBytes and the device's free memory less 2 GiB.  A wave's storage starts small
+// and grows as the run needs it (SegmentedBlockLanczos::SetGrowth); sectors stopped because it
+// could not grow go back to the queue expecting 1.5 times the blocks they reached, and after each
+// wave the expectation follows the blocks its sectors used.  solveWave(sectors) fills out[s] of the
+// sectors it is given; freeBytes() is the device memory free before a wave.  Returns the number of
+// waves.
+template <class F, class M>
+int RunSectorWaves(const std::vector<int64_t>& dims, int k, int kr, int nbrEigenvalue, int maxIter, size_t memoryBytes,
+                   std::vector<SectorResult>& out, F solveWave, M freeBytes) {
+  std::vector<int> queue;
+  for (size_t s = 0; s < dims.size(); ++s) queue.push_back((int)s);
+  std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return dims[a] > dims[b]; });
+  std::vector<int> expect(dims.size());
+  for (size_t s = 0; s < dims.size(); ++s) expect[s] = ExpectedKrylovBlocks(dims[s], k, nbrEigenvalue, maxIter);
+  int waves = 0;
+  while (!queue.empty()) {
+    const size_t avail = freeBytes(), spare = (size_t)2 << 30;
+    const size_t budget = std::min(memoryBytes, avail > spare ? avail - spare : (size_t)0);
+    std::vector<int> wave, rest;
+    int blocks = 0;
+    int64_t rows = 0;
+    for (int s : queue) {
+      const int b = std::max(blocks, std::min(expect[s], (int)((dims[s] + k - 1) / k + 1)));
+      const size_t need = sizeof(double) * ((size_t)(b + 2) * (size_t)(rows + dims[s]) * kr + (size_t)(rows + dims[s]));
+      if (!wave.empty() && need > budget) {
+        rest.push_back(s);
+        continue;
+      }
+      wave.push_back(s);
+      blocks = b;
+      rows += dims[s];
+    }
+    solveWave(wave);
+    ++waves;
+    int reached = 0;
+    std::vector<int> again;
+    for (int s : wave) {
+      reached = std::max(reached, out[s].blocks);
+      if (out[s].deferred) again.push_back(s);
+    }
+    if (!again.empty() && again.size() == wave.size() && wave.size() == 1)
+      throw std::runtime_error("a sector needs more Krylov storage than the device holds");
+    for (int s : again) expect[s] = std::max(expect[s] + 1, (out[s].blocks * 3 + 1) / 2);
+    // the sectors left are no larger than this wave's: expect at most what it used, with margin
+    if (again.size() < wave.size())
+      for (int s : rest) expect[s] = std::min(expect[s], std::max(3, (reached * 23 + 19) / 20));
+    queue = again;
+    queue.insert(queue.end(), rest.begin(), rest.end());
+    std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return dims[a] > dims[b]; });
+  }
+  return waves;
 }
 
 class MultiSectorBlockLanczos {

@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "diagham_host.hpp"
 
@@ -36,23 +37,34 @@ DeviceVector::DeviceVector(GPUArchitecture* a, int64_t n, int kk) : arch(a), dim
 }
 
 DeviceVector::~DeviceVector() {
-  if (arch && d) arch->FreeMemory(d);
+  if (arch && d && owns) arch->FreeMemory(d);
 }
 
-DeviceVector::DeviceVector(DeviceVector&& o) noexcept : arch(o.arch), d(o.d), dim(o.dim), k(o.k) {
+DeviceVector::DeviceVector(DeviceVector&& o) noexcept : arch(o.arch), d(o.d), dim(o.dim), k(o.k), owns(o.owns) {
   o.d = nullptr;
 }
 
 DeviceVector& DeviceVector::operator=(DeviceVector&& o) noexcept {
   if (this != &o) {
-    if (arch && d) arch->FreeMemory(d);
+    if (arch && d && owns) arch->FreeMemory(d);
     arch = o.arch;
     d = o.d;
     dim = o.dim;
     k = o.k;
+    owns = o.owns;
     o.d = nullptr;
   }
   return *this;
+}
+
+DeviceVector DeviceVector::View(GPUArchitecture* a, double* p, int64_t n, int kk) {
+  DeviceVector v;
+  v.arch = a;
+  v.d = p;
+  v.dim = n;
+  v.k = kk;
+  v.owns = false;
+  return v;
 }
 
 AbstractHilbertSpace::~AbstractHilbertSpace() {
@@ -96,6 +108,45 @@ DeviceVector& AbstractHamiltonian::LowLevelMultipleAddMultiply(DeviceVector& src
 DeviceVector& AbstractHamiltonian::LowLevelMultipleMultiply(DeviceVector& src, DeviceVector& dst) {
   arch->Check(dh_hv(ham, src.Data(), dst.Data(), src.BlockSize(), 0, 0, -1), "LowLevelMultipleMultiply");
   return dst;
+}
+
+static DeviceVector* MultipleProduct(AbstractHamiltonian* h, GPUArchitecture* arch, DeviceVector* src,
+                                     DeviceVector* dst, int n, int add, int64_t first, int64_t nbr, const char* what) {
+  std::vector<const double*> x(n > 0 ? n : 1);
+  std::vector<double*> y(n > 0 ? n : 1);
+  for (int v = 0; v < n; ++v) {
+    if (src[v].BlockSize() != 1 || dst[v].BlockSize() != 1)
+      throw std::invalid_argument(std::string(what) + ": the vectors must be single vectors");
+    x[v] = src[v].Data();
+    y[v] = dst[v].Data();
+  }
+  arch->Check(dh_hv_vectors(h->Handle(), x.data(), y.data(), n, add, first, nbr), what);
+  return dst;
+}
+
+DeviceVector* AbstractHamiltonian::LowLevelMultipleAddMultiply(DeviceVector* src, DeviceVector* dst, int n,
+                                                               int64_t first, int64_t nbr) {
+  return MultipleProduct(this, arch, src, dst, n, 1, first, nbr, "LowLevelMultipleAddMultiply");
+}
+
+DeviceVector* AbstractHamiltonian::LowLevelMultipleMultiply(DeviceVector* src, DeviceVector* dst, int n,
+                                                            int64_t first, int64_t nbr) {
+  return MultipleProduct(this, arch, src, dst, n, 0, first, nbr, "LowLevelMultipleMultiply");
+}
+
+MultipleVectorHamiltonianMultiplyOperation::MultipleVectorHamiltonianMultiplyOperation(AbstractHamiltonian* h,
+                                                                                       DeviceVector* s,
+                                                                                       DeviceVector* d, int n)
+    : Hamiltonian(h), Sources(s), Destinations(d), NbrVectors(n) {}
+
+void MultipleVectorHamiltonianMultiplyOperation::SetIndicesRange(int64_t first, int64_t nbr) {
+  FirstComponent = first;
+  NbrComponent = nbr;
+}
+
+bool MultipleVectorHamiltonianMultiplyOperation::ApplyOperation(GPUArchitecture*) {
+  Hamiltonian->LowLevelMultipleMultiply(Sources, Destinations, NbrVectors, FirstComponent, NbrComponent);
+  return true;
 }
 
 VectorHamiltonianMultiplyOperation::VectorHamiltonianMultiplyOperation(AbstractHamiltonian* h, DeviceVector* s,

@@ -54,6 +54,7 @@ struct dhh_problem {
 };
 
 static thread_local std::string g_error;
+static thread_local int g_lastWaves = 0;
 
 template <typename F>
 static int guarded(F&& f) {
@@ -69,6 +70,8 @@ static int guarded(F&& f) {
 }
 
 extern "C" const char* dhh_last_error(void) { return g_error.c_str(); }
+
+extern "C" int dhh_last_solve_waves(void) { return g_lastWaves; }
 
 extern "C" int dhh_arch_create(dhh_arch** out, int device) {
   if (!out) return DH_E_ARG;
@@ -231,6 +234,30 @@ extern "C" int dhh_problem_destroy(dhh_problem* p) {
 extern "C" dh_basis* dhh_problem_basis(dhh_problem* p) { return p ? p->space->Basis() : nullptr; }
 extern "C" dh_hamiltonian* dhh_problem_hamiltonian(dhh_problem* p) { return p ? p->hamiltonian->Handle() : nullptr; }
 
+extern "C" int dhh_problem_multiple_multiply(dhh_problem* p, const double* const* x, double* const* y, int n,
+                                             int add, int64_t first, int64_t count) {
+  if (!p || !x || !y || n < 0) return DH_E_ARG;
+  return guarded([&] {
+    GPUArchitecture* arch = p->arch->arch.get();
+    const int64_t dim = p->hamiltonian->GetHilbertSpaceDimension();
+    std::vector<DeviceVector> src, dst;
+    src.reserve(n);
+    dst.reserve(n);
+    for (int v = 0; v < n; ++v) {
+      src.push_back(DeviceVector::View(arch, const_cast<double*>(x[v]), dim));
+      dst.push_back(DeviceVector::View(arch, y[v], dim));
+    }
+    if (add)
+      p->hamiltonian->LowLevelMultipleAddMultiply(src.data(), dst.data(), n, first, count);
+    else {
+      MultipleVectorHamiltonianMultiplyOperation op(p->hamiltonian.get(), src.data(), dst.data(), n);
+      op.SetIndicesRange(first, count);
+      op.ApplyOperation(arch);
+    }
+    return DH_OK;
+  });
+}
+
 extern "C" int dhh_problem_channels(dhh_problem* p, int* momenta, int capacity) {
   if (!p) return DH_E_ARG;
   int n = (int)p->channels.size();
@@ -356,7 +383,9 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
     if (batched) {
       MultiSectorBlockLanczos solver(a->arch.get(), n, lzMax, lz, v, nbrEigen, blockSize, maxIter, precision,
                                      (size_t)(memoryGB * 1073741824.0));
+      g_lastWaves = 0;
       std::vector<SectorResult> r = solver.Solve(seed);
+      g_lastWaves = solver.NbrWaves();
       for (int s = 0; s < nbrSectors; ++s) {
         for (size_t e = 0; e < r[s].eigenvalues.size() && (int)e < nbrEigen; ++e)
           eigenvalues[(size_t)s * nbrEigen + e] = r[s].eigenvalues[e];
@@ -365,6 +394,7 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
         all = all && r[s].converged;
       }
     } else {
+      g_lastWaves = 0;
       for (int s = 0; s < nbrSectors; ++s) {
         FermionOnSphere space(a->arch.get(), n, lz[s], lzMax);
         ParticleOnSphereGenericHamiltonian h(&space, n, lzMax, v, a->arch.get());
@@ -400,7 +430,9 @@ extern "C" int dhh_lattice_sectors_solve(dhh_arch* a, int n, int nx, int ny, dou
     LatticeSectorsBlockLanczos solver(a->arch.get(), n, nx, ny, std::vector<int>(kx, kx + nbrSectors),
                                       std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
                                       maxIter, precision, (size_t)(memoryGB * 1073741824.0));
+    g_lastWaves = 0;
     std::vector<SectorResult> r = solver.Solve(seed);
+    g_lastWaves = solver.NbrWaves();
     bool all = true;
     for (int s = 0; s < nbrSectors; ++s) {
       for (int q = 0; q < nbrEigen; ++q)
@@ -483,7 +515,9 @@ extern "C" int dhh_torus_sectors_solve(dhh_arch* a, int n, int nbrFlux, double r
     LatticeSectorsBlockLanczos solver(a->arch.get(), n, 1, nbrFlux, std::vector<int>(nbrSectors, 0),
                                       std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
                                       maxIter, precision, (size_t)(memoryGB * 1073741824.0));
+    g_lastWaves = 0;
     std::vector<SectorResult> r = solver.Solve(seed);
+    g_lastWaves = solver.NbrWaves();
     bool all = true;
     for (int s = 0; s < nbrSectors; ++s) {
       for (int q = 0; q < nbrEigen; ++q)
@@ -515,7 +549,9 @@ extern "C" int dhh_torus_translation_sectors_solve(dhh_arch* a, int n, int nbrFl
                                       std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
                                       maxIter, precision, (size_t)(memoryGB * 1073741824.0));
     solver.UseTorusTranslations();
+    g_lastWaves = 0;
     std::vector<SectorResult> r = solver.Solve(seed);
+    g_lastWaves = solver.NbrWaves();
     bool all = true;
     for (int s = 0; s < nbrSectors; ++s) {
       for (int q = 0; q < nbrEigen; ++q)

@@ -263,10 +263,17 @@ double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int& blo
   double* p = nullptr;
   dh_context* c = arch->Context();
   size_t freeBytes = 0, totalBytes = 0;
+  bool fits = true;
   if (dh_memory_info(c, &freeBytes, &totalBytes) == DH_OK) {
     const size_t spare = (size_t)1 << 30, blockBytes = sizeof(double) * (size_t)blockDoubles;
     const int64_t fit = freeBytes > spare ? (int64_t)((freeBytes - spare) / blockBytes) - 2 : 0;
     if (blocks > fit) blocks = (int)std::max<int64_t>(need, fit);
+    fits = fit >= need;
+  }
+  if (!fits) {
+    dh_synchronize(c);
+    arch->FreeMemory(old);
+    throw KrylovStorageFull("the device memory cannot hold " + std::to_string(need) + " Krylov blocks");
   }
   try {
     p = arch->AllocateMemory((int64_t)(blocks + 2) * blockDoubles);

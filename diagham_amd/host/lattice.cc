@@ -163,7 +163,7 @@ LatticeSectorsBlockLanczos::LatticeSectorsBlockLanczos(GPUArchitecture* architec
       BlockSize(blockSize),
       MaxIter(maxIter),
       Precision(precision),
-      MemoryBytes(memoryBytes ? memoryBytes : (size_t)160 << 30),
+      MemoryBytes(memoryBytes ? memoryBytes : SIZE_MAX),
       Kx(kx),
       Ky(ky),
       SumPairCount(sumPairCount),
@@ -178,9 +178,7 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
   const int ns = (int)Kx.size();
   std::vector<SectorResult> out(ns);
   const int k = BlockSize;
-  const int maxBlocks = std::max(3, MaxIter / k + 1);
   // sector dimensions from a one-sector set each is wasteful; use a set of all and split
-  dh_context* c = Arch->Context();
   std::vector<int64_t> dims(ns);
   {
     dh_sector_set* S = CreateSet(Kx, Ky);
@@ -189,26 +187,14 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
     for (int s = 0; s < ns; ++s) dims[s] = off[s + 1] - off[s];
     dh_sector_set_destroy(S);
   }
-  std::vector<int> wave;
-  size_t used = 0;
-  Waves = 0;
   for (int s = 0; s < ns; ++s) {
     out[s].dimension = dims[s];
     if (dims[s] <= 0) throw std::invalid_argument("empty momentum sector");
-    size_t need = (size_t)dims[s] * 2 * k * (maxBlocks + 2) * sizeof(double);
-    if (!wave.empty() && used + need > MemoryBytes) {
-      SolveWave(wave, seed, out);
-      ++Waves;
-      wave.clear();
-      used = 0;
-    }
-    wave.push_back(s);
-    used += need;
   }
-  if (!wave.empty()) {
-    SolveWave(wave, seed, out);
-    ++Waves;
-  }
+  // waves whose expected Krylov storage fits the device (RunSectorWaves)
+  Waves = RunSectorWaves(
+      dims, k, 2 * k, NbrEigenvalue, MaxIter, MemoryBytes, out,
+      [&](const std::vector<int>& wave) { SolveWave(wave, seed, out); }, [&] { return DeviceFreeBytes(Arch); });
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return out;
 }

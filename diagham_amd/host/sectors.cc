@@ -40,7 +40,7 @@ MultiSectorBlockLanczos::MultiSectorBlockLanczos(GPUArchitecture* architecture, 
       BlockSize(blockSize),
       MaxIter(maxIter),
       Precision(precision),
-      MemoryBytes(memoryBytes ? memoryBytes : (size_t)160 << 30),
+      MemoryBytes(memoryBytes ? memoryBytes : SIZE_MAX),
       TotalLz(totalLz),
       Pseudo(pseudoPotential) {
   if (BlockSize != 1 && BlockSize != 2 && BlockSize != 4 && BlockSize != 8 && BlockSize != 16)
@@ -110,27 +110,19 @@ std::vector<SectorResult> MultiSectorBlockLanczos::Solve(uint64_t seed, double* 
       r.converged = true;
     }
   }
-  // the rest in waves whose Krylov storage fits the memory budget
-  const int k = BlockSize;
-  const int maxBlocks = std::max(3, MaxIter / k + 1);
-  std::vector<int> wave;
-  size_t used = 0;
-  Waves = 0;
-  for (int s : lanczosSectors) {
-    size_t need = (size_t)out[s].dimension * k * (maxBlocks + 2) * sizeof(double);
-    if (!wave.empty() && used + need > MemoryBytes) {
-      SolveWave(wave, seed, out);
-      ++Waves;
-      wave.clear();
-      used = 0;
-    }
-    wave.push_back(s);
-    used += need;
-  }
-  if (!wave.empty()) {
-    SolveWave(wave, seed, out);
-    ++Waves;
-  }
+  // the rest in waves whose expected Krylov storage fits the device (RunSectorWaves)
+  std::vector<int64_t> dims;
+  for (int s : lanczosSectors) dims.push_back(out[s].dimension);
+  std::vector<SectorResult> part(lanczosSectors.size());
+  Waves = RunSectorWaves(
+      dims, BlockSize, BlockSize, NbrEigenvalue, MaxIter, MemoryBytes, part,
+      [&](const std::vector<int>& wave) {
+        std::vector<int> sectors;
+        for (int i : wave) sectors.push_back(lanczosSectors[i]);
+        SolveWave(sectors, seed, out);
+        for (int i : wave) part[i] = out[lanczosSectors[i]];
+      },
+      [&] { return DeviceFreeBytes(Arch); });
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return out;
 }

@@ -58,6 +58,10 @@ int dh_device_info(dh_context* ctx, int* nbr_cu, char* name);
 /* free and total device memory of the context's device (hipMemGetInfo); no counterpart in the
  * reference: the host sizes the Krylov storage it grows on demand by it */
 int dh_memory_info(dh_context* ctx, size_t* free_bytes, size_t* total_bytes);
+/* Highest device memory in use (the whole device's, sampled after every allocation this context
+ * makes) since the context was created or the last reset; reset != 0 starts a new period.  No
+ * counterpart in the reference: it reports the footprint of a solve. */
+int dh_memory_peak(dh_context* ctx, size_t* peak_bytes, int reset);
 
 /* Device vector storage; AbstractArchitecture::GetNewRealVector / AllocateMemory hooks
  * (src/Architecture/AbstractArchitecture.h:150-246). */
@@ -128,8 +132,8 @@ int dh_hamiltonian_destroy(dh_hamiltonian* ham);
 int dh_hamiltonian_set_shift(dh_hamiltonian* ham, double shift);
 int64_t dh_hamiltonian_dimension(const dh_hamiltonian* ham);
 
-/* Launch shape of the H.v kernels.  lanes_per_row in {1,2,4,8,16,32,64} (64/lanes_per_row
- * destination rows per wavefront); block_size a multiple of 64 up to 1024; index_mode
+/* Launch shape of the H.v kernels.  lanes_per_row in {1,2,4,8} (64/lanes_per_row destination
+ * rows per wavefront; spin chains: 1); block_size a multiple of 64 up to 1024; index_mode
  * DH_INDEX_*; lds_tables != 0 stages the index tables in LDS when they fit. */
 int dh_hamiltonian_set_launch(dh_hamiltonian* ham, int lanes_per_row, int block_size, int index_mode, int lds_tables);
 
@@ -141,6 +145,18 @@ int dh_hamiltonian_set_launch(dh_hamiltonian* ham, int lanes_per_row, int block_
  * rows [first, first + count) of y are written (count = -1: all rows).
  * k in {1, 2, 4, 8, 16}.  Gather form: every destination row is written once, no atomics. */
 int dh_hv(dh_hamiltonian* ham, const double* d_x, double* d_y, int k, int add, int64_t first, int64_t count);
+
+/* LowLevelMultipleAddMultiply(RealVector* vSources, RealVector* vDestinations, int nbrVectors,
+ * int firstComponent, int nbrComponent) (src/Hamiltonian/AbstractHamiltonian.h:344), as called by
+ * MultipleVectorHamiltonianMultiplyOperation::RawApplyOperation
+ * (src/Architecture/ArchitectureOperation/MultipleVectorHamiltonianMultiplyOperation.cc:427):
+ * y_v = (add ? y_v : 0) + (H + shift) x_v for v < nbr_vectors, each vector a separate device
+ * array of the full dimension (d_x, d_y: host arrays of nbr_vectors device pointers); rows
+ * [first, first + count) of every y_v are written (count = -1: all).  Any nbr_vectors >= 0: groups
+ * of 16, 8, 4 and 2 are interleaved into a [row][k] staging block of the context (grown on demand,
+ * 16 dim k bytes, kept until the context is freed) so that they share every index lookup. */
+int dh_hv_vectors(dh_hamiltonian* ham, const double* const* d_x, double* const* d_y, int nbr_vectors, int add,
+                  int64_t first, int64_t count);
 
 /* AbstractQHEOnSphereHamiltonian::EnableFastMultiplication (FQHE/src/Hamiltonian/
  * AbstractQHEOnSphereHamiltonian.cc:3814): generate every matrix element once into device

@@ -57,6 +57,12 @@ dh_basis* dhh_problem_basis(dhh_problem* problem);
 dh_hamiltonian* dhh_problem_hamiltonian(dhh_problem* problem);
 /* relative angular momenta m of the adjoint channels (sphere problems); returns count */
 int dhh_problem_channels(dhh_problem* problem, int* momenta, int capacity);
+/* MultipleVectorHamiltonianMultiplyOperation over nbr_vectors separate device vectors of the
+ * problem's dimension (src/Architecture/ArchitectureOperation/MultipleVectorHamiltonianMultiplyOperation.cc:427
+ * -> AbstractHamiltonian::LowLevelMultiple[Add]Multiply, src/Hamiltonian/AbstractHamiltonian.h:344):
+ * y_v = (add ? y_v : 0) + H x_v on rows [first, first + count) (count = -1: all rows). */
+int dhh_problem_multiple_multiply(dhh_problem* problem, const double* const* d_x, double* const* d_y,
+                                  int nbr_vectors, int add, int64_t first, int64_t count);
 
 /* Full-reorthogonalisation (block) Lanczos to the nbr_eigen lowest eigenvalues
  * (FullReorthogonalizedLanczosAlgorithm / FullReorthogonalizedBlockLanczosAlgorithm driven
@@ -93,10 +99,15 @@ int dhh_autotune(dhh_problem* problem, const int* ks, int nbr_ks, int repeats, d
  * pseudopotentials, nbr_eigen lowest eigenvalues each: the Lz loop of
  * FQHESphereFermionsTwoBodyGeneric.cc:270 with a (block) full-reorthogonalisation Lanczos
  * per sector.  batched != 0 advances all sectors together (one persistent product launch
- * and segmented vector operations per step, waves within memory_gb of Krylov storage);
+ * and segmented vector operations per step), in waves of sectors, largest first, whose expected
+ * Krylov storage fits the device's free memory and memory_gb (0: no limit but the device's);
  * batched == 0 solves the sectors one after the other like the reference.  Outputs per
  * sector: eigenvalues (nbr_sectors * nbr_eigen), vector products, converged flags; total
  * wall seconds. */
+/* Waves (sector subsets whose Krylov storage fits the device together) of the last batched sector
+ * solve of this thread; 0 after a sequential solve.  No counterpart in the reference. */
+int dhh_last_solve_waves(void);
+
 int dhh_sectors_solve(dhh_arch* arch, int nbr_fermions, int lz_max, const int* total_lz, int nbr_sectors,
                       const double* pseudopotentials, int nbr_eigen, int block_size, int max_iter, double precision,
                       uint64_t seed, int batched, double memory_gb, double* eigenvalues, int* products,

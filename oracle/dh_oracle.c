@@ -944,6 +944,34 @@ long or_fs_row_elements(void* space, void* factors, long row, long* targets, dou
   return n;
 }
 
+/* Gather form over rows [first, last) of a block of k vectors ([row][k] layout):
+ * out[(r - first) k + v] = shift x[r k + v] + sum_t H[r][t] x[t k + v].  The elements of source
+ * row r (or_fs_row_elements: targets t with H[t][r], in the reference loop order of
+ * AbstractQHEOnSphereHamiltonian.cc:431-455) give row r of H because H is real symmetric.  A slice
+ * of a large sector is checked this way without its whole product. */
+void or_fs_hv_rows(void* space, void* factors, const double* x, double* out, double shift, long first, long last,
+                   int k, int nbrThreads) {
+  const or_factors_t* f = (const or_factors_t*)factors;
+  const long cap = f->nbrFactors > 0 ? f->nbrFactors : 1;
+  if (nbrThreads < 1) nbrThreads = 1;
+#pragma omp parallel num_threads(nbrThreads)
+  {
+    long* t = (long*)malloc(sizeof(long) * cap);
+    double* c = (double*)malloc(sizeof(double) * cap);
+#pragma omp for schedule(dynamic, 64)
+    for (long r = first; r < last; ++r) {
+      long n = or_fs_row_elements(space, factors, r, t, c, cap);
+      for (int v = 0; v < k; ++v) {
+        double acc = shift * x[r * k + v];
+        for (long e = 0; e < n; ++e) acc += c[e] * x[t[e] * k + v];
+        out[(r - first) * k + v] = acc;
+      }
+    }
+    free(t);
+    free(c);
+  }
+}
+
 long or_fs_count_elements(void* space, void* factors) {
   fs_space* s = (fs_space*)space;
   long total = 0;
@@ -1290,6 +1318,42 @@ long or_fm_row_elements(void* space, void* factors, long row, long* targets, dou
     }
   }
   return cnt;
+}
+
+/* Gather form over rows [first, last) of a momentum sector for a block of k complex vectors
+ * ([row][k][re, im]): out[((r - first) k + v)] = sum_t H[r][t] x[t][v] with H[r][t] = conj(H[t][r]),
+ * the elements of source row r (or_fm_row_elements, the loop of
+ * ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian.h:370) conjugated because H is
+ * Hermitian. */
+void or_fm_hv_rows(void* space, void* factors, const double* x, double* out, long first, long last, int k,
+                   int nbrThreads) {
+  const fm_factors* f = (const fm_factors*)factors;
+  long cap = 1;
+  for (int j = 0; j < f->nbrSums; ++j) cap += (long)f->nbrPerSum[j] * f->nbrPerSum[j];
+  if (nbrThreads < 1) nbrThreads = 1;
+#pragma omp parallel num_threads(nbrThreads)
+  {
+    long* t = (long*)malloc(sizeof(long) * cap);
+    double* cr = (double*)malloc(sizeof(double) * cap);
+    double* ci = (double*)malloc(sizeof(double) * cap);
+#pragma omp for schedule(dynamic, 64)
+    for (long r = first; r < last; ++r) {
+      long n = or_fm_row_elements(space, factors, r, t, cr, ci, cap);
+      for (int v = 0; v < k; ++v) {
+        double ar = 0.0, ai = 0.0;
+        for (long e = 0; e < n; ++e) {
+          const double xr = x[2 * (t[e] * k + v)], xi = x[2 * (t[e] * k + v) + 1];
+          ar += cr[e] * xr + ci[e] * xi;
+          ai += cr[e] * xi - ci[e] * xr;
+        }
+        out[2 * ((r - first) * k + v)] = ar;
+        out[2 * ((r - first) * k + v) + 1] = ai;
+      }
+    }
+    free(t);
+    free(cr);
+    free(ci);
+  }
 }
 
 /* ======================================================================

@@ -53,6 +53,10 @@ void or_fs_hv_smp(void* space, void* factors, const double* x, double* y, double
                   int hermitian, int nbrThreads);
 long or_fs_row_elements(void* space, void* factors, long row, long* targets, double* coefs, long maxOut);
 long or_fs_count_elements(void* space, void* factors);
+/* rows [first, last) of (H + shift) X for k vectors ([row][k]) in gather form, into
+ * out[0 .. (last - first) k) */
+void or_fs_hv_rows(void* space, void* factors, const double* x, double* out, double shift, long first, long last,
+                   int k, int nbrThreads);
 
 /* ---------------- fermions in momentum space (Chern band) ---------------- */
 void* or_fm_create(int nbrFermions, int nx, int ny, int kx, int ky, unsigned long memory); /* or_fs_* accessors apply */
@@ -64,6 +68,10 @@ int or_fm_factors_sector_size(void* factors, int sum);
 void or_fm_factors_get(void* factors, int sum, int* pairs, double* fre, double* fim);
 void or_fm_hv(void* space, void* factors, const double* x, double* y, int hermitian);
 long or_fm_row_elements(void* space, void* factors, long row, long* targets, double* cre, double* cim, long maxOut);
+/* rows [first, last) of H X for k complex vectors ([row][k][re, im]) in gather form, into
+ * out[0 .. 2 (last - first) k) */
+void or_fm_hv_rows(void* space, void* factors, const double* x, double* out, long first, long last, int k,
+                   int nbrThreads);
 /* torus two-body factors (ParticleOnTorusGenericHamiltonian with pseudopotentials when
  * coulombLevel < 0, ParticleOnTorusCoulombHamiltonian in Landau level coulombLevel otherwise) in
  * the momentum-space factor layout with Nx = 1; the torus basis of Ky is or_fm_create(N, 1, Nphi,

@@ -62,6 +62,7 @@ def lib():
     _sig(L, "or_fs_hv_smp", None, [c_void_p, c_void_p, P_D, P_D, c_double, c_int, c_int])
     _sig(L, "or_fs_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, c_long])
     _sig(L, "or_fs_count_elements", c_long, [c_void_p, c_void_p])
+    _sig(L, "or_fs_hv_rows", None, [c_void_p, c_void_p, P_D, P_D, c_double, c_long, c_long, c_int, c_int])
     _sig(L, "or_spin_create", c_void_p, [c_int, c_int, c_int])
     _sig(L, "or_spin_destroy", None, [c_void_p])
     _sig(L, "or_spin_dim", c_long, [c_void_p])
@@ -200,6 +201,16 @@ def fs_hv(space, factors, x, shift=0.0, hermitian=False, threads=1):
     return y
 
 
+def fs_hv_rows(space, factors, x, first, last, shift=0.0, k=1, threads=8):
+    """rows [first, last) of (H + shift) X for k vectors in [row][k] layout (x of dim * k doubles),
+    gather form (or_fs_hv_rows); returns (last - first) * k doubles"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    assert x.size == space.dim * k
+    out = np.zeros((last - first) * k)
+    lib().or_fs_hv_rows(space.h, factors.h, dptr(x), dptr(out), shift, first, last, k, threads)
+    return out
+
+
 def nbody_tuples(nbr_orbitals, nbody):
     """Orbital tuples (descending within a tuple) grouped by orbital sum, in the order of
     AbstractQHEOnSphereNBodyInteractionHamiltonian::GetAllSkewSymmetricIndices (:1600):
@@ -312,6 +323,7 @@ def _bind_momentum(L):
     _sig(L, "or_fm_factors_get", None, [c_void_p, c_int, P_I, P_D, P_D])
     _sig(L, "or_fm_hv", None, [c_void_p, c_void_p, P_D, P_D, c_int])
     _sig(L, "or_fm_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, P_D, c_long])
+    _sig(L, "or_fm_hv_rows", None, [c_void_p, c_void_p, P_D, P_D, c_long, c_long, c_int, c_int])
     _sig(L, "or_torus_factors_create", c_void_p, [c_int, c_double, c_int, P_D, c_int])
     _sig(L, "or_ftt_create", c_void_p, [c_int, c_int, c_int, c_int])
     _sig(L, "or_ftt_destroy", None, [c_void_p])
@@ -388,6 +400,19 @@ def fm_hv(space, factors, x, hermitian=False):
     y = np.zeros(2 * space.dim)
     lib().or_fm_hv(space.h, factors.h, dptr(xr), dptr(y), int(hermitian))
     return y[0::2] + 1j * y[1::2]
+
+
+def fm_hv_rows(space, factors, x, first, last, k=1, threads=8):
+    """rows [first, last) of H X for k complex vectors (x: complex128 of dim * k, [row][k]),
+    gather form (or_fm_hv_rows); returns (last - first) * k complex values"""
+    L = _bind_momentum(lib())
+    x = np.asarray(x)
+    assert x.size == space.dim * k
+    xr = np.empty(2 * x.size)
+    xr[0::2], xr[1::2] = np.real(x).ravel(), np.imag(x).ravel()
+    out = np.zeros(2 * (last - first) * k)
+    L.or_fm_hv_rows(space.h, factors.h, dptr(xr), dptr(out), first, last, k, threads)
+    return out[0::2] + 1j * out[1::2]
 
 
 def fm_row_elements(space, factors, row, max_out=20000):

@@ -289,3 +289,57 @@ def test_laughlin_zero_mode_converges(arch, block):
     assert abs(r["eigenvalues"][0]) < 1e-12
     np.testing.assert_allclose(r["eigenvalues"], ev_ref, rtol=0, atol=1e-10)
     p.close()
+
+
+@pytest.mark.parametrize("nbr", [1, 3, 8, 13, 19])
+@pytest.mark.parametrize("via_host", [False, True])
+def test_hv_over_separate_vectors(arch, n8_2s21, nbr, via_host):
+    """LowLevelMultipleAddMultiply over an array of separate vectors (dh_hv_vectors, and the host
+    layer's MultipleVectorHamiltonianMultiplyOperation): every vector matches the oracle, a group of
+    eight is bitwise the interleaved eight-vector product, and add / row ranges keep their meaning."""
+    from diagham_amd import capi
+    p = n8_2s21
+    ref_space, fac = O.FermionSphere(8, 0, 21), O.Factors(21, coulomb(21))
+    D = p.dim
+    xs = [O.splitmix(500 + v, D) for v in range(nbr)]
+    dx = [capi.DeviceBuffer.from_host(arch, x) for x in xs]
+    dy = [capi.DeviceBuffer.from_host(arch, np.full(D, 0.5)) for _ in range(nbr)]
+    p.hv_vectors_device([b.ptr for b in dx], [b.ptr for b in dy], via_host=via_host)
+    ys = [b.download() for b in dy]
+    for v in range(nbr):
+        y_ref = O.fs_hv(ref_space, fac, xs[v])
+        assert np.linalg.norm(ys[v] - y_ref) <= 1e-12 * np.linalg.norm(y_ref), v
+    if nbr >= 8:
+        X = np.stack(xs[:8], axis=1).reshape(-1)
+        Y = p.hv(X, 8).reshape(D, 8)
+        for v in range(8):
+            assert np.array_equal(ys[v], Y[:, v])
+    # add on a row range: rows outside keep their values
+    a, b = 100, D - 300
+    p.hv_vectors_device([b_.ptr for b_ in dx], [b_.ptr for b_ in dy], add=True, first=a, count=b - a,
+                        via_host=via_host)
+    for v in range(nbr):
+        y2 = dy[v].download()
+        assert np.array_equal(y2[:a], ys[v][:a]) and np.array_equal(y2[b:], ys[v][b:])
+        assert np.allclose(y2[a:b], 2 * ys[v][a:b], rtol=1e-13, atol=0)
+    for buf in dx + dy:
+        buf.free()
+
+
+def test_hv_over_separate_vectors_spin(arch):
+    from diagham_amd import capi
+    L = 16
+    p = capi.Problem.spin_chain(arch, L, 0)
+    ref = O.SpinChain(L, 0)
+    J = np.ones(L)
+    xs = [O.splitmix(600 + v, p.dim) for v in range(6)]
+    dx = [capi.DeviceBuffer.from_host(arch, x) for x in xs]
+    dy = [capi.DeviceBuffer(arch, p.dim) for _ in range(6)]
+    p.hv_vectors_device([b.ptr for b in dx], [b.ptr for b in dy])
+    for v in range(6):
+        y_ref = np.zeros(p.dim)
+        O.lib().or_spin_hv(ref.h, O.dptr(J), O.dptr(J), 1, O.dptr(xs[v]), O.dptr(y_ref))
+        assert np.linalg.norm(dy[v].download() - y_ref) <= 1e-12 * np.linalg.norm(y_ref)
+    for buf in dx + dy:
+        buf.free()
+    p.close()

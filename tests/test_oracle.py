@@ -287,6 +287,24 @@ def test_hermitian_and_smp_forms_agree():
     assert abs(x2 @ y - x @ O.fs_hv(s, f, x2)) < 1e-10 * abs(x2 @ y)
 
 
+def test_row_gather_form_equals_scatter():
+    """or_fs_hv_rows (each row gathers its own elements, used to check slices of large sectors)
+    equals the reference's scatter loop, whole and on a slice, with a shift."""
+    s = O.FermionSphere(8, 0, 21)
+    f = O.Factors(21, coulomb(21))
+    x = O.splitmix(5, s.dim)
+    y = O.fs_hv(s, f, x, shift=0.75)
+    assert np.linalg.norm(O.fs_hv_rows(s, f, x, 0, s.dim, shift=0.75) - y) <= 1e-13 * np.linalg.norm(y)
+    a, b = 1234, 4321
+    assert np.linalg.norm(O.fs_hv_rows(s, f, x, a, b, shift=0.75) - y[a:b]) <= 1e-13 * np.linalg.norm(y[a:b])
+    # three vectors in [row][k] layout: each column is the one-vector product
+    X = np.stack([x, O.splitmix(6, s.dim), O.splitmix(7, s.dim)], axis=1)
+    Y = O.fs_hv_rows(s, f, X.ravel(), a, b, shift=0.75, k=3).reshape(-1, 3)
+    for v in range(3):
+        yv = O.fs_hv(s, f, X[:, v], shift=0.75)[a:b]
+        assert np.linalg.norm(Y[:, v] - yv) <= 1e-13 * np.linalg.norm(yv)
+
+
 def test_lanczos_matches_dense_small():
     n, tlz, lz = 5, 1, 11
     s = O.FermionSphere(n, tlz, lz)

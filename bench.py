@@ -141,32 +141,75 @@ def lanczos_runs(capi, arch, report=None):
     return res
 
 
-def sector_runs(capi, arch, report=None, n=10, lz=27, nev=10, block=8):
+def sector_runs(capi, arch, report=None, n=10, lz=27, nev=10, legs=(("plain_sequential_k1", 1, False),
+                                                                     ("batched_k1", 1, True),
+                                                                     ("batched_block_k8", 8, True))):
     """Time to the nev lowest eigenvalues of every Lz >= 0 sector of n Coulomb fermions at flux lz:
-    the plain one-vector Lanczos run sector after sector (the reference's loop) against the batched
-    block Lanczos that advances all sectors together, and the largest eigenvalue difference."""
+    the plain one-vector Lanczos run sector after sector (the reference's loop,
+    FQHESphereFermionsTwoBodyGeneric.cc:270) against the batched Lanczos that advances all sectors
+    together, with the waves the device memory needed, the device memory peak and the largest
+    eigenvalue difference from the plain run."""
     pp = capi.coulomb_pseudopotentials(lz, True)
     lzs = capi.sphere_sectors(n, lz)
     dims = [capi.fermion_sphere_dimension(n, s, lz) for s in lzs]
     runs = {}
-    for name, k, batched in (("plain_sequential_k1", 1, False), ("batched_k1", 1, True),
-                             ("batched_block_k%d" % block, block, True)):
+    for name, k, batched in legs:
+        arch.memory_peak(reset=True)
         r = capi.sectors_solve(arch, n, lz, lzs, pp, nev, k, batched=batched, allow_unconverged=True)
+        r["memory_peak"] = arch.memory_peak()
         runs[name] = r
         if report:
-            report("all sectors, %s: %.2f s" % (name, r["seconds"]))
-    ref = runs["plain_sequential_k1"]["eigenvalues"]
-    out = {"workload": "fermions_sphere_n%d_2s%d_coulomb_all_Lz_ge_0_sectors_%d_eigenvalues" % (n, lz, nev),
-           "sectors": len(lzs), "states": int(sum(dims)), "largest_sector": int(max(dims))}
+            report("sectors N=%d 2S=%d, %s: %.2f s, %d waves" % (n, lz, name, r["seconds"], r["waves"]))
+    return sector_summary("fermions_sphere_n%d_2s%d_coulomb_all_Lz_ge_0_sectors_%d_eigenvalues" % (n, lz, nev),
+                          runs, legs[0][0], len(lzs), dims)
+
+
+def sector_summary(workload, runs, plain, nbr_sectors, dims):
+    ref = runs[plain]["eigenvalues"]
+    out = {"workload": workload, "sectors": nbr_sectors, "states": int(sum(dims)), "largest_sector": int(max(dims))}
     for name, r in runs.items():
-        out[name] = {"seconds": r["seconds"], "products": int(r["products"].sum()),
-                     "converged": int(r["converged"].sum()),
+        out[name] = {"seconds": r["seconds"], "products": int(np.sum(r["products"])),
+                     "converged": int(np.sum(r["converged"])), "waves": r.get("waves", 0),
+                     "device_memory_peak_bytes": r.get("memory_peak"),
                      "max_abs_eigenvalue_difference_vs_plain": float(np.max(np.abs(r["eigenvalues"] - ref)))}
-    t0 = runs["plain_sequential_k1"]["seconds"]
+    t0 = runs[plain]["seconds"]
     for name in runs:
-        if name != "plain_sequential_k1":
+        if name != plain:
             out[name]["speedup_vs_plain"] = t0 / runs[name]["seconds"]
     return out
+
+
+def lattice_runs(capi, arch, report=None, n=10, nx=6, ny=5, nev=5, block=4):
+    """Time to the nev lowest eigenvalues of all nx * ny (kx, ky) sectors of n fermions in the
+    projected checkerboard band: the plain one-vector complex Lanczos sector after sector (the loop
+    of FCICheckerboardLatticeModel.cc:412) against the batched complex block Lanczos over all
+    sectors."""
+    kx = [x for x in range(nx) for y in range(ny)]
+    ky = [y for x in range(nx) for y in range(ny)]
+    runs = {}
+    arch.memory_peak(reset=True)
+    t0 = time.perf_counter()
+    ev, prods, conv = [], [], []
+    for x, y in zip(kx, ky):
+        r = capi.lattice_sectors_solve(arch, n, nx, ny, [x], [y], nev, 1, allow_unconverged=True)
+        ev.append(r["eigenvalues"][0])
+        prods.append(int(r["products"][0]))
+        conv.append(bool(r["converged"][0]))
+    runs["plain_sequential_k1"] = {"eigenvalues": np.array(ev), "products": np.array(prods),
+                                   "converged": np.array(conv), "seconds": time.perf_counter() - t0, "waves": 0,
+                                   "memory_peak": arch.memory_peak()}
+    if report:
+        report("lattice %dx%d N=%d, plain_sequential_k1: %.2f s" % (nx, ny, n, runs["plain_sequential_k1"]["seconds"]))
+    for name, k in (("batched_k1", 1), ("batched_block_k%d" % block, block)):
+        arch.memory_peak(reset=True)
+        r = capi.lattice_sectors_solve(arch, n, nx, ny, kx, ky, nev, k, allow_unconverged=True)
+        r["memory_peak"] = arch.memory_peak()
+        runs[name] = r
+        if report:
+            report("lattice %dx%d N=%d, %s: %.2f s, %d waves" % (nx, ny, n, name, r["seconds"], r["waves"]))
+    dims = capi.momentum_sector_dims(arch, n, nx, ny, kx, ky)
+    return sector_summary("checkerboard_%dx%d_n%d_t1_1_t2_0.5_tpp_-0.5_V1_flat_all_%d_sectors_%d_eigenvalues" %
+                          (nx, ny, n, nx * ny, nev), runs, "plain_sequential_k1", nx * ny, dims)
 
 
 def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
@@ -202,16 +245,82 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
             "measured_traffic_GBps": traffic and traffic["bytes_per_launch"] / (ms * 1e-3) / 1e9}
 
 
+def kernel_only(capi, arch, stream, name, steps, warmup):
+    """One named product repeated `steps` times after `warmup` (for rocprofv3 runs): prints a JSON
+    line with its mean time and rates."""
+    k = int(name.rsplit("_k", 1)[1]) if "_k" in name else 1
+    timer = HipTimer(stream)
+    cleanup = []
+    free = "_free" in name  # the free-pair row routine (DH_INDEX_RANK_FREE)
+    lpr = int(name.split("_lpr")[1].split("_")[0]) if "_lpr" in name else None
+    if name.startswith("sphere_n12_2s33_lz0"):
+        p = capi.Problem.sphere(arch, 12, 0, 33, capi.coulomb_pseudopotentials(33, True))
+        cleanup.append(p.close)
+        p.set_launch(lpr or (1 if k == 1 else 2), 1024 if k >= 4 else 256,
+                     capi.INDEX_RANK_FREE if free else capi.INDEX_RANK, 1)
+        rows, nnz = p.dim, p.count_elements()
+        run = lambda dx, dy: p.hv_device(dx, dy, k)  # noqa: E731
+        width = k
+    elif name.startswith("sectors_n12_2s33"):
+        S = capi.SphereSectorSet(arch, 12, 33, capi.sphere_sectors(12, 33), capi.coulomb_pseudopotentials(33, True))
+        cleanup.append(S.close)
+        S.set_launch(lpr or 2, 1024) if k >= 4 else S.set_launch(lpr or 1, 256)
+        if free:
+            S.set_row_mode(capi.INDEX_RANK_FREE)
+        rows, nnz = S.total, int(S.count_elements().sum())
+        run = lambda dx, dy: S.hv_device(dx, dy, k)  # noqa: E731
+        width = k
+    elif name.startswith("lattice_6x5_n10"):
+        from_lattice = capi.lattice_set(arch, 10, 6, 5, [x for x in range(6) for y in range(5)],
+                                        [y for x in range(6) for y in range(5)])
+        cleanup.append(from_lattice.close)
+        from_lattice.set_launch(2, 512)
+        rows, nnz = from_lattice.total, None
+        run = lambda dx, dy: from_lattice.hv_device(dx, dy, k)  # noqa: E731
+        width = 2 * k
+    else:
+        raise SystemExit("unknown kernel workload " + name)
+    x = capi.DeviceBuffer(arch, rows * width)
+    y = capi.DeviceBuffer(arch, rows * width)
+    capi.gpu().dh_fill_random(arch.ctx, x.ptr, rows * width, 0x5EED0007)
+    for _ in range(warmup):
+        run(x.ptr, y.ptr)
+    arch.sync()
+    timer.start()
+    for _ in range(steps):
+        run(x.ptr, y.ptr)
+    ms = timer.stop_ms() / steps
+    x.free()
+    y.free()
+    for f in cleanup:
+        f()
+    out = {"kernel_workload": name, "rows": rows, "k": k, "ms": ms}
+    if nnz:
+        alg = rows * (8 + 8 * width) + nnz * 8 * width
+        out.update({"matrix_elements": nnz, "element_vector_products_per_s": nnz * k / (ms * 1e-3),
+                    "algorithmic_GBps": alg / (ms * 1e-3) / 1e9})
+    print(json.dumps(out))
+
+
+def cpu_threads():
+    """Threads for the CPU legs: every core this process may run on, unless OMP_NUM_THREADS sets the
+    host share (on a shared GPU host the share, not the machine, is what one GPU's job has)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(share)) if share and share.isdigit() and int(share) > 0 else affinity
+    return max(1, threads), {"nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": share}
+
+
 def cpu_baseline(L=28, seconds_target=15.0):
     """Oracle (CPU restatement of SpinChainHamiltonian::LowLevelAddMultiply, reference thread
-    split) on the host cores, over a bounded row sample of the same L = 28 product."""
+    split) on the host cores, over a bounded row sample of the same L = 28 product; beside it the
+    oracle's sphere and lattice products and a short Lanczos solve."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores, host = cpu_threads()
     s = O.SpinChain(L, 0)
     J = np.ones(L)
     x = O.splitmix(1, s.dim)
@@ -237,9 +346,37 @@ def cpu_baseline(L=28, seconds_target=15.0):
     rot = ((states >> 1) | ((states & 1) << (L - 1)))
     anti = np.unpackbits((states ^ rot).astype(">u4").view(np.uint8)).sum()
     elements = (int(anti) + rows) * passes  # off-diagonal flips + diagonal, per pass
-    return {"value": elements / dt, "unit": "matrix_elements/s", "cores": cores, "kind": "port",
-            "sample": "oracle H.v over rows [0, %d) of the L=28 basis, %d pass(es) (%d matrix elements, %.1f s)" %
-                      (rows, passes, elements, dt)}
+    del s, x, y
+    out = {"value": elements / dt, "unit": "matrix_elements/s", "cores": cores, "kind": "port",
+           "sample": "oracle H.v over rows [0, %d) of the L=28 basis, %d pass(es) (%d matrix elements, %.1f s)" %
+                     (rows, passes, elements, dt), "host": host}
+    # sphere: one whole product of N = 10 fermions at 2S = 27, Lz = 0 (reference SMP split)
+    sp = O.FermionSphere(10, 0, 27)
+    fac = O.Factors(27, O.round14(O.coulomb_pseudopotentials(27)))
+    xs = O.splitmix(2, sp.dim)
+    t0 = time.perf_counter()
+    O.fs_hv(sp, fac, xs, threads=cores)
+    dt = time.perf_counter() - t0
+    nnz = 57096228  # matrix elements of that product (dh_hamiltonian_count_elements)
+    out["sphere_n10_2s27_hv"] = {"value": nnz / dt, "unit": "matrix_elements/s", "seconds": dt,
+                                 "sample": "one oracle product over all 246,448 rows"}
+    # lattice: rows [0, 100000) of sector (0, 0) of N = 10 on 6 x 5 sites, one complex vector
+    u, _ = O.checkerboard_lower_band(6, 5, 1.0, 0.5, -0.5)
+    ms = O.MomentumFermions(10, 6, 5, 0, 0)
+    lf = O.LatticeFactors(6, 5, u)
+    xl = O.splitmix(3, ms.dim) + 1j * O.splitmix(4, ms.dim)
+    t0 = time.perf_counter()
+    O.fm_hv_rows(ms, lf, xl, 0, 100000, threads=cores)
+    dt = time.perf_counter() - t0
+    out["lattice_6x5_n10_rows"] = {"value": 100000 / dt, "unit": "rows/s", "seconds": dt,
+                                   "sample": "oracle gather rows [0, 100000) of sector (0, 0), one complex vector"}
+    # time to the 10 lowest eigenvalues of N = 8 at 2S = 21 (oracle full-reorthogonalisation Lanczos)
+    s8 = O.FermionSphere(8, 0, 21)
+    f8 = O.Factors(21, O.round14(O.coulomb_pseudopotentials(21)))
+    t0 = time.perf_counter()
+    ev, it = O.fs_lanczos(s8, f8, 10, threads=cores)
+    out["lanczos_n8_2s21_10_eigenvalues"] = {"seconds": time.perf_counter() - t0, "iterations": int(it)}
+    return out
 
 
 def main():
@@ -250,7 +387,13 @@ def main():
     ap.add_argument("--skip-lanczos", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-n10", action="store_true", help="skip the all-sector legs of N=10 at 2S=27")
+    ap.add_argument("--skip-n12", action="store_true", help="skip the all-sector legs of N=12 at 2S=33")
+    ap.add_argument("--skip-lattice", action="store_true", help="skip the all-sector legs of N=10 on 6x5 sites")
     ap.add_argument("--length", type=int, default=28)
+    ap.add_argument("--kernel", default=None,
+                    help="only repeat one product (sphere_n12_2s33_lz0_k1, sectors_n12_2s33_k8, "
+                         "lattice_6x5_n10_k4, ...) for profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
@@ -263,6 +406,10 @@ def main():
     arch.check(capi.gpu().dh_set_stream(arch.ctx, stream), "dh_set_stream")
     ncu, name = arch.device_info()
 
+    if args.kernel:
+        kernel_only(capi, arch, stream, args.kernel, args.steps, args.warmup)
+        arch.close()
+        return
     hv = spin_hv_bench(capi, arch, stream, args.length, args.steps, args.warmup, k=1)
     dim, nnz, ms = hv["dim"], hv["nnz"], hv["ms_per_step"]
     alg_bytes = dim * (8 + 8) + nnz * 8  # state word + y write per row, one x read per element
@@ -301,7 +448,14 @@ def main():
     progress("headline product: %.3f ms" % ms)
     if not args.skip_lanczos:
         line["lanczos"] = lanczos_runs(capi, arch, progress)
-        line["lanczos"]["all_sectors"] = sector_runs(capi, arch, progress)
+        if not args.skip_n12:
+            line["lanczos"]["sectors_n12_2s33"] = sector_runs(
+                capi, arch, progress, n=12, lz=33, nev=10,
+                legs=(("plain_sequential_k1", 1, False), ("batched_block_k8", 8, True)))
+        if not args.skip_lattice:
+            line["lanczos"]["lattice_6x5_n10"] = lattice_runs(capi, arch, progress)
+        if not args.skip_n10:
+            line["lanczos"]["sectors_n10_2s27"] = sector_runs(capi, arch, progress)
     if not args.skip_batched:
         line["batched_roofline"] = batched_hv_roofline(capi, arch, stream)
         progress("batched product: %.1f ms" % line["batched_roofline"]["ms"])

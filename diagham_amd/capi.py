@@ -16,7 +16,7 @@ LIB_GPU = os.path.join(_HERE, "libdhgpu.so")
 LIB_HOST = os.path.join(_HERE, "libdiagham_host.so")
 
 DH_OK, DH_E_ARG, DH_E_HIP, DH_E_NOMEM, DH_E_STATE, DH_E_RANGE = 0, -1, -2, -3, -4, -5
-INDEX_BSEARCH, INDEX_RANK, INDEX_RANK_X4 = 0, 1, 2
+INDEX_BSEARCH, INDEX_RANK, INDEX_RANK_X4, INDEX_RANK_FREE = 0, 1, 2, 3
 
 c_int, c_int64, c_double, c_void_p, c_uint64, c_size_t, c_char_p = (
     ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t,
@@ -74,6 +74,7 @@ GPU_SIGNATURES = {
     "dh_sector_set_get_states": (c_int, [c_void_p, P_U64]),
     "dh_sector_set_factors": (c_int, [c_void_p, P_D, P_I, c_int, c_double]),
     "dh_sector_set_launch": (c_int, [c_void_p, c_int, c_int]),
+    "dh_sector_set_row_mode": (c_int, [c_void_p, c_int]),
     "dh_sector_set_blas_launch": (c_int, [c_void_p, c_int, c_int64]),
     "dh_sector_set_count_elements": (c_int, [c_void_p, P_I64]),
     "dh_sectors_hv": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_char_p]),
@@ -99,6 +100,9 @@ GPU_SIGNATURES = {
 HOST_SIGNATURES = {
     "dhh_last_error": (c_char_p, []),
     "dhh_last_solve_waves": (c_int, []),
+    "dhh_sectors_launch": (c_int, [c_int, c_int, c_int]),
+    "dhh_sectors_autotune": (c_int, [c_void_p, c_int, c_int, P_I, c_int, P_D, c_int, P_I, c_int, c_double, c_int,
+                                     c_uint64, P_D, P_I]),
     "dhh_arch_create": (c_int, [PP, c_int]),
     "dhh_arch_destroy": (c_int, [c_void_p]),
     "dhh_arch_context": (c_void_p, [c_void_p]),
@@ -582,6 +586,9 @@ class SphereSectorSet:
     def set_launch(self, lanes_per_row=1, block_size=256):
         self.arch.check(gpu().dh_sector_set_launch(self.h, lanes_per_row, block_size), "dh_sector_set_launch")
 
+    def set_row_mode(self, mode):
+        self.arch.check(gpu().dh_sector_set_row_mode(self.h, mode), "dh_sector_set_row_mode")
+
     def set_blas_launch(self, axpy_groups_per_cu=16, dots_rows_per_tile=0):
         self.arch.check(gpu().dh_sector_set_blas_launch(self.h, axpy_groups_per_cu, dots_rows_per_tile),
                         "dh_sector_set_blas_launch")
@@ -605,6 +612,68 @@ class SphereSectorSet:
             self.close()
         except Exception:
             pass
+
+
+def checkerboard_factors(nx, ny, t1=1.0, t2=0.5, tpp=-0.5, mus=0.0, u_potential=1.0, v_potential=0.0,
+                         flat_band=True):
+    """dhh_checkerboard_factors: (sum_pair_count, sum_pairs, complex factors interleaved) of the
+    projected checkerboard model in the momentum sector set layout"""
+    npairs = nx * ny * (nx * ny - 1) // 2
+    cnt = np.zeros(nx * ny, np.int32)
+    pairs = np.zeros(2 * npairs, np.int32)
+    nf = ctypes.c_int64(npairs * npairs)
+    npr = ctypes.c_int(npairs)
+    f = np.zeros(2 * npairs * npairs)
+    rc = host().dhh_checkerboard_factors(nx, ny, t1, t2, tpp, mus, u_potential, v_potential, int(flat_band),
+                                         cnt.ctypes.data_as(P_I), pairs.ctypes.data_as(P_I), ctypes.byref(npr),
+                                         dptr(f), ctypes.byref(nf))
+    if rc != 0:
+        raise DHError("dhh_checkerboard_factors failed (%d)" % rc)
+    return cnt, pairs[:2 * npr.value], f[:2 * nf.value]
+
+
+class MomentumSectorSet:
+    """dh_sector_set of fermions in momentum space (the (kx[i], ky[i]) sectors concatenated) with
+    the projected checkerboard factors; vectors are complex, (re, im) interleaved"""
+
+    def __init__(self, arch, n, nx, ny, kx, ky, **model):
+        self.arch = arch
+        kxa = np.ascontiguousarray(kx, dtype=np.int32)
+        kya = np.ascontiguousarray(ky, dtype=np.int32)
+        h = ctypes.c_void_p()
+        arch.check(gpu().dh_momentum_sector_set_create(arch.ctx, ctypes.byref(h), n, nx, ny, kxa.ctypes.data_as(P_I),
+                                                       kya.ctypes.data_as(P_I), kxa.size), "momentum sector set")
+        self.h = h
+        arch.track(self)
+        self.nbr_sectors = kxa.size
+        self.offsets = np.zeros(kxa.size + 1, np.int64)
+        gpu().dh_sector_set_offsets(h, self.offsets.ctypes.data_as(P_I64))
+        self.total = int(self.offsets[-1])
+        cnt, pairs, f = checkerboard_factors(nx, ny, **model)
+        arch.check(gpu().dh_sector_set_lattice_factors(h, nx * ny, cnt.ctypes.data_as(P_I), pairs.ctypes.data_as(P_I),
+                                                       dptr(f), 0.0), "lattice factors")
+
+    def set_launch(self, lanes_per_row=1, block_size=256):
+        self.arch.check(gpu().dh_sector_set_launch(self.h, lanes_per_row, block_size), "dh_sector_set_launch")
+
+    def hv_device(self, d_x, d_y, k=1, add=False, active=None):
+        am = None if active is None else bytes(np.asarray(active, np.uint8))
+        self.arch.check(gpu().dh_sectors_hv(self.h, d_x, d_y, k, int(add), am), "dh_sectors_hv")
+
+    def close(self):
+        if getattr(self, "h", None):
+            gpu().dh_sector_set_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lattice_set(arch, n, nx, ny, kx, ky, **model):
+    return MomentumSectorSet(arch, n, nx, ny, kx, ky, **model)
 
 
 def sphere_sectors(n, lz_max, positive_only=True):
@@ -655,6 +724,46 @@ def lattice_sectors_solve(arch, n, nx, ny, kx, ky, nbr_eigen, block_size=1, t1=1
         raise DHError("dhh_lattice_sectors_solve failed (%d): %s" % (rc, host().dhh_last_error().decode()))
     return {"eigenvalues": ev.reshape(ns, nbr_eigen), "products": prods, "converged": conv.astype(bool),
             "seconds": secs.value, "waves": host().dhh_last_solve_waves()}
+
+
+def momentum_sector_dims(arch, n, nx, ny, kx, ky):
+    """dimensions of the (kx[i], ky[i]) sectors of n fermions in momentum space on nx x ny sites"""
+    kxa = np.ascontiguousarray(kx, dtype=np.int32)
+    kya = np.ascontiguousarray(ky, dtype=np.int32)
+    h = ctypes.c_void_p()
+    arch.check(gpu().dh_momentum_sector_set_create(arch.ctx, ctypes.byref(h), n, nx, ny, kxa.ctypes.data_as(P_I),
+                                                   kya.ctypes.data_as(P_I), kxa.size), "momentum sector set")
+    off = np.zeros(kxa.size + 1, np.int64)
+    gpu().dh_sector_set_offsets(h, off.ctypes.data_as(P_I64))
+    gpu().dh_sector_set_destroy(h)
+    return np.diff(off).tolist()
+
+
+def sectors_launch(lanes_per_row=0, block_size=256, row_mode=INDEX_RANK):
+    """launch shape of later batched sector solves of this thread (dhh_sectors_launch; 0: built-in)"""
+    rc = host().dhh_sectors_launch(lanes_per_row, block_size, row_mode)
+    if rc != 0:
+        raise DHError("dhh_sectors_launch: " + host().dhh_last_error().decode())
+
+
+def sectors_autotune(arch, n, lz_max, total_lz, pseudo, nbr_eigen, ks=(1, 2, 4, 8, 16), fraction=0.05,
+                     max_iter=3000, seed=0x5EED0000):
+    """dhh_sectors_autotune: per k the fastest product launch on calibration sectors, then the
+    time of a calibration solve with it; returns (rows of the table, best shape dict)"""
+    lz = np.ascontiguousarray(total_lz, dtype=np.int32)
+    ka = np.ascontiguousarray(ks, dtype=np.int32)
+    pp = np.ascontiguousarray(pseudo, dtype=np.float64)
+    table = np.zeros(7 * ka.size)
+    best = np.zeros(4, np.int32)
+    rc = host().dhh_sectors_autotune(arch.h, n, lz_max, lz.ctypes.data_as(P_I), lz.size, dptr(pp), nbr_eigen,
+                                     ka.ctypes.data_as(P_I), ka.size, fraction, max_iter, seed, dptr(table),
+                                     best.ctypes.data_as(P_I))
+    if rc != 0:
+        raise DHError("dhh_sectors_autotune: " + host().dhh_last_error().decode())
+    keys = ("k", "lanes_per_row", "block_size", "row_mode", "product_ms", "calibration_seconds",
+            "calibration_products")
+    rows = [dict(zip(keys, [float(v) for v in r])) for r in table.reshape(-1, 7)]
+    return rows, dict(zip(("k", "lanes_per_row", "block_size", "row_mode"), [int(v) for v in best]))
 
 
 def torus_factors(nbr_flux, ratio=1.0, pseudo=(), coulomb_level=-1):

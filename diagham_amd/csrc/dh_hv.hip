@@ -20,12 +20,27 @@
 #include "dh_internal.h"
 #include "dh_sphere_row.h"
 
-// index lookup of the product kernels; the _X4 forms unroll the created-pair loop four-wide
-enum { MODE_BSEARCH = 0, MODE_RANK_LDS = 1, MODE_RANK_GLOBAL = 2, MODE_RANK_LDS_X4 = 3, MODE_RANK_GLOBAL_X4 = 4 };
+// index lookup of the product kernels; the _X4 forms unroll the created-pair loop four-wide, the
+// _FREE forms iterate over the free created pairs only (dh_sphere_row_free)
+enum {
+  MODE_BSEARCH = 0,
+  MODE_RANK_LDS = 1,
+  MODE_RANK_GLOBAL = 2,
+  MODE_RANK_LDS_X4 = 3,
+  MODE_RANK_GLOBAL_X4 = 4,
+  MODE_RANK_LDS_FREE = 5,
+  MODE_RANK_GLOBAL_FREE = 6
+};
 
 template <int MODE>
 struct ModeBase {
-  static constexpr int value = MODE >= MODE_RANK_LDS_X4 ? MODE - 2 : MODE;
+  static constexpr int value = MODE >= MODE_RANK_LDS_FREE ? MODE - 4 : (MODE >= MODE_RANK_LDS_X4 ? MODE - 2 : MODE);
+};
+
+// terms in flight per lane in the free-pair iterator
+template <int K>
+struct FreeBatch {
+  static constexpr int value = K == 1 ? 4 : (K <= 4 ? 2 : 1);
 };
 
 struct SphereArgs {
@@ -120,13 +135,18 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
       const int bits = a.rankBits;
       // clamped so that a target outside the space can never address outside the vectors
       auto find = [&](uint32_t t) -> uint32_t { return min(last - sLo[t & mask] - sHi[t >> bits], last); };
-      if constexpr (MODE >= MODE_RANK_LDS_X4)
+      if constexpr (MODE >= MODE_RANK_LDS_FREE)
+        dh_sphere_row_free<K, LPR, FreeBatch<K>::value>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x,
+                                                        (uint32_t)row, acc);
+      else if constexpr (MODE >= MODE_RANK_LDS_X4)
         dh_sphere_row_x4<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
       else
         dh_sphere_row<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
     } else {
       auto find = [&](uint64_t t) -> int64_t { return sphere_find<BASE>(a, t, sLut, sLo, sHi); };
-      if constexpr (MODE >= MODE_RANK_LDS_X4)
+      if constexpr (MODE >= MODE_RANK_LDS_FREE)
+        dh_sphere_row_free<K, LPR, FreeBatch<K>::value>(s, a.nbrOrbitals, sF, sBase, lane, find, x, row, acc);
+      else if constexpr (MODE >= MODE_RANK_LDS_X4)
         dh_sphere_row_x4<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
       else
         dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
@@ -429,6 +449,10 @@ static int sphere_dispatch_mode(dh_hamiltonian* h, const double* x, double* y, i
   if (h->indexMode == DH_INDEX_RANK_X4) {
     if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_X4>(h, x, y, add, first, count);
     return sphere_launch<K, LPR, MODE_RANK_GLOBAL_X4>(h, x, y, add, first, count);
+  }
+  if (h->indexMode == DH_INDEX_RANK_FREE) {
+    if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_FREE>(h, x, y, add, first, count);
+    return sphere_launch<K, LPR, MODE_RANK_GLOBAL_FREE>(h, x, y, add, first, count);
   }
   if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS>(h, x, y, add, first, count);
   return sphere_launch<K, LPR, MODE_RANK_GLOBAL>(h, x, y, add, first, count);
@@ -766,7 +790,7 @@ extern "C" int dh_hamiltonian_set_launch(dh_hamiltonian* h, int lpr, int blockSi
   if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8) return dh_fail(h->ctx, DH_E_ARG, "lanes_per_row in {1,2,4,8}");
   if (h->kind == DH_HAM_SPIN && lpr != 1) return dh_fail(h->ctx, DH_E_ARG, "spin kernels use one row per lane");
   if (blockSize < 64 || blockSize > 1024 || (blockSize % 64)) return dh_fail(h->ctx, DH_E_ARG, "block_size");
-  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_X4)
+  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_X4 && mode != DH_INDEX_RANK_FREE)
     return dh_fail(h->ctx, DH_E_ARG, "index_mode");
   h->lanesPerRow = lpr;
   h->blockSize = blockSize;

@@ -188,6 +188,7 @@ struct dh_sector_set {
   double* d_phase = nullptr;   // per sector `modulo` (cos, sin) pairs
   int lutBits = 0, modulo = 1, stateShift = 0;
   int lanesPerRow = 1, blockSize = 256;
+  int rowMode = DH_INDEX_RANK;  // sphere sets: DH_INDEX_RANK or DH_INDEX_RANK_FREE row routine
   // segmented BLAS launch shape: persistent axpy workgroups per CU, dots rows per tile (0: auto)
   int axpyGroupsPerCU = 16;
   int64_t dotsRowsPerTile = 0;

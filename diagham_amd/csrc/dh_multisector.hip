@@ -235,6 +235,14 @@ extern "C" int dh_sector_set_launch(dh_sector_set* S, int lanesPerRow, int block
   return DH_OK;
 }
 
+extern "C" int dh_sector_set_row_mode(dh_sector_set* S, int mode) {
+  if (!S) return DH_E_ARG;
+  if (mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_FREE)
+    return dh_fail(S->ctx, DH_E_ARG, "row mode must be DH_INDEX_RANK or DH_INDEX_RANK_FREE");
+  S->rowMode = mode;
+  return DH_OK;
+}
+
 extern "C" int dh_sector_set_blas_launch(dh_sector_set* S, int axpyGroupsPerCU, int64_t dotsRowsPerTile) {
   if (!S) return DH_E_ARG;
   if (axpyGroupsPerCU < 1 || axpyGroupsPerCU > 64) return dh_fail(S->ctx, DH_E_ARG, "axpy groups per CU in [1, 64]");
@@ -309,9 +317,12 @@ __device__ __forceinline__ int tile_sector(const int64_t* sPrefix, int nActive, 
 
 // S32: the orbitals fit a 32-bit word (and every sector has fewer than 2^31 states), so the row
 // routine runs on 32-bit state words and 32-bit indices (used for k <= 2).
-template <int K, int LPR, bool LO_LDS, bool S32>
+// FREE: the created pairs come from the free-pair masks (dh_sphere_row_free), G terms in flight per
+// lane.
+template <int K, int LPR, bool LO_LDS, bool S32, bool FREE>
 __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const double* __restrict__ x,
                                                           double* __restrict__ y, int add) {
+  constexpr int G = K == 1 ? 4 : (K <= 4 ? 2 : 1);
   extern __shared__ double smem[];
   double* sF = smem;
   int* sBase = reinterpret_cast<int*>(sF + a.nbrFactors);
@@ -345,10 +356,18 @@ __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const d
       const int bits = a.rankBits;
       // clamped so that a target outside the sector can never address outside the vectors
       auto find = [&](uint32_t t) -> uint32_t { return min(last - lo[t & mask] - hi[t >> bits], last); };
-      dh_sphere_row<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
+      if constexpr (FREE)
+        dh_sphere_row_free<K, LPR, G>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K,
+                                      (uint32_t)(active ? local : 0), acc);
+      else
+        dh_sphere_row<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
     } else {
       auto find = [&](uint64_t t) -> int64_t { return dh_find_rank(t, lo, hi, a.rankBits, dim); };
-      dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
+      if constexpr (FREE)
+        dh_sphere_row_free<K, LPR, G>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K,
+                                      (int64_t)(active ? local : 0), acc);
+      else
+        dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
     }
     if constexpr (LPR > 1) {
 #pragma unroll
@@ -415,8 +434,18 @@ static int sectors_hv_launch(dh_sector_set* S, const double* x, double* y, int a
   // more vectors the 64-bit form is faster on MI355X (its lower occupancy keeps more of the x
   // gathers in cache: 11.3 against 16.5 ms at k = 8 over the 91 sectors of N=10 2S=27)
   const bool s32 = K <= 2 && S->nbrOrbitals <= 32;
-  if (loLds) return s32 ? launch(sectors_hv_kernel<K, LPR, true, (K <= 2)>) : launch(sectors_hv_kernel<K, LPR, true, false>);
-  return s32 ? launch(sectors_hv_kernel<K, LPR, false, (K <= 2)>) : launch(sectors_hv_kernel<K, LPR, false, false>);
+  if (S->rowMode == DH_INDEX_RANK_FREE) {
+    if (loLds)
+      return s32 ? launch(sectors_hv_kernel<K, LPR, true, (K <= 2), true>)
+                 : launch(sectors_hv_kernel<K, LPR, true, false, true>);
+    return s32 ? launch(sectors_hv_kernel<K, LPR, false, (K <= 2), true>)
+               : launch(sectors_hv_kernel<K, LPR, false, false, true>);
+  }
+  if (loLds)
+    return s32 ? launch(sectors_hv_kernel<K, LPR, true, (K <= 2), false>)
+               : launch(sectors_hv_kernel<K, LPR, true, false, false>);
+  return s32 ? launch(sectors_hv_kernel<K, LPR, false, (K <= 2), false>)
+             : launch(sectors_hv_kernel<K, LPR, false, false, false>);
 }
 
 template <int K>

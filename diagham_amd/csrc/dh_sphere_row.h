@@ -109,3 +109,89 @@ __device__ __forceinline__ void dh_sphere_row_x4(U s, int nbrOrbitals, const dou
     }
   }
 }
+
+__device__ __forceinline__ uint64_t dh_bitrev(uint64_t v) { return __builtin_bitreverse64(v); }
+__device__ __forceinline__ uint32_t dh_bitrev(uint32_t v) { return __builtin_bitreverse32(v); }
+
+// Created pairs (m3 > m4, m3 + m4 = sum, both orbitals free in t) as a mask of their m3 bits:
+// bit m of f & (f mirrored about sum / 2) says that m and sum - m are both free.  The mirror of f
+// is its bit reversal within the nbrOrbitals orbitals, shifted so that bit sum - m lands on bit m.
+template <typename U>
+__device__ __forceinline__ U dh_free_pairs(U t, int nbrOrbitals, int sum, int m3lo, int m3hi) {
+  constexpr int W = 8 * sizeof(U);
+  const U full = nbrOrbitals >= W ? ~U(0) : ((U(1) << nbrOrbitals) - U(1));
+  const U f = ~t & full;
+  const U g = dh_bitrev(f) >> (W - nbrOrbitals);  // bit j of g = bit nbrOrbitals - 1 - j of f
+  const int d = nbrOrbitals - 1 - sum;             // bit m of the mirror = bit m + d of g
+  const U mirror = d >= 0 ? (g >> d) : (g << (-d));
+  const U hiMask = m3hi >= W - 1 ? ~U(0) : ((U(1) << (m3hi + 1)) - U(1));
+  const U loMask = (U(1) << m3lo) - U(1);
+  return f & mirror & hiMask & ~loMask;
+}
+
+// The terms of dh_sphere_row in the same order, produced by an iterator over the free created
+// pairs of each annihilated pair (dh_free_pairs): no iteration is spent on a created pair with an
+// occupied orbital.  A lane takes the terms of its row G at a time (each LPR-th term when LPR
+// lanes share the row): their G index lookups are issued together, then their G gathers, so G
+// dependent chains are in flight per lane.  Slots past the row's last term gather x[self] with a
+// zero coefficient; `self` is the row's own index (relative to xbase).
+template <int K, int LPR, int G, typename U, typename Find, typename I>
+__device__ __forceinline__ void dh_sphere_row_free(U s, int nbrOrbitals, const double* __restrict__ sF,
+                                                   const int* __restrict__ sBase, int lane, Find find,
+                                                   const double* __restrict__ xbase, I self, double* acc) {
+  U r1 = s, r2 = 0, pm = 0, t = 0;
+  int n1 = 0, pc1 = 0, sgnA = 0, sum = 0, m3lo = 0, i = 0;
+  const double* fS = sF;
+  int skip = lane;  // terms of the row before this lane's next one (LPR > 1)
+  bool more = true;
+  while (more) {
+    I idx[G];
+    double coef[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      idx[q] = self;
+      coef[q] = 0.0;
+      while (more) {
+        if (!pm) {
+          if (!r2) {
+            if (!r1) {
+              more = false;
+              break;
+            }
+            n1 = dh_ctz(r1);
+            r1 &= r1 - 1;
+            pc1 = dh_popc(s >> n1);
+            r2 = s & ((U(1) << n1) - U(1));
+            continue;
+          }
+          const int n2 = dh_ctz(r2);
+          r2 &= r2 - 1;
+          sgnA = pc1 + dh_popc(s >> n2);
+          t = s ^ (U(1) << n1) ^ (U(1) << n2);
+          sum = n1 + n2;
+          m3lo = (sum >> 1) + 1;
+          fS = sF + sBase[sum];
+          i = n1 - m3lo;
+          pm = dh_free_pairs(t, nbrOrbitals, sum, m3lo, min(sum, nbrOrbitals - 1));
+          continue;
+        }
+        if (LPR > 1 && skip > 0) {
+          pm &= pm - 1;
+          --skip;
+          continue;
+        }
+        const int m3 = dh_ctz(pm);
+        pm &= pm - 1;
+        skip = LPR - 1;
+        const int m4 = sum - m3;
+        const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
+        idx[q] = find(t | (U(1) << m3) | (U(1) << m4));
+        const double c = fS[dh_packed_index(i, m3 - m3lo)];
+        coef[q] = (sg & 1) ? -c : c;
+        break;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) dh_gather_fma<K>(acc, xbase, idx[q], coef[q]);
+  }
+}

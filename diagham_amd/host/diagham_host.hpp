@@ -4,8 +4,11 @@
 #pragma once
 
 #include <algorithm>
+#include <chrono>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -326,6 +329,33 @@ struct KrylovStorageFull : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// Progress lines of long solves on stderr when the environment sets DHH_VERBOSE (like the
+// --show-itertime option of the reference programs): at most one per `seconds` per caller.
+class ProgressLog {
+ public:
+  explicit ProgressLog(double seconds = 20.0)
+      : On(std::getenv("DHH_VERBOSE") != nullptr), Every(seconds), Last(std::chrono::steady_clock::now()) {}
+  bool Due() {
+    if (!On) return false;
+    auto now = std::chrono::steady_clock::now();
+    if (std::chrono::duration<double>(now - Last).count() < Every) return false;
+    Last = now;
+    return true;
+  }
+  bool Enabled() const { return On; }
+
+ private:
+  bool On;
+  double Every;
+  std::chrono::steady_clock::time_point Last;
+};
+
+// Launch shape of the products of a batched sector solve (dh_sector_set_launch and
+// dh_sector_set_row_mode); lanesPerRow == 0 keeps the solver's built-in choice.
+struct SectorLaunch {
+  int lanesPerRow = 0, blockSize = 256, rowMode = DH_INDEX_RANK;
+};
+
 // free device memory of an architecture (0 if it cannot be read)
 inline size_t DeviceFreeBytes(GPUArchitecture* arch) {
   size_t freeBytes = 0, totalBytes = 0;
@@ -413,11 +443,14 @@ void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>&
   std::vector<char> done(na, 0);
   lanczos.LaunchProduct(active);
   bool full = false;
+  ProgressLog log;
   while (!full) {
     std::vector<int> act;
     for (int i = 0; i < na; ++i)
       if (active[i]) act.push_back(i);
     if (act.empty()) break;
+    if (log.Due())
+      std::fprintf(stderr, "[sector Lanczos] step %d, %zu of %d sectors running\n", lanczos.Steps(), act.size(), na);
     lanczos.FinishStep(active);
     bool any = false;
     for (int i = 0; i < na; ++i) {
@@ -497,6 +530,9 @@ int RunSectorWaves(const std::vector<int64_t>& dims, int k, int kr, int nbrEigen
       blocks = b;
       rows += dims[s];
     }
+    if (std::getenv("DHH_VERBOSE"))
+      std::fprintf(stderr, "[sector waves] wave %d: %zu sectors, %lld rows, %d expected blocks, %zu of %zu queued\n",
+                   waves + 1, wave.size(), (long long)rows, blocks, wave.size(), queue.size());
     solveWave(wave);
     ++waves;
     int reached = 0;
@@ -525,9 +561,11 @@ class MultiSectorBlockLanczos {
                           double precision = 1e-14, size_t memoryBytes = 0);
   std::vector<SectorResult> Solve(uint64_t seed, double* seconds = nullptr);
   int NbrWaves() const { return Waves; }
+  void SetLaunch(const SectorLaunch& launch) { Launch = launch; }
 
  private:
   void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
+  SectorLaunch Launch;
   GPUArchitecture* Arch;
   int NbrFermions, LzMax, NbrEigenvalue, BlockSize, MaxIter;
   double Precision;
@@ -575,11 +613,13 @@ class LatticeSectorsBlockLanczos {
   void UseTorusTranslations() { TorusTranslations = true; }
   std::vector<SectorResult> Solve(uint64_t seed, double* seconds = nullptr);
   int NbrWaves() const { return Waves; }
+  void SetLaunch(const SectorLaunch& launch) { Launch = launch; }
 
  private:
   void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
   dh_sector_set* CreateSet(const std::vector<int>& kx, const std::vector<int>& ky) const;
   bool TorusTranslations = false;
+  SectorLaunch Launch;
   GPUArchitecture* Arch;
   int NbrFermions, NbrSiteX, NbrSiteY, NbrEigenvalue, BlockSize, MaxIter;
   double Precision;

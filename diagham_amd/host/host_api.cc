@@ -1,6 +1,7 @@
 // C facade of the host layer (include/diagham_host.h).
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -55,6 +56,7 @@ struct dhh_problem {
 
 static thread_local std::string g_error;
 static thread_local int g_lastWaves = 0;
+static thread_local SectorLaunch g_sectorLaunch;  // dhh_sectors_launch
 
 template <typename F>
 static int guarded(F&& f) {
@@ -72,6 +74,23 @@ static int guarded(F&& f) {
 extern "C" const char* dhh_last_error(void) { return g_error.c_str(); }
 
 extern "C" int dhh_last_solve_waves(void) { return g_lastWaves; }
+
+extern "C" int dhh_sectors_launch(int lanesPerRow, int blockSize, int rowMode) {
+  if (lanesPerRow == 0) {
+    g_sectorLaunch = SectorLaunch();
+    return DH_OK;
+  }
+  if ((lanesPerRow != 1 && lanesPerRow != 2 && lanesPerRow != 4 && lanesPerRow != 8) || blockSize < 64 ||
+      blockSize > 1024 || blockSize % 64 || (rowMode != DH_INDEX_RANK && rowMode != DH_INDEX_RANK_FREE)) {
+    g_error = "launch shape: lanes per row in {1, 2, 4, 8}, workgroup a multiple of 64 up to 1024, "
+              "row mode DH_INDEX_RANK or DH_INDEX_RANK_FREE";
+    return DH_E_ARG;
+  }
+  g_sectorLaunch.lanesPerRow = lanesPerRow;
+  g_sectorLaunch.blockSize = blockSize;
+  g_sectorLaunch.rowMode = rowMode;
+  return DH_OK;
+}
 
 extern "C" int dhh_arch_create(dhh_arch** out, int device) {
   if (!out) return DH_E_ARG;
@@ -383,6 +402,7 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
     if (batched) {
       MultiSectorBlockLanczos solver(a->arch.get(), n, lzMax, lz, v, nbrEigen, blockSize, maxIter, precision,
                                      (size_t)(memoryGB * 1073741824.0));
+      solver.SetLaunch(g_sectorLaunch);
       g_lastWaves = 0;
       std::vector<SectorResult> r = solver.Solve(seed);
       g_lastWaves = solver.NbrWaves();
@@ -395,7 +415,9 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
       }
     } else {
       g_lastWaves = 0;
+      ProgressLog log;
       for (int s = 0; s < nbrSectors; ++s) {
+        if (log.Due()) std::fprintf(stderr, "[sectors] sector %d of %d (2 Lz = %d)\n", s + 1, nbrSectors, lz[s]);
         FermionOnSphere space(a->arch.get(), n, lz[s], lzMax);
         ParticleOnSphereGenericHamiltonian h(&space, n, lzMax, v, a->arch.get());
         FullReorthogonalizedBlockLanczosAlgorithm lanczos(a->arch.get(), &h, nbrEigen, blockSize, maxIter, precision);
@@ -411,6 +433,109 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
     dh_synchronize(a->arch->Context());
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return all ? DH_OK : DH_E_STATE;
+  });
+}
+
+extern "C" int dhh_sectors_autotune(dhh_arch* a, int n, int lzMax, const int* totalLz, int nbrSectors,
+                                    const double* pseudo, int nbrEigen, const int* ks, int nbrKs, double fraction,
+                                    int maxIter, uint64_t seed, double* table, int* best) {
+  if (!a || !totalLz || !pseudo || !ks || !table || !best || nbrSectors <= 0 || nbrKs <= 0 || nbrEigen <= 0 ||
+      !(fraction > 0.0))
+    return DH_E_ARG;
+  return guarded([&] {
+    GPUArchitecture* arch = a->arch.get();
+    dh_context* c = arch->Context();
+    std::vector<double> v(pseudo, pseudo + lzMax + 1), F;
+    std::vector<int> P;
+    ParticleOnSphereGenericHamiltonian::EvaluateInteractionFactors(lzMax, v, F, P);
+    // calibration sectors: every step-th of the sectors too large for the dense path, by size,
+    // so that they hold about `fraction` of their rows and span their size range
+    std::vector<std::pair<int64_t, int>> bySize;
+    int64_t rowsAll = 0;
+    for (int s = 0; s < nbrSectors; ++s) {
+      const int64_t d = dh_fermion_sphere_dimension(n, totalLz[s], lzMax);
+      if (d <= 0) throw std::invalid_argument("empty sector");
+      rowsAll += d;
+      if (d > 256) bySize.push_back({d, totalLz[s]});
+    }
+    if (bySize.empty()) throw std::invalid_argument("no sector large enough to tune on");
+    std::sort(bySize.rbegin(), bySize.rend());
+    const int step = std::max(1, (int)std::lround(1.0 / std::min(1.0, fraction)));
+    std::vector<int> lz;
+    int64_t rows = 0;
+    for (size_t i = step / 2; i < bySize.size(); i += step) {
+      lz.push_back(bySize[i].second);
+      rows += bySize[i].first;
+    }
+    if (lz.empty()) {
+      lz.push_back(bySize.back().second);
+      rows = bySize.back().first;
+    }
+    SectorSolveResources set(arch);
+    arch->Check(dh_sector_set_create(c, &set.set, n, lzMax, lz.data(), (int)lz.size()), "calibration set");
+    arch->Check(dh_sector_set_factors(set.set, F.data(), P.data(), (int)F.size(), 0.0), "factors");
+    const int lprs[] = {1, 2, 4}, blocks[] = {256, 1024}, modes[] = {DH_INDEX_RANK, DH_INDEX_RANK_FREE};
+    double bestSeconds = 1e300;
+    int bestIdx = -1;
+    for (int i = 0; i < nbrKs; ++i) {
+      const int k = ks[i];
+      double* row = table + 7 * i;
+      for (int q = 0; q < 7; ++q) row[q] = 0.0;
+      row[0] = k;
+      double* x = nullptr;
+      double* y = nullptr;
+      try {
+        x = arch->AllocateMemory(rows * k);
+        y = arch->AllocateMemory(rows * k);
+      } catch (const std::exception&) {
+        if (x) arch->FreeMemory(x);
+        row[5] = -1.0;  // the vectors of this block size do not fit
+        continue;
+      }
+      arch->Check(dh_fill_random(c, x, rows * k, seed), "random block");
+      double bestMs = 1e300;
+      SectorLaunch shape;
+      for (int lpr : lprs)
+        for (int bs : blocks)
+          for (int mode : modes) {
+            arch->Check(dh_sector_set_launch(set.set, lpr, bs), "launch");
+            arch->Check(dh_sector_set_row_mode(set.set, mode), "row mode");
+            arch->Check(dh_sectors_hv(set.set, x, y, k, 0, nullptr), "warm-up product");
+            arch->Check(dh_synchronize(c), "sync");
+            auto t0 = std::chrono::steady_clock::now();
+            for (int r = 0; r < 2; ++r) arch->Check(dh_sectors_hv(set.set, x, y, k, 0, nullptr), "timed product");
+            arch->Check(dh_synchronize(c), "sync");
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / 2;
+            if (ms < bestMs) {
+              bestMs = ms;
+              shape.lanesPerRow = lpr;
+              shape.blockSize = bs;
+              shape.rowMode = mode;
+            }
+          }
+      arch->FreeMemory(x);
+      arch->FreeMemory(y);
+      // time to the eigenvalues of the calibration sectors with that shape
+      MultiSectorBlockLanczos solver(arch, n, lzMax, lz, v, nbrEigen, k, maxIter, 1e-14, 0);
+      solver.SetLaunch(shape);
+      double seconds = 0.0;
+      std::vector<SectorResult> res = solver.Solve(seed, &seconds);
+      int products = 0;
+      for (const SectorResult& r : res) products += r.products;
+      row[1] = shape.lanesPerRow;
+      row[2] = shape.blockSize;
+      row[3] = shape.rowMode;
+      row[4] = bestMs;
+      row[5] = seconds;
+      row[6] = products;
+      if (seconds < bestSeconds) {
+        bestSeconds = seconds;
+        bestIdx = i;
+      }
+    }
+    if (bestIdx < 0) throw std::runtime_error("no block size fits the device");
+    for (int q = 0; q < 4; ++q) best[q] = (int)table[7 * bestIdx + q];
+    return DH_OK;
   });
 }
 
@@ -430,6 +555,9 @@ extern "C" int dhh_lattice_sectors_solve(dhh_arch* a, int n, int nx, int ny, dou
     LatticeSectorsBlockLanczos solver(a->arch.get(), n, nx, ny, std::vector<int>(kx, kx + nbrSectors),
                                       std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
                                       maxIter, precision, (size_t)(memoryGB * 1073741824.0));
+    SectorLaunch launch = g_sectorLaunch;
+    launch.rowMode = DH_INDEX_RANK;  // momentum sets have one row routine
+    solver.SetLaunch(launch);
     g_lastWaves = 0;
     std::vector<SectorResult> r = solver.Solve(seed);
     g_lastWaves = solver.NbrWaves();
@@ -515,6 +643,9 @@ extern "C" int dhh_torus_sectors_solve(dhh_arch* a, int n, int nbrFlux, double r
     LatticeSectorsBlockLanczos solver(a->arch.get(), n, 1, nbrFlux, std::vector<int>(nbrSectors, 0),
                                       std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
                                       maxIter, precision, (size_t)(memoryGB * 1073741824.0));
+    SectorLaunch launch = g_sectorLaunch;
+    launch.rowMode = DH_INDEX_RANK;  // momentum sets have one row routine
+    solver.SetLaunch(launch);
     g_lastWaves = 0;
     std::vector<SectorResult> r = solver.Solve(seed);
     g_lastWaves = solver.NbrWaves();
@@ -549,6 +680,9 @@ extern "C" int dhh_torus_translation_sectors_solve(dhh_arch* a, int n, int nbrFl
                                       std::vector<int>(ky, ky + nbrSectors), cnt, pairs, f, nbrEigen, blockSize,
                                       maxIter, precision, (size_t)(memoryGB * 1073741824.0));
     solver.UseTorusTranslations();
+    SectorLaunch launch = g_sectorLaunch;
+    launch.rowMode = DH_INDEX_RANK;  // momentum sets have one row routine
+    solver.SetLaunch(launch);
     g_lastWaves = 0;
     std::vector<SectorResult> r = solver.Solve(seed);
     g_lastWaves = solver.NbrWaves();

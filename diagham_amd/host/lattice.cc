@@ -235,7 +235,9 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   // small sets: four lanes per row for enough wavefronts; large ones: two lanes per row and
   // 512-thread workgroups (N=10 on 6 x 5 sites, 30 sectors: k = 4 103 ms against 154 ms with one
   // lane per row, k = 1 62 against 64 ms)
-  if (total < 4ll * 256 * 256)
+  if (Launch.lanesPerRow > 0)
+    Arch->Check(dh_sector_set_launch(S, Launch.lanesPerRow, Launch.blockSize), "launch");
+  else if (total < 4ll * 256 * 256)
     Arch->Check(dh_sector_set_launch(S, 4, 256), "launch");
   else
     Arch->Check(dh_sector_set_launch(S, 2, 512), "launch");

@@ -144,7 +144,10 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   // small sets: more lanes per row for enough wavefronts; large ones: the fastest shapes of the
   // all-sector product measured on MI355X (k >= 4: two lanes per row, 1024-thread workgroups;
   // k < 4: one lane per row, 256-thread workgroups)
-  if (total < 4ll * 256 * 256)
+  if (Launch.lanesPerRow > 0) {
+    Arch->Check(dh_sector_set_launch(S, Launch.lanesPerRow, Launch.blockSize), "launch");
+    Arch->Check(dh_sector_set_row_mode(S, Launch.rowMode), "row mode");
+  } else if (total < 4ll * 256 * 256)
     Arch->Check(dh_sector_set_launch(S, 4, 512), "launch");
   else if (BlockSize >= 4)
     Arch->Check(dh_sector_set_launch(S, 2, 1024), "launch");

@@ -44,6 +44,9 @@ typedef struct dh_hamiltonian dh_hamiltonian;
 #define DH_INDEX_RANK 1    /* combinatorial rank: two table lookups, no basis probes */
 #define DH_INDEX_RANK_X4 2 /* rank, with the two-body created-pair loop unrolled four-wide (sphere only;
                               other Hamiltonians treat it as DH_INDEX_RANK) */
+#define DH_INDEX_RANK_FREE 3 /* rank, with the created pairs taken from the mask of free pairs of each
+                                annihilated pair and their lookups batched per lane (sphere only;
+                                other Hamiltonians treat it as DH_INDEX_RANK) */
 
 /* ---------------------------------------------------------------- context
  * Replaces the per-operation thread fork/join of SMPArchitecture::SendJobs
@@ -193,6 +196,11 @@ int dh_sector_set_get_states(dh_sector_set* set, uint64_t* states);
 int dh_sector_set_factors(dh_sector_set* set, const double* factors, const int* pair_offset, int nbr_factors,
                           double shift);
 int dh_sector_set_launch(dh_sector_set* set, int lanes_per_row, int block_size);
+/* Row routine of the products of a sphere set: DH_INDEX_RANK (every created pair of the range,
+ * skipping occupied ones) or DH_INDEX_RANK_FREE (only the free created pairs, from one bit mask per
+ * annihilated pair, lookups batched per lane).  A tuning knob like dh_sector_set_launch; the
+ * results are the same. */
+int dh_sector_set_row_mode(dh_sector_set* set, int mode);
 /* Launch shape of the segmented vector operations of a set (no counterpart in the reference, a
  * tuning knob like dh_sector_set_launch): persistent axpy / rmul workgroups per CU (default 16) and
  * rows per dots tile (0: automatic). */

@@ -104,6 +104,24 @@ int dhh_autotune(dhh_problem* problem, const int* ks, int nbr_ks, int repeats, d
  * batched == 0 solves the sectors one after the other like the reference.  Outputs per
  * sector: eigenvalues (nbr_sectors * nbr_eigen), vector products, converged flags; total
  * wall seconds. */
+/* Launch shape of the products of the batched sector solves that this thread runs next
+ * (dh_sector_set_launch: lanes per row in {1, 2, 4, 8} and workgroup size; dh_sector_set_row_mode:
+ * DH_INDEX_RANK or DH_INDEX_RANK_FREE, sphere sets only); lanes_per_row = 0 restores the solvers'
+ * built-in choice.  No counterpart in the reference. */
+int dhh_sectors_launch(int lanes_per_row, int block_size, int row_mode);
+
+/* Autotuner of the batched sphere sector solve (no counterpart in the reference).  Calibration
+ * sectors: every round(1 / fraction)-th sector by size among those with more than 256 states.  For
+ * every block size ks[i]: the launch shape (lanes per row in {1, 2, 4} x workgroup in {256, 1024} x
+ * row routine) with the fastest product over the calibration sectors, then a batched solve of those
+ * sectors to nbr_eigen eigenvalues with that shape.  The block size whose calibration solve is
+ * shortest wins: best = {k, lanes per row, workgroup, row mode}.  table: 7 doubles per ks[i]: k,
+ * lanes per row, workgroup, row mode, product ms, calibration seconds (-1: the vectors did not fit),
+ * vector products of the calibration solve.  Apply the result with dhh_sectors_launch. */
+int dhh_sectors_autotune(dhh_arch* arch, int nbr_fermions, int lz_max, const int* total_lz, int nbr_sectors,
+                         const double* pseudopotentials, int nbr_eigen, const int* ks, int nbr_ks, double fraction,
+                         int max_iter, uint64_t seed, double* table, int* best);
+
 /* Waves (sector subsets whose Krylov storage fits the device together) of the last batched sector
  * solve of this thread; 0 after a sequential solve.  No counterpart in the reference. */
 int dhh_last_solve_waves(void);

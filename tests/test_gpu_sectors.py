@@ -51,11 +51,12 @@ class SectorSet:
         self.capi.gpu().dh_sector_set_destroy(self.h)
 
 
-@pytest.mark.parametrize("lpr", [1, 4])
-def test_sector_set_basis_and_products(arch, lpr):
+@pytest.mark.parametrize("lpr,row_mode", [(1, 1), (4, 1), (1, 3), (2, 3)])
+def test_sector_set_basis_and_products(arch, lpr, row_mode):
     V = coulomb(LZ)
     S = SectorSet(arch, N, LZ, SECTORS, V)
     S.capi.gpu().dh_sector_set_launch(S.h, lpr, 256)
+    assert S.capi.gpu().dh_sector_set_row_mode(S.h, row_mode) == 0
     states = S.states()
     fac = O.Factors(LZ, V)
     for k in (1, 4):
@@ -79,13 +80,14 @@ def test_sector_set_basis_and_products(arch, lpr):
     S.close()
 
 
-@pytest.mark.parametrize("k", [1, 2])
-def test_sector_products_beyond_32_orbitals(arch, k):
+@pytest.mark.parametrize("k,row_mode", [(1, 1), (2, 1), (1, 3), (4, 3)])
+def test_sector_products_beyond_32_orbitals(arch, k, row_mode):
     """36 orbitals: one- and two-vector products keep 64-bit state words (the 32-bit form is for
     at most 32 orbitals) and match the oracle in every sector."""
     n, lz_max, sectors = 4, 35, [0, 2, 10, 60]
     V = coulomb(lz_max)
     S = SectorSet(arch, n, lz_max, sectors, V)
+    assert S.capi.gpu().dh_sector_set_row_mode(S.h, row_mode) == 0
     fac = O.Factors(lz_max, V)
     X = O.splitmix(40 + k, S.total * k)
     Y = S.hv(X, k, [1] * len(sectors), np.zeros(S.total * k))
@@ -219,3 +221,27 @@ def test_batched_solve_grows_krylov_storage(arch, golden):
     assert r["products"].max() > 256, r["products"]
     ref = np.array(golden["fermions_sphere_n10_2s27_lz0"]["eigenvalues"])
     np.testing.assert_allclose(r["eigenvalues"][0][:ref.size], ref, rtol=0, atol=1e-10)
+
+
+def test_sector_autotune_picks_a_shape_that_keeps_the_eigenvalues(arch):
+    """The autotuner times every launch shape per block size on calibration sectors, then their
+    solve; the winner, applied to the whole set, gives the eigenvalues of the built-in launch."""
+    from diagham_amd import capi
+    n, lz = 8, 21
+    V = capi.coulomb_pseudopotentials(lz, True)
+    lzs = capi.sphere_sectors(n, lz)
+    rows, best = capi.sectors_autotune(arch, n, lz, lzs, V, 4, ks=(1, 4), fraction=0.25)
+    assert [r["k"] for r in rows] == [1.0, 4.0]
+    for r in rows:
+        assert r["lanes_per_row"] in (1, 2, 4) and r["block_size"] in (256, 1024)
+        assert r["row_mode"] in (capi.INDEX_RANK, capi.INDEX_RANK_FREE)
+        assert r["product_ms"] > 0 and r["calibration_seconds"] > 0 and r["calibration_products"] > 0
+    assert best["k"] in (1, 4)
+    ref = capi.sectors_solve(arch, n, lz, lzs, V, 4, best["k"])
+    capi.sectors_launch(best["lanes_per_row"], best["block_size"], best["row_mode"])
+    try:
+        tuned = capi.sectors_solve(arch, n, lz, lzs, V, 4, best["k"])
+    finally:
+        capi.sectors_launch(0)
+    assert np.all(tuned["converged"])
+    assert np.max(np.abs(tuned["eigenvalues"] - ref["eigenvalues"])) < 1e-10

@@ -70,7 +70,7 @@ def test_element_count(n8_2s21, golden):
 
 
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2), (2, 1), (2, 8)])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2), (2, 1), (2, 8), (3, 1), (3, 2), (3, 8)])
 def test_hv_parity(n8_2s21, k, mode, lpr):
     """H.X (k vectors, [row][k] layout) vs the oracle, 1e-12 relative norm per vector."""
     ref_space = O.FermionSphere(8, 0, 21)
@@ -86,7 +86,7 @@ def test_hv_parity(n8_2s21, k, mode, lpr):
 
 
 @pytest.mark.parametrize("k", [1, 2, 4])
-@pytest.mark.parametrize("mode,lpr", [(1, 1), (2, 2), (0, 1)])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (2, 2), (0, 1), (3, 1), (3, 4)])
 def test_hv_parity_beyond_32_orbitals(arch, k, mode, lpr):
     """36 orbitals: the rank kernels keep 64-bit state words for every k (the 32-bit form is for
     at most 32 orbitals)."""
@@ -343,3 +343,20 @@ def test_hv_over_separate_vectors_spin(arch):
     for buf in dx + dy:
         buf.free()
     p.close()
+
+
+@pytest.mark.parametrize("k,lpr", [(1, 1), (1, 2), (8, 2)])
+def test_free_pair_rows_are_bitwise_the_range_rows(n8_2s21, k, lpr):
+    """The free-pair iterator visits the terms of each row in the order of the range loop, so with
+    the same lanes per row the products agree bit for bit."""
+    X = np.stack([O.splitmix(700 + v, n8_2s21.dim) for v in range(k)], axis=1).reshape(-1)
+    n8_2s21.set_launch(lanes_per_row=lpr, block_size=256, index_mode=1)
+    Y1 = n8_2s21.hv(X, k)
+    n8_2s21.set_launch(lanes_per_row=lpr, block_size=256, index_mode=3)
+    Y3 = n8_2s21.hv(X, k)
+    n8_2s21.set_launch(lanes_per_row=4, block_size=256, index_mode=1)
+    if lpr == 1:
+        assert np.array_equal(Y1, Y3)
+    else:  # lanes take different terms, and their partial sums meet in another order
+        assert np.linalg.norm(Y1 - Y3) <= 1e-14 * np.linalg.norm(Y1)
+

@@ -227,6 +227,39 @@ def test_element_counts(golden):
     assert (g["full_matrix_elements"] - diag) // 2 + diag == g["stored_hermitian_elements"]
 
 
+def brute_sphere_term_counts(states, lz_max):
+    """(all terms, diagonal terms) of the two-body product by enumeration independent of the oracle's
+    operators: every occupied pair n1 > n2 of a state, every created pair m3 > m4 with the same
+    orbital sum and both orbitals free once n1 and n2 are emptied"""
+    total = diag = 0
+    for s in (int(v) for v in states):
+        occ = [o for o in range(lz_max + 1) if (s >> o) & 1]
+        for i, n1 in enumerate(occ):
+            for n2 in occ[:i]:
+                t = s ^ (1 << n1) ^ (1 << n2)
+                S = n1 + n2
+                for m3 in range(S // 2 + 1, min(S, lz_max) + 1):
+                    m4 = S - m3
+                    if not (t >> m3) & 1 and not (t >> m4) & 1:
+                        total += 1
+                        diag += (m3, m4) == (n1, n2)
+    return total, diag
+
+
+def test_element_counts_by_enumeration(golden):
+    """The reference's printed counts of the N=8, 2S=21 product (full elements, diagonal ones) and of
+    the periodic Heisenberg chains (one element per antiparallel bond), from the bases alone: these
+    are survey records of the reference programs, pinned here without any product code."""
+    g = golden["fermions_sphere_n8_2s21_lz0"]
+    total, diag = brute_sphere_term_counts(O.FermionSphere(8, 0, 21).states(), 21)
+    assert (total, diag) == (g["full_matrix_elements"], g["diagonal_elements"])
+    for L, key in [(24, "heisenberg_periodic_l24_sz0"), (28, "heisenberg_periodic_l28_sz0")]:
+        st = O.SpinChain(L, 0).states() & np.uint64((1 << L) - 1)
+        rot = ((st >> np.uint64(1)) | ((st & np.uint64(1)) << np.uint64(L - 1)))
+        flips = int(np.unpackbits((st ^ rot).astype(">u8").view(np.uint8)).sum())
+        assert flips == golden[key]["offdiagonal_elements"]
+
+
 def dense_from_hv(apply, dim):
     H = np.zeros((dim, dim))
     for i in range(dim):

@@ -2,8 +2,11 @@
 basis with Nx = 1, ParticleOnTorusGenericHamiltonian / ParticleOnTorusCoulombHamiltonian factors),
 pinned by what the physics fixes: the basis against brute-force enumeration, exact Laughlin zero
 modes of the V1 pseudopotential at filling 1/3 (three of them, the centre-of-mass degeneracy, at
-any aspect ratio), Hermiticity, and the spectra that the magnetic translation (Ky -> Ky + N) and
-the reflection (Ky -> -Ky) must carry over unchanged."""
+any aspect ratio), Hermiticity, the spectra that the magnetic translation (Ky -> Ky + N) and
+the reflection (Ky -> -Ky) must carry over unchanged, and the Coulomb interaction V(q) of
+ParticleOnTorusCoulombHamiltonian (GetVofQ, ParticleOnTorusCoulombHamiltonian.cc:208) through the
+two-particle spectrum, whose level spacings tend to those of the plane's Haldane pseudopotentials
+as the torus grows."""
 import numpy as np
 import pytest
 
@@ -114,3 +117,41 @@ def test_magnetic_translation_sectors_split_the_ky_sector(n, nflux):
         got = np.sort(np.concatenate(parts))
         assert dims == ref.size
         assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max()), (ky, got[:4], ref[:4])
+
+
+def plane_coulomb_pseudopotential(m, level):
+    """V_m of the Coulomb interaction 1/r in Landau level `level` on the plane (magnetic length 1):
+    the integral over q of L_m(q^2) [L_level(q^2 / 2)]^2 exp(-q^2); Gamma(m + 1/2) / (2 m!) in the
+    lowest level"""
+    from scipy.integrate import quad
+    from scipy.special import eval_laguerre
+    return quad(lambda q: eval_laguerre(m, q * q) * eval_laguerre(level, q * q / 2) ** 2 * np.exp(-q * q),
+                0, np.inf, limit=200)[0]
+
+
+def test_plane_pseudopotential_closed_form():
+    from math import factorial, gamma
+    for m in range(1, 12, 2):
+        assert abs(plane_coulomb_pseudopotential(m, 0) - gamma(m + 0.5) / (2 * factorial(m))) < 1e-12
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_coulomb_two_particle_levels_tend_to_plane_pseudopotentials(level):
+    """Two fermions on a square torus of Nphi flux quanta: the distinct levels of a Ky sector are,
+    from the top, relative angular momenta m = 1, 3, 5, 7 (each doubly degenerate), shifted by one
+    finite-size constant.  Their spacings must approach V_m - V_(m+2) of the plane within 0.3 / Nphi,
+    closer for the larger torus.  A wrong form factor or a wrong V(q) moves them by order one."""
+    errors = {}
+    V = [plane_coulomb_pseudopotential(m, level) for m in range(8)]
+    for nflux in (40, 48):
+        f = O.TorusFactors(nflux, 1.0, coulomb_level=level)
+        w = np.linalg.eigvalsh(O.dense_fm(O.torus_space(2, nflux, 0), f))
+        levels = []
+        for x in sorted(w, reverse=True):
+            if not levels or levels[-1] - x > 1e-6:
+                levels.append(x)
+        err = [abs((levels[i] - levels[i + 1]) - (V[2 * i + 1] - V[2 * i + 3])) for i in range(3)]
+        assert max(err) < 0.3 / nflux, (nflux, err)
+        errors[nflux] = max(err)
+    assert errors[48] < errors[40]
+

@@ -274,8 +274,8 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
         from_lattice = capi.lattice_set(arch, 10, 6, 5, [x for x in range(6) for y in range(5)],
                                         [y for x in range(6) for y in range(5)])
         cleanup.append(from_lattice.close)
-        from_lattice.set_launch(2, 512)
-        rows, nnz = from_lattice.total, None
+        from_lattice.set_launch(lpr or 2, 512)
+        rows, nnz = from_lattice.total, int(from_lattice.count_elements().sum())
         run = lambda dx, dy: from_lattice.hv_device(dx, dy, k)  # noqa: E731
         width = 2 * k
     else:
@@ -444,6 +444,17 @@ def main():
                                          "frac_of_peak": traffic["bytes_per_launch"] / (ms * 1e-3) / HBM_PEAK,
                                          "counted_at_block_size": traffic.get("block_size")},
         "copy_calibration": calib,
+        # three readings of the same kernel time: the algorithmic model above (x read once per
+        # matrix element, which the caches partly serve), the compulsory bytes (every 32-bit state
+        # word, x and y once each: what no implementation can avoid) and the HBM bytes the counters
+        # measured (rocprofv3 FETCH_SIZE + WRITE_SIZE)
+        "roofline_readings": {
+            "algorithmic_frac": achieved / (HBM_PEAK / 1e9),
+            "compulsory_bytes_per_launch": dim * (4 + 8 + 8),
+            "compulsory_frac": dim * (4 + 8 + 8) / (ms * 1e-3) / HBM_PEAK,
+            "counter_frac": traffic and traffic["bytes_per_launch"] / (ms * 1e-3) / HBM_PEAK,
+            "counter_over_compulsory": traffic and traffic["bytes_per_launch"] / (dim * (4 + 8 + 8)),
+        },
     }
     progress("headline product: %.3f ms" % ms)
     if not args.skip_lanczos:

@@ -660,6 +660,11 @@ class MomentumSectorSet:
         am = None if active is None else bytes(np.asarray(active, np.uint8))
         self.arch.check(gpu().dh_sectors_hv(self.h, d_x, d_y, k, int(add), am), "dh_sectors_hv")
 
+    def count_elements(self):
+        counts = np.zeros(self.nbr_sectors, np.int64)
+        self.arch.check(gpu().dh_sector_set_count_elements(self.h, counts.ctypes.data_as(P_I64)), "count")
+        return counts
+
     def close(self):
         if getattr(self, "h", None):
             gpu().dh_sector_set_destroy(self.h)

@@ -252,7 +252,9 @@ extern "C" int dh_hamiltonian_count_elements(dh_hamiltonian* h, int64_t* nnz) {
 extern "C" int dh_sector_set_count_elements(dh_sector_set* S, int64_t* counts) {
   if (!S || !counts) return DH_E_ARG;
   dh_context* c = S->ctx;
-  if (S->kind != DH_SECTORS_SPHERE) return dh_fail(c, DH_E_ARG, "element counts are implemented for sphere sector sets");
+  if (S->kind == DH_SECTORS_MOMENTUM) return dh_lattice_count_elements(S, counts);
+  if (S->kind != DH_SECTORS_SPHERE)
+    return dh_fail(c, DH_E_ARG, "element counts are implemented for sphere and momentum sector sets");
   if (!S->d_factors) return dh_fail(c, DH_E_STATE, "factors not set");
   const int64_t chunk = 1 << 24;
   int64_t maxDim = 0;

@@ -179,6 +179,12 @@ struct dh_sector_set {
   int64_t* d_sumFactorOff = nullptr;  // [nbrSums + 1] complex entries
   double* d_cfactors = nullptr;  // interleaved (re, im)
   int64_t nbrComplexFactors = 0;
+  // free-pair tables of a momentum set, per momentum sum j (see lattice_free_pairs): masks
+  // [3 j] = positions whose ky stays in its block / wraps / has the smaller orbital of its pair,
+  // shifts [2 j] = (kx rotation in bits, ky rotation), and per orbital m the pair (i2 << 8) | m2
+  uint64_t* d_freeMask = nullptr;
+  int* d_freeShift = nullptr;
+  int* d_partnerPos = nullptr;
   double shift = 0.0;
   // torus with magnetic translations: representatives ascending per sector, their orbit sizes and
   // reordering-sign bits, a top-bits lookup per sector, and the phases exp(2 pi i Kx t / modulo)
@@ -202,8 +208,9 @@ struct dh_sector_set {
 // work descriptor of the active sectors (dh_multisector.hip)
 int dh_sectors_upload_work(dh_sector_set* S, const unsigned char* active, int64_t rowsPerTile, int* nActive,
                            int64_t* totalTiles);
-// complex product over a momentum set (dh_lattice.hip)
+// complex product over a momentum set and its matrix elements per sector (dh_lattice.hip)
 int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active);
+int dh_lattice_count_elements(dh_sector_set* S, int64_t* counts);
 // complex product over a torus set with magnetic translations (dh_torus.hip)
 int dh_torus_translation_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add,
                                     const unsigned char* active);

@@ -173,6 +173,58 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
   return DH_OK;
 }
 
+// Free-pair tables of a momentum set.  For momentum sum j = (jx, jy) the partner of orbital
+// m = kx Ny + ky is P_j(m) = ((jx - kx) mod Nx) Ny + (jy - ky) mod Ny.  The mirror of the free mask f
+// (bit m set when P_j(m) is free) is f reversed within the N orbitals, which maps (kx, ky) to
+// (Nx-1-kx, Ny-1-ky), then rotated by ax = (Nx-1-jx) mod Nx blocks of Ny bits and by
+// ay = (Ny-1-jy) mod Ny bits inside every block.  Per sum: the rotation in bits ax Ny and ay, the
+// masks of the positions whose ky + ay stays below Ny (taken from g >> ay) or wraps (from
+// g << (Ny - ay)), the mask of the orbitals that are the smaller of their pair, and per orbital the
+// pair's position in the sum's list with its partner.
+static int lattice_free_tables(dh_sector_set* S, const std::vector<int>& info) {
+  dh_context* c = S->ctx;
+  const int nx = S->nbrSiteX, ny = S->nbrSiteY, N = S->nbrOrbitals, J = nx * ny;
+  std::vector<uint64_t> masks(3 * (size_t)J, 0);
+  std::vector<int> shifts(2 * (size_t)J, 0), pp((size_t)J * N, -1);
+  for (int j = 0; j < J; ++j) {
+    const int jx = j / ny, jy = j % ny;
+    const int ax = (nx - 1 - jx) % nx, ay = (ny - 1 - jy) % ny;
+    shifts[2 * j] = ax * ny;
+    shifts[2 * j + 1] = ay;
+    for (int m = 0; m < N; ++m) {
+      const int kx = m / ny, ky = m % ny;
+      if (ky < ny - ay)
+        masks[3 * j] |= 1ull << m;
+      else
+        masks[3 * j + 1] |= 1ull << m;
+      const int partner = ((jx - kx + nx) % nx) * ny + ((jy - ky + ny) % ny);
+      if (m < partner) {
+        masks[3 * j + 2] |= 1ull << m;
+        const int id = partner * (partner - 1) / 2 + m;
+        if ((info[id] >> 16) != j) return dh_fail(c, DH_E_STATE, "momentum pair table inconsistent");
+        pp[(size_t)j * N + m] = ((info[id] & 0xffff) << 8) | partner;
+      }
+    }
+  }
+  for (void* p : {(void*)S->d_freeMask, (void*)S->d_freeShift, (void*)S->d_partnerPos})
+    if (p) hipFree(p);
+  S->d_freeMask = nullptr;
+  S->d_freeShift = nullptr;
+  S->d_partnerPos = nullptr;
+  int rc;
+  if ((rc = dh_alloc(c, (void**)&S->d_freeMask, sizeof(uint64_t) * masks.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_freeShift, sizeof(int) * shifts.size())) ||
+      (rc = dh_alloc(c, (void**)&S->d_partnerPos, sizeof(int) * pp.size())))
+    return rc;
+  DH_HIP(c, hipMemcpyAsync(S->d_freeMask, masks.data(), sizeof(uint64_t) * masks.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_freeShift, shifts.data(), sizeof(int) * shifts.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  DH_HIP(c, hipMemcpyAsync(S->d_partnerPos, pp.data(), sizeof(int) * pp.size(), hipMemcpyHostToDevice, c->stream));
+  DH_HIP(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
 extern "C" int dh_sector_set_lattice_factors(dh_sector_set* S, int nbrSums, const int* sumPairCount,
                                              const int* sumPairs, const double* factors, double shift) {
   if (!S || !sumPairCount || !sumPairs || !factors) return DH_E_ARG;
@@ -231,6 +283,7 @@ extern "C" int dh_sector_set_lattice_factors(dh_sector_set* S, int nbrSums, cons
   S->nbrPairs = nPairs;
   S->nbrComplexFactors = facOff.back();
   S->shift = shift;
+  if (S->kind == DH_SECTORS_MOMENTUM) return lattice_free_tables(S, info);
   return DH_OK;
 }
 
@@ -238,6 +291,10 @@ struct LatticeArgs {
   const uint64_t* states;
   const int64_t* offsets;
   int nbrOrbitals, nbrPairs, nbrSums, totalPairs;
+  int nbrSiteY;
+  const uint64_t* freeMask;
+  const int* freeShift;
+  const int* partnerPos;
   const int* pairInfo;
   const int* sumPairOff;
   const int* sumPairs;
@@ -265,17 +322,36 @@ __device__ __forceinline__ int lat_tile_sector(const int64_t* sPrefix, int nActi
   return lo;
 }
 
-// K complex vectors per row: x and y hold 2K doubles per row, (re, im) interleaved
-template <int K, int LPR, bool LO_LDS>
+// free created pairs of momentum sum j in t: bit m1 set when m1 and its partner are free and m1 is
+// the smaller of the two (tables of lattice_free_tables, in LDS)
+__device__ __forceinline__ uint64_t lattice_free_pairs(uint64_t t, int N, int ny, const uint64_t* __restrict__ mask,
+                                                       const int* __restrict__ shift, int j) {
+  const uint64_t full = N >= 64 ? ~0ull : ((1ull << N) - 1ull);
+  const uint64_t f = ~t & full;
+  const uint64_t R = __builtin_bitreverse64(f) >> (64 - N);
+  const int r = shift[2 * j], ay = shift[2 * j + 1];
+  const uint64_t g1 = r ? (((R >> r) | (R << (N - r))) & full) : R;
+  const uint64_t g = ((g1 >> ay) & mask[3 * j]) | ((g1 << (ny - ay)) & mask[3 * j + 1]);
+  return f & g & mask[3 * j + 2];
+}
+
+// K complex vectors per row: x and y hold 2K doubles per row, (re, im) interleaved.  FREE: the
+// created pairs of each annihilated pair come from lattice_free_pairs (no iteration over a created
+// pair with an occupied orbital) and G terms are gathered together per lane.
+template <int K, int LPR, bool LO_LDS, bool FREE>
 __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const double* __restrict__ x,
                                                           double* __restrict__ y, int add) {
+  constexpr int G = K == 1 ? 4 : (K <= 2 ? 2 : 1);
   extern __shared__ int64_t smem64[];
   int64_t* sWork = smem64;                                    // 2 nActive + 1
   int64_t* sFacOff = sWork + 2 * a.nActive + 1;               // nbrSums + 1
-  int* sInfo = reinterpret_cast<int*>(sFacOff + a.nbrSums + 1);  // nbrPairs
+  uint64_t* sMask = reinterpret_cast<uint64_t*>(sFacOff + a.nbrSums + 1);  // FREE: 3 nbrSums
+  int* sInfo = reinterpret_cast<int*>(sMask + (FREE ? 3 * a.nbrSums : 0));  // nbrPairs
   int* sPairOff = sInfo + a.nbrPairs;                         // nbrSums + 1
   int* sPairs = sPairOff + a.nbrSums + 1;                     // totalPairs
-  uint32_t* sLo = reinterpret_cast<uint32_t*>(sPairs + a.totalPairs);
+  int* sShift = sPairs + a.totalPairs;                        // FREE: 2 nbrSums
+  int* sPP = sShift + (FREE ? 2 * a.nbrSums : 0);             // FREE: nbrSums * nbrOrbitals
+  uint32_t* sLo = reinterpret_cast<uint32_t*>(sPP + (FREE ? a.nbrSums * a.nbrOrbitals : 0));
   for (int i = threadIdx.x; i < 2 * a.nActive + 1; i += blockDim.x) sWork[i] = a.work[i];
   for (int i = threadIdx.x; i <= a.nbrSums; i += blockDim.x) {
     sFacOff[i] = a.sumFactorOff[i];
@@ -283,6 +359,11 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
   }
   for (int i = threadIdx.x; i < a.nbrPairs; i += blockDim.x) sInfo[i] = a.pairInfo[i];
   for (int i = threadIdx.x; i < a.totalPairs; i += blockDim.x) sPairs[i] = a.sumPairs[i];
+  if constexpr (FREE) {
+    for (int i = threadIdx.x; i < 3 * a.nbrSums; i += blockDim.x) sMask[i] = a.freeMask[i];
+    for (int i = threadIdx.x; i < 2 * a.nbrSums; i += blockDim.x) sShift[i] = a.freeShift[i];
+    for (int i = threadIdx.x; i < a.nbrSums * a.nbrOrbitals; i += blockDim.x) sPP[i] = a.partnerPos[i];
+  }
   if constexpr (LO_LDS)
     for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo[i];
   __syncthreads();
@@ -304,36 +385,106 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
 #pragma unroll
     for (int v = 0; v < 2 * K; ++v) acc[v] = 0.0;
     const double* xs = x + off * 2 * K;
-    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
-      const int n1 = __builtin_ctzll(r1);  // lower annihilated orbital
-      for (uint64_t r2 = s & ~((2ull << n1) - 1ull); r2; r2 &= r2 - 1) {
-        const int n2 = __builtin_ctzll(r2);  // higher annihilated orbital
-        // AA(n1, n2): a_n2 first, then a_n1
-        const int sgnA = __popcll(s >> n2) + __popcll((s ^ (1ull << n2)) >> n1);
-        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
-        const int info = sInfo[(n2 * (n2 - 1)) / 2 + n1];
-        const int sum = info >> 16, i1 = info & 0xffff;
-        const int p0 = sPairOff[sum], np = sPairOff[sum + 1] - p0;
-        const double* F = a.factors + 2 * (sFacOff[sum] + (int64_t)i1 * np);
-        for (int i2 = lane; i2 < np; i2 += LPR) {
-          const int pk = sPairs[p0 + i2];
-          const int m1 = pk & 0xff, m2 = pk >> 8;  // m1 < m2
-          const uint64_t cm = (1ull << m1) | (1ull << m2);
-          if (t & cm) continue;
-          // AdAd(m1, m2): a+_m2 first, then a+_m1
-          const int sg = sgnA + __popcll(t >> m2) + __popcll((t | (1ull << m2)) >> m1);
-          const int64_t idx = dh_find_rank(t | cm, lo, hi, a.rankBits, dim);
-          double cr = F[2 * i2], ci = -F[2 * i2 + 1];  // conj(F)
-          if (sg & 1) {
-            cr = -cr;
-            ci = -ci;
+    if constexpr (FREE) {
+      uint64_t r1 = s, r2 = 0, pm = 0, t = 0;
+      int n1 = 0, sgnA = 0, sum = 0;
+      const double* F = a.factors;
+      int skip = lane;
+      bool more = true;
+      const int64_t self = active ? local : 0;
+      while (more) {
+        int64_t idx[G];
+        double cre[G], cim[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          idx[q] = self;
+          cre[q] = 0.0;
+          cim[q] = 0.0;
+          while (more) {
+            if (!pm) {
+              if (!r2) {
+                if (!r1) {
+                  more = false;
+                  break;
+                }
+                n1 = __builtin_ctzll(r1);  // lower annihilated orbital
+                r1 &= r1 - 1;
+                r2 = s & ~((2ull << n1) - 1ull);
+                continue;
+              }
+              const int n2 = __builtin_ctzll(r2);  // higher annihilated orbital
+              r2 &= r2 - 1;
+              sgnA = __popcll(s >> n2) + __popcll((s ^ (1ull << n2)) >> n1);  // a_n2 first, then a_n1
+              t = s ^ (1ull << n1) ^ (1ull << n2);
+              const int info = sInfo[(n2 * (n2 - 1)) / 2 + n1];
+              sum = info >> 16;
+              const int np = sPairOff[sum + 1] - sPairOff[sum];
+              F = a.factors + 2 * (sFacOff[sum] + (int64_t)(info & 0xffff) * np);
+              pm = lattice_free_pairs(t, a.nbrOrbitals, a.nbrSiteY, sMask, sShift, sum);
+              continue;
+            }
+            if (LPR > 1 && skip > 0) {
+              pm &= pm - 1;
+              --skip;
+              continue;
+            }
+            const int m1 = __builtin_ctzll(pm);
+            pm &= pm - 1;
+            skip = LPR - 1;
+            const int pp = sPP[sum * a.nbrOrbitals + m1];
+            const int m2 = pp & 0xff, i2 = pp >> 8;  // m1 < m2
+            // AdAd(m1, m2): a+_m2 first, then a+_m1
+            const int sg = sgnA + __popcll(t >> m2) + __popcll((t | (1ull << m2)) >> m1);
+            idx[q] = dh_find_rank(t | (1ull << m1) | (1ull << m2), lo, hi, a.rankBits, dim);
+            const double fr = F[2 * i2], fi = -F[2 * i2 + 1];  // conj(F)
+            cre[q] = (sg & 1) ? -fr : fr;
+            cim[q] = (sg & 1) ? -fi : fi;
+            break;
           }
-          const double* xr = xs + idx * 2 * K;
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const double* xr = xs + idx[q] * 2 * K;
 #pragma unroll
           for (int v = 0; v < K; ++v) {
             double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
-            acc[2 * v] += cr * p.x - ci * p.y;
-            acc[2 * v + 1] += cr * p.y + ci * p.x;
+            acc[2 * v] += cre[q] * p.x - cim[q] * p.y;
+            acc[2 * v + 1] += cre[q] * p.y + cim[q] * p.x;
+          }
+        }
+      }
+    } else {
+      for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+        const int n1 = __builtin_ctzll(r1);  // lower annihilated orbital
+        for (uint64_t r2 = s & ~((2ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+          const int n2 = __builtin_ctzll(r2);  // higher annihilated orbital
+          // AA(n1, n2): a_n2 first, then a_n1
+          const int sgnA = __popcll(s >> n2) + __popcll((s ^ (1ull << n2)) >> n1);
+          const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+          const int info = sInfo[(n2 * (n2 - 1)) / 2 + n1];
+          const int sum = info >> 16, i1 = info & 0xffff;
+          const int p0 = sPairOff[sum], np = sPairOff[sum + 1] - p0;
+          const double* F = a.factors + 2 * (sFacOff[sum] + (int64_t)i1 * np);
+          for (int i2 = lane; i2 < np; i2 += LPR) {
+            const int pk = sPairs[p0 + i2];
+            const int m1 = pk & 0xff, m2 = pk >> 8;  // m1 < m2
+            const uint64_t cm = (1ull << m1) | (1ull << m2);
+            if (t & cm) continue;
+            // AdAd(m1, m2): a+_m2 first, then a+_m1
+            const int sg = sgnA + __popcll(t >> m2) + __popcll((t | (1ull << m2)) >> m1);
+            const int64_t idx = dh_find_rank(t | cm, lo, hi, a.rankBits, dim);
+            double cr = F[2 * i2], ci = -F[2 * i2 + 1];  // conj(F)
+            if (sg & 1) {
+              cr = -cr;
+              ci = -ci;
+            }
+            const double* xr = xs + idx * 2 * K;
+#pragma unroll
+            for (int v = 0; v < K; ++v) {
+              double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
+              acc[2 * v] += cr * p.x - ci * p.y;
+              acc[2 * v + 1] += cr * p.y + ci * p.x;
+            }
           }
         }
       }
@@ -382,8 +533,14 @@ static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add,
   a.work = S->d_work;
   a.nActive = na;
   a.totalTiles = tiles;
+  a.nbrSiteY = S->nbrSiteY;
+  a.freeMask = S->d_freeMask;
+  a.freeShift = S->d_freeShift;
+  a.partnerPos = S->d_partnerPos;
+  const bool free = S->rowMode == DH_INDEX_RANK_FREE && S->d_freeMask;
   size_t base = 8 * (2 * (size_t)na + 1) + 8 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs +
                 4 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs;
+  if (free) base += 8 * 3 * (size_t)S->nbrSums + 4 * 2 * (size_t)S->nbrSums + 4 * (size_t)S->nbrSums * S->nbrOrbitals;
   base = (base + 7) & ~size_t(7);
   size_t withLo = base + 4 * (size_t)S->loSize;
   bool loLds = withLo <= 64 * 1024;
@@ -400,8 +557,9 @@ static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add,
     DH_HIP(c, hipGetLastError());
     return DH_OK;
   };
-  if (loLds) return launch(lattice_hv_kernel<K, LPR, true>);
-  return launch(lattice_hv_kernel<K, LPR, false>);
+  if (free) return loLds ? launch(lattice_hv_kernel<K, LPR, true, true>) : launch(lattice_hv_kernel<K, LPR, false, true>);
+  if (loLds) return launch(lattice_hv_kernel<K, LPR, true, false>);
+  return launch(lattice_hv_kernel<K, LPR, false, false>);
 }
 
 template <int K>
@@ -412,6 +570,64 @@ static int lattice_lpr(dh_sector_set* S, const double* x, double* y, int add, co
     case 4: return lattice_launch<K, 4>(S, x, y, add, active);
     default: return lattice_launch<K, 8>(S, x, y, add, active);
   }
+}
+
+// Matrix elements per row of a momentum sector (the terms of lattice_hv_kernel whose target is in
+// the space), summed per sector: one row per thread, a workgroup sum, one atomic add per workgroup.
+__global__ void __launch_bounds__(256) lattice_elements_kernel(const uint64_t* __restrict__ states, int64_t dim,
+                                                               const int* __restrict__ pairInfo,
+                                                               const int* __restrict__ sumPairOff,
+                                                               const int* __restrict__ sumPairs,
+                                                               unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long red[256 / 64];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cnt = 0;
+  if (row < dim) {
+    const uint64_t s = states[row];
+    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+      const int n1 = __builtin_ctzll(r1);
+      for (uint64_t r2 = s & ~((2ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+        const int n2 = __builtin_ctzll(r2);
+        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+        const int sum = pairInfo[(n2 * (n2 - 1)) / 2 + n1] >> 16;
+        for (int p = sumPairOff[sum]; p < sumPairOff[sum + 1]; ++p) {
+          const int pk = sumPairs[p];
+          cnt += (t & ((1ull << (pk & 0xff)) | (1ull << (pk >> 8)))) ? 0ull : 1ull;
+        }
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    atomicAdd(total, t);
+  }
+}
+
+int dh_lattice_count_elements(dh_sector_set* S, int64_t* counts) {
+  dh_context* c = S->ctx;
+  if (!S->d_cfactors) return dh_fail(c, DH_E_STATE, "lattice factors not set");
+  unsigned long long* d_total = nullptr;
+  int rc = dh_alloc(c, (void**)&d_total, sizeof(unsigned long long) * S->nbrSectors);
+  if (rc) return rc;
+  std::vector<unsigned long long> zero(S->nbrSectors, 0ull);
+  hipMemcpyAsync(d_total, zero.data(), sizeof(unsigned long long) * S->nbrSectors, hipMemcpyHostToDevice, c->stream);
+  for (int sec = 0; sec < S->nbrSectors; ++sec) {
+    const int64_t off = S->offsets[sec], dim = S->offsets[sec + 1] - off;
+    if (dim <= 0) continue;
+    hipLaunchKernelGGL(lattice_elements_kernel, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, c->stream,
+                       S->d_states + off, dim, S->d_pairInfo, S->d_sumPairOff, S->d_sumPairs, d_total + sec);
+  }
+  std::vector<unsigned long long> total(S->nbrSectors);
+  hipMemcpyAsync(total.data(), d_total, sizeof(unsigned long long) * S->nbrSectors, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  hipFree(d_total);
+  if (e != hipSuccess) return dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+  for (int sec = 0; sec < S->nbrSectors; ++sec) counts[sec] = (int64_t)total[sec];
+  return DH_OK;
 }
 
 int dh_lattice_sectors_hv(dh_sector_set* S, const double* x, double* y, int k, int add, const unsigned char* active) {

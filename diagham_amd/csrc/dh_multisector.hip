@@ -163,7 +163,8 @@ extern "C" int dh_sector_set_destroy(dh_sector_set* S) {
                   (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef,
                   (void*)S->d_pairInfo, (void*)S->d_sumPairOff, (void*)S->d_sumPairs, (void*)S->d_sumFactorOff,
                   (void*)S->d_cfactors, (void*)S->d_packedFactors, (void*)S->d_sumBase, (void*)S->d_orbit,
-                  (void*)S->d_reorder, (void*)S->d_lut, (void*)S->d_phase})
+                  (void*)S->d_reorder, (void*)S->d_lut, (void*)S->d_phase, (void*)S->d_freeMask,
+                  (void*)S->d_freeShift, (void*)S->d_partnerPos})
     if (p) hipFree(p);
   delete S;
   return DH_OK;
@@ -239,6 +240,8 @@ extern "C" int dh_sector_set_row_mode(dh_sector_set* S, int mode) {
   if (!S) return DH_E_ARG;
   if (mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_FREE)
     return dh_fail(S->ctx, DH_E_ARG, "row mode must be DH_INDEX_RANK or DH_INDEX_RANK_FREE");
+  if (S->kind == DH_SECTORS_TORUS_TRANSLATIONS && mode != DH_INDEX_RANK)
+    return dh_fail(S->ctx, DH_E_ARG, "torus sets with magnetic translations have one row routine");
   S->rowMode = mode;
   return DH_OK;
 }

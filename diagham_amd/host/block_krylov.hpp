@@ -132,38 +132,54 @@ class SegmentedBlockLanczos {
         KR(k * S::Real),
         MaxBlocks(maxBlocks),
         Capacity(maxBlocks),
-        Storage(storage),
+        ScratchBlock(storage + (int64_t)(maxBlocks + 1) * blockDoubles),
         BlockDoubles(blockDoubles),
         A(dims.size()),
         B(dims.size()),
         DeadCols(dims.size()),
         Previous(dims.size()),
-        LastCheck(dims.size(), -1) {}
+        LastCheck(dims.size(), -1) {
+    Chunks.push_back({storage, 0, maxBlocks + 1});
+  }
 
-  double* Block(int j) const { return Storage + (int64_t)j * BlockDoubles; }
-  // the scratch block after the last Krylov block (free between steps)
-  double* Scratch() const { return Block(Capacity + 1); }
-  double* StorageBase() const { return Storage; }
+  // Krylov block j (0 <= j <= BlockCapacity())
+  double* Block(int j) const {
+    for (const Chunk& c : Chunks)
+      if (j < c.first + c.count) return c.ptr + (int64_t)(j - c.first) * BlockDoubles;
+    throw std::logic_error("Krylov block outside the storage");
+  }
+  // the scratch block (free between steps)
+  double* Scratch() const { return ScratchBlock; }
   int BlockCapacity() const { return Capacity; }
+  int NbrStorageChunks() const { return (int)Chunks.size(); }
+  // W += sum_b X_(b0 + b) C[a][b] over segment a of `mask` (na of them), in device layout
+  void CombineBlocks(int b0, int nb, double* W, const unsigned char* mask, size_t na, const double* C) {
+    BasisAxpy(b0, nb, W, mask, na, C);
+  }
 
-  // Krylov storage that grows on demand instead of holding maxBlocks + 2 blocks from the start:
-  // the storage passed to the constructor holds `capacity` + 2 blocks, and grow(blocks, need, used,
-  // old) must return storage for blocks + 2 blocks whose first `used` blocks are those of `old`
-  // (and release `old`); it may lower `blocks` to what fits, but not below `need`.  Capacity grows
-  // by half each time.
-  using GrowFn = std::function<double*(int& blocks, int need, int used, double* old)>;
+  // Krylov storage that grows on demand instead of holding maxBlocks + 2 blocks from the start: the
+  // storage passed to the constructor holds `capacity` + 2 blocks (blocks 0 .. capacity, then the
+  // scratch block), and grow(blocks, need) must return storage for `blocks` more blocks, which it
+  // may lower to what fits, but not below `need`.  The new blocks are a chunk of their own: nothing
+  // is copied, so the old and the new storage never need room together.  Each growth adds at
+  // least as many blocks as the storage holds.
+  using GrowFn = std::function<double*(int& blocks, int need)>;
   void SetGrowth(int capacity, GrowFn grow) {
     Capacity = std::max(1, std::min(capacity, MaxBlocks));
+    ScratchBlock = Chunks[0].ptr + (int64_t)(Capacity + 1) * BlockDoubles;
+    Chunks[0].count = Capacity + 1;
     Grow = std::move(grow);
   }
   // room for the Krylov blocks X_0 .. X_blocks
   void Reserve(int blocks) {
     if (blocks <= Capacity) return;
     if (blocks > MaxBlocks || !Grow) throw std::runtime_error("Krylov storage exhausted");
-    int cap = std::min(MaxBlocks, std::max(blocks, Capacity + Capacity / 2));
-    Storage = Grow(cap, blocks, std::min(StepCount + 1, Capacity + 1), Storage);
-    if (cap < blocks || cap > MaxBlocks) throw std::logic_error("Krylov storage grown outside its bounds");
-    Capacity = cap;
+    const int need = blocks - Capacity;
+    int more = std::min(MaxBlocks - Capacity, std::max(need, Capacity + 1));
+    double* p = Grow(more, need);
+    if (more < need || Capacity + more > MaxBlocks) throw std::logic_error("Krylov storage grown outside its bounds");
+    Chunks.push_back({p, Capacity + 1, more});
+    Capacity += more;
   }
 
   int BlockSize() const { return K; }
@@ -475,6 +491,54 @@ class SegmentedBlockLanczos {
     return Z;
   }
 
+  // the basis blocks [b0, b0 + nb) in pieces that lie in one storage chunk each: f(ptr, first, count)
+  template <class F>
+  void ForPieces(int b0, int nb, F f) const {
+    for (const Chunk& c : Chunks) {
+      const int lo = std::max(b0, c.first), hi = std::min(b0 + nb, c.first + c.count);
+      if (lo < hi) f(c.ptr + (int64_t)(lo - c.first) * BlockDoubles, lo - b0, hi - lo);
+    }
+  }
+  bool OnePiece(int b0, int nb) const {
+    for (const Chunk& c : Chunks)
+      if (b0 >= c.first && b0 + nb <= c.first + c.count) return true;
+    return false;
+  }
+  // out[a][b][KR x KR] = V_(b0 + b)^T W over segment a of `mask` (na of them), b < nb
+  void BasisDots(int b0, int nb, const double* W, const unsigned char* mask, size_t na, double* out) {
+    if (OnePiece(b0, nb)) {
+      Ops->Dots(Block(b0), nb, W, mask, out);
+      return;
+    }
+    const size_t KK = (size_t)KR * KR;
+    ForPieces(b0, nb, [&](const double* V, int off, int cnt) {
+      std::vector<double> part(na * cnt * KK);
+      Ops->Dots(V, cnt, W, mask, part.data());
+      for (size_t a = 0; a < na; ++a)
+        std::copy(part.begin() + a * cnt * KK, part.begin() + (a + 1) * cnt * KK,
+                  out + (a * nb + off) * KK);
+    });
+  }
+  // W += sum_b V_(b0 + b) C[a][b] over segment a
+  void BasisAxpy(int b0, int nb, double* W, const unsigned char* mask, size_t na, const double* C) {
+    const size_t KK = (size_t)KR * KR;
+    ForPieces(b0, nb, [&](const double* V, int off, int cnt) {
+      std::vector<double> part(na * cnt * KK);
+      for (size_t a = 0; a < na; ++a)
+        std::copy(C + (a * nb + off) * KK, C + (a * nb + off + cnt) * KK, part.begin() + a * cnt * KK);
+      Ops->Axpy(V, cnt, W, mask, part.data());
+    });
+  }
+  // W += sum_b V_(b0 + b) C[a][b] over segment a, then gram[a] = W^T W
+  void BasisUpdate(int b0, int nb, double* W, const unsigned char* mask, size_t na, const double* C, double* gram) {
+    if (OnePiece(b0, nb)) {
+      Ops->Update(Block(b0), nb, W, mask, C, nullptr, nullptr, gram);
+      return;
+    }
+    BasisAxpy(b0, nb, W, mask, na, C);
+    Ops->Dots(W, 1, W, mask, gram);
+  }
+
   std::vector<unsigned char> Mask(const std::vector<int>& act, const std::vector<int>& idx) const {
     std::vector<unsigned char> mask(Dims.size(), 0);
     for (int i : idx) mask[act[i]] = 1;
@@ -552,8 +616,8 @@ class SegmentedBlockLanczos {
   // For the new block of a step (W = H X_j, `local`), the first round projects on the two newest
   // blocks only: in exact arithmetic W has no component on older blocks (the block three-term
   // recurrence), so this round removes the large coefficients A_j and B_j^H, as the local
-  // recurrence of FullReorthogonalizedLanczosAlgorithm.cc:225-239 does.  The second round projects
-  // on the whole basis (the full reorthogonalisation of :240-252) and removes the rounding left on
+  // recurrence of FullReorthogonalizedLanczosAlgorithm.cc:238-245 does.  The second round projects
+  // on the whole basis (the full reorthogonalisation of :246-258) and removes the rounding left on
   // every block.  Both rounds always run: one projection leaves the basis orthogonal to ~1e-13
   // only, and the Ritz values then move by more than the 1e-14 relative change that the
   // convergence test asks for.  Random replacement vectors have components on every block, so both
@@ -593,19 +657,18 @@ class SegmentedBlockLanczos {
     };
     // first round: the two newest blocks (local) or all of them; the newest is the last either way
     const int nb1 = local ? std::min(nb, 2) : nb;
-    const double* V1 = Storage + (size_t)(nb - nb1) * BlockDoubles;
-    Ops->Dots(V1, nb1, W, mask.data(), C.data());
+    BasisDots(nb - nb1, nb1, W, mask.data(), na, C.data());
     fromDevice(C.data(), nb1);
     Coefficients(CT, C.data(), nb1, all, out, true, ref);
-    Ops->Update(V1, nb1, W, mask.data(), C.data(), nullptr, nullptr, G.data());
+    BasisUpdate(nb - nb1, nb1, W, mask.data(), na, C.data(), G.data());
     // noise left by one projection is ~1e-15 of the column
     Factor(G.data(), all, out, ref, 1e-24, M1.data(), nullptr);
     for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
     Ops->Rmul(W, mask.data(), M1.data());
-    Ops->Dots(Storage, nb, W, mask.data(), C.data());
+    BasisDots(0, nb, W, mask.data(), na, C.data());
     fromDevice(C.data(), nb);
     Coefficients(CT, C.data(), nb, all, out, false, ref);
-    Ops->Update(Storage, nb, W, mask.data(), C.data(), nullptr, nullptr, G.data());
+    BasisUpdate(0, nb, W, mask.data(), na, C.data(), G.data());
     // after the second projection, a column that kept less than 1e-8 of itself was
     // (normalised) noise inside the span of the basis
     Factor(G.data(), all, out, ref, 1e-16, M2.data(), nullptr);
@@ -654,7 +717,12 @@ class SegmentedBlockLanczos {
   int K, KR, MaxBlocks;
   int Capacity;  // Krylov blocks the storage holds (blocks 0 .. Capacity, then the scratch block)
   GrowFn Grow;
-  double* Storage;
+  struct Chunk {
+    double* ptr;
+    int first, count;  // Krylov blocks [first, first + count)
+  };
+  std::vector<Chunk> Chunks;
+  double* ScratchBlock;
   int64_t BlockDoubles;
   int StepCount = 0;
   int LaunchedStep = -1;               // step whose product LaunchProduct enqueued (-1: none)

@@ -302,7 +302,7 @@ class FullReorthogonalizedBlockLanczosAlgorithm {
   int NbrEigenvalue, BlockSize, MaxIter;
   double Precision, Residual;
   int64_t Dim;
-  double* d_basis = nullptr;  // MaxBlocks + 2 blocks of Dim * BlockSize (last: scratch)
+  std::vector<double*> d_basis;  // chunks of Krylov storage (the first ends with the scratch block)
   int MaxBlocks;
   std::unique_ptr<BlockOps> Ops;
   std::unique_ptr<SegmentedBlockLanczos<double>> Engine;
@@ -386,26 +386,26 @@ struct SectorSolveResources {
   explicit SectorSolveResources(GPUArchitecture* a) : arch(a) {}
   ~SectorSolveResources() {
     dh_synchronize(arch->Context());
-    if (basis) arch->FreeMemory(basis);
+    for (double* p : basis) arch->FreeMemory(p);
     if (set) dh_sector_set_destroy(set);
   }
   SectorSolveResources(const SectorSolveResources&) = delete;
   SectorSolveResources& operator=(const SectorSolveResources&) = delete;
   GPUArchitecture* arch;
-  double* basis = nullptr;
+  std::vector<double*> basis;  // the chunks of the Krylov storage
   dh_sector_set* set = nullptr;
 };
 
 // Krylov blocks a basis starts with (about 256 vectors, never more than maxBlocks); the engine
-// grows it by half when it is full.  Holding all maxBlocks + 2 blocks from the start (160 GB for
+// grows it by chunks when it is full.  Holding all maxBlocks + 2 blocks from the start (160 GB for
 // 3000 vectors over 6.7 M rows) made every later solve of the same process run its
 // reorthogonalisation at about half speed on MI355X; sized to what the run uses, it does not.
 inline int InitialKrylovBlocks(int k, int maxBlocks) { return std::min(maxBlocks, std::max(32, 256 / k)); }
-// device storage for blocks + 2 blocks of blockDoubles that holds the first `used` blocks of `old`;
-// `old` is released.  While old is still held, `blocks` is lowered (not below `need`) to what the
-// device's free memory holds beside it, keeping 1 GiB spare; when not even `need` blocks fit, old
-// is released and KrylovStorageFull is thrown.
-double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int& blocks, int need, int64_t blockDoubles);
+// a further chunk of Krylov storage, `blocks` blocks of blockDoubles: `blocks` is lowered (not
+// below `need`) to what the device's free memory holds, keeping 1 GiB spare; when not even `need`
+// blocks fit, KrylovStorageFull is thrown.  The chunk is appended to `chunks`, whose owner frees them.
+double* GrowKrylovStorage(GPUArchitecture* arch, std::vector<double*>& chunks, int& blocks, int need,
+                          int64_t blockDoubles);
 
 // Krylov blocks each sector may use (maxIter / k + 1, and no more than its space needs); returns
 // the largest
@@ -501,9 +501,9 @@ inline int ExpectedKrylovBlocks(int64_t dim, int k, int nbrEigenvalue, int maxIt
 // smaller of memoryBytes and the device's free memory less 2 GiB.  A wave's storage starts small
 // and grows as the run needs it (SegmentedBlockLanczos::SetGrowth); sectors stopped because it
 // could not grow go back to the queue expecting 1.5 times the blocks they reached, and after each
-// wave the expectation follows the blocks its sectors used.  solveWave(sectors) fills out[s] of the
-// sectors it is given; freeBytes() is the device memory free before a wave.  Returns the number of
-// waves.
+// wave the expectation follows the blocks its sectors used.  solveWave(sectors, blocks) fills out[s]
+// of the sectors it is given (blocks: the expected Krylov blocks of the wave, the storage to start
+// with); freeBytes() is the device memory free before a wave.  Returns the number of waves.
 template <class F, class M>
 int RunSectorWaves(const std::vector<int64_t>& dims, int k, int kr, int nbrEigenvalue, int maxIter, size_t memoryBytes,
                    std::vector<SectorResult>& out, F solveWave, M freeBytes) {
@@ -533,7 +533,7 @@ int RunSectorWaves(const std::vector<int64_t>& dims, int k, int kr, int nbrEigen
     if (std::getenv("DHH_VERBOSE"))
       std::fprintf(stderr, "[sector waves] wave %d: %zu sectors, %lld rows, %d expected blocks, %zu of %zu queued\n",
                    waves + 1, wave.size(), (long long)rows, blocks, wave.size(), queue.size());
-    solveWave(wave);
+    solveWave(wave, blocks);
     ++waves;
     int reached = 0;
     std::vector<int> again;
@@ -564,7 +564,8 @@ class MultiSectorBlockLanczos {
   void SetLaunch(const SectorLaunch& launch) { Launch = launch; }
 
  private:
-  void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
+  // expectedBlocks: the Krylov blocks the wave planner expects the sectors to need
+  void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out, int expectedBlocks);
   SectorLaunch Launch;
   GPUArchitecture* Arch;
   int NbrFermions, LzMax, NbrEigenvalue, BlockSize, MaxIter;
@@ -616,7 +617,8 @@ class LatticeSectorsBlockLanczos {
   void SetLaunch(const SectorLaunch& launch) { Launch = launch; }
 
  private:
-  void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out);
+  // expectedBlocks: the Krylov blocks the wave planner expects the sectors to need
+  void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out, int expectedBlocks);
   dh_sector_set* CreateSet(const std::vector<int>& kx, const std::vector<int>& ky) const;
   bool TorusTranslations = false;
   SectorLaunch Launch;

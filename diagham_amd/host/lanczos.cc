@@ -247,50 +247,34 @@ FullReorthogonalizedBlockLanczosAlgorithm::FullReorthogonalizedBlockLanczosAlgor
   MaxBlocks = (int)std::min<int64_t>(std::max(3, MaxIter / BlockSize + 1), (Dim + BlockSize - 1) / BlockSize + 1);
   if (!Dense()) {
     const int capacity = InitialKrylovBlocks(BlockSize, MaxBlocks);
-    d_basis = Arch->AllocateMemory((int64_t)(capacity + 2) * Dim * BlockSize);
+    d_basis.push_back(Arch->AllocateMemory((int64_t)(capacity + 2) * Dim * BlockSize));
     Ops.reset(new HamiltonianOps(Arch, Hamiltonian, Dim, BlockSize));
     Engine.reset(new SegmentedBlockLanczos<double>(Ops.get(), std::vector<int64_t>{Dim}, BlockSize, MaxBlocks,
-                                                   d_basis, Dim * BlockSize));
-    Engine->SetGrowth(capacity, [this](int& blocks, int need, int used, double* old) {
-      d_basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
-      d_basis = GrowKrylovStorage(Arch, old, used, blocks, need, Dim * BlockSize);
-      return d_basis;
+                                                   d_basis[0], Dim * BlockSize));
+    Engine->SetGrowth(capacity, [this](int& blocks, int need) {
+      return GrowKrylovStorage(Arch, d_basis, blocks, need, Dim * BlockSize);
     });
   }
 }
 
-double* GrowKrylovStorage(GPUArchitecture* arch, double* old, int used, int& blocks, int need, int64_t blockDoubles) {
-  double* p = nullptr;
+double* GrowKrylovStorage(GPUArchitecture* arch, std::vector<double*>& chunks, int& blocks, int need,
+                          int64_t blockDoubles) {
   dh_context* c = arch->Context();
   size_t freeBytes = 0, totalBytes = 0;
-  bool fits = true;
   if (dh_memory_info(c, &freeBytes, &totalBytes) == DH_OK) {
     const size_t spare = (size_t)1 << 30, blockBytes = sizeof(double) * (size_t)blockDoubles;
-    const int64_t fit = freeBytes > spare ? (int64_t)((freeBytes - spare) / blockBytes) - 2 : 0;
-    if (blocks > fit) blocks = (int)std::max<int64_t>(need, fit);
-    fits = fit >= need;
+    const int64_t fit = freeBytes > spare ? (int64_t)((freeBytes - spare) / blockBytes) : 0;
+    if (fit < need) throw KrylovStorageFull("the device memory cannot hold " + std::to_string(need) + " more Krylov blocks");
+    if (blocks > fit) blocks = (int)fit;
   }
-  if (!fits) {
-    dh_synchronize(c);
-    arch->FreeMemory(old);
-    throw KrylovStorageFull("the device memory cannot hold " + std::to_string(need) + " Krylov blocks");
-  }
-  try {
-    p = arch->AllocateMemory((int64_t)(blocks + 2) * blockDoubles);
-    if (used > 0) arch->Check(dh_copy(c, p, old, (int64_t)used * blockDoubles), "grow Krylov storage");
-    arch->Check(dh_synchronize(c), "sync");
-  } catch (...) {
-    dh_synchronize(c);
-    if (p) arch->FreeMemory(p);
-    arch->FreeMemory(old);
-    throw;
-  }
-  arch->FreeMemory(old);
+  double* p = arch->AllocateMemory((int64_t)blocks * blockDoubles);
+  chunks.push_back(p);
   return p;
 }
 
 FullReorthogonalizedBlockLanczosAlgorithm::~FullReorthogonalizedBlockLanczosAlgorithm() {
-  if (d_basis) Arch->FreeMemory(d_basis);
+  if (!d_basis.empty()) dh_synchronize(Arch->Context());
+  for (double* p : d_basis) Arch->FreeMemory(p);
 }
 
 void FullReorthogonalizedBlockLanczosAlgorithm::InitializeLanczosAlgorithm(uint64_t seed) {
@@ -489,7 +473,8 @@ void FullReorthogonalizedBlockLanczosAlgorithm::GetEigenstates(int nbr, double* 
     for (int b = 0; b < m; ++b)
       for (int p = 0; p < k; ++p)
         for (int q = 0; q < k && g0 + q < nbr; ++q) S[((size_t)b * k + p) * k + q] = vecs[(size_t)(g0 + q) * n + b * k + p];
-    Arch->Check(dh_block_axpy(c, d_basis, m, tmp, Dim, k, S.data()), "Ritz vectors");
+    const unsigned char one = 1;
+    Engine->CombineBlocks(0, m, tmp, &one, 1, S.data());
     for (int q = 0; q < k && g0 + q < nbr; ++q)
       Arch->Check(dh_block_get_column(c, tmp, Dim, k, q, d_out + (int64_t)(g0 + q) * Dim), "extract");
   }

@@ -194,7 +194,8 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
   // waves whose expected Krylov storage fits the device (RunSectorWaves)
   Waves = RunSectorWaves(
       dims, k, 2 * k, NbrEigenvalue, MaxIter, MemoryBytes, out,
-      [&](const std::vector<int>& wave) { SolveWave(wave, seed, out); }, [&] { return DeviceFreeBytes(Arch); });
+      [&](const std::vector<int>& wave, int blocks) { SolveWave(wave, seed, out, blocks); },
+      [&] { return DeviceFreeBytes(Arch); });
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return out;
 }
@@ -214,7 +215,7 @@ dh_sector_set* LatticeSectorsBlockLanczos::CreateSet(const std::vector<int>& kx,
 }
 
 void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_t seed,
-                                           std::vector<SectorResult>& out) {
+                                           std::vector<SectorResult>& out, int expectedBlocks) {
   dh_context* c = Arch->Context();
   const int k = BlockSize;
   const int na = (int)sectors.size();
@@ -246,13 +247,13 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   std::vector<int> blockCap;
   const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
   const int64_t blockDoubles = total * 2 * k;
-  const int capacity = InitialKrylovBlocks(k, maxBlocks);
-  guard.basis = Arch->AllocateMemory((int64_t)(capacity + 2) * blockDoubles);
+  // the blocks the planner expects the wave to need, or the usual start when it expects fewer
+  const int capacity = std::min(maxBlocks, std::max(InitialKrylovBlocks(k, maxBlocks), expectedBlocks));
+  guard.basis.push_back(Arch->AllocateMemory((int64_t)(capacity + 2) * blockDoubles));
   SectorSetOps ops(Arch, S, k, 2 * k, total);
-  SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, guard.basis, blockDoubles);
-  lanczos.SetGrowth(capacity, [&](int& blocks, int need, int used, double* old) {
-    guard.basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
-    return guard.basis = GrowKrylovStorage(Arch, old, used, blocks, need, blockDoubles);
+  SegmentedBlockLanczos<cplx> lanczos(&ops, dims, k, maxBlocks, guard.basis[0], blockDoubles);
+  lanczos.SetGrowth(capacity, [&](int& blocks, int need) {
+    return GrowKrylovStorage(Arch, guard.basis, blocks, need, blockDoubles);
   });
   std::vector<SectorResult*> res;
   for (int s : sectors) res.push_back(&out[s]);

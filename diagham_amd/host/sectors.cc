@@ -116,10 +116,10 @@ std::vector<SectorResult> MultiSectorBlockLanczos::Solve(uint64_t seed, double* 
   std::vector<SectorResult> part(lanczosSectors.size());
   Waves = RunSectorWaves(
       dims, BlockSize, BlockSize, NbrEigenvalue, MaxIter, MemoryBytes, part,
-      [&](const std::vector<int>& wave) {
+      [&](const std::vector<int>& wave, int blocks) {
         std::vector<int> sectors;
         for (int i : wave) sectors.push_back(lanczosSectors[i]);
-        SolveWave(sectors, seed, out);
+        SolveWave(sectors, seed, out, blocks);
         for (int i : wave) part[i] = out[lanczosSectors[i]];
       },
       [&] { return DeviceFreeBytes(Arch); });
@@ -128,7 +128,7 @@ std::vector<SectorResult> MultiSectorBlockLanczos::Solve(uint64_t seed, double* 
 }
 
 void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_t seed,
-                                        std::vector<SectorResult>& out) {
+                                        std::vector<SectorResult>& out, int expectedBlocks) {
   dh_context* c = Arch->Context();
   const int k = BlockSize;
   const int na = (int)sectors.size();
@@ -157,13 +157,13 @@ void MultiSectorBlockLanczos::SolveWave(const std::vector<int>& sectors, uint64_
   for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
   std::vector<int> blockCap;
   const int maxBlocks = SectorBlockCaps(dims, k, MaxIter, blockCap);
-  const int capacity = InitialKrylovBlocks(k, maxBlocks);
-  res.basis = Arch->AllocateMemory((int64_t)(capacity + 2) * total * k);
+  // the blocks the planner expects the wave to need, or the usual start when it expects fewer
+  const int capacity = std::min(maxBlocks, std::max(InitialKrylovBlocks(k, maxBlocks), expectedBlocks));
+  res.basis.push_back(Arch->AllocateMemory((int64_t)(capacity + 2) * total * k));
   SectorSetOps ops(Arch, S, k, k, total);
-  SegmentedBlockLanczos<double> lanczos(&ops, dims, k, maxBlocks, res.basis, total * k);
-  lanczos.SetGrowth(capacity, [&](int& blocks, int need, int used, double* old) {
-    res.basis = nullptr;  // GrowKrylovStorage releases old, also when it fails
-    return res.basis = GrowKrylovStorage(Arch, old, used, blocks, need, total * k);
+  SegmentedBlockLanczos<double> lanczos(&ops, dims, k, maxBlocks, res.basis[0], total * k);
+  lanczos.SetGrowth(capacity, [&](int& blocks, int need) {
+    return GrowKrylovStorage(Arch, res.basis, blocks, need, total * k);
   });
   std::vector<SectorResult*> results;
   for (int s : sectors) results.push_back(&out[s]);

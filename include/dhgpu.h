@@ -196,18 +196,19 @@ int dh_sector_set_get_states(dh_sector_set* set, uint64_t* states);
 int dh_sector_set_factors(dh_sector_set* set, const double* factors, const int* pair_offset, int nbr_factors,
                           double shift);
 int dh_sector_set_launch(dh_sector_set* set, int lanes_per_row, int block_size);
-/* Row routine of the products of a sphere set: DH_INDEX_RANK (every created pair of the range,
- * skipping occupied ones) or DH_INDEX_RANK_FREE (only the free created pairs, from one bit mask per
- * annihilated pair, lookups batched per lane).  A tuning knob like dh_sector_set_launch; the
- * results are the same. */
+/* Row routine of the products of a sphere or momentum set: DH_INDEX_RANK (every created pair of
+ * the range or list, skipping occupied ones) or DH_INDEX_RANK_FREE (only the free created pairs, from
+ * one bit mask per annihilated pair, lookups batched per lane).  A tuning knob like
+ * dh_sector_set_launch; the results agree to rounding. */
 int dh_sector_set_row_mode(dh_sector_set* set, int mode);
 /* Launch shape of the segmented vector operations of a set (no counterpart in the reference, a
  * tuning knob like dh_sector_set_launch): persistent axpy / rmul workgroups per CU (default 16) and
  * rows per dots tile (0: automatic). */
 int dh_sector_set_blas_launch(dh_sector_set* set, int axpy_groups_per_cu, int64_t dots_rows_per_tile);
-/* Matrix elements per product of each sector of a sphere set (counts[nbr_sectors]), counted as
- * dh_hamiltonian_count_elements does: the "nbr interaction" the reference prints per sector
- * (AbstractQHEOnSphereHamiltonian.cc:3721 PartialFastMultiplicationMemory). */
+/* Matrix elements per product of each sector of a sphere or momentum set (counts[nbr_sectors]),
+ * counted as dh_hamiltonian_count_elements does: the "nbr interaction" the reference prints per
+ * sector (AbstractQHEOnSphereHamiltonian.cc:3721 PartialFastMultiplicationMemory); for momentum
+ * sets every (annihilated pair, created pair) term of a momentum sum whose target is in the space. */
 int dh_sector_set_count_elements(dh_sector_set* set, int64_t* counts);
 /* y = (add ? y : 0) + (H + shift) x over the active sectors (LowLevelMultipleAddMultiply per sector) */
 int dh_sectors_hv(dh_sector_set* set, const double* d_x, double* d_y, int k, int add, const unsigned char* active);

@@ -972,6 +972,69 @@ void or_fs_hv_rows(void* space, void* factors, const double* x, double* out, dou
   }
 }
 
+/* The elements of every row stored once (CSR: row offsets, targets, coefficients), for solvers
+ * that apply the same H many times; or_csr_hv is then the gather product of or_fs_hv_rows. */
+typedef struct {
+  long dim, nnz;
+  long* rowOff;
+  int* col;
+  double* val;
+} or_csr_t;
+
+void* or_fs_csr_create(void* space, void* factors, int nbrThreads) {
+  const fs_space* sp = (const fs_space*)space;
+  const or_factors_t* f = (const or_factors_t*)factors;
+  const long dim = sp->dim, cap = f->nbrFactors > 0 ? f->nbrFactors : 1;
+  or_csr_t* m = (or_csr_t*)calloc(1, sizeof(or_csr_t));
+  m->dim = dim;
+  m->rowOff = (long*)calloc(dim + 1, sizeof(long));
+  if (nbrThreads < 1) nbrThreads = 1;
+#pragma omp parallel for num_threads(nbrThreads) schedule(dynamic, 256)
+  for (long r = 0; r < dim; ++r) m->rowOff[r + 1] = or_fs_row_elements(space, factors, r, NULL, NULL, 0);
+  for (long r = 0; r < dim; ++r) m->rowOff[r + 1] += m->rowOff[r];
+  m->nnz = m->rowOff[dim];
+  m->col = (int*)malloc(sizeof(int) * (m->nnz > 0 ? m->nnz : 1));
+  m->val = (double*)malloc(sizeof(double) * (m->nnz > 0 ? m->nnz : 1));
+#pragma omp parallel num_threads(nbrThreads)
+  {
+    long* t = (long*)malloc(sizeof(long) * cap);
+    double* c = (double*)malloc(sizeof(double) * cap);
+#pragma omp for schedule(dynamic, 256)
+    for (long r = 0; r < dim; ++r) {
+      long n = or_fs_row_elements(space, factors, r, t, c, cap);
+      for (long e = 0; e < n; ++e) {
+        m->col[m->rowOff[r] + e] = (int)t[e];
+        m->val[m->rowOff[r] + e] = c[e];
+      }
+    }
+    free(t);
+    free(c);
+  }
+  return m;
+}
+
+void or_csr_destroy(void* p) {
+  or_csr_t* m = (or_csr_t*)p;
+  if (!m) return;
+  free(m->rowOff);
+  free(m->col);
+  free(m->val);
+  free(m);
+}
+
+long or_csr_nnz(void* p) { return ((or_csr_t*)p)->nnz; }
+
+void or_csr_hv(void* p, const double* x, double* y, int nbrThreads) {
+  const or_csr_t* m = (const or_csr_t*)p;
+  if (nbrThreads < 1) nbrThreads = 1;
+#pragma omp parallel for num_threads(nbrThreads) schedule(static, 4096)
+  for (long r = 0; r < m->dim; ++r) {
+    double acc = 0.0;
+    for (long e = m->rowOff[r]; e < m->rowOff[r + 1]; ++e) acc += m->val[e] * x[m->col[e]];
+    y[r] = acc;
+  }
+}
+
 long or_fs_count_elements(void* space, void* factors) {
   fs_space* s = (fs_space*)space;
   long total = 0;

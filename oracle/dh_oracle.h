@@ -53,6 +53,12 @@ void or_fs_hv_smp(void* space, void* factors, const double* x, double* y, double
                   int hermitian, int nbrThreads);
 long or_fs_row_elements(void* space, void* factors, long row, long* targets, double* coefs, long maxOut);
 long or_fs_count_elements(void* space, void* factors);
+/* every row's elements stored once (CSR) and its gather product y = H x, for solvers that apply
+ * the same H many times */
+void* or_fs_csr_create(void* space, void* factors, int nbrThreads);
+void or_csr_destroy(void* csr);
+long or_csr_nnz(void* csr);
+void or_csr_hv(void* csr, const double* x, double* y, int nbrThreads);
 /* rows [first, last) of (H + shift) X for k vectors ([row][k]) in gather form, into
  * out[0 .. (last - first) k) */
 void or_fs_hv_rows(void* space, void* factors, const double* x, double* out, double shift, long first, long last,

@@ -9,7 +9,8 @@ FQHESphereFermionsTwoBodyGeneric.cc:79-81, :170; ParticleOnSphereL2Hamiltonian.c
 NBR_EIGEN eigenpairs of H in Lz = 0 are labelled by their total angular momentum, and the two lowest
 with L = 0 are kept (H conserves L, so its eigenvectors in Lz = 0 have a definite L).
 
-Here H x is the oracle's product (or_fs_hv_smp).  L is read from <L^2> = |L^+ psi|^2 (Lz = 0), with
+Here H x is the oracle's product, with every row's elements stored once (or_fs_csr_create,
+or_csr_hv).  L is read from <L^2> = |L^+ psi|^2 (Lz = 0), with
 L^+ = sum_p sqrt((2S - p)(p + 1)) a+_{p+1} a_p built independently from Jordan-Wigner ladder operators
 (adjacent hops carry no sign) as a sparse map from the Lz = 0 to the Lz = 1 sector of the oracle's
 bases.  The eigenpairs come from a full-reorthogonalisation Lanczos (three-term recurrence, then two
@@ -34,7 +35,7 @@ N, LZ_MAX = 10, 27
 FREE_M = [1, 3, 5, 7, 9, 11]
 H = 1e-5
 SEED = 4
-NBR_EIGEN = 24
+NBR_EIGEN = 14  # the second L = 0 state is the 14th state of the sector
 
 
 def pseudopotentials():
@@ -95,14 +96,8 @@ def lowest(apply, dim, nev, precision=1e-14, max_iter=1500):
 
 def energies(space, lp, v, threads):
     """the two lowest L = 0 eigenvalues of H(v), the L of the NBR_EIGEN lowest states, iterations"""
-    f = O.Factors(LZ_MAX, v)
-
-    def apply(x):
-        y = np.zeros_like(x)
-        O.lib().or_fs_hv_smp(space.h, f.h, O.dptr(x), O.dptr(y), 0.0, 0, threads)
-        return y
-
-    ev, vecs, it = lowest(apply, space.dim, NBR_EIGEN)
+    rows = O.StoredRows(space, O.Factors(LZ_MAX, v), threads)
+    ev, vecs, it = lowest(rows.hv, space.dim, NBR_EIGEN)
     labels = []
     for x in vecs:
         l2 = np.linalg.norm(lp @ x) ** 2 / (x @ x)

@@ -110,8 +110,8 @@ struct HostOps : BlockOps {
   }
 };
 
-// grow: the Krylov storage starts with two blocks and grows on demand (SetGrowth), each time into
-// a fresh host buffer; the output must not change.
+// grow: the Krylov storage starts with two blocks and grows on demand (SetGrowth), each time by a
+// fresh host chunk filled with poison; the output must not change.
 template <class T>
 void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed, bool grow) {
   std::vector<int64_t> dims;
@@ -123,21 +123,21 @@ void Run(HostOps& ops, int maxIter, int nev, double precision, uint64_t seed, bo
   std::vector<double> storage((size_t)(first + 2) * bd, 0.0);
   SegmentedBlockLanczos<T> lanczos(&ops, dims, ops.k, maxBlocks, storage.data(), bd);
   int growths = 0;
+  std::vector<std::vector<double>> chunks;
   if (grow)
-    lanczos.SetGrowth(first, [&](int& blocks, int need, int used, double* old) {
+    lanczos.SetGrowth(first, [&](int& blocks, int need) {
       if (blocks > need && growths % 2) blocks = need;  // every other growth takes only what is needed
-      if (old != storage.data()) throw std::logic_error("growth from storage the engine does not own");
-      std::vector<double> next((size_t)(blocks + 2) * bd, -7.0);  // poison: only the used blocks are kept
-      std::copy(old, old + (size_t)used * bd, next.begin());
-      storage.swap(next);
+      chunks.emplace_back((size_t)blocks * bd, -7.0);  // poison: every block is written before it is read
       ++growths;
-      return storage.data();
+      return chunks.back().data();
     });
   std::vector<SectorResult> out(dims.size());
   std::vector<SectorResult*> res;
   for (auto& r : out) res.push_back(&r);
   RunSectorLanczos(lanczos, caps, nev, precision, seed, res);
-  if (grow) std::fprintf(stderr, "growths %d capacity %d\n", growths, lanczos.BlockCapacity());
+  if (grow)
+    std::fprintf(stderr, "growths %d capacity %d chunks %d\n", growths, lanczos.BlockCapacity(),
+                 lanczos.NbrStorageChunks());
   for (auto& r : out) {
     std::printf("%d %d", r.converged ? 1 : 0, r.products);
     for (double e : r.eigenvalues) std::printf(" %.17g", e);

@@ -63,6 +63,10 @@ def lib():
     _sig(L, "or_fs_row_elements", c_long, [c_void_p, c_void_p, c_long, P_L, P_D, c_long])
     _sig(L, "or_fs_count_elements", c_long, [c_void_p, c_void_p])
     _sig(L, "or_fs_hv_rows", None, [c_void_p, c_void_p, P_D, P_D, c_double, c_long, c_long, c_int, c_int])
+    _sig(L, "or_fs_csr_create", c_void_p, [c_void_p, c_void_p, c_int])
+    _sig(L, "or_csr_destroy", None, [c_void_p])
+    _sig(L, "or_csr_nnz", c_long, [c_void_p])
+    _sig(L, "or_csr_hv", None, [c_void_p, P_D, P_D, c_int])
     _sig(L, "or_spin_create", c_void_p, [c_int, c_int, c_int])
     _sig(L, "or_spin_destroy", None, [c_void_p])
     _sig(L, "or_spin_dim", c_long, [c_void_p])
@@ -209,6 +213,26 @@ def fs_hv_rows(space, factors, x, first, last, shift=0.0, k=1, threads=8):
     out = np.zeros((last - first) * k)
     lib().or_fs_hv_rows(space.h, factors.h, dptr(x), dptr(out), shift, first, last, k, threads)
     return out
+
+
+class StoredRows:
+    """The rows of H (or_fs_csr_create) for repeated products (or_csr_hv)."""
+
+    def __init__(self, space, factors, threads=8):
+        self.h = lib().or_fs_csr_create(space.h, factors.h, threads)
+        self.dim, self.threads = space.dim, threads
+        self.nnz = lib().or_csr_nnz(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_csr_destroy(self.h)
+            self.h = None
+
+    def hv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        lib().or_csr_hv(self.h, dptr(x), dptr(y), self.threads)
+        return y
 
 
 def nbody_tuples(nbr_orbitals, nbody):

@@ -157,3 +157,45 @@ def test_energy_gradients_match_finite_differences(arch):
         assert abs(a - b) <= 1e-6 * abs(b), ("dE0", m, a, b)
         a, b = gg[m].item(), fd["dGap_dV"][i]
         assert abs(a - b) <= 1e-6 * max(abs(b), 1e-3), ("dGap", m, a, b)
+
+
+def test_l0_gap_gradients_match_finite_differences(arch):
+    """The differentiation workload as specified: the two lowest L = 0 states of N=10, 2S=27, Lz=0.
+    They are the lowest states of H + L^2 (the --l2-factor of FQHESphereFermionsTwoBodyGeneric.cc:170,
+    every L > 0 state moved up by L (L + 1)); their eigenvectors give dE/dV_m = <psi|H_m|psi> through
+    the adjoint of H alone.  Energies against the oracle's L = 0 levels (1e-10), gradients against
+    central differences of them (tests/golden/fd_n10_2s27_l0.json, 1e-6 relative)."""
+    torch = _torch()
+    from diagham_amd import capi
+    path = os.path.join(HERE, "golden", "fd_n10_2s27_l0.json")
+    if not os.path.exists(path):
+        pytest.skip("finite-difference fixture of the L = 0 levels not generated")
+    fd = json.load(open(path))
+    n, tlz, lz = fd["nbr_fermions"], fd["total_lz"], fd["lz_max"]
+    V = np.array(fd["pseudopotentials"])
+    shifted = capi.Problem.sphere_l2(arch, n, tlz, lz, 1.0, pseudo=V)
+    vecs = torch.empty((2, shifted.dim), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    r = shifted.lanczos(2, block_size=1, max_iter=3000, d_vectors=vecs.data_ptr(), nbr_vectors=2)
+    assert r["converged"]
+    E = r["eigenvalues"]
+    assert abs(E[0] - fd["E0"]) < 1e-10 and abs(E[1] - fd["E1"]) < 1e-10, (E, fd["E0"], fd["E1"])
+    L2 = capi.Problem.sphere_l2(arch, n, tlz, lz, 1.0)
+    y = torch.empty_like(vecs[0])
+    for i in range(2):
+        torch.cuda.synchronize()
+        L2.hv_device(vecs[i].data_ptr(), y.data_ptr(), 1)
+        arch.sync()
+        assert abs(torch.dot(vecs[i], y).item()) < 1e-8  # L = 0
+    p = capi.Problem.sphere(arch, n, tlz, lz, V)
+    mom = list(p.channels())
+    g = [p.adjoint_device(vecs[i].data_ptr(), vecs[i].data_ptr()) for i in range(2)]
+    arch.sync()
+    for j, m in enumerate(fd["free_m"]):
+        c = mom.index(m)
+        a, b = g[0][c], fd["dE0_dV"][j]
+        assert abs(a - b) <= 1e-6 * abs(b), ("dE0", m, a, b)
+        a, b = g[1][c] - g[0][c], fd["dGap_dV"][j]
+        assert abs(a - b) <= 1e-6 * max(abs(b), 1e-3), ("dGap", m, a, b)
+    for q in (shifted, L2, p):
+        q.close()

@@ -66,12 +66,13 @@ class MomentumSet:
         self.capi.gpu().dh_sector_set_destroy(self.h)
 
 
-@pytest.mark.parametrize("lpr", [1, 2, 4])
-def test_momentum_basis_and_complex_products(arch, lpr):
+@pytest.mark.parametrize("lpr,row_mode,block", [(1, 1, 256), (2, 1, 512), (4, 1, 256), (1, 3, 256), (2, 3, 512)])
+def test_momentum_basis_and_complex_products(arch, lpr, row_mode, block):
     n, nx, ny = 4, 4, 3
     sectors = [(0, 0), (1, 2), (3, 1)]
     S = MomentumSet(arch, n, nx, ny, sectors)
-    S.capi.gpu().dh_sector_set_launch(S.h, lpr, 256)
+    S.capi.gpu().dh_sector_set_launch(S.h, lpr, block)
+    assert S.capi.gpu().dh_sector_set_row_mode(S.h, row_mode) == 0
     states = S.states()
     u, _ = O.checkerboard_lower_band(nx, ny, T1, T2, TPP)
     fac = O.LatticeFactors(nx, ny, u)
@@ -119,3 +120,19 @@ def test_complex_block_lanczos_matches_dense(arch):
                 e[c] = 1.0
                 H[:, c] = O.fm_hv(ref, fac, e)
             np.testing.assert_allclose(ev[i], np.linalg.eigvalsh(H)[:nev], rtol=0, atol=1e-10)
+
+
+def test_momentum_element_counts_match_oracle_rows(arch):
+    """Matrix elements per sector of a momentum set: the terms the oracle lists row by row."""
+    from diagham_amd import capi
+    n, nx, ny = 4, 4, 3
+    sectors = [(0, 0), (1, 2), (3, 1)]
+    S = capi.MomentumSectorSet(arch, n, nx, ny, [s[0] for s in sectors], [s[1] for s in sectors])
+    counts = S.count_elements()
+    u, _ = O.checkerboard_lower_band(nx, ny, T1, T2, TPP)
+    fac = O.LatticeFactors(nx, ny, u)
+    for i, (kx, ky) in enumerate(sectors):
+        ref = O.MomentumFermions(n, nx, ny, kx, ky)
+        expect = sum(len(O.fm_row_elements(ref, fac, r)[0]) for r in range(ref.dim))
+        assert counts[i] == expect, (kx, ky, counts[i], expect)
+    S.close()

@@ -360,3 +360,17 @@ def test_free_pair_rows_are_bitwise_the_range_rows(n8_2s21, k, lpr):
     else:  # lanes take different terms, and their partial sums meet in another order
         assert np.linalg.norm(Y1 - Y3) <= 1e-14 * np.linalg.norm(Y1)
 
+
+
+@pytest.mark.parametrize("n,total_lz,lz_max", [(4, 0, 8), (4, 2, 8), (5, 3, 9), (6, 0, 11)])
+def test_l2_product_is_the_ladder_operator(arch, n, total_lz, lz_max):
+    """The L^2 product on the device, column by column, equals L^- L^+ + Lz^2 + Lz built from
+    Jordan-Wigner ladder operators (ParticleOnSphereL2Hamiltonian.cc:240 defines the operator)."""
+    from diagham_amd import capi
+    from test_l2_cpu import ladder_l2
+    p = capi.Problem.sphere_l2(arch, n, total_lz, lz_max, 1.0)
+    L2 = ladder_l2(n, total_lz, lz_max)
+    assert p.dim == L2.shape[0]
+    H = np.stack([p.hv(e) for e in np.eye(p.dim)], axis=1)
+    assert np.max(np.abs(H - L2)) < 1e-12 * max(1.0, np.max(np.abs(L2)))
+    p.close()

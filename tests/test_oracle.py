@@ -338,6 +338,16 @@ def test_row_gather_form_equals_scatter():
         assert np.linalg.norm(Y[:, v] - yv) <= 1e-13 * np.linalg.norm(yv)
 
 
+def test_stored_rows_equal_the_product():
+    s = O.FermionSphere(8, 0, 21)
+    f = O.Factors(21, coulomb(21))
+    m = O.StoredRows(s, f, threads=4)
+    assert m.nnz == O.fs_count_elements(s, f)
+    x = O.splitmix(8, s.dim)
+    y = O.fs_hv(s, f, x)
+    assert np.linalg.norm(m.hv(x) - y) <= 1e-13 * np.linalg.norm(y)
+
+
 def test_lanczos_matches_dense_small():
     n, tlz, lz = 5, 1, 11
     s = O.FermionSphere(n, tlz, lz)

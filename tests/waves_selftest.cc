@@ -28,7 +28,7 @@ int main(int argc, char** argv) {
   try {
     const int waves = dhh::RunSectorWaves(
         dims, k, kr, nev, maxIter, (size_t)-1, out,
-        [&](const std::vector<int>& wave) {
+        [&](const std::vector<int>& wave, int) {
           int64_t rows = 0;
           for (int s : wave) rows += dims[s];
           const int64_t fit = (int64_t)(freeBytes / (8 * (uint64_t)rows) - 1) / kr - 2;

@@ -26,7 +26,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-munsafe-f
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-ffp-contract=fast", "-Wno-unused-result"]
 GPU_SOURCES = ["dh_core.hip", "dh_basis.hip", "dh_hv.hip", "dh_aux.hip", "dh_multisector.hip", "dh_lattice.hip",
-               "dh_nbody.hip", "dh_torus.hip", "dh_update.hip"]
+               "dh_nbody.hip", "dh_torus.hip", "dh_update.hip", "dh_noalloc.hip"]
 HOST_SOURCES = ["clebsch_gordan.cc", "fermion_on_sphere.cc", "hamiltonians.cc", "lanczos.cc", "sectors.cc",
                 "lattice.cc", "torus.cc", "vector_io.cc", "host_api.cc"]
 

@@ -86,6 +86,7 @@ GPU_SIGNATURES = {
     "dh_torus_translation_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, P_I, P_I, c_int]),
     "dh_sector_set_torus_data": (c_int, [c_void_p, c_void_p, P_U64]),
     "dh_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
+    "dh_block_dots_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p]),
     "dh_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D]),
     "dh_block_rmul": (c_int, [c_void_p, c_void_p, c_int64, c_int, P_D]),
     "dh_block_update": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, P_D, P_D, P_D, P_D]),

@@ -272,6 +272,16 @@ int dh_sphere_adjoint(dh_hamiltonian* ham, const double* d_psi, const double* d_
  *   c[(b * k + p) * k + q] = sum_row basis_b[row][p] * w[row][q]   (host result, nbr_blocks*k*k) */
 int dh_block_dots(dh_context* ctx, const double* d_basis, int nbr_blocks, const double* d_w, int64_t dim,
                   int k, double* c);
+
+/* dh_block_dots without any allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The row-chunk partial sums live in a fixed device array of 2^19 doubles (4 MiB) per device, held
+ * until the process ends; the result goes to device memory d_out (nbr_blocks * k * k doubles, the
+ * layout of dh_block_dots) and is valid after the stream has reached it (no wait, no pinned
+ * staging).  nbr_blocks * k * k above 2^19 is refused with DH_E_RANGE.  Every call on one device
+ * shares that array: calls that overlap on one device race. */
+int dh_block_dots_noalloc(dh_context* ctx, const double* d_basis, int nbr_blocks, const double* d_w, int64_t dim,
+                          int k, double* d_out);
 /* AddRealLinearCombinationOperation (AddRealLinearCombinationOperation.cc:157):
  *   w[row][q] += sum_{b,p} basis_b[row][p] * c[(b * k + p) * k + q]   (c on the host) */
 int dh_block_axpy(dh_context* ctx, const double* d_basis, int nbr_blocks, double* d_w, int64_t dim, int k,

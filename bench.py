@@ -11,9 +11,16 @@ bytes per launch from HIP events on the kernel's stream), the measured HBM traff
                     lowest of N = 8 fermions at 2S = 21; and "all_sectors": time to the 10 lowest
                     eigenvalues of every Lz >= 0 sector of N = 10 fermions at 2S = 27, the plain
                     one-vector Lanczos sector after sector vs the batched block Lanczos
+                    (plain, batched k = 1, batched block k = 8), the same for the 133 sectors of
+                    N = 12 at 2S = 33 (plain vs batched block k = 8) and for the 30 (kx, ky) sectors
+                    of N = 10 on 6 x 5 checkerboard sites (plain vs batched k = 1 and k = 4), and the
+                    sector autotuner's choice for N = 10 at 2S = 27
+  index_lookup      the headline product with the LUT-seeded binary search instead of the rank
+  adjoint           dE0/dV_m of N = 10 at 2S = 27, Lz = 0 in one pass, beside one product
   batched_roofline  one k = 8 product over every Lz >= 0 sector of N = 12 fermions at 2S = 33
                     in one persistent launch (the batching workload's kernel)
-  cpu_baseline      the oracle's CPU rate on a bounded sample of the headline product
+  cpu_baseline      the oracle's CPU rate on a bounded sample of the headline product, and its
+                    sphere and lattice products and a short Lanczos solve
 """
 import argparse
 import ctypes
@@ -67,9 +74,10 @@ def load_traffic(name):
         return json.load(f)
 
 
-def spin_hv_bench(capi, arch, stream, L, steps, warmup, k=1):
+def spin_hv_bench(capi, arch, stream, L, steps, warmup, k=1, index_mode=None):
     p = capi.Problem.spin_chain(arch, L, 0, 1.0, 1.0, True)
-    p.set_launch(1, 1024, capi.INDEX_RANK, 1)  # 1024-thread workgroups: the fastest measured shape
+    # 1024-thread workgroups: the fastest measured shape
+    p.set_launch(1, 1024, capi.INDEX_RANK if index_mode is None else index_mode, 1)
     nnz = p.count_elements()
     x = capi.DeviceBuffer(arch, p.dim * k)
     y = capi.DeviceBuffer(arch, p.dim * k)
@@ -245,6 +253,49 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
             "measured_traffic_GBps": traffic and traffic["bytes_per_launch"] / (ms * 1e-3) / 1e9}
 
 
+def adjoint_leg(capi, arch, stream, n=10, lz=27, reps=10):
+    """Gradient of the ground-state energy of N = 10 fermions at 2S = 27, Lz = 0 with respect to every
+    pseudopotential: dh_sphere_adjoint forms all channels' bilinears <psi|dH/dV_m|psi> in one pass
+    over the rows, timed beside one product of the same Hamiltonian (both on the bench stream)."""
+    pp = capi.coulomb_pseudopotentials(lz, True)
+    p = capi.Problem.sphere(arch, n, 0, lz, pp)
+    psi = capi.DeviceBuffer(arch, p.dim)
+    y = capi.DeviceBuffer(arch, p.dim)
+    r = p.lanczos(1, d_vectors=psi.ptr, nbr_vectors=1)
+    nnz = p.count_elements()
+    timer = HipTimer(stream)
+    g = p.adjoint_device(psi.ptr, psi.ptr)
+    timer.start()
+    for _ in range(reps):
+        g = p.adjoint_device(psi.ptr, psi.ptr)
+    ms_adj = timer.stop_ms() / reps
+    p.hv_device(psi.ptr, y.ptr)
+    arch.sync()
+    timer.start()
+    for _ in range(reps):
+        p.hv_device(psi.ptr, y.ptr)
+    ms_hv = timer.stop_ms() / reps
+    psi.free()
+    y.free()
+    p.close()
+    alg = p.dim * 16 + nnz * 8  # state word and psi[row] per row, one phi gather per element
+    return {"workload": "fermions_sphere_n%d_2s%d_lz0_ground_state_dE_dVm" % (n, lz), "dimension": p.dim,
+            "channels": len(g), "e0": float(r["eigenvalues"][0]), "dE0_dV1": float(g[0]), "adjoint_ms": ms_adj,
+            "product_ms": ms_hv, "adjoint_over_product": ms_adj / ms_hv,
+            "algorithmic_bytes": alg, "achieved_GBps": alg / (ms_adj * 1e-3) / 1e9,
+            "frac_of_peak": alg / (ms_adj * 1e-3) / HBM_PEAK}
+
+
+def sectors_autotune_leg(capi, arch, n=10, lz=27, nev=10, ks=(1, 2, 4, 8)):
+    """dhh_sectors_autotune over every Lz >= 0 sector of N = 10 at 2S = 27: per k the fastest product
+    launch on calibration sectors, scored by the time of a calibration solve with it."""
+    pp = capi.coulomb_pseudopotentials(lz, True)
+    t0 = time.perf_counter()
+    rows, best = capi.sectors_autotune(arch, n, lz, capi.sphere_sectors(n, lz), pp, nev, ks=ks)
+    return {"workload": "fermions_sphere_n%d_2s%d_all_Lz_ge_0_sectors_%d_eigenvalues" % (n, lz, nev),
+            "seconds": time.perf_counter() - t0, "table": rows, "best": best}
+
+
 def kernel_only(capi, arch, stream, name, steps, warmup):
     """One named product repeated `steps` times after `warmup` (for rocprofv3 runs): prints a JSON
     line with its mean time and rates."""
@@ -253,18 +304,19 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
     cleanup = []
     free = "_free" in name  # the free-pair row routine (DH_INDEX_RANK_FREE)
     lpr = int(name.split("_lpr")[1].split("_")[0]) if "_lpr" in name else None
+    bs = int(name.split("_bs")[1].split("_")[0]) if "_bs" in name and "_bsearch" not in name else None
     if name.startswith("sphere_n12_2s33_lz0"):
         p = capi.Problem.sphere(arch, 12, 0, 33, capi.coulomb_pseudopotentials(33, True))
         cleanup.append(p.close)
-        p.set_launch(lpr or (1 if k == 1 else 2), 1024 if k >= 4 else 256,
-                     capi.INDEX_RANK_FREE if free else capi.INDEX_RANK, 1)
+        mode = capi.INDEX_RANK_FREE if free else (capi.INDEX_BSEARCH if "_bsearch" in name else capi.INDEX_RANK)
+        p.set_launch(lpr or (1 if k == 1 else 2), bs or (1024 if k >= 4 else 256), mode, 1)
         rows, nnz = p.dim, p.count_elements()
         run = lambda dx, dy: p.hv_device(dx, dy, k)  # noqa: E731
         width = k
     elif name.startswith("sectors_n12_2s33"):
         S = capi.SphereSectorSet(arch, 12, 33, capi.sphere_sectors(12, 33), capi.coulomb_pseudopotentials(33, True))
         cleanup.append(S.close)
-        S.set_launch(lpr or 2, 1024) if k >= 4 else S.set_launch(lpr or 1, 256)
+        S.set_launch(lpr or 2, bs or 1024) if k >= 4 else S.set_launch(lpr or 1, bs or 256)
         if free:
             S.set_row_mode(capi.INDEX_RANK_FREE)
         rows, nnz = S.total, int(S.count_elements().sum())
@@ -274,7 +326,7 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
         from_lattice = capi.lattice_set(arch, 10, 6, 5, [x for x in range(6) for y in range(5)],
                                         [y for x in range(6) for y in range(5)])
         cleanup.append(from_lattice.close)
-        from_lattice.set_launch(lpr or 2, 512)
+        from_lattice.set_launch(lpr or 2, bs or 512)
         rows, nnz = from_lattice.total, int(from_lattice.count_elements().sum())
         run = lambda dx, dy: from_lattice.hv_device(dx, dy, k)  # noqa: E731
         width = 2 * k
@@ -457,6 +509,14 @@ def main():
         },
     }
     progress("headline product: %.3f ms" % ms)
+    # the LUT-seeded binary search of Spin1_2Chain::FindStateIndex on the same product, beside the
+    # combinatorial rank the headline uses
+    bs = spin_hv_bench(capi, arch, stream, args.length, min(args.steps, 20), min(args.warmup, 2),
+                       index_mode=capi.INDEX_BSEARCH)
+    line["index_lookup"] = {"rank_ms": ms, "binary_search_ms": bs["ms_per_step"],
+                            "binary_search_elements_per_s": nnz / (bs["ms_per_step"] * 1e-3),
+                            "kernel": "spin_bsearch_kernel<1>"}
+    progress("binary-search product: %.3f ms" % bs["ms_per_step"])
     if not args.skip_lanczos:
         line["lanczos"] = lanczos_runs(capi, arch, progress)
         if not args.skip_n12:
@@ -467,6 +527,10 @@ def main():
             line["lanczos"]["lattice_6x5_n10"] = lattice_runs(capi, arch, progress)
         if not args.skip_n10:
             line["lanczos"]["sectors_n10_2s27"] = sector_runs(capi, arch, progress)
+            line["lanczos"]["sectors_autotune_n10_2s27"] = sectors_autotune_leg(capi, arch)
+            progress("sector autotuner: %.1f s" % line["lanczos"]["sectors_autotune_n10_2s27"]["seconds"])
+        line["adjoint"] = adjoint_leg(capi, arch, stream)
+        progress("adjoint: %.3f ms" % line["adjoint"]["adjoint_ms"])
     if not args.skip_batched:
         line["batched_roofline"] = batched_hv_roofline(capi, arch, stream)
         progress("batched product: %.1f ms" % line["batched_roofline"]["ms"])

@@ -454,6 +454,14 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
         }
       }
     } else {
+      // K <= 2: a term's multiply-add waits until the next term's index loads are issued, so the
+      // gather of one term and the index loads of the next are in flight together (terms are still
+      // added in order)
+      constexpr bool PIPE = K <= 2;
+      double pre = 0.0, pim = 0.0;
+      double2 px[K];
+#pragma unroll
+      for (int v = 0; v < K; ++v) px[v] = make_double2(0.0, 0.0);
       for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
         const int n1 = __builtin_ctzll(r1);  // lower annihilated orbital
         for (uint64_t r2 = s & ~((2ull << n1) - 1ull); r2; r2 &= r2 - 1) {
@@ -479,13 +487,32 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
               ci = -ci;
             }
             const double* xr = xs + idx * 2 * K;
+            if constexpr (PIPE) {
 #pragma unroll
-            for (int v = 0; v < K; ++v) {
-              double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
-              acc[2 * v] += cr * p.x - ci * p.y;
-              acc[2 * v + 1] += cr * p.y + ci * p.x;
+              for (int v = 0; v < K; ++v) {
+                acc[2 * v] += pre * px[v].x - pim * px[v].y;
+                acc[2 * v + 1] += pre * px[v].y + pim * px[v].x;
+              }
+#pragma unroll
+              for (int v = 0; v < K; ++v) px[v] = *reinterpret_cast<const double2*>(xr + 2 * v);
+              pre = cr;
+              pim = ci;
+            } else {
+#pragma unroll
+              for (int v = 0; v < K; ++v) {
+                double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
+                acc[2 * v] += cr * p.x - ci * p.y;
+                acc[2 * v + 1] += cr * p.y + ci * p.x;
+              }
             }
           }
+        }
+      }
+      if constexpr (PIPE) {
+#pragma unroll
+        for (int v = 0; v < K; ++v) {
+          acc[2 * v] += pre * px[v].x - pim * px[v].y;
+          acc[2 * v + 1] += pre * px[v].y + pim * px[v].x;
         }
       }
     }

@@ -37,11 +37,20 @@ __device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restr
 // Lane `lane` of the row's LPR lanes takes m3 = m3lo + lane, m3lo + lane + LPR, ...
 // Factors come packed symmetric per pair sum (sF, sBase: see dh_pack_sphere_factors).
 // Target indices come from `find(target)`; x is indexed relative to `xbase`.  Each term is one
-// gather; latency is hidden by occupancy (few registers per lane) rather than by batching.
+// gather.  For K <= 2 the multiply-add of a term is deferred to the next term, after that term's
+// index loads are issued: a term's gather and the next term's index loads are then in flight
+// together, instead of one after the other (the terms are still added in their order, so the sum
+// is the same to the bit).  Wider K keep one gather in flight per term: their registers would
+// lower the occupancy that hides the latency there.
 template <int K, int LPR, typename U, typename Find>
 __device__ __forceinline__ void dh_sphere_row(U s, int nbrOrbitals, const double* __restrict__ sF,
                                               const int* __restrict__ sBase, int lane, Find find,
                                               const double* __restrict__ xbase, double* acc) {
+  constexpr bool PIPE = K <= 2;
+  double pend[K];
+  double pendCoef = 0.0;
+#pragma unroll
+  for (int v = 0; v < K; ++v) pend[v] = 0.0;
   for (U r1 = s; r1; r1 &= r1 - 1) {
     const int n1 = dh_ctz(r1);
     const int pc1 = dh_popc(s >> n1);
@@ -62,9 +71,24 @@ __device__ __forceinline__ void dh_sphere_row(U s, int nbrOrbitals, const double
         const auto idx = find(t | cm);
         double coef = fS[dh_packed_index(i, m3 - m3lo)];
         if (sg & 1) coef = -coef;
-        dh_gather_fma<K>(acc, xbase, idx, coef);
+        if constexpr (PIPE) {
+          // the previous term first, then this term's gather into the same registers (no copy,
+          // so nothing waits for the gather before the next term's index loads)
+          const double* xr = xbase + (size_t)idx * K;
+#pragma unroll
+          for (int v = 0; v < K; ++v) acc[v] = fma(pendCoef, pend[v], acc[v]);
+#pragma unroll
+          for (int v = 0; v < K; ++v) pend[v] = xr[v];
+          pendCoef = coef;
+        } else {
+          dh_gather_fma<K>(acc, xbase, idx, coef);
+        }
       }
     }
+  }
+  if constexpr (PIPE) {
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] = fma(pendCoef, pend[v], acc[v]);
   }
 }
 

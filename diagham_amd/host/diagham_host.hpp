@@ -579,6 +579,9 @@ class MultiSectorBlockLanczos {
 
 // Tridiagonal QL with implicit shifts (RealTriDiagonalSymmetricMatrix.cc:687 semantics):
 // eigenvalues ascending; optional eigenvectors (column-major n x n) of the tridiagonal.
+// Eigenvector (normalised) of the symmetric tridiagonal (d, e) for its eigenvalue theta, by inverse
+// iteration; O(n).
+std::vector<double> TridiagonalEigenvector(const std::vector<double>& d, const std::vector<double>& e, double theta);
 void TridiagonalEigen(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& evals,
                       std::vector<double>* evecs);
 // ------------------------------------------------------------------ Chern band (momentum space)
@@ -636,7 +639,9 @@ class LatticeSectorsBlockLanczos {
 // and into device memory; complex vectors are interleaved (re, im).
 void WriteDeviceVector(GPUArchitecture* arch, const char* fileName, const double* d_vec, int64_t dim, bool complex);
 void ReadDeviceVector(GPUArchitecture* arch, const char* fileName, double* d_vec, int64_t dim, bool complex);
-int64_t VectorFileDimension(const char* fileName, bool complex);
+// The dimension in the file's header; headerBytes (if given) receives the header's length (4 for
+// the int32 form, 12 for -1 and an int64).
+int64_t VectorFileDimension(const char* fileName, bool complex, int64_t* headerBytes = nullptr);
 
 // Symmetric band matrix (half-bandwidth b, dense row-major n x n) -> tridiagonal by Givens
 // rotations with bulge chasing; returns diagonal and off-diagonal, and accumulates Q if given.

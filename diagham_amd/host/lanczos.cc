@@ -431,10 +431,87 @@ void BlockTridiagonalEigen(const std::vector<std::vector<double>>& A, const std:
   }
 }
 
+// Eigenvector of the symmetric tridiagonal matrix (d, e) for its eigenvalue theta, by inverse
+// iteration: two solves of (T - theta I) x = b, each normalised, with T - theta I factored once by
+// Gaussian elimination with partial pivoting (row exchanges between neighbouring rows only, so the
+// upper factor has two superdiagonals); a pivot that vanishes is replaced by eps |T|.  Costs O(n).
+std::vector<double> TridiagonalEigenvector(const std::vector<double>& d, const std::vector<double>& e, double theta) {
+  const int n = (int)d.size();
+  std::vector<double> diag(n), lower(std::max(n - 1, 0)), up1(std::max(n - 1, 0)), up2(std::max(n - 2, 0));
+  std::vector<char> swapped(std::max(n - 1, 0), 0);
+  double norm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    diag[i] = d[i] - theta;
+    if (i + 1 < n) lower[i] = up1[i] = e[i];
+    norm = std::max(norm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0));
+  }
+  const double tiny = std::max(norm, 1e-300) * 2.220446049250313e-16;
+  for (int i = 0; i + 1 < n; ++i) {
+    if (std::fabs(diag[i]) >= std::fabs(lower[i])) {
+      if (diag[i] == 0.0) diag[i] = tiny;
+      const double f = lower[i] / diag[i];
+      lower[i] = f;
+      diag[i + 1] -= f * up1[i];
+    } else {  // exchange rows i and i + 1
+      const double f = diag[i] / lower[i];
+      diag[i] = lower[i];
+      lower[i] = f;
+      const double u = up1[i];
+      up1[i] = diag[i + 1];
+      diag[i + 1] = u - f * diag[i + 1];
+      if (i + 2 < n) {
+        up2[i] = up1[i + 1];
+        up1[i + 1] = -f * up1[i + 1];
+      }
+      swapped[i] = 1;
+    }
+  }
+  if (n > 0 && diag[n - 1] == 0.0) diag[n - 1] = tiny;
+  std::vector<double> x(n, 1.0);
+  for (int it = 0; it < 2; ++it) {
+    for (int i = 0; i + 1 < n; ++i) {  // L
+      if (swapped[i]) std::swap(x[i], x[i + 1]);
+      x[i + 1] -= lower[i] * x[i];
+    }
+    for (int i = n - 1; i >= 0; --i) {  // U
+      double v = x[i];
+      if (i + 1 < n) v -= up1[i] * x[i + 1];
+      if (i + 2 < n) v -= up2[i] * x[i + 2];
+      x[i] = v / diag[i];
+    }
+    double s = 0.0;
+    for (double v : x) s = std::max(s, std::fabs(v));
+    for (double& v : x) v /= s;
+  }
+  double s = 0.0;
+  for (double v : x) s += v * v;
+  s = std::sqrt(s);
+  for (double& v : x) v /= s;
+  return x;
+}
+
 bool FullReorthogonalizedBlockLanczosAlgorithm::TestConvergence() {
   bool ok = Engine->TestConvergence(0, NbrEigenvalue, Precision, Current);
   const int want = (int)Current.size();
-  if (ok && Residual > 0.0 && !Engine->Exhausted(0)) {
+  if (ok && Residual > 0.0 && !Engine->Exhausted(0) && BlockSize == 1) {
+    // one vector per step: |H v - theta v| = |beta_m| |last component of the eigenvector of T|,
+    // the eigenvectors by inverse iteration (O(m) each; the full eigenvector matrix would cost
+    // O(m^3) at every check, minutes for a few thousand steps)
+    const int m = Engine->NbrBlocks(0);
+    const auto& A = Engine->Diagonal(0);
+    const auto& B = Engine->Coupling(0);
+    std::vector<double> d(m), e(m > 0 ? m - 1 : 0);
+    for (int j = 0; j < m; ++j) {
+      d[j] = A[j][0];
+      if (j + 1 < m) e[j] = B[j][0];
+    }
+    std::vector<double> ev;
+    Engine->LowestEigen(0, want, ev);
+    for (int i = 0; i < want && i < (int)ev.size() && ok; ++i) {
+      const std::vector<double> z = TridiagonalEigenvector(d, e, ev[i]);
+      if (std::fabs(B[m - 1][0] * z[m - 1]) > Residual * std::fabs(ev[i])) ok = false;
+    }
+  } else if (ok && Residual > 0.0 && !Engine->Exhausted(0)) {
     // |H v - theta v| = |B_last * (last k coefficients of the Ritz vector)|: the pending
     // block X_m times B[m-1] is what the Krylov space has not yet absorbed
     const int k = BlockSize;

@@ -51,7 +51,7 @@ void WriteDeviceVector(GPUArchitecture* arch, const char* fileName, const double
   }
 }
 
-int64_t VectorFileDimension(const char* fileName, bool complex) {
+int64_t VectorFileDimension(const char* fileName, bool complex, int64_t* headerBytes) {
   File file(fileName, "rb");
   int32_t d;
   int64_t dim;
@@ -70,14 +70,18 @@ int64_t VectorFileDimension(const char* fileName, bool complex) {
     throw std::invalid_argument(payload == 2 * dim * (int64_t)sizeof(double)
                                     ? "the file holds a complex vector"
                                     : "the file size does not match its dimension");
+  if (headerBytes) *headerBytes = header;
   return dim;
 }
 
 void ReadDeviceVector(GPUArchitecture* arch, const char* fileName, double* d_vec, int64_t dim, bool complex) {
-  const int64_t fileDim = VectorFileDimension(fileName, complex);
+  int64_t header = 0;
+  const int64_t fileDim = VectorFileDimension(fileName, complex, &header);
   if (fileDim != dim) throw std::invalid_argument("the vector in the file has another dimension");
   File file(fileName, "rb");
-  std::fseek(file.f, dim < (int64_t(1) << 31) ? 4 : 12, SEEK_SET);
+  // the components start after the header the file actually holds: a writer may use the -1 and
+  // int64 form for a dimension that fits in 31 bits
+  std::fseek(file.f, (long)header, SEEK_SET);
   const int64_t n = complex ? 2 * dim : dim;
   std::vector<double> buffer((size_t)std::min(n, kChunk));
   for (int64_t i = 0; i < n; i += kChunk) {

@@ -161,8 +161,10 @@ def test_energy_gradients_match_finite_differences(arch):
 
 def test_l0_gap_gradients_match_finite_differences(arch):
     """The differentiation workload as specified: the two lowest L = 0 states of N=10, 2S=27, Lz=0.
-    They are the lowest states of H + L^2 (the --l2-factor of FQHESphereFermionsTwoBodyGeneric.cc:170,
-    every L > 0 state moved up by L (L + 1)); their eigenvectors give dE/dV_m = <psi|H_m|psi> through
+    They are the lowest states of H + c L^2 (the --l2-factor of FQHESphereFermionsTwoBodyGeneric.cc:170,
+    every L > 0 state moved up by c L (L + 1)); c = 0.1 lifts the L = 2 and 3 states that lie below
+    the second L = 0 state (0.31 above the first) over it, and keeps the spectrum narrow enough for
+    the Ritz vectors to converge quickly.  The eigenvectors give dE/dV_m = <psi|H_m|psi> through
     the adjoint of H alone.  Energies against the oracle's L = 0 levels (1e-10), gradients against
     central differences of them (tests/golden/fd_n10_2s27_l0.json, 1e-6 relative)."""
     torch = _torch()
@@ -173,10 +175,13 @@ def test_l0_gap_gradients_match_finite_differences(arch):
     fd = json.load(open(path))
     n, tlz, lz = fd["nbr_fermions"], fd["total_lz"], fd["lz_max"]
     V = np.array(fd["pseudopotentials"])
-    shifted = capi.Problem.sphere_l2(arch, n, tlz, lz, 1.0, pseudo=V)
+    shifted = capi.Problem.sphere_l2(arch, n, tlz, lz, 0.1, pseudo=V)
     vecs = torch.empty((2, shifted.dim), dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    r = shifted.lanczos(2, block_size=1, max_iter=3000, d_vectors=vecs.data_ptr(), nbr_vectors=2)
+    # the gradients are bilinears of the Ritz vectors, whose error is first order in the residual
+    # (the eigenvalues' error is second order): the residual test converges the vectors as well
+    r = shifted.lanczos(2, block_size=1, max_iter=3000, residual=1e-11, d_vectors=vecs.data_ptr(),
+                        nbr_vectors=2)
     assert r["converged"]
     E = r["eigenvalues"]
     assert abs(E[0] - fd["E0"]) < 1e-10 and abs(E[1] - fd["E1"]) < 1e-10, (E, fd["E0"], fd["E1"])
@@ -191,11 +196,15 @@ def test_l0_gap_gradients_match_finite_differences(arch):
     mom = list(p.channels())
     g = [p.adjoint_device(vecs[i].data_ptr(), vecs[i].data_ptr()) for i in range(2)]
     arch.sync()
+    # 1e-6 relative, plus the finite differences' own noise: each energy of the fixture is known to
+    # its Lanczos stopping rule, 1e-14 relative, and a central difference divides that by 2h (the
+    # gap's derivative takes four energies); the gap's derivatives are small differences of ~10
+    noise = 1e-14 * max(abs(fd["E0"]), abs(fd["E1"])) / (2 * fd["h"])
     for j, m in enumerate(fd["free_m"]):
         c = mom.index(m)
         a, b = g[0][c], fd["dE0_dV"][j]
-        assert abs(a - b) <= 1e-6 * abs(b), ("dE0", m, a, b)
+        assert abs(a - b) <= 1e-6 * abs(b) + 2 * noise, ("dE0", m, a, b)
         a, b = g[1][c] - g[0][c], fd["dGap_dV"][j]
-        assert abs(a - b) <= 1e-6 * max(abs(b), 1e-3), ("dGap", m, a, b)
+        assert abs(a - b) <= 1e-6 * abs(b) + 4 * noise, ("dGap", m, a, b)
     for q in (shifted, L2, p):
         q.close()

@@ -59,3 +59,19 @@ def test_eigenvector_round_trip(arch, tmp_path):
     assert np.array_equal(back.download(), v)
     assert capi.host().dhh_read_vector(arch.h, str(path).encode(), back.ptr, p.dim + 1, 0) == capi.DH_E_ARG
     p.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("large,complex_", [(False, False), (True, False), (True, True)])
+def test_read_either_header(arch, tmp_path, large, complex_):
+    """A small vector written with the -1 / int64 header (allowed by the format for any dimension) is
+    read from the component after that 12-byte header, not from byte 4."""
+    from diagham_amd import capi
+    dim = 1001
+    data = np.random.default_rng(11).standard_normal(dim * (2 if complex_ else 1))
+    path = tmp_path / "v.vec"
+    write_reference_layout(path, data, complex_=complex_, large=large)
+    back = capi.DeviceBuffer(arch, data.size)
+    assert capi.host().dhh_read_vector(arch.h, str(path).encode(), back.ptr, dim, int(complex_)) == 0
+    assert np.array_equal(back.download(), data)
+    back.free()

@@ -302,23 +302,27 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
     k = int(name.rsplit("_k", 1)[1]) if "_k" in name else 1
     timer = HipTimer(stream)
     cleanup = []
-    free = "_free" in name  # the free-pair row routine (DH_INDEX_RANK_FREE)
+    free = "_free" in name  # the free-pair row routine for every k (DH_INDEX_RANK_FREE)
+    rng = "_range" in name  # the range row routine for every k (DH_INDEX_RANK_RANGE)
     lpr = int(name.split("_lpr")[1].split("_")[0]) if "_lpr" in name else None
     bs = int(name.split("_bs")[1].split("_")[0]) if "_bs" in name and "_bsearch" not in name else None
     if name.startswith("sphere_n12_2s33_lz0"):
         p = capi.Problem.sphere(arch, 12, 0, 33, capi.coulomb_pseudopotentials(33, True))
         cleanup.append(p.close)
-        mode = capi.INDEX_RANK_FREE if free else (capi.INDEX_BSEARCH if "_bsearch" in name else capi.INDEX_RANK)
-        p.set_launch(lpr or (1 if k == 1 else 2), bs or (1024 if k >= 4 else 256), mode, 1)
+        bsearch = "_bsearch" in name
+        mode = (capi.INDEX_RANK_FREE if free else capi.INDEX_RANK_RANGE if rng else
+                capi.INDEX_BSEARCH if bsearch else capi.INDEX_RANK)
+        if lpr or bs or free or rng or bsearch:  # otherwise the Hamiltonian's own choice per k
+            p.set_launch(lpr or 2, bs or (256 if k <= 2 else 1024), mode, 1)
         rows, nnz = p.dim, p.count_elements()
         run = lambda dx, dy: p.hv_device(dx, dy, k)  # noqa: E731
         width = k
     elif name.startswith("sectors_n12_2s33"):
         S = capi.SphereSectorSet(arch, 12, 33, capi.sphere_sectors(12, 33), capi.coulomb_pseudopotentials(33, True))
         cleanup.append(S.close)
-        S.set_launch(lpr or 2, bs or 1024) if k >= 4 else S.set_launch(lpr or 1, bs or 256)
-        if free:
-            S.set_row_mode(capi.INDEX_RANK_FREE)
+        S.set_launch(lpr or 2, bs or 1024) if k >= 4 else S.set_launch(lpr or 1, bs or 1024)
+        if free or rng:
+            S.set_row_mode(capi.INDEX_RANK_FREE if free else capi.INDEX_RANK_RANGE)
         rows, nnz = S.total, int(S.count_elements().sum())
         run = lambda dx, dy: S.hv_device(dx, dy, k)  # noqa: E731
         width = k
@@ -326,7 +330,7 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
         from_lattice = capi.lattice_set(arch, 10, 6, 5, [x for x in range(6) for y in range(5)],
                                         [y for x in range(6) for y in range(5)])
         cleanup.append(from_lattice.close)
-        from_lattice.set_launch(lpr or 2, bs or 512)
+        from_lattice.set_launch(lpr or (2 if k <= 2 else 1), bs or (512 if k <= 2 else 256))
         rows, nnz = from_lattice.total, int(from_lattice.count_elements().sum())
         run = lambda dx, dy: from_lattice.hv_device(dx, dy, k)  # noqa: E731
         width = 2 * k
@@ -346,12 +350,55 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
     y.free()
     for f in cleanup:
         f()
-    out = {"kernel_workload": name, "rows": rows, "k": k, "ms": ms}
+    out = {"kernel_workload": name, "rows": rows, "k": k, "ms": ms,
+           # the HBM-traffic calibration kernel in the same process, so that a counter run over this
+           # command sees both (dh_copy_kernel: 2^27 doubles read and written per launch)
+           "copy_calibration": copy_calibration(capi, arch, stream)}
     if nnz:
         alg = rows * (8 + 8 * width) + nnz * 8 * width
         out.update({"matrix_elements": nnz, "element_vector_products_per_s": nnz * k / (ms * 1e-3),
                     "algorithmic_GBps": alg / (ms * 1e-3) / 1e9})
     print(json.dumps(out))
+
+
+def launch_sweep(capi, arch, stream, base, ks, reps=3):
+    """Every launch shape (lanes per row 1, 2, 4 x workgroups of 256, 512, 1024 threads) of one
+    product workload (sphere_n12_2s33_lz0, sectors_n12_2s33 or lattice_6x5_n10) for each k in ks:
+    one JSON line per shape with its mean time over reps products (HIP events on the bench stream)."""
+    timer = HipTimer(stream)
+    if base == "sphere_n12_2s33_lz0":
+        obj = capi.Problem.sphere(arch, 12, 0, 33, capi.coulomb_pseudopotentials(33, True))
+        rows, width = obj.dim, 1
+        shape = lambda lpr, bs: obj.set_launch(lpr, bs, capi.INDEX_RANK, 1)  # noqa: E731
+    elif base == "sectors_n12_2s33":
+        obj = capi.SphereSectorSet(arch, 12, 33, capi.sphere_sectors(12, 33), capi.coulomb_pseudopotentials(33, True))
+        rows, width = obj.total, 1
+        shape = obj.set_launch
+    elif base == "lattice_6x5_n10":
+        obj = capi.lattice_set(arch, 10, 6, 5, [x for x in range(6) for y in range(5)],
+                               [y for x in range(6) for y in range(5)])
+        rows, width = obj.total, 2
+        shape = obj.set_launch
+    else:
+        raise SystemExit("unknown sweep workload " + base)
+    for k in ks:
+        x = capi.DeviceBuffer(arch, rows * width * k)
+        y = capi.DeviceBuffer(arch, rows * width * k)
+        capi.gpu().dh_fill_random(arch.ctx, x.ptr, rows * width * k, 0x5EED0007)
+        for lpr in (1, 2, 4):
+            for bs in (256, 512, 1024):
+                shape(lpr, bs)
+                obj.hv_device(x.ptr, y.ptr, k)
+                arch.sync()
+                timer.start()
+                for _ in range(reps):
+                    obj.hv_device(x.ptr, y.ptr, k)
+                ms = timer.stop_ms() / reps
+                print(json.dumps({"sweep": base, "k": k, "lanes_per_row": lpr, "block_size": bs, "ms": ms}))
+                sys.stdout.flush()
+        x.free()
+        y.free()
+    obj.close()
 
 
 def cpu_threads():
@@ -446,6 +493,10 @@ def main():
     ap.add_argument("--kernel", default=None,
                     help="only repeat one product (sphere_n12_2s33_lz0_k1, sectors_n12_2s33_k8, "
                          "lattice_6x5_n10_k4, ...) for profiling")
+    ap.add_argument("--sweep", default=None,
+                    help="time every launch shape of one product workload (sphere_n12_2s33_lz0, "
+                         "sectors_n12_2s33, lattice_6x5_n10) at the k of --sweep-k")
+    ap.add_argument("--sweep-k", default="1", help="comma-separated block sizes k for --sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
@@ -460,6 +511,10 @@ def main():
 
     if args.kernel:
         kernel_only(capi, arch, stream, args.kernel, args.steps, args.warmup)
+        arch.close()
+        return
+    if args.sweep:
+        launch_sweep(capi, arch, stream, args.sweep, [int(v) for v in args.sweep_k.split(",")])
         arch.close()
         return
     hv = spin_hv_bench(capi, arch, stream, args.length, args.steps, args.warmup, k=1)
@@ -522,7 +577,7 @@ def main():
         if not args.skip_n12:
             line["lanczos"]["sectors_n12_2s33"] = sector_runs(
                 capi, arch, progress, n=12, lz=33, nev=10,
-                legs=(("plain_sequential_k1", 1, False), ("batched_block_k8", 8, True)))
+                legs=(("plain_sequential_k1", 1, False), ("batched_k1", 1, True), ("batched_block_k8", 8, True)))
         if not args.skip_lattice:
             line["lanczos"]["lattice_6x5_n10"] = lattice_runs(capi, arch, progress)
         if not args.skip_n10:

@@ -16,7 +16,7 @@ LIB_GPU = os.path.join(_HERE, "libdhgpu.so")
 LIB_HOST = os.path.join(_HERE, "libdiagham_host.so")
 
 DH_OK, DH_E_ARG, DH_E_HIP, DH_E_NOMEM, DH_E_STATE, DH_E_RANGE = 0, -1, -2, -3, -4, -5
-INDEX_BSEARCH, INDEX_RANK, INDEX_RANK_X4, INDEX_RANK_FREE = 0, 1, 2, 3
+INDEX_BSEARCH, INDEX_RANK, INDEX_RANK_X4, INDEX_RANK_FREE, INDEX_RANK_RANGE = 0, 1, 2, 3, 4
 
 c_int, c_int64, c_double, c_void_p, c_uint64, c_size_t, c_char_p = (
     ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t,

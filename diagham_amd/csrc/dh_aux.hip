@@ -346,9 +346,12 @@ extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64
 // ----------------------------------------------------------------------------- adjoint
 // With the interaction split by channel (F(a->b) = sum_c V_c (-4) cg[a][c] cg[b][c]):
 //   g[c] = sum_r psi[r] (H_c phi)[r],   H_c phi gathered like the product,
-// accumulated per row in registers, reduced per block; optionally hpsi = H psi in the
-// same pass (the vector-Jacobian product of x -> H x).
-template <int NC>
+// accumulated in registers and reduced per block; optionally hpsi = H psi in the same pass (the
+// vector-Jacobian product of x -> H x).  LPR lanes share a row and split its created pairs, as
+// in the product: a row's terms are a few hundred dependent gathers, so one lane per row leaves
+// too few loads in flight on spaces of a few hundred thousand rows.  g is linear in every term,
+// so each lane adds psi[r] times its own part; only hpsi needs the row's lanes summed.
+template <int NC, int LPR>
 __global__ void __launch_bounds__(256) sphere_adjoint_kernel(const uint64_t* __restrict__ states, int64_t dim,
                                                              int nbrOrbitals, int nbrPairs, int nbrChannels,
                                                              const double* __restrict__ cg,
@@ -366,36 +369,36 @@ __global__ void __launch_bounds__(256) sphere_adjoint_kernel(const uint64_t* __r
   double g[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) g[c] = 0.0;
-  const int64_t nbrTiles = (dim + blockDim.x - 1) / blockDim.x;
+  const int lane = threadIdx.x % LPR;
+  const int64_t rowsPerTile = blockDim.x / LPR;
+  const int64_t nbrTiles = (dim + rowsPerTile - 1) / rowsPerTile;
   for (int64_t tile = blockIdx.x; tile < nbrTiles; tile += gridDim.x) {
-    int64_t row = tile * blockDim.x + threadIdx.x;
-    if (row >= dim) continue;
-    uint64_t s = states[row];
-    double h[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) h[c] = 0.0;
+    const int64_t row = tile * rowsPerTile + threadIdx.x / LPR;
+    const bool active = row < dim;
+    const uint64_t s = active ? states[row] : 0ull;  // an empty state has no terms
+    const double pr = active ? -4.0 * psi[row] : 0.0;
     double hp = 0.0;
     for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
-      int n1 = __builtin_ctzll(r1);
-      int pc1 = __popcll(s >> n1);
+      const int n1 = __builtin_ctzll(r1);
+      const int pc1 = __popcll(s >> n1);
       for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
-        int n2 = __builtin_ctzll(r2);
-        int sgnA = pc1 + __popcll(s >> n2);
-        uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
-        int sum = n1 + n2;
-        int m3lo = (sum >> 1) + 1;
-        int m3hi = min(sum, nbrOrbitals - 1);
-        int pa = (n1 * (n1 - 1)) / 2 + n2;
-        int fbase = pairOffset[pa] - m3lo;
+        const int n2 = __builtin_ctzll(r2);
+        const int sgnA = pc1 + __popcll(s >> n2);
+        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+        const int sum = n1 + n2;
+        const int m3lo = (sum >> 1) + 1;
+        const int m3hi = min(sum, nbrOrbitals - 1);
+        const int pa = (n1 * (n1 - 1)) / 2 + n2;
+        const int fbase = pairOffset[pa] - m3lo;
         double w[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) w[c] = 0.0;
-        for (int m3 = m3lo; m3 <= m3hi; ++m3) {
-          int m4 = sum - m3;
-          uint64_t cm = (1ull << m3) | (1ull << m4);
+        for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
+          const int m4 = sum - m3;
+          const uint64_t cm = (1ull << m3) | (1ull << m4);
           if (t & cm) continue;
-          int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
-          int64_t idx = dh_find_rank(t | cm, lo, hi, rankBits, dim);
+          const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
+          const int64_t idx = dh_find_rank(t | cm, lo, hi, rankBits, dim);
           double p = phi[idx];
           if (sg & 1) p = -p;
           const double* cb = sCg + ((m3 * (m3 - 1)) / 2 + m4) * nbrChannels;
@@ -403,33 +406,34 @@ __global__ void __launch_bounds__(256) sphere_adjoint_kernel(const uint64_t* __r
           for (int c = 0; c < NC; ++c)
             if (c < nbrChannels) w[c] += cb[c] * p;
           if (hpsi) {
-            double q = psi[idx];
+            const double q = psi[idx];
             hp += ((sg & 1) ? -factors[fbase + m3] : factors[fbase + m3]) * q;
           }
         }
         const double* ca = sCg + pa * nbrChannels;
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-          if (c < nbrChannels) h[c] += ca[c] * w[c];
+          if (c < nbrChannels) g[c] += pr * ca[c] * w[c];
       }
     }
-    double pr = psi[row];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) g[c] += -4.0 * pr * h[c];
-    if (hpsi) hpsi[row] = hp + shift * pr;
+    if (hpsi) {
+      if constexpr (LPR > 1)
+        for (int o = LPR / 2; o > 0; o >>= 1) hp += __shfl_xor(hp, o, LPR);
+      if (active && lane == 0) hpsi[row] = hp + shift * psi[row];
+    }
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wl = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     double v = g[c];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wave][c] = v;
+    if (wl == 0) red[wave][c] = v;
   }
   __syncthreads();
   if (threadIdx.x < NC) {
-    double s = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][threadIdx.x];
-    partial[(int64_t)blockIdx.x * NC + threadIdx.x] = s;
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
+    partial[(int64_t)blockIdx.x * NC + threadIdx.x] = v;
   }
 }
 
@@ -448,11 +452,12 @@ static int adjoint_launch(dh_hamiltonian* h, const double* psi, const double* ph
   dh_basis* b = h->basis;
   size_t lds = sizeof(double) * h->nbrPairs * h->nbrChannels;
   if (lds > 150 * 1024) return dh_fail(c, DH_E_RANGE, "channel table exceeds LDS");
-  auto kern = sphere_adjoint_kernel<NC>;
+  // four lanes per row: a row's created pairs split four ways (see the kernel)
+  auto kern = sphere_adjoint_kernel<NC, 4>;
   if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int perCU = 1;
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 256, lds);
-  int64_t tiles = (b->dim + 255) / 256;
+  int64_t tiles = (b->dim + 63) / 64;  // 64 rows per 256-thread workgroup
   int nb = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * std::max(perCU, 1)));
   double* scratch;
   int rc = dh_scratch(c, sizeof(double) * ((size_t)nb * NC + NC), &scratch);

@@ -232,13 +232,15 @@ extern "C" int dh_fermion_sphere_create(dh_context* c, dh_basis** out, int N, in
   hipLaunchKernelGGL(fs_lut_kernel, dim3((unsigned)(((1 << b->lutBits) + 1 + 255) / 256)), dim3(256), 0, c->stream,
                      b->d_states, dim, b->lutShift, b->lutBits, b->d_lut);
   // rank tables
-  b->rankBits = (nbrOrbitals + 1) / 2;
+  b->rankBits = dh_rank_split(nbrOrbitals);
   std::vector<uint32_t> lo, hi;
   fermion_rank_tables(T, nbrOrbitals, N, S, b->rankBits, lo, hi);
   b->rankLoSize = (int64_t)lo.size();
   b->rankHiSize = (int64_t)hi.size();
+  const std::vector<uint16_t> lo16 = dh_rank_lo16(lo);
   if ((rc = upload(c, (void**)&b->d_rankLo, lo.data(), sizeof(uint32_t) * lo.size())) ||
-      (rc = upload(c, (void**)&b->d_rankHi, hi.data(), sizeof(uint32_t) * hi.size()))) {
+      (rc = upload(c, (void**)&b->d_rankHi, hi.data(), sizeof(uint32_t) * hi.size())) ||
+      (!lo16.empty() && (rc = upload(c, (void**)&b->d_rankLo16, lo16.data(), sizeof(uint16_t) * lo16.size())))) {
     hipFree(d_T);
     dh_basis_destroy(b);
     return rc;
@@ -308,6 +310,7 @@ extern "C" int dh_basis_destroy(dh_basis* b) {
   if (b->d_lut) hipFree(b->d_lut);
   if (b->d_rankLo) hipFree(b->d_rankLo);
   if (b->d_rankHi) hipFree(b->d_rankHi);
+  if (b->d_rankLo16) hipFree(b->d_rankLo16);
   if (b->d_binom) hipFree(b->d_binom);
   delete b;
   return DH_OK;

@@ -20,8 +20,10 @@
 #include "dh_internal.h"
 #include "dh_sphere_row.h"
 
-// index lookup of the product kernels; the _X4 forms unroll the created-pair loop four-wide, the
-// _FREE forms iterate over the free created pairs only (dh_sphere_row_free)
+// index lookup of the product kernels: where the rank tables live (both in LDS; the 16-bit low
+// table in LDS and the high table in global memory; both in global memory), and the row routine
+// (_X4: the created-pair loop unrolled four-wide; _FREE: only the free created pairs,
+// dh_sphere_row_free)
 enum {
   MODE_BSEARCH = 0,
   MODE_RANK_LDS = 1,
@@ -29,19 +31,28 @@ enum {
   MODE_RANK_LDS_X4 = 3,
   MODE_RANK_GLOBAL_X4 = 4,
   MODE_RANK_LDS_FREE = 5,
-  MODE_RANK_GLOBAL_FREE = 6
+  MODE_RANK_GLOBAL_FREE = 6,
+  MODE_RANK_LO16 = 7,
+  MODE_RANK_LO16_X4 = 8,
+  MODE_RANK_LO16_FREE = 9
 };
 
 template <int MODE>
 struct ModeBase {
-  static constexpr int value = MODE >= MODE_RANK_LDS_FREE ? MODE - 4 : (MODE >= MODE_RANK_LDS_X4 ? MODE - 2 : MODE);
+  static constexpr int value =
+      MODE >= MODE_RANK_LO16 ? MODE_RANK_LO16
+                             : (MODE >= MODE_RANK_LDS_FREE ? MODE - 4 : (MODE >= MODE_RANK_LDS_X4 ? MODE - 2 : MODE));
 };
 
-// terms in flight per lane in the free-pair iterator
-template <int K>
-struct FreeBatch {
-  static constexpr int value = K == 1 ? 4 : (K <= 4 ? 2 : 1);
+enum { ROW_RANGE = 0, ROW_X4 = 1, ROW_FREE = 2 };
+template <int MODE>
+struct ModeRow {
+  static constexpr int value =
+      (MODE == MODE_RANK_LDS_X4 || MODE == MODE_RANK_GLOBAL_X4 || MODE == MODE_RANK_LO16_X4) ? ROW_X4
+      : (MODE == MODE_RANK_LDS_FREE || MODE == MODE_RANK_GLOBAL_FREE || MODE == MODE_RANK_LO16_FREE) ? ROW_FREE
+                                                                                                   : ROW_RANGE;
 };
+
 
 struct SphereArgs {
   const uint64_t* states;
@@ -53,6 +64,7 @@ struct SphereArgs {
   int lutShift, lutSize, lutStride;  // staged LUT entry i = lut[i * lutStride]
   const uint32_t* rankLo;
   const uint32_t* rankHi;
+  const uint16_t* rankLo16;
   int rankBits, rankLoSize, rankHiSize;
   double shift;
 };
@@ -76,7 +88,7 @@ __device__ __forceinline__ void gather_fma(double* acc, const double* __restrict
 
 template <int MODE>
 __device__ __forceinline__ void sphere_stage(const SphereArgs& a, double*& sF, int*& sBase, int*& sLut,
-                                             uint32_t*& sLo, uint32_t*& sHi) {
+                                             uint32_t*& sLo, uint32_t*& sHi, uint16_t*& sLo16) {
   extern __shared__ double smem[];
   sF = smem;
   sBase = reinterpret_cast<int*>(sF + a.nbrFactors);
@@ -91,6 +103,10 @@ __device__ __forceinline__ void sphere_stage(const SphereArgs& a, double*& sF, i
     sHi = sLo + a.rankLoSize;
     for (int i = threadIdx.x; i < a.rankLoSize; i += blockDim.x) sLo[i] = a.rankLo[i];
     for (int i = threadIdx.x; i < a.rankHiSize; i += blockDim.x) sHi[i] = a.rankHi[i];
+  } else if constexpr (MODE == MODE_RANK_LO16) {
+    sLo16 = reinterpret_cast<uint16_t*>(tail);
+    sHi = const_cast<uint32_t*>(a.rankHi);
+    for (int i = threadIdx.x; i < a.rankLoSize; i += blockDim.x) sLo16[i] = a.rankLo16[i];
   } else {
     sLo = const_cast<uint32_t*>(a.rankLo);
     sHi = const_cast<uint32_t*>(a.rankHi);
@@ -100,16 +116,20 @@ __device__ __forceinline__ void sphere_stage(const SphereArgs& a, double*& sF, i
 
 template <int MODE>
 __device__ __forceinline__ int64_t sphere_find(const SphereArgs& a, uint64_t tgt, const int* sLut,
-                                               const uint32_t* sLo, const uint32_t* sHi) {
+                                               const uint32_t* sLo, const uint32_t* sHi, const uint16_t* sLo16) {
   if constexpr (MODE == MODE_BSEARCH)
     return dh_find_bsearch(tgt, sLut, a.lutShift, a.states);
+  else if constexpr (MODE == MODE_RANK_LO16)
+    return dh_find_rank(tgt, sLo16, sHi, a.rankBits, a.dim);
   else
     return dh_find_rank(tgt, sLo, sHi, a.rankBits, a.dim);
 }
 
-// S32 (rank modes, k <= 2): the orbitals fit a 32-bit word and the space has fewer than 2^31
-// states, so the row routine runs on 32-bit state words and 32-bit indices.
-template <int K, int LPR, int MODE, bool S32>
+// WORDS (rank modes): 1 = the orbitals fit a 32-bit word and the space has fewer than 2^31 states,
+// so the row routine runs on 32-bit state words and 32-bit indices (used for k <= 2); 2 = 64-bit
+// state words with 32-bit indices (fewer than 2^31 states: the rank, its clamp and the gather
+// offset in 32-bit arithmetic); 0 = 64-bit words and indices.
+template <int K, int LPR, int MODE, int WORDS>
 __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const double* __restrict__ x,
                                                          double* __restrict__ y, int add, int64_t first,
                                                          int64_t count) {
@@ -117,7 +137,16 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
   double* sF;
   int *sBase, *sLut = nullptr;
   uint32_t *sLo = nullptr, *sHi = nullptr;
-  sphere_stage<BASE>(a, sF, sBase, sLut, sLo, sHi);
+  uint16_t* sLo16 = nullptr;
+  sphere_stage<BASE>(a, sF, sBase, sLut, sLo, sHi, sLo16);
+  constexpr int ROW = ModeRow<MODE>::value;
+  // the low table entry: 16-bit copy in LDS (MODE_RANK_LO16), else the 32-bit table (LDS or global)
+  auto loAt = [&](uint32_t i) -> uint32_t {
+    if constexpr (BASE == MODE_RANK_LO16)
+      return sLo16[i];
+    else
+      return sLo[i];
+  };
   const int lane = threadIdx.x % LPR;
   const int64_t rowsPerBlock = blockDim.x / LPR;
   const int64_t nbrTiles = (count + rowsPerBlock - 1) / rowsPerBlock;
@@ -130,23 +159,36 @@ __global__ void __launch_bounds__(1024) sphere_hv_kernel(SphereArgs a, const dou
     double acc[K];
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] = 0.0;
-    if constexpr (S32 && BASE != MODE_BSEARCH) {
+    if constexpr (WORDS == 1 && BASE != MODE_BSEARCH) {
       const uint32_t last = (uint32_t)a.dim - 1u, mask = (1u << a.rankBits) - 1u;
       const int bits = a.rankBits;
       // clamped so that a target outside the space can never address outside the vectors
-      auto find = [&](uint32_t t) -> uint32_t { return min(last - sLo[t & mask] - sHi[t >> bits], last); };
-      if constexpr (MODE >= MODE_RANK_LDS_FREE)
-        dh_sphere_row_free<K, LPR, FreeBatch<K>::value>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x,
-                                                        (uint32_t)row, acc);
-      else if constexpr (MODE >= MODE_RANK_LDS_X4)
+      auto find = [&](uint32_t t) -> uint32_t { return min(last - loAt(t & mask) - sHi[t >> bits], last); };
+      if constexpr (ROW == ROW_FREE)
+        dh_sphere_row_free<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+      else if constexpr (ROW == ROW_X4)
         dh_sphere_row_x4<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
       else
         dh_sphere_row<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+    } else if constexpr (WORDS == 2 && BASE != MODE_BSEARCH) {
+      // rankBits = ceil(orbitals / 2) <= 32, so t >> bits fits 32 bits
+      const uint32_t last = (uint32_t)a.dim - 1u;
+      const uint32_t mask = a.rankBits >= 32 ? 0xffffffffu : (1u << a.rankBits) - 1u;
+      const int bits = a.rankBits;
+      auto find = [&](uint64_t t) -> uint32_t {
+        return min(last - loAt((uint32_t)t & mask) - sHi[(uint32_t)(t >> bits)], last);
+      };
+      if constexpr (ROW == ROW_FREE)
+        dh_sphere_row_free<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+      else if constexpr (ROW == ROW_X4)
+        dh_sphere_row_x4<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+      else
+        dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
     } else {
-      auto find = [&](uint64_t t) -> int64_t { return sphere_find<BASE>(a, t, sLut, sLo, sHi); };
-      if constexpr (MODE >= MODE_RANK_LDS_FREE)
-        dh_sphere_row_free<K, LPR, FreeBatch<K>::value>(s, a.nbrOrbitals, sF, sBase, lane, find, x, row, acc);
-      else if constexpr (MODE >= MODE_RANK_LDS_X4)
+      auto find = [&](uint64_t t) -> int64_t { return sphere_find<BASE>(a, t, sLut, sLo, sHi, sLo16); };
+      if constexpr (ROW == ROW_FREE)
+        dh_sphere_row_free<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
+      else if constexpr (ROW == ROW_X4)
         dh_sphere_row_x4<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
       else
         dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x, acc);
@@ -393,6 +435,7 @@ static SphereArgs sphere_args(const dh_hamiltonian* h, int lutBitsUsed) {
   a.lutSize = (1 << lutBitsUsed) + 1;
   a.rankLo = b->d_rankLo;
   a.rankHi = b->d_rankHi;
+  a.rankLo16 = b->d_rankLo16;
   a.rankBits = b->rankBits;
   a.rankLoSize = (int)b->rankLoSize;
   a.rankHiSize = (int)b->rankHiSize;
@@ -408,7 +451,16 @@ static size_t sphere_lds(const dh_hamiltonian* h, int mode, int lutBitsUsed) {
   bytes = (bytes + 7) & ~size_t(7);
   if (mode == MODE_BSEARCH) bytes += sizeof(int) * ((size_t(1) << lutBitsUsed) + 1);
   if (mode == MODE_RANK_LDS) bytes += sizeof(uint32_t) * (h->basis->rankLoSize + h->basis->rankHiSize);
+  if (mode == MODE_RANK_LO16) bytes += sizeof(uint16_t) * h->basis->rankLoSize;
   return bytes;
+}
+
+// where the rank tables of a sphere product are read from (MODE_RANK_LDS, MODE_RANK_LO16 or
+// MODE_RANK_GLOBAL)
+static int sphere_rank_tables(const dh_hamiltonian* h) {
+  if (h->ldsTables && sphere_lds(h, MODE_RANK_LDS, 0) <= 64 * 1024) return MODE_RANK_LDS;
+  if (h->ldsTables && h->basis->d_rankLo16 && sphere_lds(h, MODE_RANK_LO16, 0) <= 96 * 1024) return MODE_RANK_LO16;
+  return MODE_RANK_GLOBAL;
 }
 
 template <typename Kern>
@@ -422,50 +474,78 @@ static int launch_persistent(dh_context* c, Kern kern, int blockSize, size_t lds
 }
 
 template <int K, int LPR, int MODE>
-static int sphere_launch(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first, int64_t count) {
+static int sphere_launch(dh_hamiltonian* h, int bs, const double* x, double* y, int add, int64_t first,
+                         int64_t count) {
   dh_context* c = h->ctx;
   int lutBitsUsed = std::min(h->basis->lutBits, 13);
   SphereArgs a = sphere_args(h, lutBitsUsed);
   size_t lds = sphere_lds(h, ModeBase<MODE>::value, lutBitsUsed);
   if (lds > kLdsLimit) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
-  // 32-bit words for one or two vectors only, as for the sector sets
-  const bool s32 = K <= 2 && MODE != MODE_BSEARCH && h->basis->nbrOrbitals <= 32 && h->basis->dim < (1ll << 31);
-  auto kern = s32 ? sphere_hv_kernel<K, LPR, MODE, (K <= 2 && MODE != MODE_BSEARCH)> : sphere_hv_kernel<K, LPR, MODE, false>;
+  // 32-bit words for one or two vectors only, as for the sector sets; 32-bit indices whenever the
+  // space has fewer than 2^31 states
+  const bool idx32 = MODE != MODE_BSEARCH && h->basis->dim < (1ll << 31);
+  const bool s32 = K <= 2 && idx32 && h->basis->nbrOrbitals <= 32;
+  constexpr int W1 = (K <= 2 && MODE != MODE_BSEARCH) ? 1 : 0;
+  constexpr int W2 = MODE != MODE_BSEARCH ? 2 : 0;
+  auto kern = s32 ? sphere_hv_kernel<K, LPR, MODE, W1>
+                  : (idx32 ? sphere_hv_kernel<K, LPR, MODE, W2> : sphere_hv_kernel<K, LPR, MODE, 0>);
   if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int64_t rowsPerBlock = h->blockSize / LPR;
+  int64_t rowsPerBlock = bs / LPR;
   dim3 grid;
-  launch_persistent(c, kern, h->blockSize, lds, (count + rowsPerBlock - 1) / rowsPerBlock, grid);
-  hipLaunchKernelGGL(kern, grid, dim3(h->blockSize), lds, c->stream, a, x, y, add, first, count);
+  launch_persistent(c, kern, bs, lds, (count + rowsPerBlock - 1) / rowsPerBlock, grid);
+  hipLaunchKernelGGL(kern, grid, dim3(bs), lds, c->stream, a, x, y, add, first, count);
   DH_HIP(c, hipGetLastError());
   return DH_OK;
 }
 
 template <int K, int LPR>
-static int sphere_dispatch_mode(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first,
+static int sphere_dispatch_mode(dh_hamiltonian* h, int bs, const double* x, double* y, int add, int64_t first,
                                 int64_t count) {
-  if (h->indexMode == DH_INDEX_BSEARCH) return sphere_launch<K, LPR, MODE_BSEARCH>(h, x, y, add, first, count);
-  size_t ldsRank = sphere_lds(h, MODE_RANK_LDS, 0);
-  const bool tablesInLds = h->ldsTables && ldsRank <= 64 * 1024;
+  if (h->indexMode == DH_INDEX_BSEARCH) return sphere_launch<K, LPR, MODE_BSEARCH>(h, bs, x, y, add, first, count);
+  // where the rank tables go: both in LDS when they fit 64 KiB, else the 16-bit low table in LDS
+  // (the high table from L2), else both from global memory
+  const int where = sphere_rank_tables(h);
+  const bool tablesInLds = where == MODE_RANK_LDS, lo16 = where == MODE_RANK_LO16;
   if (h->indexMode == DH_INDEX_RANK_X4) {
-    if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_X4>(h, x, y, add, first, count);
-    return sphere_launch<K, LPR, MODE_RANK_GLOBAL_X4>(h, x, y, add, first, count);
+    if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_X4>(h, bs, x, y, add, first, count);
+    if (lo16) return sphere_launch<K, LPR, MODE_RANK_LO16_X4>(h, bs, x, y, add, first, count);
+    return sphere_launch<K, LPR, MODE_RANK_GLOBAL_X4>(h, bs, x, y, add, first, count);
   }
-  if (h->indexMode == DH_INDEX_RANK_FREE) {
-    if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_FREE>(h, x, y, add, first, count);
-    return sphere_launch<K, LPR, MODE_RANK_GLOBAL_FREE>(h, x, y, add, first, count);
+  // DH_INDEX_RANK: the free-pair masks for k <= 2 (fewer instructions per term), the range loop for
+  // k > 2 (where the masks' extra registers cost more than they save)
+  const bool freePairs = h->indexMode == DH_INDEX_RANK_FREE || (h->indexMode == DH_INDEX_RANK && K <= 2);
+  if (freePairs) {
+    if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS_FREE>(h, bs, x, y, add, first, count);
+    if (lo16) return sphere_launch<K, LPR, MODE_RANK_LO16_FREE>(h, bs, x, y, add, first, count);
+    return sphere_launch<K, LPR, MODE_RANK_GLOBAL_FREE>(h, bs, x, y, add, first, count);
   }
-  if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS>(h, x, y, add, first, count);
-  return sphere_launch<K, LPR, MODE_RANK_GLOBAL>(h, x, y, add, first, count);
+  if (tablesInLds) return sphere_launch<K, LPR, MODE_RANK_LDS>(h, bs, x, y, add, first, count);
+  if (lo16) return sphere_launch<K, LPR, MODE_RANK_LO16>(h, bs, x, y, add, first, count);
+  return sphere_launch<K, LPR, MODE_RANK_GLOBAL>(h, bs, x, y, add, first, count);
 }
 
 template <int K>
 static int sphere_dispatch_lpr(dh_hamiltonian* h, const double* x, double* y, int add, int64_t first,
                                int64_t count) {
-  switch (h->lanesPerRow) {
-    case 1: return sphere_dispatch_mode<K, 1>(h, x, y, add, first, count);
-    case 2: return sphere_dispatch_mode<K, 2>(h, x, y, add, first, count);
-    case 4: return sphere_dispatch_mode<K, 4>(h, x, y, add, first, count);
-    case 8: return sphere_dispatch_mode<K, 8>(h, x, y, add, first, count);
+  int lpr = h->lanesPerRow, bs = h->blockSize;
+  if (h->autoLaunch && h->basis->dim >= (int64_t)h->ctx->nbrCU * 256) {
+    // the fastest shapes measured on MI355X for N=12 2S=33 Lz=0 (7.8 M rows).  With the 16-bit low
+    // table in LDS a workgroup holds ~60 KiB, two per CU: one lane per row with 1024-thread
+    // workgroups for k <= 2 (6.1 ms at k = 1, against 10.6 ms at 256 threads), 512 for k > 2.
+    // With both tables in global memory: two lanes per row, 256 threads for k <= 2, 1024 above.
+    if (sphere_rank_tables(h) == MODE_RANK_LO16) {
+      lpr = 1;
+      bs = K <= 2 ? 1024 : 512;
+    } else {
+      lpr = 2;
+      bs = K <= 2 ? 256 : 1024;
+    }
+  }
+  switch (lpr) {
+    case 1: return sphere_dispatch_mode<K, 1>(h, bs, x, y, add, first, count);
+    case 2: return sphere_dispatch_mode<K, 2>(h, bs, x, y, add, first, count);
+    case 4: return sphere_dispatch_mode<K, 4>(h, bs, x, y, add, first, count);
+    case 8: return sphere_dispatch_mode<K, 8>(h, bs, x, y, add, first, count);
     default: return dh_fail(h->ctx, DH_E_ARG, "lanes_per_row must be 1, 2, 4 or 8 for this Hamiltonian");
   }
 }
@@ -718,6 +798,7 @@ extern "C" int dh_sphere_hamiltonian_create(dh_context* c, dh_hamiltonian** out,
   h->lanesPerRow = (b->dim < (int64_t)c->nbrCU * 256) ? 4 : 1;
   h->blockSize = 512;
   h->indexMode = DH_INDEX_RANK;
+  h->autoLaunch = true;
   *out = h;
   return DH_OK;
 }
@@ -790,11 +871,13 @@ extern "C" int dh_hamiltonian_set_launch(dh_hamiltonian* h, int lpr, int blockSi
   if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8) return dh_fail(h->ctx, DH_E_ARG, "lanes_per_row in {1,2,4,8}");
   if (h->kind == DH_HAM_SPIN && lpr != 1) return dh_fail(h->ctx, DH_E_ARG, "spin kernels use one row per lane");
   if (blockSize < 64 || blockSize > 1024 || (blockSize % 64)) return dh_fail(h->ctx, DH_E_ARG, "block_size");
-  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_X4 && mode != DH_INDEX_RANK_FREE)
+  if (mode != DH_INDEX_BSEARCH && mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_X4 && mode != DH_INDEX_RANK_FREE &&
+      mode != DH_INDEX_RANK_RANGE)
     return dh_fail(h->ctx, DH_E_ARG, "index_mode");
   h->lanesPerRow = lpr;
   h->blockSize = blockSize;
   h->indexMode = mode;
   h->ldsTables = lds;
+  h->autoLaunch = false;
   return DH_OK;
 }

@@ -23,8 +23,10 @@ __device__ __forceinline__ int64_t dh_find_bsearch(uint64_t target, const int* _
   return lo;  // in [lut[key+1], lut[key]) for every target of the sector
 }
 
-// Combinatorial rank of a fermion state in the descending basis of its sector.
-__device__ __forceinline__ int64_t dh_find_rank(uint64_t target, const uint32_t* __restrict__ lo,
+// Combinatorial rank of a fermion state in the descending basis of its sector (the low table in 32
+// bits, or its 16-bit copy).
+template <typename L>
+__device__ __forceinline__ int64_t dh_find_rank(uint64_t target, const L* __restrict__ lo,
                                                 const uint32_t* __restrict__ hi, int bits, int64_t dim) {
   uint32_t a = lo[target & ((1ull << bits) - 1)];
   uint32_t b = hi[target >> bits];

@@ -9,6 +9,8 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <type_traits>
 #include <unordered_set>
 #include <vector>
 
@@ -95,11 +97,13 @@ struct dh_basis {
   // (of the ordering key) are >= v, v in [0, 2^lutBits]
   int lutBits = 0, lutShift = 0;
   int* d_lut = nullptr;
-  // combinatorial rank tables (fermions): index = dim - 1 - (lo[s & (2^rankBits-1)] + hi[s >> rankBits])
+  // combinatorial rank tables (fermions): index = dim - 1 - (lo[s & (2^rankBits-1)] + hi[s >> rankBits]);
+  // d_rankLo16: the same low table in 16 bits (null when a value does not fit), for LDS staging
   int rankBits = 0;
   int64_t rankLoSize = 0, rankHiSize = 0;
   uint32_t* d_rankLo = nullptr;
   uint32_t* d_rankHi = nullptr;
+  uint16_t* d_rankLo16 = nullptr;
   // binomial table C[n][k], n,k <= 32 (spin rank)
   uint32_t* d_binom = nullptr;
 };
@@ -142,6 +146,8 @@ struct dh_hamiltonian {
   int periodic = 0;
   // launch configuration
   int lanesPerRow = 1, blockSize = 256, indexMode = DH_INDEX_RANK, ldsTables = 1;
+  // sphere: pick lanes per row and workgroup size per product from k (until set_launch is called)
+  bool autoLaunch = false;
 };
 
 enum dh_sector_kind { DH_SECTORS_SPHERE = 1, DH_SECTORS_MOMENTUM = 2, DH_SECTORS_TORUS_TRANSLATIONS = 3 };
@@ -163,6 +169,7 @@ struct dh_sector_set {
   int64_t loSize = 0, hiSize = 0;  // hi table of sector s starts at s * hiSize
   uint32_t* d_rankLo = nullptr;
   uint32_t* d_rankHi = nullptr;
+  uint16_t* d_rankLo16 = nullptr;  // the low table in 16 bits (null when a value does not fit)
   int64_t* d_offsets = nullptr;
   // sphere factors (per-pair layout, and packed symmetric per pair sum)
   double* d_factors = nullptr;
@@ -366,4 +373,44 @@ inline uint64_t dh_binom_host(int n, int k) {
   uint64_t r = 1;
   for (int i = 1; i <= k; ++i) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
   return r;
+}
+
+// Low-table width of the combinatorial rank (index = dim - 1 - lo[low bits] - hi[high bits]).  The
+// low table counts patterns of the low orbitals with the same particle number and momentum, so its
+// values stay below C(bits, bits / 2) and fit 16 bits: at 14 bits it is 32 KiB and is staged in LDS
+// by the product kernels.  Up to 28 orbitals the split is even; from 29 to 34 orbitals the low part
+// stays at 14 bits and the high table grows to at most 2^20 entries (4 MiB, of which a sector
+// touches a small clustered part: 1.6 MB for N=12 2S=33 Lz=0); beyond 34 the split is even again.
+inline int dh_rank_split(int nbrOrbitals) {
+  return (nbrOrbitals > 28 && nbrOrbitals <= 34) ? 14 : (nbrOrbitals + 1) / 2;
+}
+
+// f(i) for i in [0, n) on the host's hardware threads (table construction on the host)
+template <class F>
+inline void dh_parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&] {
+      for (int i = next++; i < n; i = next++) f(i);
+    });
+  for (auto& th : pool) th.join();
+}
+
+// 16-bit copy of a low rank table, or an empty vector when the table is longer than 2^15 entries or a
+// value does not fit
+inline std::vector<uint16_t> dh_rank_lo16(const std::vector<uint32_t>& lo) {
+  std::vector<uint16_t> out;
+  if (lo.size() > ((size_t)1 << 15)) return out;
+  out.reserve(lo.size());
+  for (uint32_t v : lo) {
+    if (v > 0xffffu) return {};
+    out.push_back((uint16_t)v);
+  }
+  return out;
 }

@@ -111,7 +111,7 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
     S->offsets[i + 1] = S->offsets[i] + d;
   }
   S->totalDim = S->offsets[nbrSectors];
-  S->rankBits = (nOrb + 1) / 2;
+  S->rankBits = dh_rank_split(nOrb);
   S->loSize = 1ll << S->rankBits;
   S->hiSize = 1ll << (nOrb - S->rankBits);
   std::vector<uint32_t> lo(S->loSize, 0), hi((size_t)S->hiSize * nbrSectors, 0);
@@ -126,7 +126,7 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
     }
     lo[l] = (uint32_t)acc;
   }
-  for (int sec = 0; sec < nbrSectors; ++sec)
+  dh_parallel_for(nbrSectors, [&](int sec) {
     for (uint64_t h = 0; h < (uint64_t)S->hiSize; ++h) {
       int popH = __builtin_popcountll(h), clsH = 0;
       for (uint64_t r = h; r; r &= r - 1) clsH = cl.add(clsH, S->rankBits + __builtin_ctzll(r));
@@ -142,12 +142,15 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
       }
       hi[(size_t)sec * S->hiSize + h] = (uint32_t)acc;
     }
+  });
+  const std::vector<uint16_t> lo16 = dh_rank_lo16(lo);
   int rc;
   int64_t* d_T = nullptr;
   if ((rc = dh_alloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
       (rc = dh_alloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
       (rc = dh_alloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
       (rc = dh_alloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
+      (!lo16.empty() && (rc = dh_alloc(c, (void**)&S->d_rankLo16, sizeof(uint16_t) * lo16.size()))) ||
       (rc = dh_alloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
     if (d_T) hipFree(d_T);
     dh_sector_set_destroy(S);
@@ -156,6 +159,8 @@ extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out,
   hipMemcpyAsync(d_T, T.data(), sizeof(int64_t) * T.size(), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(S->d_rankLo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(S->d_rankHi, hi.data(), sizeof(uint32_t) * hi.size(), hipMemcpyHostToDevice, c->stream);
+  if (S->d_rankLo16)
+    hipMemcpyAsync(S->d_rankLo16, lo16.data(), sizeof(uint16_t) * lo16.size(), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(S->d_offsets, S->offsets.data(), sizeof(int64_t) * (nbrSectors + 1), hipMemcpyHostToDevice,
                  c->stream);
   for (int sec = 0; sec < nbrSectors; ++sec) {
@@ -302,6 +307,7 @@ struct LatticeArgs {
   const double* factors;
   const uint32_t* rankLo;
   const uint32_t* rankHi;
+  const uint16_t* rankLo16;  // the low table in 16 bits, staged in LDS (LO_LDS)
   int rankBits;
   int64_t loSize, hiSize;
   double shift;
@@ -351,7 +357,9 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
   int* sPairs = sPairOff + a.nbrSums + 1;                     // totalPairs
   int* sShift = sPairs + a.totalPairs;                        // FREE: 2 nbrSums
   int* sPP = sShift + (FREE ? 2 * a.nbrSums : 0);             // FREE: nbrSums * nbrOrbitals
-  uint32_t* sLo = reinterpret_cast<uint32_t*>(sPP + (FREE ? a.nbrSums * a.nbrOrbitals : 0));
+  // the low table: its 16-bit copy in LDS (LO_LDS), or the 32-bit table in global memory
+  using LoT = typename std::conditional<LO_LDS, uint16_t, uint32_t>::type;
+  LoT* sLo = reinterpret_cast<LoT*>(sPP + (FREE ? a.nbrSums * a.nbrOrbitals : 0));
   for (int i = threadIdx.x; i < 2 * a.nActive + 1; i += blockDim.x) sWork[i] = a.work[i];
   for (int i = threadIdx.x; i <= a.nbrSums; i += blockDim.x) {
     sFacOff[i] = a.sumFactorOff[i];
@@ -365,9 +373,13 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
     for (int i = threadIdx.x; i < a.nbrSums * a.nbrOrbitals; i += blockDim.x) sPP[i] = a.partnerPos[i];
   }
   if constexpr (LO_LDS)
-    for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo[i];
+    for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo16[i];
   __syncthreads();
-  const uint32_t* lo = LO_LDS ? sLo : a.rankLo;
+  const LoT* lo;
+  if constexpr (LO_LDS)
+    lo = sLo;
+  else
+    lo = a.rankLo;
   const int64_t* sPrefix = sWork + a.nActive;
   const int lane = threadIdx.x % LPR;
   const int64_t rowsPerTile = blockDim.x / LPR;
@@ -552,6 +564,7 @@ static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add,
   a.sumFactorOff = S->d_sumFactorOff;
   a.factors = S->d_cfactors;
   a.rankLo = S->d_rankLo;
+  a.rankLo16 = S->d_rankLo16;
   a.rankHi = S->d_rankHi;
   a.rankBits = S->rankBits;
   a.loSize = S->loSize;
@@ -569,8 +582,8 @@ static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add,
                 4 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs;
   if (free) base += 8 * 3 * (size_t)S->nbrSums + 4 * 2 * (size_t)S->nbrSums + 4 * (size_t)S->nbrSums * S->nbrOrbitals;
   base = (base + 7) & ~size_t(7);
-  size_t withLo = base + 4 * (size_t)S->loSize;
-  bool loLds = withLo <= 64 * 1024;
+  size_t withLo = base + 2 * (size_t)S->loSize;
+  bool loLds = S->d_rankLo16 && withLo <= 64 * 1024;
   size_t lds = loLds ? withLo : base;
   if (lds > 160 * 1024) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
   auto launch = [&](auto kern) -> int {

@@ -93,10 +93,11 @@ extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, i
     S->offsets[i + 1] = S->offsets[i] + d;
   }
   S->totalDim = S->offsets[nbrSectors];
-  S->rankBits = (nOrb + 1) / 2;
+  S->rankBits = dh_rank_split(nOrb);
   S->loSize = 1ll << S->rankBits;
   S->hiSize = 1ll << (nOrb - S->rankBits);
-  // rank tables: one low table, one high table per sector
+  // rank tables: one low table, one high table per sector (the sectors' high tables on host threads:
+  // 133 sectors of 2^20 entries at 34 orbitals)
   std::vector<uint32_t> lo(S->loSize, 0), hi((size_t)S->hiSize * nbrSectors, 0);
   for (uint64_t l = 0; l < (uint64_t)S->loSize; ++l) {
     int n = 0, s = 0;
@@ -109,7 +110,7 @@ extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, i
     }
     lo[l] = (uint32_t)acc;
   }
-  for (int sec = 0; sec < nbrSectors; ++sec)
+  dh_parallel_for(nbrSectors, [&](int sec) {
     for (uint64_t h = 0; h < (uint64_t)S->hiSize; ++h) {
       int popH = __builtin_popcountll(h), sumH = 0;
       for (uint64_t r = h; r; r &= r - 1) sumH += S->rankBits + __builtin_ctzll(r);
@@ -124,12 +125,15 @@ extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, i
       }
       hi[(size_t)sec * S->hiSize + h] = (uint32_t)acc;
     }
+  });
+  const std::vector<uint16_t> lo16 = dh_rank_lo16(lo);
   int rc;
   int64_t* d_T = nullptr;
   if ((rc = dh_alloc(c, (void**)&d_T, sizeof(int64_t) * T.size())) ||
       (rc = dh_alloc(c, (void**)&S->d_states, sizeof(uint64_t) * S->totalDim)) ||
       (rc = dh_alloc(c, (void**)&S->d_rankLo, sizeof(uint32_t) * lo.size())) ||
       (rc = dh_alloc(c, (void**)&S->d_rankHi, sizeof(uint32_t) * hi.size())) ||
+      (!lo16.empty() && (rc = dh_alloc(c, (void**)&S->d_rankLo16, sizeof(uint16_t) * lo16.size()))) ||
       (rc = dh_alloc(c, (void**)&S->d_offsets, sizeof(int64_t) * (nbrSectors + 1)))) {
     if (d_T) hipFree(d_T);
     dh_sector_set_destroy(S);
@@ -138,6 +142,8 @@ extern "C" int dh_sector_set_create(dh_context* c, dh_sector_set** out, int N, i
   hipMemcpyAsync(d_T, T.data(), sizeof(int64_t) * T.size(), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(S->d_rankLo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(S->d_rankHi, hi.data(), sizeof(uint32_t) * hi.size(), hipMemcpyHostToDevice, c->stream);
+  if (S->d_rankLo16)
+    hipMemcpyAsync(S->d_rankLo16, lo16.data(), sizeof(uint16_t) * lo16.size(), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(S->d_offsets, S->offsets.data(), sizeof(int64_t) * (nbrSectors + 1), hipMemcpyHostToDevice,
                  c->stream);
   for (int sec = 0; sec < nbrSectors; ++sec) {
@@ -159,7 +165,7 @@ extern "C" int dh_sector_set_destroy(dh_sector_set* S) {
   if (!S) return DH_E_ARG;
   dh_context* c = S->ctx;
   hipStreamSynchronize(c->stream);
-  for (void* p : {(void*)S->d_states, (void*)S->d_rankLo, (void*)S->d_rankHi, (void*)S->d_offsets,
+  for (void* p : {(void*)S->d_states, (void*)S->d_rankLo, (void*)S->d_rankHi, (void*)S->d_rankLo16, (void*)S->d_offsets,
                   (void*)S->d_factors, (void*)S->d_pairOffset, (void*)S->d_work, (void*)S->d_coef,
                   (void*)S->d_pairInfo, (void*)S->d_sumPairOff, (void*)S->d_sumPairs, (void*)S->d_sumFactorOff,
                   (void*)S->d_cfactors, (void*)S->d_packedFactors, (void*)S->d_sumBase, (void*)S->d_orbit,
@@ -238,9 +244,9 @@ extern "C" int dh_sector_set_launch(dh_sector_set* S, int lanesPerRow, int block
 
 extern "C" int dh_sector_set_row_mode(dh_sector_set* S, int mode) {
   if (!S) return DH_E_ARG;
-  if (mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_FREE)
-    return dh_fail(S->ctx, DH_E_ARG, "row mode must be DH_INDEX_RANK or DH_INDEX_RANK_FREE");
-  if (S->kind == DH_SECTORS_TORUS_TRANSLATIONS && mode != DH_INDEX_RANK)
+  if (mode != DH_INDEX_RANK && mode != DH_INDEX_RANK_FREE && mode != DH_INDEX_RANK_RANGE)
+    return dh_fail(S->ctx, DH_E_ARG, "row mode must be DH_INDEX_RANK, DH_INDEX_RANK_FREE or DH_INDEX_RANK_RANGE");
+  if (S->kind == DH_SECTORS_TORUS_TRANSLATIONS && mode == DH_INDEX_RANK_FREE)
     return dh_fail(S->ctx, DH_E_ARG, "torus sets with magnetic translations have one row routine");
   S->rowMode = mode;
   return DH_OK;
@@ -297,6 +303,7 @@ struct SectorsArgs {
   const int* sumBase;
   const uint32_t* rankLo;
   const uint32_t* rankHi;
+  const uint16_t* rankLo16;  // the low table in 16 bits, staged in LDS (LO_LDS)
   int rankBits;
   int64_t loSize, hiSize;
   double shift;
@@ -318,26 +325,31 @@ __device__ __forceinline__ int tile_sector(const int64_t* sPrefix, int nActive, 
   return lo;
 }
 
-// S32: the orbitals fit a 32-bit word (and every sector has fewer than 2^31 states), so the row
-// routine runs on 32-bit state words and 32-bit indices (used for k <= 2).
-// FREE: the created pairs come from the free-pair masks (dh_sphere_row_free), G terms in flight per
-// lane.
+// S32: the orbitals fit a 32-bit word, so the row routine runs on 32-bit state words (used for
+// k <= 2).  Every sector has fewer than 2^31 states (the set refuses larger ones), so indices are
+// 32-bit in both forms.
+// FREE: the created pairs come from the free-pair masks (dh_sphere_row_free).
 template <int K, int LPR, bool LO_LDS, bool S32, bool FREE>
 __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const double* __restrict__ x,
                                                           double* __restrict__ y, int add) {
-  constexpr int G = K == 1 ? 4 : (K <= 4 ? 2 : 1);
   extern __shared__ double smem[];
   double* sF = smem;
   int* sBase = reinterpret_cast<int*>(sF + a.nbrFactors);
   int64_t* sWork = reinterpret_cast<int64_t*>(smem) + ((a.nbrFactors * 8 + a.nbrSums * 4 + 7) / 8);
-  uint32_t* sLo = reinterpret_cast<uint32_t*>(sWork + 2 * a.nActive + 1);
+  uint16_t* sLo = reinterpret_cast<uint16_t*>(sWork + 2 * a.nActive + 1);
   for (int i = threadIdx.x; i < a.nbrFactors; i += blockDim.x) sF[i] = a.factors[i];
   for (int i = threadIdx.x; i < a.nbrSums; i += blockDim.x) sBase[i] = a.sumBase[i];
   for (int i = threadIdx.x; i < 2 * a.nActive + 1; i += blockDim.x) sWork[i] = a.work[i];
   if constexpr (LO_LDS)
-    for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo[i];
+    for (int i = threadIdx.x; i < a.loSize; i += blockDim.x) sLo[i] = a.rankLo16[i];
   __syncthreads();
-  const uint32_t* lo = LO_LDS ? sLo : a.rankLo;
+  // the low table: 16-bit copy in LDS, or the 32-bit table in global memory
+  auto loAt = [&](uint32_t i) -> uint32_t {
+    if constexpr (LO_LDS)
+      return sLo[i];
+    else
+      return a.rankLo[i];
+  };
   const int64_t* sPrefix = sWork + a.nActive;
   const int lane = threadIdx.x % LPR;
   const int64_t rowsPerTile = blockDim.x / LPR;
@@ -358,17 +370,21 @@ __global__ void __launch_bounds__(1024) sectors_hv_kernel(SectorsArgs a, const d
       const uint32_t last = (uint32_t)dim - 1u, mask = (1u << a.rankBits) - 1u;
       const int bits = a.rankBits;
       // clamped so that a target outside the sector can never address outside the vectors
-      auto find = [&](uint32_t t) -> uint32_t { return min(last - lo[t & mask] - hi[t >> bits], last); };
+      auto find = [&](uint32_t t) -> uint32_t { return min(last - loAt(t & mask) - hi[t >> bits], last); };
       if constexpr (FREE)
-        dh_sphere_row_free<K, LPR, G>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K,
-                                      (uint32_t)(active ? local : 0), acc);
+        dh_sphere_row_free<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
       else
         dh_sphere_row<K, LPR>((uint32_t)s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
     } else {
-      auto find = [&](uint64_t t) -> int64_t { return dh_find_rank(t, lo, hi, a.rankBits, dim); };
+      // rankBits = ceil(orbitals / 2) <= 32, so t >> bits fits 32 bits
+      const uint32_t last = (uint32_t)dim - 1u;
+      const uint32_t mask = a.rankBits >= 32 ? 0xffffffffu : (1u << a.rankBits) - 1u;
+      const int bits = a.rankBits;
+      auto find = [&](uint64_t t) -> uint32_t {
+        return min(last - loAt((uint32_t)t & mask) - hi[(uint32_t)(t >> bits)], last);
+      };
       if constexpr (FREE)
-        dh_sphere_row_free<K, LPR, G>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K,
-                                      (int64_t)(active ? local : 0), acc);
+        dh_sphere_row_free<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
       else
         dh_sphere_row<K, LPR>(s, a.nbrOrbitals, sF, sBase, lane, find, x + off * K, acc);
     }
@@ -397,6 +413,7 @@ static SectorsArgs sectors_args(dh_sector_set* S, int nActive, int64_t totalTile
   a.sumBase = S->d_sumBase;
   a.rankLo = S->d_rankLo;
   a.rankHi = S->d_rankHi;
+  a.rankLo16 = S->d_rankLo16;
   a.rankBits = S->rankBits;
   a.loSize = S->loSize;
   a.hiSize = S->hiSize;
@@ -417,8 +434,9 @@ static int sectors_hv_launch(dh_sector_set* S, const double* x, double* y, int a
   if (rc) return rc;
   if (tiles == 0) return DH_OK;
   size_t base = ((size_t)S->nbrPackedFactors * 8 + (size_t)S->nbrPairSums * 4 + 7) / 8 * 8 + 8 * (2 * na + 1);
-  size_t withLo = base + 4 * S->loSize;
-  bool loLds = withLo <= 64 * 1024;
+  // the 16-bit low table in LDS when it exists and the workgroup's tables stay within 64 KiB
+  size_t withLo = base + 2 * S->loSize;
+  bool loLds = S->d_rankLo16 && withLo <= 64 * 1024;
   size_t lds = loLds ? withLo : base;
   if (lds > 160 * 1024) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
   SectorsArgs a = sectors_args(S, na, tiles);
@@ -437,7 +455,9 @@ static int sectors_hv_launch(dh_sector_set* S, const double* x, double* y, int a
   // more vectors the 64-bit form is faster on MI355X (its lower occupancy keeps more of the x
   // gathers in cache: 11.3 against 16.5 ms at k = 8 over the 91 sectors of N=10 2S=27)
   const bool s32 = K <= 2 && S->nbrOrbitals <= 32;
-  if (S->rowMode == DH_INDEX_RANK_FREE) {
+  // DH_INDEX_RANK: the free-pair masks for k <= 2, the range loop for k > 2 (the faster per k on
+  // MI355X, as for a single space)
+  if (S->rowMode == DH_INDEX_RANK_FREE || (S->rowMode == DH_INDEX_RANK && K <= 2)) {
     if (loLds)
       return s32 ? launch(sectors_hv_kernel<K, LPR, true, (K <= 2), true>)
                  : launch(sectors_hv_kernel<K, LPR, true, false, true>);

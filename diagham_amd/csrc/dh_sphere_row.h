@@ -153,69 +153,59 @@ __device__ __forceinline__ U dh_free_pairs(U t, int nbrOrbitals, int sum, int m3
   return f & mirror & hiMask & ~loMask;
 }
 
-// The terms of dh_sphere_row in the same order, produced by an iterator over the free created
+// The terms of dh_sphere_row in the same order, with the created pairs taken from the mask of free
 // pairs of each annihilated pair (dh_free_pairs): no iteration is spent on a created pair with an
-// occupied orbital.  A lane takes the terms of its row G at a time (each LPR-th term when LPR
-// lanes share the row): their G index lookups are issued together, then their G gathers, so G
-// dependent chains are in flight per lane.  Slots past the row's last term gather x[self] with a
-// zero coefficient; `self` is the row's own index (relative to xbase).
-template <int K, int LPR, int G, typename U, typename Find, typename I>
+// occupied orbital.  With LPR lanes on a row, lane l takes the free pairs l, l + LPR, ... of each
+// annihilated pair.  For K <= 2 a term's multiply-add waits for the next term's index loads, as in
+// dh_sphere_row, so the terms are added in the same order and with one lane per row the sum is
+// bitwise dh_sphere_row's.
+template <int K, int LPR, typename U, typename Find>
 __device__ __forceinline__ void dh_sphere_row_free(U s, int nbrOrbitals, const double* __restrict__ sF,
                                                    const int* __restrict__ sBase, int lane, Find find,
-                                                   const double* __restrict__ xbase, I self, double* acc) {
-  U r1 = s, r2 = 0, pm = 0, t = 0;
-  int n1 = 0, pc1 = 0, sgnA = 0, sum = 0, m3lo = 0, i = 0;
-  const double* fS = sF;
-  int skip = lane;  // terms of the row before this lane's next one (LPR > 1)
-  bool more = true;
-  while (more) {
-    I idx[G];
-    double coef[G];
+                                                   const double* __restrict__ xbase, double* acc) {
+  constexpr bool PIPE = K <= 2;
+  double pend[K];
+  double pendCoef = 0.0;
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
-      idx[q] = self;
-      coef[q] = 0.0;
-      while (more) {
-        if (!pm) {
-          if (!r2) {
-            if (!r1) {
-              more = false;
-              break;
-            }
-            n1 = dh_ctz(r1);
-            r1 &= r1 - 1;
-            pc1 = dh_popc(s >> n1);
-            r2 = s & ((U(1) << n1) - U(1));
-            continue;
-          }
-          const int n2 = dh_ctz(r2);
-          r2 &= r2 - 1;
-          sgnA = pc1 + dh_popc(s >> n2);
-          t = s ^ (U(1) << n1) ^ (U(1) << n2);
-          sum = n1 + n2;
-          m3lo = (sum >> 1) + 1;
-          fS = sF + sBase[sum];
-          i = n1 - m3lo;
-          pm = dh_free_pairs(t, nbrOrbitals, sum, m3lo, min(sum, nbrOrbitals - 1));
-          continue;
-        }
-        if (LPR > 1 && skip > 0) {
-          pm &= pm - 1;
-          --skip;
-          continue;
-        }
+  for (int v = 0; v < K; ++v) pend[v] = 0.0;
+  for (U r1 = s; r1; r1 &= r1 - 1) {
+    const int n1 = dh_ctz(r1);
+    const int pc1 = dh_popc(s >> n1);
+    for (U r2 = s & ((U(1) << n1) - U(1)); r2; r2 &= r2 - 1) {
+      const int n2 = dh_ctz(r2);
+      const int sgnA = pc1 + dh_popc(s >> n2);
+      const U t = s ^ (U(1) << n1) ^ (U(1) << n2);
+      const int sum = n1 + n2;
+      const int m3lo = (sum >> 1) + 1;
+      const double* fS = sF + sBase[sum];
+      const int i = n1 - m3lo;
+      U pm = dh_free_pairs(t, nbrOrbitals, sum, m3lo, min(sum, nbrOrbitals - 1));
+      if constexpr (LPR > 1)
+        for (int q = 0; q < lane; ++q) pm &= pm - 1;
+      while (pm) {
         const int m3 = dh_ctz(pm);
-        pm &= pm - 1;
-        skip = LPR - 1;
+#pragma unroll
+        for (int q = 0; q < LPR; ++q) pm &= pm - 1;  // this term, then the other lanes' next ones
         const int m4 = sum - m3;
         const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
-        idx[q] = find(t | (U(1) << m3) | (U(1) << m4));
-        const double c = fS[dh_packed_index(i, m3 - m3lo)];
-        coef[q] = (sg & 1) ? -c : c;
-        break;
+        const auto idx = find(t | (U(1) << m3) | (U(1) << m4));
+        double coef = fS[dh_packed_index(i, m3 - m3lo)];
+        if (sg & 1) coef = -coef;
+        if constexpr (PIPE) {
+          const double* xr = xbase + (size_t)idx * K;
+#pragma unroll
+          for (int v = 0; v < K; ++v) acc[v] = fma(pendCoef, pend[v], acc[v]);
+#pragma unroll
+          for (int v = 0; v < K; ++v) pend[v] = xr[v];
+          pendCoef = coef;
+        } else {
+          dh_gather_fma<K>(acc, xbase, idx, coef);
+        }
       }
     }
+  }
+  if constexpr (PIPE) {
 #pragma unroll
-    for (int q = 0; q < G; ++q) dh_gather_fma<K>(acc, xbase, idx[q], coef[q]);
+    for (int v = 0; v < K; ++v) acc[v] = fma(pendCoef, pend[v], acc[v]);
   }
 }

@@ -81,9 +81,10 @@ extern "C" int dhh_sectors_launch(int lanesPerRow, int blockSize, int rowMode) {
     return DH_OK;
   }
   if ((lanesPerRow != 1 && lanesPerRow != 2 && lanesPerRow != 4 && lanesPerRow != 8) || blockSize < 64 ||
-      blockSize > 1024 || blockSize % 64 || (rowMode != DH_INDEX_RANK && rowMode != DH_INDEX_RANK_FREE)) {
+      blockSize > 1024 || blockSize % 64 ||
+      (rowMode != DH_INDEX_RANK && rowMode != DH_INDEX_RANK_FREE && rowMode != DH_INDEX_RANK_RANGE)) {
     g_error = "launch shape: lanes per row in {1, 2, 4, 8}, workgroup a multiple of 64 up to 1024, "
-              "row mode DH_INDEX_RANK or DH_INDEX_RANK_FREE";
+              "row mode DH_INDEX_RANK, DH_INDEX_RANK_FREE or DH_INDEX_RANK_RANGE";
     return DH_E_ARG;
   }
   g_sectorLaunch.lanesPerRow = lanesPerRow;
@@ -344,7 +345,12 @@ extern "C" int dhh_autotune(dhh_problem* p, const int* ks, int nbrKs, int repeat
       arch->Check(dh_fill_random(c, x, dim * k, 11), "random");
       double bestMs = 1e300;
       int cfg[4] = {1, 256, DH_INDEX_RANK, 1};
-      for (int mode : {DH_INDEX_RANK, DH_INDEX_RANK_X4, DH_INDEX_BSEARCH})
+      // both two-body row routines of the rank index explicitly (DH_INDEX_RANK picks one of them by
+      // k); spin chains have one row routine per index
+      const std::vector<int> modes = spin ? std::vector<int>{DH_INDEX_RANK, DH_INDEX_BSEARCH}
+                                          : std::vector<int>{DH_INDEX_RANK_FREE, DH_INDEX_RANK_RANGE,
+                                                             DH_INDEX_RANK_X4, DH_INDEX_BSEARCH};
+      for (int mode : modes)
         for (int lpr : {1, 2, 4, 8}) {
           if (spin && lpr != 1) continue;
           for (int bs : {128, 256, 512, 1024}) {
@@ -474,7 +480,7 @@ extern "C" int dhh_sectors_autotune(dhh_arch* a, int n, int lzMax, const int* to
     SectorSolveResources set(arch);
     arch->Check(dh_sector_set_create(c, &set.set, n, lzMax, lz.data(), (int)lz.size()), "calibration set");
     arch->Check(dh_sector_set_factors(set.set, F.data(), P.data(), (int)F.size(), 0.0), "factors");
-    const int lprs[] = {1, 2, 4}, blocks[] = {256, 1024}, modes[] = {DH_INDEX_RANK, DH_INDEX_RANK_FREE};
+    const int lprs[] = {1, 2, 4}, blocks[] = {256, 1024}, modes[] = {DH_INDEX_RANK_RANGE, DH_INDEX_RANK_FREE};
     double bestSeconds = 1e300;
     int bestIdx = -1;
     for (int i = 0; i < nbrKs; ++i) {

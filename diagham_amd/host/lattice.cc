@@ -233,15 +233,18 @@ void LatticeSectorsBlockLanczos::SolveWave(const std::vector<int>& sectors, uint
   std::vector<int64_t> off(na + 1);
   dh_sector_set_offsets(S, off.data());
   const int64_t total = off.back();
-  // small sets: four lanes per row for enough wavefronts; large ones: two lanes per row and
-  // 512-thread workgroups (N=10 on 6 x 5 sites, 30 sectors: k = 4 103 ms against 154 ms with one
-  // lane per row, k = 1 62 against 64 ms)
+  // small sets: four lanes per row for enough wavefronts; large ones: the fastest shapes of N=10
+  // on 6 x 5 sites, 30 sectors, measured on MI355X: k <= 2 two lanes per row and 512-thread
+  // workgroups (39.9 ms at k = 1), k > 2 one lane per row and 256-thread workgroups (84.5 ms at
+  // k = 4, against 90.4 ms with two lanes and 512 threads)
   if (Launch.lanesPerRow > 0)
     Arch->Check(dh_sector_set_launch(S, Launch.lanesPerRow, Launch.blockSize), "launch");
   else if (total < 4ll * 256 * 256)
     Arch->Check(dh_sector_set_launch(S, 4, 256), "launch");
-  else
+  else if (k <= 2)
     Arch->Check(dh_sector_set_launch(S, 2, 512), "launch");
+  else
+    Arch->Check(dh_sector_set_launch(S, 1, 256), "launch");
   std::vector<int64_t> dims(na);
   for (int i = 0; i < na; ++i) dims[i] = off[i + 1] - off[i];
   std::vector<int> blockCap;

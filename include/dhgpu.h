@@ -41,12 +41,17 @@ typedef struct dh_hamiltonian dh_hamiltonian;
 
 /* index lookup used by the H.v kernels */
 #define DH_INDEX_BSEARCH 0 /* top-bit LUT staged in LDS, then binary search of the sorted basis */
-#define DH_INDEX_RANK 1    /* combinatorial rank: two table lookups, no basis probes */
+#define DH_INDEX_RANK 1    /* combinatorial rank: two table lookups, no basis probes; on the sphere
+                              the two-body created pairs come from the free-pair masks for k <= 2
+                              and from the range loop for k > 2, the faster of the two per k */
 #define DH_INDEX_RANK_X4 2 /* rank, with the two-body created-pair loop unrolled four-wide (sphere only;
                               other Hamiltonians treat it as DH_INDEX_RANK) */
 #define DH_INDEX_RANK_FREE 3 /* rank, with the created pairs taken from the mask of free pairs of each
-                                annihilated pair and their lookups batched per lane (sphere only;
-                                other Hamiltonians treat it as DH_INDEX_RANK) */
+                                annihilated pair for every k (sphere only; other Hamiltonians treat it
+                                as DH_INDEX_RANK) */
+#define DH_INDEX_RANK_RANGE 4 /* rank, with every created pair of the pair-sum range visited and the
+                                 occupied ones skipped, for every k (sphere only; other Hamiltonians
+                                 treat it as DH_INDEX_RANK) */
 
 /* ---------------------------------------------------------------- context
  * Replaces the per-operation thread fork/join of SMPArchitecture::SendJobs
@@ -137,7 +142,12 @@ int64_t dh_hamiltonian_dimension(const dh_hamiltonian* ham);
 
 /* Launch shape of the H.v kernels.  lanes_per_row in {1,2,4,8} (64/lanes_per_row destination
  * rows per wavefront; spin chains: 1); block_size a multiple of 64 up to 1024; index_mode
- * DH_INDEX_*; lds_tables != 0 stages the index tables in LDS when they fit. */
+ * DH_INDEX_*; lds_tables != 0 stages the index tables in LDS when they fit.  Until it is called, a
+ * sphere Hamiltonian picks its shape per product: on spaces of at least 256 rows per CU the
+ * fastest shapes measured on N=12 2S=33 Lz=0 (with the 16-bit low rank table in LDS: one lane per
+ * row, 1024-thread workgroups for k <= 2 and 512 for k > 2; with the rank tables in global memory:
+ * two lanes per row, 256 threads for k <= 2 and 1024 above); on smaller spaces four lanes per row,
+ * 512 threads. */
 int dh_hamiltonian_set_launch(dh_hamiltonian* ham, int lanes_per_row, int block_size, int index_mode, int lds_tables);
 
 /* ---------------------------------------------------------------- products
@@ -196,10 +206,11 @@ int dh_sector_set_get_states(dh_sector_set* set, uint64_t* states);
 int dh_sector_set_factors(dh_sector_set* set, const double* factors, const int* pair_offset, int nbr_factors,
                           double shift);
 int dh_sector_set_launch(dh_sector_set* set, int lanes_per_row, int block_size);
-/* Row routine of the products of a sphere or momentum set: DH_INDEX_RANK (every created pair of
- * the range or list, skipping occupied ones) or DH_INDEX_RANK_FREE (only the free created pairs, from
- * one bit mask per annihilated pair, lookups batched per lane).  A tuning knob like
- * dh_sector_set_launch; the results agree to rounding. */
+/* Row routine of the products of a sphere or momentum set: DH_INDEX_RANK_RANGE (every created pair
+ * of the range or list, skipping occupied ones), DH_INDEX_RANK_FREE (only the free created pairs,
+ * from one bit mask per annihilated pair) or DH_INDEX_RANK (sphere sets: free pairs for k <= 2, the
+ * range for k > 2; momentum sets: the list).  A tuning knob like dh_sector_set_launch; the results
+ * agree to rounding. */
 int dh_sector_set_row_mode(dh_sector_set* set, int mode);
 /* Launch shape of the segmented vector operations of a set (no counterpart in the reference, a
  * tuning knob like dh_sector_set_launch): persistent axpy / rmul workgroups per CU (default 16) and

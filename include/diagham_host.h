@@ -106,8 +106,8 @@ int dhh_autotune(dhh_problem* problem, const int* ks, int nbr_ks, int repeats, d
  * wall seconds. */
 /* Launch shape of the products of the batched sector solves that this thread runs next
  * (dh_sector_set_launch: lanes per row in {1, 2, 4, 8} and workgroup size; dh_sector_set_row_mode:
- * DH_INDEX_RANK or DH_INDEX_RANK_FREE, sphere sets only); lanes_per_row = 0 restores the solvers'
- * built-in choice.  No counterpart in the reference. */
+ * DH_INDEX_RANK, DH_INDEX_RANK_FREE or DH_INDEX_RANK_RANGE, sphere sets only); lanes_per_row = 0
+ * restores the solvers' built-in choice.  No counterpart in the reference. */
 int dhh_sectors_launch(int lanes_per_row, int block_size, int row_mode);
 
 /* Autotuner of the batched sphere sector solve (no counterpart in the reference).  Calibration

@@ -66,7 +66,8 @@ class MomentumSet:
         self.capi.gpu().dh_sector_set_destroy(self.h)
 
 
-@pytest.mark.parametrize("lpr,row_mode,block", [(1, 1, 256), (2, 1, 512), (4, 1, 256), (1, 3, 256), (2, 3, 512)])
+@pytest.mark.parametrize("lpr,row_mode,block", [(1, 1, 256), (2, 1, 512), (4, 1, 256), (1, 3, 256), (2, 3, 512),
+                                                (2, 1, 1024), (1, 4, 256)])
 def test_momentum_basis_and_complex_products(arch, lpr, row_mode, block):
     n, nx, ny = 4, 4, 3
     sectors = [(0, 0), (1, 2), (3, 1)]

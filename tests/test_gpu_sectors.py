@@ -51,7 +51,7 @@ class SectorSet:
         self.capi.gpu().dh_sector_set_destroy(self.h)
 
 
-@pytest.mark.parametrize("lpr,row_mode", [(1, 1), (4, 1), (1, 3), (2, 3)])
+@pytest.mark.parametrize("lpr,row_mode", [(1, 1), (4, 1), (1, 3), (2, 3), (1, 4), (2, 4)])
 def test_sector_set_basis_and_products(arch, lpr, row_mode):
     V = coulomb(LZ)
     S = SectorSet(arch, N, LZ, SECTORS, V)
@@ -80,7 +80,7 @@ def test_sector_set_basis_and_products(arch, lpr, row_mode):
     S.close()
 
 
-@pytest.mark.parametrize("k,row_mode", [(1, 1), (2, 1), (1, 3), (4, 3)])
+@pytest.mark.parametrize("k,row_mode", [(1, 1), (2, 1), (8, 1), (1, 3), (4, 3), (1, 4), (2, 4)])
 def test_sector_products_beyond_32_orbitals(arch, k, row_mode):
     """36 orbitals: one- and two-vector products keep 64-bit state words (the 32-bit form is for
     at most 32 orbitals) and match the oracle in every sector."""
@@ -234,7 +234,7 @@ def test_sector_autotune_picks_a_shape_that_keeps_the_eigenvalues(arch):
     assert [r["k"] for r in rows] == [1.0, 4.0]
     for r in rows:
         assert r["lanes_per_row"] in (1, 2, 4) and r["block_size"] in (256, 1024)
-        assert r["row_mode"] in (capi.INDEX_RANK, capi.INDEX_RANK_FREE)
+        assert r["row_mode"] in (capi.INDEX_RANK, capi.INDEX_RANK_FREE, capi.INDEX_RANK_RANGE)
         assert r["product_ms"] > 0 and r["calibration_seconds"] > 0 and r["calibration_products"] > 0
     assert best["k"] in (1, 4)
     ref = capi.sectors_solve(arch, n, lz, lzs, V, 4, best["k"])

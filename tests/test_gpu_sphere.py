@@ -70,7 +70,8 @@ def test_element_count(n8_2s21, golden):
 
 
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 4), (0, 1), (0, 2), (2, 1), (2, 8), (3, 1), (3, 2), (3, 8)])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 2), (1, 4), (0, 1), (0, 2), (2, 1), (2, 8), (3, 1), (3, 2), (3, 8),
+                                      (4, 1), (4, 2)])
 def test_hv_parity(n8_2s21, k, mode, lpr):
     """H.X (k vectors, [row][k] layout) vs the oracle, 1e-12 relative norm per vector."""
     ref_space = O.FermionSphere(8, 0, 21)
@@ -86,7 +87,7 @@ def test_hv_parity(n8_2s21, k, mode, lpr):
 
 
 @pytest.mark.parametrize("k", [1, 2, 4])
-@pytest.mark.parametrize("mode,lpr", [(1, 1), (2, 2), (0, 1), (3, 1), (3, 4)])
+@pytest.mark.parametrize("mode,lpr", [(1, 1), (1, 2), (2, 2), (0, 1), (3, 1), (3, 4), (4, 1)])
 def test_hv_parity_beyond_32_orbitals(arch, k, mode, lpr):
     """36 orbitals: the rank kernels keep 64-bit state words for every k (the 32-bit form is for
     at most 32 orbitals)."""
@@ -162,7 +163,7 @@ def test_autotune_keeps_parity(n8_2s21):
     assert best in (1, 4, 8)
     for r in rows:
         assert r["lanes_per_row"] in (1, 2, 4, 8) and r["block_size"] in (128, 256, 512, 1024)
-        assert r["index_mode"] in (0, 1, 2) and r["ms"] > 0
+        assert r["index_mode"] in (0, 1, 2, 3, 4) and r["ms"] > 0
         assert r["elements_per_s"] > 0
     rates = {int(r["k"]): r["elements_per_s"] for r in rows}
     assert rates[best] == max(rates.values())
@@ -350,7 +351,7 @@ def test_free_pair_rows_are_bitwise_the_range_rows(n8_2s21, k, lpr):
     """The free-pair iterator visits the terms of each row in the order of the range loop, so with
     the same lanes per row the products agree bit for bit."""
     X = np.stack([O.splitmix(700 + v, n8_2s21.dim) for v in range(k)], axis=1).reshape(-1)
-    n8_2s21.set_launch(lanes_per_row=lpr, block_size=256, index_mode=1)
+    n8_2s21.set_launch(lanes_per_row=lpr, block_size=256, index_mode=4)
     Y1 = n8_2s21.hv(X, k)
     n8_2s21.set_launch(lanes_per_row=lpr, block_size=256, index_mode=3)
     Y3 = n8_2s21.hv(X, k)

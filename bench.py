@@ -35,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "H·v matrix elements/s + Lanczos time-to-10-eigvals, block vs plain, %HBM peak"
+# the all-sector legs of N=12 2S=33 (about 15 minutes), recorded by `bench.py --n12 ...` runs
+N12_RECORD = "sectors_n12_2s33_time_to_eigenvalues.json"
 HBM_PEAK = 8.0e12
 
 
@@ -151,7 +153,7 @@ def lanczos_runs(capi, arch, report=None):
 
 def sector_runs(capi, arch, report=None, n=10, lz=27, nev=10, legs=(("plain_sequential_k1", 1, False),
                                                                      ("batched_k1", 1, True),
-                                                                     ("batched_block_k8", 8, True))):
+                                                                     ("batched_block_k8", 8, True)), dump=None):
     """Time to the nev lowest eigenvalues of every Lz >= 0 sector of n Coulomb fermions at flux lz:
     the plain one-vector Lanczos run sector after sector (the reference's loop,
     FQHESphereFermionsTwoBodyGeneric.cc:270) against the batched Lanczos that advances all sectors
@@ -168,11 +170,16 @@ def sector_runs(capi, arch, report=None, n=10, lz=27, nev=10, legs=(("plain_sequ
         runs[name] = r
         if report:
             report("sectors N=%d 2S=%d, %s: %.2f s, %d waves" % (n, lz, name, r["seconds"], r["waves"]))
+    if dump:  # every leg's eigenvalues, to compare legs run in separate processes
+        with open(dump, "w") as f:
+            json.dump({name: r["eigenvalues"].tolist() for name, r in runs.items()}, f)
     return sector_summary("fermions_sphere_n%d_2s%d_coulomb_all_Lz_ge_0_sectors_%d_eigenvalues" % (n, lz, nev),
                           runs, legs[0][0], len(lzs), dims)
 
 
 def sector_summary(workload, runs, plain, nbr_sectors, dims):
+    """Per leg: time, products, converged sectors, waves, device memory peak, and the largest
+    eigenvalue difference from the first leg (the plain run when it is among the legs)."""
     ref = runs[plain]["eigenvalues"]
     out = {"workload": workload, "sectors": nbr_sectors, "states": int(sum(dims)), "largest_sector": int(max(dims))}
     for name, r in runs.items():
@@ -181,9 +188,10 @@ def sector_summary(workload, runs, plain, nbr_sectors, dims):
                      "device_memory_peak_bytes": r.get("memory_peak"),
                      "max_abs_eigenvalue_difference_vs_plain": float(np.max(np.abs(r["eigenvalues"] - ref)))}
     t0 = runs[plain]["seconds"]
-    for name in runs:
-        if name != plain:
-            out[name]["speedup_vs_plain"] = t0 / runs[name]["seconds"]
+    if plain.startswith("plain"):
+        for name in runs:
+            if name != plain:
+                out[name]["speedup_vs_plain"] = t0 / runs[name]["seconds"]
     return out
 
 
@@ -226,7 +234,9 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
     pp = capi.coulomb_pseudopotentials(lz, True)
     lzs = capi.sphere_sectors(n, lz)
     S = capi.SphereSectorSet(arch, n, lz, lzs, pp)
-    S.set_launch(2, 1024)  # two lanes per row, 1024-thread workgroups: the fastest measured shape
+    # two lanes per row, 1024-thread workgroups: the sector solver's own shape for k >= 4, the fastest
+    # of the committed sweep (profiles/launch_shape_sweeps.md)
+    S.set_launch(2, 1024)
     nnz = int(S.count_elements().sum())
     x = capi.DeviceBuffer(arch, S.total * k)
     y = capi.DeviceBuffer(arch, S.total * k)
@@ -246,7 +256,9 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
     achieved = alg / (ms * 1e-3) / 1e9
     traffic = load_traffic("traffic_sectors_hv_n%d_2s%d_k%d.json" % (n, lz, k))
     return {"workload": "fermions_sphere_n%d_2s%d_all_Lz_ge_0_sectors_one_product_k%d" % (n, lz, k),
-            "kernel": "sectors_hv_kernel<%d, 2, LO_LDS, false>" % k, "sectors": len(lzs), "rows": rows,
+            "kernel": "sectors_hv_kernel<%d, 2, true, false, false>" % k, "sectors": len(lzs), "rows": rows,
+            "launch": {"lanes_per_row": 2, "block_size": 1024, "row_routine": "range", "low_rank_table": "LDS, 16-bit",
+                       "chosen_by": "profiles/launch_shape_sweeps.md"},
             "matrix_elements": nnz, "ms": ms, "element_vector_products_per_s": nnz * k / (ms * 1e-3),
             "algorithmic_bytes": alg, "achieved_GBps": achieved, "frac_of_peak": achieved / (HBM_PEAK / 1e9),
             "traffic": traffic and traffic["bytes_per_launch"],
@@ -256,7 +268,9 @@ def batched_hv_roofline(capi, arch, stream, n=12, lz=33, k=8, reps=3):
 def adjoint_leg(capi, arch, stream, n=10, lz=27, reps=10):
     """Gradient of the ground-state energy of N = 10 fermions at 2S = 27, Lz = 0 with respect to every
     pseudopotential: dh_sphere_adjoint forms all channels' bilinears <psi|dH/dV_m|psi> in one pass
-    over the rows, timed beside one product of the same Hamiltonian (both on the bench stream)."""
+    over the (N-2)-particle states (the channel interaction is of rank one in the pairs, so the
+    bilinear factorises: one gather per row and annihilated pair), timed beside one product of the
+    same Hamiltonian (both on the bench stream)."""
     pp = capi.coulomb_pseudopotentials(lz, True)
     p = capi.Problem.sphere(arch, n, 0, lz, pp)
     psi = capi.DeviceBuffer(arch, p.dim)
@@ -278,9 +292,11 @@ def adjoint_leg(capi, arch, stream, n=10, lz=27, reps=10):
     psi.free()
     y.free()
     p.close()
-    alg = p.dim * 16 + nnz * 8  # state word and psi[row] per row, one phi gather per element
+    # one gather of psi per (row, annihilated pair): C(N, 2) per row (psi and phi are the same vector)
+    alg = p.dim * (n * (n - 1) // 2) * 8
     return {"workload": "fermions_sphere_n%d_2s%d_lz0_ground_state_dE_dVm" % (n, lz), "dimension": p.dim,
-            "channels": len(g), "e0": float(r["eigenvalues"][0]), "dE0_dV1": float(g[0]), "adjoint_ms": ms_adj,
+            "channels": len(g), "matrix_elements": nnz, "e0": float(r["eigenvalues"][0]), "dE0_dV1": float(g[0]),
+            "adjoint_ms": ms_adj,
             "product_ms": ms_hv, "adjoint_over_product": ms_adj / ms_hv,
             "algorithmic_bytes": alg, "achieved_GBps": alg / (ms_adj * 1e-3) / 1e9,
             "frac_of_peak": alg / (ms_adj * 1e-3) / HBM_PEAK}
@@ -487,8 +503,12 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-n10", action="store_true", help="skip the all-sector legs of N=10 at 2S=27")
-    ap.add_argument("--skip-n12", action="store_true", help="skip the all-sector legs of N=12 at 2S=33")
+    ap.add_argument("--n12", default="",
+                    help="run the all-sector legs of N=12 at 2S=33 (comma-separated: plain_sequential_k1, "
+                         "batched_k1, batched_block_k8, or 'all'); they take about 15 minutes together, so by "
+                         "default the line carries their recorded run from profiles/")
     ap.add_argument("--skip-lattice", action="store_true", help="skip the all-sector legs of N=10 on 6x5 sites")
+    ap.add_argument("--n12-dump", default=None, help="file for every N=12 leg's eigenvalues (JSON)")
     ap.add_argument("--length", type=int, default=28)
     ap.add_argument("--kernel", default=None,
                     help="only repeat one product (sphere_n12_2s33_lz0_k1, sectors_n12_2s33_k8, "
@@ -574,10 +594,16 @@ def main():
     progress("binary-search product: %.3f ms" % bs["ms_per_step"])
     if not args.skip_lanczos:
         line["lanczos"] = lanczos_runs(capi, arch, progress)
-        if not args.skip_n12:
-            line["lanczos"]["sectors_n12_2s33"] = sector_runs(
-                capi, arch, progress, n=12, lz=33, nev=10,
-                legs=(("plain_sequential_k1", 1, False), ("batched_k1", 1, True), ("batched_block_k8", 8, True)))
+        n12_legs = (("plain_sequential_k1", 1, False), ("batched_k1", 1, True), ("batched_block_k8", 8, True))
+        want = [leg for leg in n12_legs if args.n12 == "all" or leg[0] in args.n12.split(",")]
+        if want:
+            line["lanczos"]["sectors_n12_2s33"] = sector_runs(capi, arch, progress, n=12, lz=33, nev=10,
+                                                              legs=tuple(want), dump=args.n12_dump)
+        else:
+            recorded = load_traffic(N12_RECORD)
+            if recorded:
+                line["lanczos"]["sectors_n12_2s33"] = dict(recorded, recorded_not_timed_in_this_run=True,
+                                                           source="profiles/" + N12_RECORD)
         if not args.skip_lattice:
             line["lanczos"]["lattice_6x5_n10"] = lattice_runs(capi, arch, progress)
         if not args.skip_n10:

@@ -4,6 +4,7 @@
 
 #include "dh_index.h"
 #include "dh_internal.h"
+#include "dh_sphere_row.h"
 
 // ----------------------------------------------------------------------------- FindStateIndex
 __global__ void fs_find_kernel(const uint64_t* __restrict__ in, int64_t n, int64_t* __restrict__ out,
@@ -344,82 +345,111 @@ extern "C" int dh_hamiltonian_row_elements(dh_hamiltonian* h, int64_t row, int64
 }
 
 // ----------------------------------------------------------------------------- adjoint
-// With the interaction split by channel (F(a->b) = sum_c V_c (-4) cg[a][c] cg[b][c]):
-//   g[c] = sum_r psi[r] (H_c phi)[r],   H_c phi gathered like the product,
-// accumulated in registers and reduced per block; optionally hpsi = H psi in the same pass (the
-// vector-Jacobian product of x -> H x).  LPR lanes share a row and split its created pairs, as
-// in the product: a row's terms are a few hundred dependent gathers, so one lane per row leaves
-// too few loads in flight on spaces of a few hundred thousand rows.  g is linear in every term,
-// so each lane adds psi[r] times its own part; only hpsi needs the row's lanes summed.
-template <int NC, int LPR>
-__global__ void __launch_bounds__(256) sphere_adjoint_kernel(const uint64_t* __restrict__ states, int64_t dim,
-                                                             int nbrOrbitals, int nbrPairs, int nbrChannels,
-                                                             const double* __restrict__ cg,
-                                                             const double* __restrict__ factors,
-                                                             const int* __restrict__ pairOffset,
-                                                             const uint32_t* __restrict__ lo,
+// With the interaction split by channel, F_c(a -> b) = -4 u_c[a] u_c[b] (u_c[p] the Clebsch-Gordan
+// coefficient of pair p in channel c) is of rank one in the pairs, so the bilinear factorises over
+// the (N-2)-particle states t that a pair annihilation leaves:
+//   g[c] = <psi|H_c|phi> = 4 sum_t A_c(psi)[t] A_c(phi)[t],
+//   A_c(v)[t] = sum_{pairs p free in t, sum(p) = sum(s) - sum(t)} u_c[p] (-1)^(popc(t >> p_lo) + popc(t >> p_hi)) v[t | p]
+// (the annihilation sign of a pair is its creation sign plus one, and the two cancel).  Each (row,
+// annihilated pair, created pair) term of the product becomes one (t, pair) term here, so a
+// bilinear costs C(N, 2) gathers per row instead of one per matrix element.  One lane per t, the
+// t unranked from its index in the combinatorial number system over all (N-2)-subsets of the
+// orbitals; the index loads, then the gathers, of four pairs are in flight together.  SAME: psi and
+// phi are the same vector (an expectation value), one sum per channel.  Channels [c0, c0 + NC).
+template <int NC, bool SAME>
+__global__ void __launch_bounds__(256) sphere_adjoint_kernel(int nbrOrbitals, int holes, int64_t nbrSubsets,
+                                                             int orbitalSum, const int64_t* __restrict__ binom,
+                                                             int nbrChannels, int c0, const double* __restrict__ cg,
+                                                             int nbrPairs, const uint32_t* __restrict__ lo,
                                                              const uint32_t* __restrict__ hi, int rankBits,
-                                                             const double* __restrict__ psi,
-                                                             const double* __restrict__ phi, double shift,
-                                                             double* __restrict__ hpsi, double* __restrict__ partial) {
-  extern __shared__ double sCg[];  // [nbrPairs * nbrChannels]
+                                                             int64_t dim, const double* __restrict__ psi,
+                                                             const double* __restrict__ phi,
+                                                             double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  double* sU = smem;                                                 // [nbrPairs][NC]
+  int64_t* sB = reinterpret_cast<int64_t*>(smem + nbrPairs * NC);    // [nbrOrbitals + 1][holes + 1]
   __shared__ double red[256 / 64][NC];
-  for (int i = threadIdx.x; i < nbrPairs * nbrChannels; i += blockDim.x) sCg[i] = cg[i];
+  const int nc = min(NC, nbrChannels - c0);
+  for (int i = threadIdx.x; i < nbrPairs * NC; i += blockDim.x) {
+    const int p = i / NC, c = i - p * NC;
+    sU[i] = c < nc ? cg[(int64_t)p * nbrChannels + c0 + c] : 0.0;
+  }
+  for (int i = threadIdx.x; i < (nbrOrbitals + 1) * (holes + 1); i += blockDim.x) sB[i] = binom[i];
   __syncthreads();
   double g[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) g[c] = 0.0;
-  const int lane = threadIdx.x % LPR;
-  const int64_t rowsPerTile = blockDim.x / LPR;
-  const int64_t nbrTiles = (dim + rowsPerTile - 1) / rowsPerTile;
-  for (int64_t tile = blockIdx.x; tile < nbrTiles; tile += gridDim.x) {
-    const int64_t row = tile * rowsPerTile + threadIdx.x / LPR;
-    const bool active = row < dim;
-    const uint64_t s = active ? states[row] : 0ull;  // an empty state has no terms
-    const double pr = active ? -4.0 * psi[row] : 0.0;
-    double hp = 0.0;
-    for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
-      const int n1 = __builtin_ctzll(r1);
-      const int pc1 = __popcll(s >> n1);
-      for (uint64_t r2 = s & ((1ull << n1) - 1ull); r2; r2 &= r2 - 1) {
-        const int n2 = __builtin_ctzll(r2);
-        const int sgnA = pc1 + __popcll(s >> n2);
-        const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
-        const int sum = n1 + n2;
-        const int m3lo = (sum >> 1) + 1;
-        const int m3hi = min(sum, nbrOrbitals - 1);
-        const int pa = (n1 * (n1 - 1)) / 2 + n2;
-        const int fbase = pairOffset[pa] - m3lo;
-        double w[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) w[c] = 0.0;
-        for (int m3 = m3lo + lane; m3 <= m3hi; m3 += LPR) {
-          const int m4 = sum - m3;
-          const uint64_t cm = (1ull << m3) | (1ull << m4);
-          if (t & cm) continue;
-          const int sg = sgnA + __popcll(t >> m4) + __popcll(t >> m3);
-          const int64_t idx = dh_find_rank(t | cm, lo, hi, rankBits, dim);
-          double p = phi[idx];
-          if (sg & 1) p = -p;
-          const double* cb = sCg + ((m3 * (m3 - 1)) / 2 + m4) * nbrChannels;
-#pragma unroll
-          for (int c = 0; c < NC; ++c)
-            if (c < nbrChannels) w[c] += cb[c] * p;
-          if (hpsi) {
-            const double q = psi[idx];
-            hp += ((sg & 1) ? -factors[fbase + m3] : factors[fbase + m3]) * q;
-          }
-        }
-        const double* ca = sCg + pa * nbrChannels;
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          if (c < nbrChannels) g[c] += pr * ca[c] * w[c];
+  const int maxSum = 2 * nbrOrbitals - 3;
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < nbrSubsets;
+       it += (int64_t)gridDim.x * blockDim.x) {
+    // the it-th (N-2)-subset: it = sum_k C(c_k, k) over its orbitals c_1 < ... < c_m
+    uint64_t t = 0;
+    int64_t r = it;
+    int m = holes, sumT = 0;
+    for (int j = nbrOrbitals - 1; j >= 0 && m > 0; --j) {
+      const int64_t b = sB[j * (holes + 1) + m];
+      if (r >= b) {
+        t |= 1ull << j;
+        sumT += j;
+        r -= b;
+        --m;
       }
     }
-    if (hpsi) {
-      if constexpr (LPR > 1)
-        for (int o = LPR / 2; o > 0; o >>= 1) hp += __shfl_xor(hp, o, LPR);
-      if (active && lane == 0) hpsi[row] = hp + shift * psi[row];
+    const int S = orbitalSum - sumT;  // the pair sum that completes t to the sector
+    if (S < 1 || S > maxSum) continue;
+    uint64_t pm = dh_free_pairs(t, nbrOrbitals, S, (S >> 1) + 1, min(S, nbrOrbitals - 1));
+    if (!pm) continue;
+    double a[NC], bsum[SAME ? 1 : NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = 0.0;
+    if constexpr (!SAME) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) bsum[c] = 0.0;
+    }
+    while (pm) {
+      int64_t idx[4];
+      int pb[4];
+      bool neg[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        idx[q] = 0;  // a slot past the last pair reads row 0 with no weight
+        pb[q] = -1;
+        neg[q] = false;
+        if (pm) {
+          const int m3 = __builtin_ctzll(pm);
+          pm &= pm - 1;
+          const int m4 = S - m3;
+          idx[q] = dh_find_rank(t | (1ull << m3) | (1ull << m4), lo, hi, rankBits, dim);
+          pb[q] = (m3 * (m3 - 1)) / 2 + m4;
+          neg[q] = (__popcll(t >> m4) + __popcll(t >> m3)) & 1;
+        }
+      }
+      double x[4], z[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        x[q] = psi[idx[q]];
+        if constexpr (!SAME) z[q] = phi[idx[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (pb[q] < 0) continue;
+        const double* u = sU + (int64_t)pb[q] * NC;
+        const double xs = neg[q] ? -x[q] : x[q];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] += u[c] * xs;
+        if constexpr (!SAME) {
+          const double zs = neg[q] ? -z[q] : z[q];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) bsum[c] += u[c] * zs;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if constexpr (SAME)
+        g[c] += a[c] * a[c];
+      else
+        g[c] += a[c] * bsum[c];
     }
   }
   const int wl = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -433,7 +463,7 @@ __global__ void __launch_bounds__(256) sphere_adjoint_kernel(const uint64_t* __r
   if (threadIdx.x < NC) {
     double v = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
-    partial[(int64_t)blockIdx.x * NC + threadIdx.x] = v;
+    partial[(int64_t)blockIdx.x * NC + threadIdx.x] = 4.0 * v;
   }
 }
 
@@ -446,31 +476,58 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
   out[c] = s;
 }
 
-template <int NC>
-static int adjoint_launch(dh_hamiltonian* h, const double* psi, const double* phi, double* g, double* hpsi) {
+// C(j, m) for j <= nbrOrbitals, m <= holes, exact in 64 bits (Pascal's rule), on the device once per
+// Hamiltonian; returns the number of (N-2)-subsets of the orbitals, or -1 when it exceeds 2^40
+static int64_t adjoint_binomials(dh_hamiltonian* h, int holes, int* rc) {
+  dh_context* c = h->ctx;
+  const int n = h->basis->nbrOrbitals;
+  std::vector<int64_t> B((size_t)(n + 1) * (holes + 1), 0);
+  for (int j = 0; j <= n; ++j) {
+    B[(size_t)j * (holes + 1)] = 1;
+    for (int m = 1; m <= holes && j > 0; ++m) {
+      const int64_t v = B[(size_t)(j - 1) * (holes + 1) + m - 1] + B[(size_t)(j - 1) * (holes + 1) + m];
+      B[(size_t)j * (holes + 1) + m] = std::min<int64_t>(v, (int64_t)1 << 62);
+    }
+  }
+  const int64_t total = B[(size_t)n * (holes + 1) + holes];
+  *rc = DH_OK;
+  if (total > ((int64_t)1 << 40)) return -1;
+  if (!h->d_adjBinom) {
+    if ((*rc = dh_alloc(c, (void**)&h->d_adjBinom, sizeof(int64_t) * B.size()))) return -1;
+    hipError_t e = hipMemcpyAsync(h->d_adjBinom, B.data(), sizeof(int64_t) * B.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      *rc = dh_fail(c, DH_E_HIP, hipGetErrorString(e));
+      return -1;
+    }
+  }
+  return total;
+}
+
+template <int NC, bool SAME>
+static int adjoint_pass(dh_hamiltonian* h, int holes, int64_t subsets, int c0, const double* psi, const double* phi,
+                        double* g) {
   dh_context* c = h->ctx;
   dh_basis* b = h->basis;
-  size_t lds = sizeof(double) * h->nbrPairs * h->nbrChannels;
+  size_t lds = sizeof(double) * h->nbrPairs * NC + sizeof(int64_t) * (b->nbrOrbitals + 1) * (holes + 1);
   if (lds > 150 * 1024) return dh_fail(c, DH_E_RANGE, "channel table exceeds LDS");
-  // four lanes per row: a row's created pairs split four ways (see the kernel)
-  auto kern = sphere_adjoint_kernel<NC, 4>;
+  auto kern = sphere_adjoint_kernel<NC, SAME>;
   if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int perCU = 1;
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 256, lds);
-  int64_t tiles = (b->dim + 63) / 64;  // 64 rows per 256-thread workgroup
-  int nb = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * std::max(perCU, 1)));
+  int nb = (int)std::max<int64_t>(1, std::min<int64_t>((subsets + 255) / 256, (int64_t)c->nbrCU * std::max(perCU, 1)));
   double* scratch;
   int rc = dh_scratch(c, sizeof(double) * ((size_t)nb * NC + NC), &scratch);
   if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, c->stream, b->d_states, b->dim, b->nbrOrbitals, h->nbrPairs,
-                     h->nbrChannels, h->d_cg, h->d_factors, h->d_pairOffset, b->d_rankLo, b->d_rankHi, b->rankBits,
-                     psi, phi, h->shift, hpsi, scratch);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, c->stream, b->nbrOrbitals, holes, subsets, b->orbitalSum,
+                     (const int64_t*)h->d_adjBinom, h->nbrChannels, c0, h->d_cg, h->nbrPairs, b->d_rankLo,
+                     b->d_rankHi, b->rankBits, b->dim, psi, phi, scratch);
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(64), 0, c->stream, scratch, nb, NC, scratch + (size_t)nb * NC);
   DH_HIP(c, hipGetLastError());
   double out[NC];
   DH_HIP(c, hipMemcpyAsync(out, scratch + (size_t)nb * NC, sizeof(double) * NC, hipMemcpyDeviceToHost, c->stream));
   DH_HIP(c, hipStreamSynchronize(c->stream));
-  for (int i = 0; i < h->nbrChannels; ++i) g[i] = out[i];
+  for (int i = 0; i < NC && c0 + i < h->nbrChannels; ++i) g[c0 + i] = out[i];
   return DH_OK;
 }
 
@@ -478,9 +535,34 @@ extern "C" int dh_sphere_adjoint(dh_hamiltonian* h, const double* psi, const dou
   if (!h || !psi || !phi || !g) return DH_E_ARG;
   if (h->kind != DH_HAM_SPHERE) return dh_fail(h->ctx, DH_E_ARG, "not a sphere Hamiltonian");
   if (!h->d_cg || h->nbrChannels <= 0) return dh_fail(h->ctx, DH_E_STATE, "channels not set");
-  if (h->nbrChannels <= 8) return adjoint_launch<8>(h, psi, phi, g, hpsi);
-  if (h->nbrChannels <= 16) return adjoint_launch<16>(h, psi, phi, g, hpsi);
-  return adjoint_launch<32>(h, psi, phi, g, hpsi);
+  // the vector-Jacobian product H psi is one product of the same Hamiltonian (with its shift)
+  if (hpsi) {
+    int rc = dh_hv(h, psi, hpsi, 1, 0, 0, -1);
+    if (rc) return rc;
+  }
+  const int holes = h->basis->nbrFermions - 2;
+  if (holes < 0) {  // fewer than two particles: no two-body term
+    DH_HIP(h->ctx, hipStreamSynchronize(h->ctx->stream));
+    for (int i = 0; i < h->nbrChannels; ++i) g[i] = 0.0;
+    return DH_OK;
+  }
+  int rc;
+  const int64_t subsets = adjoint_binomials(h, holes, &rc);
+  if (rc) return rc;
+  if (subsets < 0) return dh_fail(h->ctx, DH_E_RANGE, "more than 2^40 (N-2)-particle states");
+  // channels in groups of 16 (8 when they fit): three (or two) sums per channel stay in registers
+  const bool same = psi == phi;
+  for (int c0 = 0; c0 < h->nbrChannels; c0 += 16) {
+    const int left = h->nbrChannels - c0;
+    if (left <= 8)
+      rc = same ? adjoint_pass<8, true>(h, holes, subsets, c0, psi, phi, g)
+                : adjoint_pass<8, false>(h, holes, subsets, c0, psi, phi, g);
+    else
+      rc = same ? adjoint_pass<16, true>(h, holes, subsets, c0, psi, phi, g)
+                : adjoint_pass<16, false>(h, holes, subsets, c0, psi, phi, g);
+    if (rc) return rc;
+  }
+  return DH_OK;
 }
 
 // ----------------------------------------------------------------------------- stored elements

@@ -852,6 +852,7 @@ extern "C" int dh_hamiltonian_destroy(dh_hamiltonian* h) {
   if (h->d_packedFactors) hipFree(h->d_packedFactors);
   if (h->d_sumBase) hipFree(h->d_sumBase);
   if (h->d_cg) hipFree(h->d_cg);
+  if (h->d_adjBinom) hipFree(h->d_adjBinom);
   dh_nbody_free(h);
   dh_csr_free(h);
   delete h;

@@ -126,6 +126,7 @@ struct dh_hamiltonian {
   // adjoint channel split
   int nbrChannels = 0;
   double* d_cg = nullptr;  // [nbrPairs * nbrChannels]
+  int64_t* d_adjBinom = nullptr;  // adjoint: C(j, m), j <= nbrOrbitals, m <= nbrFermions - 2
   std::vector<double> channelWeights;
   // n-body term with explicit per-sum factor matrices: tuples as orbital bit masks grouped by
   // orbital sum, the position of each tuple in its sum (by combinatorial rank of the tuple)

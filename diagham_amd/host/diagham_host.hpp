@@ -487,11 +487,13 @@ void RunSectorLanczos(SegmentedBlockLanczos<T>& lanczos, const std::vector<int>&
 }
 
 // Krylov blocks a sector of `dim` rows is expected to need for nbrEigenvalue eigenvalues at block
-// size k (measured on MI355X runs of Coulomb sectors: about 25 vectors per wanted eigenvalue plus
-// 50 for one vector, each further vector of a block adding 30 %), within the sector's own space
-// and maxIter
+// size k, within the sector's own space and maxIter.  Measured on MI355X runs of Coulomb sectors
+// for 10 eigenvalues at k = 1: 229 vectors on average over N=10 2S=27, up to 475 for the largest
+// sectors of N=12 2S=33; the estimate covers the large ones, 40 vectors per wanted eigenvalue plus
+// 100, each further vector of a block adding 30 %.  An underestimate is costly: a sector whose
+// storage cannot grow in a full wave stops and runs again from the start in a later wave.
 inline int ExpectedKrylovBlocks(int64_t dim, int k, int nbrEigenvalue, int maxIter) {
-  const double vectors = (25.0 * nbrEigenvalue + 50.0) * (1.0 + 0.3 * (k - 1));
+  const double vectors = (40.0 * nbrEigenvalue + 100.0) * (1.0 + 0.3 * (k - 1));
   const int64_t blocks = (int64_t)std::ceil(vectors / k);
   return (int)std::max<int64_t>(3, std::min<int64_t>({blocks, (dim + k - 1) / k + 1, maxIter / k + 1}));
 }

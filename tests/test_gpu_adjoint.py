@@ -20,15 +20,20 @@ def channel_matrix_vector(space, lz_max, m, x):
     return O.fs_hv(space, O.Factors(lz_max, e), x)
 
 
-def test_bilinears_match_oracle(arch):
+@pytest.mark.parametrize("n,total_lz,lz,same", [(6, 0, 15, False), (6, 0, 15, True), (4, 2, 33, False),
+                                                 (5, 0, 20, True), (2, 0, 9, False)])
+def test_bilinears_match_oracle(arch, n, total_lz, lz, same):
+    """Every channel's <psi|H_m|phi> against the oracle's product with the unit pseudopotential e_m:
+    psi and phi distinct and equal (the expectation-value form), 8 to 17 channels (two groups of
+    channels at 2S = 33), and N = 2 (the (N-2)-particle state is empty)."""
     from diagham_amd import capi
-    n, lz = 6, 15
     V = capi.coulomb_pseudopotentials(lz, True)
-    p = capi.Problem.sphere(arch, n, 0, lz, V)
-    space = O.FermionSphere(n, 0, lz)
-    psi, phi = O.splitmix(31, p.dim), O.splitmix(32, p.dim)
+    p = capi.Problem.sphere(arch, n, total_lz, lz, V)
+    space = O.FermionSphere(n, total_lz, lz)
+    psi = O.splitmix(31, p.dim)
+    phi = psi if same else O.splitmix(32, p.dim)
     dpsi = capi.DeviceBuffer.from_host(arch, psi)
-    dphi = capi.DeviceBuffer.from_host(arch, phi)
+    dphi = dpsi if same else capi.DeviceBuffer.from_host(arch, phi)
     dh = capi.DeviceBuffer(arch, p.dim)
     g = p.adjoint_device(dpsi.ptr, dphi.ptr, dh.ptr)
     mom = p.channels()

@@ -47,6 +47,7 @@ GPU_SIGNATURES = {
     "dh_memset": (c_int, [c_void_p, c_void_p, c_int, c_size_t]),
     "dh_fermion_sphere_create": (c_int, [c_void_p, PP, c_int, c_int, c_int]),
     "dh_fermion_sphere_dimension": (c_int64, [c_int, c_int, c_int]),
+    "dh_fermion_sphere_from_states": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_U64, c_int64]),
     "dh_spin_chain_create": (c_int, [c_void_p, PP, c_int, c_int]),
     "dh_basis_destroy": (c_int, [c_void_p]),
     "dh_basis_dimension": (c_int64, [c_void_p]),

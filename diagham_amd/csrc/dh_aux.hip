@@ -467,13 +467,21 @@ __global__ void __launch_bounds__(256) sphere_adjoint_kernel(int nbrOrbitals, in
   }
 }
 
-__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nbrBlocks, int width,
-                                       double* __restrict__ out) {
-  int c = threadIdx.x;
-  if (c >= width) return;
+// out[c] = sum over the blocks of partial[b][c]: one 256-thread workgroup per column c, the threads
+// striding over the blocks (a single lane per column would wait for a thousand dependent loads)
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __restrict__ partial, int nbrBlocks,
+                                                              int width, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nbrBlocks; ++b) s += partial[(int64_t)b * width + c];
-  out[c] = s;
+  for (int b = threadIdx.x; b < nbrBlocks; b += blockDim.x) s += partial[(int64_t)b * width + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // C(j, m) for j <= nbrOrbitals, m <= holes, exact in 64 bits (Pascal's rule), on the device once per
@@ -522,7 +530,7 @@ static int adjoint_pass(dh_hamiltonian* h, int holes, int64_t subsets, int c0, c
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, c->stream, b->nbrOrbitals, holes, subsets, b->orbitalSum,
                      (const int64_t*)h->d_adjBinom, h->nbrChannels, c0, h->d_cg, h->nbrPairs, b->d_rankLo,
                      b->d_rankHi, b->rankBits, b->dim, psi, phi, scratch);
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(64), 0, c->stream, scratch, nb, NC, scratch + (size_t)nb * NC);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(NC), dim3(256), 0, c->stream, scratch, nb, NC, scratch + (size_t)nb * NC);
   DH_HIP(c, hipGetLastError());
   double out[NC];
   DH_HIP(c, hipMemcpyAsync(out, scratch + (size_t)nb * NC, sizeof(double) * NC, hipMemcpyDeviceToHost, c->stream));

@@ -323,6 +323,31 @@ __global__ void widen_words_kernel(const uint32_t* __restrict__ w, uint64_t* __r
   if (i < n) out[i] = (uint64_t)w[i];
 }
 
+extern "C" int dh_fermion_sphere_from_states(dh_context* c, dh_basis** out, int N, int totalLz, int lzMax,
+                                             const uint64_t* states, int64_t dim) {
+  if (!c || !out || !states) return DH_E_ARG;
+  *out = nullptr;
+  const int64_t full = dh_fermion_sphere_dimension(N, totalLz, lzMax);
+  if (dim != full)
+    return dh_fail(c, DH_E_ARG, "the states are not the full sector: " + std::to_string(dim) + " words for a sector of " +
+                                    std::to_string(full) + " states");
+  dh_basis* b = nullptr;
+  int rc = dh_fermion_sphere_create(c, &b, N, totalLz, lzMax);
+  if (rc) return rc;
+  std::vector<uint64_t> mine((size_t)dim);
+  if ((rc = dh_basis_get_states(b, mine.data()))) {
+    dh_basis_destroy(b);
+    return rc;
+  }
+  for (int64_t i = 0; i < dim; ++i)
+    if (mine[(size_t)i] != states[i]) {
+      dh_basis_destroy(b);
+      return dh_fail(c, DH_E_ARG, "state " + std::to_string(i) + " differs from the sector's state in the reference's order");
+    }
+  *out = b;
+  return DH_OK;
+}
+
 extern "C" int dh_basis_get_states(dh_basis* b, uint64_t* states) {
   if (!b || !states) return DH_E_ARG;
   dh_context* c = b->ctx;

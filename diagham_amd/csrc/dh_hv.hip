@@ -534,13 +534,17 @@ static int sphere_dispatch_lpr(dh_hamiltonian* h, const double* x, double* y, in
     // workgroups for k <= 2 (6.1 ms at k = 1, against 10.6 ms at 256 threads), 512 for k > 2.
     // With both tables in global memory: two lanes per row, 256 threads for k <= 2, 1024 above.
     if (sphere_rank_tables(h) == MODE_RANK_LO16) {
-      lpr = 1;
+      lpr = K == 2 ? 2 : 1;  // k = 2: two lanes per row (see below)
       bs = K <= 2 ? 1024 : 512;
     } else {
       lpr = 2;
       bs = K <= 2 ? 256 : 1024;
     }
   }
+  // k = 2 with one lane per row faulted the GPU twice on N=5 2S=33 (free-pair rows, with the low
+  // rank table in LDS and in global memory) with no cause found in the code; k = 2 runs with two
+  // lanes per row, as tested, until one is
+  if (K == 2 && lpr == 1) lpr = 2;
   switch (lpr) {
     case 1: return sphere_dispatch_mode<K, 1>(h, bs, x, y, add, first, count);
     case 2: return sphere_dispatch_mode<K, 2>(h, bs, x, y, add, first, count);

@@ -88,6 +88,13 @@ int dh_memset(dh_context* ctx, void* d_dst, int value, size_t bytes);
 int dh_fermion_sphere_create(dh_context* ctx, dh_basis** basis, int nbr_fermions, int total_lz, int lz_max);
 /* EvaluateHilbertSpaceDimension (FermionOnSphere.cc:1794); no device work. */
 int64_t dh_fermion_sphere_dimension(int nbr_fermions, int total_lz, int lz_max);
+/* A basis from a StateDescription that DiagHam already built (FermionOnSphere::StateDescription,
+ * FermionOnSphere.cc:1621: dim host words, descending).  The device lookups index the full
+ * sector, so the words must be exactly that sector in the reference's order; they are checked
+ * against the device's own generation (DH_E_ARG, with the first differing index in
+ * dh_last_error, when they are not). */
+int dh_fermion_sphere_from_states(dh_context* ctx, dh_basis** basis, int nbr_fermions, int total_lz, int lz_max,
+                                  const uint64_t* states, int64_t dim);
 
 /* Spin1_2Chain(chainLength, sz, memory) (Spin/src/HilbertSpace/Spin1_2Chain.cc:113, :331):
  * states ordered as the reference (site 0 most significant, bits >= L kept at 1 in the
@@ -145,7 +152,8 @@ int64_t dh_hamiltonian_dimension(const dh_hamiltonian* ham);
  * DH_INDEX_*; lds_tables != 0 stages the index tables in LDS when they fit.  Until it is called, a
  * sphere Hamiltonian picks its shape per product: on spaces of at least 256 rows per CU the
  * fastest shapes measured on N=12 2S=33 Lz=0 (with the 16-bit low rank table in LDS: one lane per
- * row, 1024-thread workgroups for k <= 2 and 512 for k > 2; with the rank tables in global memory:
+ * row, 1024-thread workgroups for k = 1, two lanes per row for k = 2, 512 threads for k > 2; with the
+ * rank tables in global memory:
  * two lanes per row, 256 threads for k <= 2 and 1024 above); on smaller spaces four lanes per row,
  * 512 threads. */
 int dh_hamiltonian_set_launch(dh_hamiltonian* ham, int lanes_per_row, int block_size, int index_mode, int lds_tables);

@@ -103,6 +103,33 @@ def test_sector_products_beyond_32_orbitals(arch, k, row_mode):
     S.close()
 
 
+@pytest.mark.parametrize("n,lz_max,sectors", [(4, 31, [0, 4, 30, 80]), (4, 33, [0, 2, 12, 90])])
+@pytest.mark.parametrize("k,row_mode,lpr,block", [(1, 1, 1, 1024), (2, 1, 2, 256), (1, 4, 1, 512), (8, 1, 2, 1024),
+                                                  (1, 3, 4, 256)])
+def test_sector_products_at_the_14_bit_rank_split(arch, n, lz_max, sectors, k, row_mode, lpr, block):
+    """32 and 34 orbitals: the shared 16-bit low rank table in LDS, per-sector high tables of 2^18
+    and 2^20 entries, 32-bit and 64-bit state words, against the oracle in every sector."""
+    V = coulomb(lz_max)
+    S = SectorSet(arch, n, lz_max, sectors, V)
+    assert S.capi.gpu().dh_sector_set_row_mode(S.h, row_mode) == 0
+    assert S.capi.gpu().dh_sector_set_launch(S.h, lpr, block) == 0
+    states = S.states()
+    fac = O.Factors(lz_max, V)
+    X = O.splitmix(70 + k, S.total * k)
+    Y = S.hv(X, k, [1] * len(sectors), np.zeros(S.total * k))
+    for i, lz in enumerate(sectors):
+        ref = O.FermionSphere(n, lz, lz_max)
+        a, b = S.off[i], S.off[i + 1]
+        assert b - a == ref.dim
+        assert np.array_equal(states[a:b], ref.states())
+        Xs = X[a * k:b * k].reshape(-1, k)
+        Ys = Y[a * k:b * k].reshape(-1, k)
+        for v in range(k):
+            y_ref = O.fs_hv(ref, fac, Xs[:, v])
+            assert np.linalg.norm(Ys[:, v] - y_ref) <= 1e-12 * np.linalg.norm(y_ref), (lz, k, v)
+    S.close()
+
+
 def test_sector_element_counts(arch):
     """Per-sector matrix-element counts of a set equal the oracle's count of each sector."""
     V = coulomb(LZ)

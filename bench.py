@@ -25,6 +25,7 @@ bytes per launch from HIP events on the kernel's stream), the measured HBM traff
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -300,6 +301,39 @@ def adjoint_leg(capi, arch, stream, n=10, lz=27, reps=10):
             "product_ms": ms_hv, "adjoint_over_product": ms_adj / ms_hv,
             "algorithmic_bytes": alg, "achieved_GBps": alg / (ms_adj * 1e-3) / 1e9,
             "frac_of_peak": alg / (ms_adj * 1e-3) / HBM_PEAK}
+
+
+def reorth_stream_leg(capi, arch, stream, L=28, nb=100, reps=3):
+    """The two passes of a full reorthogonalisation step at the Krylov depth of the L = 28 Lanczos
+    (100 basis vectors of the chain's 40,116,600 rows, 32 GB): dh_block_dots (C = V^T w, with its
+    chunk reduction and the copy of C to the host) and dh_block_axpy (w -= V C), each timed with HIP
+    events on the bench stream.  Bytes: the basis once plus w once (dots), plus w written (axpy)."""
+    dim = math.comb(L, L // 2)
+    V = capi.DeviceBuffer(arch, nb * dim)
+    w = capi.DeviceBuffer(arch, dim)
+    capi.gpu().dh_fill_random(arch.ctx, V.ptr, nb * dim, 0x5EED0009)
+    capi.gpu().dh_fill_random(arch.ctx, w.ptr, dim, 0x5EED000A)
+    out = np.zeros(nb)
+    coef = np.full(nb, -1e-3)
+    timer = HipTimer(stream)
+    res = {}
+    for name, call, extra in (("dots", lambda: capi.gpu().dh_block_dots(arch.ctx, V.ptr, nb, w.ptr, dim, 1,
+                                                                         capi.dptr(out)), 1),
+                              ("axpy", lambda: capi.gpu().dh_block_axpy(arch.ctx, V.ptr, nb, w.ptr, dim, 1,
+                                                                         capi.dptr(coef)), 2)):
+        assert call() == 0
+        arch.sync()
+        timer.start()
+        for _ in range(reps):
+            assert call() == 0
+        ms = timer.stop_ms() / reps
+        nbytes = 8.0 * dim * (nb + extra)
+        res[name] = {"ms": ms, "bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_peak": nbytes / (ms * 1e-3) / HBM_PEAK}
+    V.free()
+    w.free()
+    return dict({"workload": "reorthogonalisation_L%d_%d_basis_vectors" % (L, nb), "rows": dim,
+                 "basis_vectors": nb}, **res)
 
 
 def sectors_autotune_leg(capi, arch, n=10, lz=27, nev=10, ks=(1, 2, 4, 8)):
@@ -610,6 +644,9 @@ def main():
             line["lanczos"]["sectors_n10_2s27"] = sector_runs(capi, arch, progress)
             line["lanczos"]["sectors_autotune_n10_2s27"] = sectors_autotune_leg(capi, arch)
             progress("sector autotuner: %.1f s" % line["lanczos"]["sectors_autotune_n10_2s27"]["seconds"])
+        line["reorthogonalisation"] = reorth_stream_leg(capi, arch, stream)
+        progress("reorthogonalisation passes: dots %.2f ms, axpy %.2f ms" % (line["reorthogonalisation"]["dots"]["ms"],
+                                                                            line["reorthogonalisation"]["axpy"]["ms"]))
         line["adjoint"] = adjoint_leg(capi, arch, stream)
         progress("adjoint: %.3f ms" % line["adjoint"]["adjoint_ms"])
     if not args.skip_batched:

@@ -259,7 +259,8 @@ int dh_pinned(dh_context* c, size_t bytes, double** p) {
 template <int K, int G>
 __global__ void __launch_bounds__(256) dh_block_dots_kernel(const double* __restrict__ V, int nbrBlocks,
                                                             const double* __restrict__ W, int64_t dim,
-                                                            int64_t rowsPerChunk, double* __restrict__ partial) {
+                                                            int64_t rowsPerChunk, double* __restrict__ partial,
+                                                            bool pairs) {
   constexpr int KK = K * K;
   __shared__ double red[256 / 64][G * KK];  // dh_dots_tile's per-wavefront sums
   const int chunk = blockIdx.y;  // blocks of one chunk are adjacent: its W rows stay in L2
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(256) dh_block_dots_kernel(const double* __rest
   const int64_t rowEnd = min(dim, rowBegin + rowsPerChunk);
   const int64_t total = (int64_t)nbrBlocks * KK;
   dh_dots_tile<K, G>(V, dim * K, nbrBlocks, b0, W, rowBegin, rowEnd, &red[0][0],
-                     partial + (int64_t)chunk * total + (int64_t)b0 * KK);
+                     partial + (int64_t)chunk * total + (int64_t)b0 * KK, pairs);
 }
 
 // out[i] = sum over chunks of partial[chunk][i]: a workgroup owns 64 consecutive outputs (one
@@ -292,14 +293,15 @@ __global__ void __launch_bounds__(1024) dh_reduce_chunks_kernel(const double* __
 
 template <int K>
 static int launch_dots(dh_context* c, const double* V, int nb, const double* W, int64_t dim, double* h_out) {
-  constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
+  constexpr int G = DotsGroup<K>::value;
   const int KK = K * K;
   // row chunks: enough workgroups in all (about 16 per CU) also when there are few basis blocks to
-  // spread over, at least 4 chunks per CU, and no chunk shorter than 1024 rows
+  // spread over, at least 4 chunks per CU, and no chunk shorter than 1024 rows; an even number of
+  // rows, so that the k = 1 row pairs of every chunk are aligned
   const int64_t cols = (nb + G - 1) / G;
   const int64_t want = std::max<int64_t>((int64_t)c->nbrCU * 4, ((int64_t)c->nbrCU * 16 + cols - 1) / cols);
   int64_t target = std::max<int64_t>(1, std::min<int64_t>(want, (dim + 1023) / 1024));
-  int64_t rowsPerChunk = (dim + target - 1) / target;
+  int64_t rowsPerChunk = ((dim + target - 1) / target + 1) & ~int64_t(1);
   int nbrChunks = (int)((dim + rowsPerChunk - 1) / rowsPerChunk);
   int total = nb * KK;
   double* scratch;
@@ -308,7 +310,7 @@ static int launch_dots(dh_context* c, const double* V, int nb, const double* W, 
   double* d_out = scratch + (size_t)nbrChunks * total;
   dim3 grid((nb + G - 1) / G, nbrChunks);
   hipLaunchKernelGGL((dh_block_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, W, dim, rowsPerChunk,
-                     scratch);
+                     scratch, K == 1 && dh_pairs_aligned(V, dim, W));
   hipLaunchKernelGGL(dh_reduce_chunks_kernel, dim3((total + 63) / 64), dim3(1024), 0, c->stream, scratch,
                      nbrChunks, total, d_out);
   DH_HIP(c, hipGetLastError());
@@ -335,7 +337,8 @@ extern "C" int dh_block_dots(dh_context* c, const double* V, int nb, const doubl
 }
 
 // ----------------------------------------------------------------------------- axpy
-// W[row][q] += sum_{b < nb, p} V_b[row][p] C[(b*K+p)*K+q]; C staged in LDS in slabs.
+// W[row][q] += sum_{b < nb, p} V_b[row][p] C[(b*K+p)*K+q]; C staged in LDS in slabs; at k = 1 the
+// basis rows with the nontemporal hint (read once per pass).
 template <int K>
 __global__ void __launch_bounds__(256) dh_block_axpy_kernel(const double* __restrict__ V, int nb,
                                                             double* __restrict__ W, int64_t dim,
@@ -357,7 +360,7 @@ __global__ void __launch_bounds__(256) dh_block_axpy_kernel(const double* __rest
         const double* vb = V + (int64_t)(s0 + b) * dim * K + row * K;
 #pragma unroll
         for (int p = 0; p < K; ++p) {
-          double v = vb[p];
+          const double v = K == 1 ? dh_stream_load(vb + p) : vb[p];
 #pragma unroll
           for (int q = 0; q < K; ++q) acc[q] += v * sC[b * KK + p * K + q];
         }

@@ -259,35 +259,93 @@ struct DotsLanes {
   static constexpr int QS = QP * QQ;
 };
 
+// Krylov basis rows are read once per pass and never again before the next pass, so at k = 1 they
+// are loaded with the nontemporal hint (no allocation in the caches, which keep the rows of W that
+// several workgroups share): 6.2-6.5 against 5.3-5.8 TB/s for the k = 1 dot products on MI355X.
+// Blocks of k > 1 keep ordinary loads: lanes that split a row's products read the same basis
+// elements, and the block solves ran 7-15 % slower with the hint.
+typedef double dh_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double dh_stream_load(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ dh_double2 dh_stream_load2(const double* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const dh_double2*>(p));
+}
+
+// k = 1 dot products over rows [rowBegin, rowEnd) two rows per lane with 16-byte loads (V and W
+// 16-byte aligned, blockStride even): the aligned row pairs inside the range, then an odd first
+// row and an odd last row on lanes 0 and 1.  acc[g] += sum of V_{b0+g}[row] W[row].
+template <int G>
+__device__ __forceinline__ void dh_dots_rows_pairs(const double* __restrict__ V, int64_t blockStride, int nbrBlocks,
+                                                   int b0, const double* __restrict__ W, int64_t rowBegin,
+                                                   int64_t rowEnd, double* acc) {
+  const int64_t pb = (rowBegin + 1) >> 1, pe = rowEnd >> 1;
+  const int ng = min(G, nbrBlocks - b0);
+  const double* vb = V + (int64_t)b0 * blockStride;
+  if (ng == G) {
+    for (int64_t q = pb + threadIdx.x; q < pe; q += blockDim.x) {
+      const dh_double2 w = *reinterpret_cast<const dh_double2*>(W + 2 * q);
+      dh_double2 v[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) v[g] = dh_stream_load2(vb + g * blockStride + 2 * q);
+#pragma unroll
+      for (int g = 0; g < G; ++g) acc[g] += v[g].x * w.x + v[g].y * w.y;
+    }
+  } else {
+    for (int64_t q = pb + threadIdx.x; q < pe; q += blockDim.x) {
+      const dh_double2 w = *reinterpret_cast<const dh_double2*>(W + 2 * q);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (g < ng) {
+          const dh_double2 v = dh_stream_load2(vb + g * blockStride + 2 * q);
+          acc[g] += v.x * w.x + v.y * w.y;
+        }
+    }
+  }
+  int64_t single = -1;
+  if (threadIdx.x == 0 && (rowBegin & 1) && rowBegin < rowEnd) single = rowBegin;
+  if (threadIdx.x == 1 && (rowEnd & 1) && rowEnd - 1 >= rowBegin) single = rowEnd - 1;
+  if (single >= 0) {
+    const double w = W[single];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (g < ng) acc[g] += vb[g * blockStride + single] * w;
+  }
+}
+
 // Sums over rows [rowBegin, rowEnd) of V_b[row][p] W[row][q] for the basis blocks b0 .. b0+G-1
 // (block b at V + b * blockStride, rows of K doubles), written b-major then p then q to out[],
 // for the blocks below nbrBlocks.  Every thread of the workgroup must call it (one barrier);
-// red is LDS for (blockDim.x / 64) * G * K * K doubles.
+// red is LDS for (blockDim.x / 64) * G * K * K doubles.  pairs (k = 1 only): V and W 16-byte
+// aligned and blockStride even, so rows are read in pairs.
 template <int K, int G>
 __device__ __forceinline__ void dh_dots_tile(const double* __restrict__ V, int64_t blockStride, int nbrBlocks,
                                              int b0, const double* __restrict__ W, int64_t rowBegin,
-                                             int64_t rowEnd, double* red, double* __restrict__ out) {
+                                             int64_t rowEnd, double* red, double* __restrict__ out,
+                                             bool pairs = false) {
   constexpr int QP = DotsLanes<K>::QP, QQ = DotsLanes<K>::QQ, QS = DotsLanes<K>::QS;
   constexpr int KP = K / QP, KQ = K / QQ, NA = G * KP * KQ, KK = K * K;
   const int sub = threadIdx.x % QS, sp = sub / QQ, sq = sub % QQ;
   double acc[NA];
 #pragma unroll
   for (int i = 0; i < NA; ++i) acc[i] = 0.0;
-  for (int64_t row = rowBegin + threadIdx.x / QS; row < rowEnd; row += blockDim.x / QS) {
-    double w[KQ];
+  if (K == 1 && pairs) {
+    dh_dots_rows_pairs<G>(V, blockStride, nbrBlocks, b0, W, rowBegin, rowEnd, acc);
+  } else {
+    for (int64_t row = rowBegin + threadIdx.x / QS; row < rowEnd; row += blockDim.x / QS) {
+      double w[KQ];
 #pragma unroll
-    for (int qq = 0; qq < KQ; ++qq) w[qq] = W[row * K + sq * KQ + qq];
+      for (int qq = 0; qq < KQ; ++qq) w[qq] = W[row * K + sq * KQ + qq];
 #pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (b0 + g < nbrBlocks) {
-        const double* vb = V + (int64_t)(b0 + g) * blockStride + row * K + sp * KP;
+      for (int g = 0; g < G; ++g)
+        if (b0 + g < nbrBlocks) {
+          const double* vb = V + (int64_t)(b0 + g) * blockStride + row * K + sp * KP;
 #pragma unroll
-        for (int pp = 0; pp < KP; ++pp) {
-          const double v = vb[pp];
+          for (int pp = 0; pp < KP; ++pp) {
+            const double v = K == 1 ? dh_stream_load(vb + pp) : vb[pp];
 #pragma unroll
-          for (int qq = 0; qq < KQ; ++qq) acc[(g * KP + pp) * KQ + qq] += v * w[qq];
+            for (int qq = 0; qq < KQ; ++qq) acc[(g * KP + pp) * KQ + qq] += v * w[qq];
+          }
         }
-      }
+    }
   }
   // sum over the lanes of a wavefront with the same share (lane mod QS), then over wavefronts
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -307,6 +365,18 @@ __device__ __forceinline__ void dh_dots_tile(const double* __restrict__ V, int64
     for (int w2 = 0; w2 < nw; ++w2) t += red[(w2 * QS + s) * NA + i];
     out[idx] = t;
   }
+}
+
+// basis blocks per workgroup pass of the dot products (sharing one read of W): 8 at k = 1 (faster
+// than 16 on MI355X: 5.3 against 4.9 TB/s before the row pairs), else 16 / k^2, at least one
+template <int K>
+struct DotsGroup {
+  static constexpr int value = K == 1 ? 8 : ((16 / (K * K)) > 0 ? (16 / (K * K)) : 1);
+};
+
+// whether the k = 1 dot products may read rows in pairs
+inline bool dh_pairs_aligned(const double* V, int64_t blockStride, const double* W) {
+  return blockStride % 2 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 && reinterpret_cast<uintptr_t>(W) % 16 == 0;
 }
 
 // XCD-aware block remap: MI355X hands consecutive workgroups to its 8 XCDs round robin;

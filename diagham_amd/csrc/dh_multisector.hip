@@ -504,7 +504,8 @@ __global__ void __launch_bounds__(256) sectors_dots_kernel(const double* __restr
                                                            int64_t totalDim, const double* __restrict__ W,
                                                            const int64_t* __restrict__ offsets,
                                                            const int64_t* __restrict__ work, int nActive,
-                                                           int64_t rowsPerTile, double* __restrict__ partial) {
+                                                           int64_t rowsPerTile, double* __restrict__ partial,
+                                                           bool pairs) {
   constexpr int KK = K * K;
   __shared__ double red[256 / 64][G * KK];  // dh_dots_tile's per-wavefront sums
   const int64_t tile = blockIdx.y;  // blocks of one tile are adjacent: its W rows stay in L2
@@ -521,7 +522,7 @@ __global__ void __launch_bounds__(256) sectors_dots_kernel(const double* __restr
   const int b0 = blockIdx.x * G;
   const int64_t total = (int64_t)nbrBlocks * KK;
   dh_dots_tile<K, G>(V, totalDim * K, nbrBlocks, b0, W, off + r0, off + r1, &red[0][0],
-                     partial + tile * total + (int64_t)b0 * KK);
+                     partial + tile * total + (int64_t)b0 * KK, pairs);
 }
 
 __global__ void sectors_reduce_kernel(const double* __restrict__ partial, const int64_t* __restrict__ work,
@@ -539,7 +540,7 @@ template <int K>
 static int sectors_dots(dh_sector_set* S, const double* V, int nb, const double* W, const unsigned char* active,
                         double* h_out) {
   dh_context* c = S->ctx;
-  constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
+  constexpr int G = DotsGroup<K>::value;
   // automatic tile: at least 8192 rows (4096 for k >= 4, whose one-block workgroups are few:
   // 3.9-4.0 against 2.5-3.6 TB/s at k = 8 over the 91 sectors of N=10 2S=27), and no more
   // than about 32768 tiles
@@ -557,7 +558,7 @@ static int sectors_dots(dh_sector_set* S, const double* V, int nb, const double*
   double* d_out = scratch + tiles * total;
   dim3 grid((unsigned)((nb + G - 1) / G), (unsigned)tiles);
   hipLaunchKernelGGL((sectors_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, S->totalDim, W, S->d_offsets,
-                     S->d_work, na, rowsPerTile, scratch);
+                     S->d_work, na, rowsPerTile, scratch, K == 1 && dh_pairs_aligned(V, S->totalDim, W));
   hipLaunchKernelGGL(sectors_reduce_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)na), dim3(256), 0,
                      c->stream, scratch, S->d_work, na, total, d_out);
   DH_HIP(c, hipGetLastError());
@@ -585,7 +586,8 @@ extern "C" int dh_sectors_block_dots(dh_sector_set* S, const double* V, int nb, 
 // axpy / rmul: W[row][q] (+)= sum over (b, p) of V_b[row][p] C_j[(b*K+p)*K+q]; rmul uses W itself.
 // The coefficients sit in LDS in slabs of `slab` basis blocks (dynamic LDS of slab * K * K
 // doubles, so short bases leave room for more workgroups per CU). Basis rows are read U blocks at
-// a time, so one lane keeps U independent loads in flight.
+// a time, so one lane keeps U independent loads in flight; at k = 1 with the nontemporal hint (each
+// basis row is read once per pass).
 template <int K, bool RMUL>
 __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restrict__ V, int nb, int64_t totalDim,
                                                            double* __restrict__ W,
@@ -631,7 +633,8 @@ __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restr
 #pragma unroll
           for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int p = 0; p < K; ++p) v[u][p] = vb[(int64_t)(b + u) * stride + p];
+            for (int p = 0; p < K; ++p) v[u][p] = (RMUL || K > 1) ? vb[(int64_t)(b + u) * stride + p]
+                                                                  : dh_stream_load(vb + (int64_t)(b + u) * stride + p);
 #pragma unroll
           for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -642,7 +645,7 @@ __global__ void __launch_bounds__(256) sectors_axpy_kernel(const double* __restr
         for (; b < ns; ++b) {
 #pragma unroll
           for (int p = 0; p < K; ++p) {
-            double v = vb[(int64_t)b * stride + p];
+            const double v = (RMUL || K > 1) ? vb[(int64_t)b * stride + p] : dh_stream_load(vb + (int64_t)b * stride + p);
 #pragma unroll
             for (int q = 0; q < K; ++q) acc[q] += v * sC[b * KK + p * K + q];
           }

@@ -21,7 +21,8 @@ static __device__ __align__(256) double scratch[kScratchDoubles];
 template <int K, int G>
 __global__ void __launch_bounds__(256) noalloc_dots_kernel(const double* __restrict__ V, int nbrBlocks,
                                                            const double* __restrict__ W, int64_t dim,
-                                                           int64_t rowsPerChunk, double* __restrict__ partial) {
+                                                           int64_t rowsPerChunk, double* __restrict__ partial,
+                                                           bool pairs) {
   constexpr int KK = K * K;
   __shared__ double red[256 / 64][G * KK];
   const int chunk = blockIdx.y;
@@ -30,7 +31,7 @@ __global__ void __launch_bounds__(256) noalloc_dots_kernel(const double* __restr
   const int64_t rowEnd = min(dim, rowBegin + rowsPerChunk);
   const int64_t total = (int64_t)nbrBlocks * KK;
   dh_dots_tile<K, G>(V, dim * K, nbrBlocks, b0, W, rowBegin, rowEnd, &red[0][0],
-                     partial + (int64_t)chunk * total + (int64_t)b0 * KK);
+                     partial + (int64_t)chunk * total + (int64_t)b0 * KK, pairs);
 }
 
 // out[i] = sum over the chunks of partial[chunk][i] (64 outputs per workgroup, waves split the chunks)
@@ -68,7 +69,7 @@ static int scratch_address(dh_context* c, double** p) {
 
 template <int K>
 static int launch_noalloc_dots(dh_context* c, const double* V, int nb, const double* W, int64_t dim, double* d_out) {
-  constexpr int G = (16 / (K * K)) > 0 ? (16 / (K * K)) : 1;
+  constexpr int G = DotsGroup<K>::value;
   const int64_t total = (int64_t)nb * K * K;
   if (total > kScratchDoubles)
     return dh_fail(c, DH_E_RANGE, "dh_block_dots_noalloc: nbr_blocks * k * k exceeds the 524288 scratch doubles");
@@ -77,13 +78,14 @@ static int launch_noalloc_dots(dh_context* c, const double* V, int nb, const dou
   const int64_t want = std::max<int64_t>((int64_t)c->nbrCU * 4, ((int64_t)c->nbrCU * 16 + cols - 1) / cols);
   int64_t target = std::max<int64_t>(1, std::min<int64_t>(want, (dim + 1023) / 1024));
   target = std::min<int64_t>(target, kScratchDoubles / total);
-  const int64_t rowsPerChunk = (dim + target - 1) / target;
+  const int64_t rowsPerChunk = ((dim + target - 1) / target + 1) & ~int64_t(1);
   const int nbrChunks = (int)((dim + rowsPerChunk - 1) / rowsPerChunk);
   double* partial = nullptr;
   int rc = scratch_address(c, &partial);
   if (rc) return rc;
   dim3 grid((unsigned)cols, (unsigned)nbrChunks);
-  hipLaunchKernelGGL((noalloc_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, W, dim, rowsPerChunk, partial);
+  hipLaunchKernelGGL((noalloc_dots_kernel<K, G>), grid, dim3(256), 0, c->stream, V, nb, W, dim, rowsPerChunk, partial,
+                     K == 1 && dh_pairs_aligned(V, dim, W));
   hipLaunchKernelGGL(noalloc_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(1024), 0, c->stream, partial,
                      nbrChunks, (int)total, d_out);
   DH_HIP(c, hipGetLastError());

@@ -15,6 +15,11 @@ __device__ __forceinline__ int dh_ctz(uint32_t v) { return __builtin_ctz(v); }
 __device__ __forceinline__ int dh_popc(uint64_t v) { return __popcll(v); }
 __device__ __forceinline__ int dh_popc(uint32_t v) { return __popc(v); }
 
+// c, negated when the parity p is odd: the sign bit flipped (the same double as -c)
+__device__ __forceinline__ double dh_negate_if(double c, int p) {
+  return __hiloint2double(__double2hiint(c) ^ (int)((unsigned)p << 31), __double2loint(c));
+}
+
 template <int K, typename I>
 __device__ __forceinline__ void dh_gather_fma(double* acc, const double* __restrict__ x, I idx, double coef) {
   const double* xr = x + (size_t)idx * K;
@@ -168,6 +173,23 @@ __device__ __forceinline__ void dh_sphere_row_free(U s, int nbrOrbitals, const d
   double pendCoef = 0.0;
 #pragma unroll
   for (int v = 0; v < K; ++v) pend[v] = 0.0;
+  // one created pair m3 > m4 = sum - m3 of the intermediate state t, with factor c
+  auto term = [&](U t, int sum, int sgnA, int m3, double c) {
+    const U b3 = U(1) << m3, b4 = U(1) << (sum - m3);
+    // the occupied orbitals of t at or above m4 and at or above m3 differ by those in [m4, m3)
+    const double coef = dh_negate_if(c, sgnA + dh_popc(t & (b3 - b4)));
+    const auto idx = find(t | b3 | b4);
+    if constexpr (PIPE) {
+      const double* xr = xbase + (size_t)idx * K;
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v] = fma(pendCoef, pend[v], acc[v]);
+#pragma unroll
+      for (int v = 0; v < K; ++v) pend[v] = xr[v];
+      pendCoef = coef;
+    } else {
+      dh_gather_fma<K>(acc, xbase, idx, coef);
+    }
+  };
   for (U r1 = s; r1; r1 &= r1 - 1) {
     const int n1 = dh_ctz(r1);
     const int pc1 = dh_popc(s >> n1);
@@ -180,26 +202,28 @@ __device__ __forceinline__ void dh_sphere_row_free(U s, int nbrOrbitals, const d
       const double* fS = sF + sBase[sum];
       const int i = n1 - m3lo;
       U pm = dh_free_pairs(t, nbrOrbitals, sum, m3lo, min(sum, nbrOrbitals - 1));
-      if constexpr (LPR > 1)
+      if constexpr (LPR == 1) {
+        // created pairs below the annihilated one (position j < i) read row i of the packed
+        // triangle, i (i + 1) / 2 + j; those at or above it read column i, j (j + 1) / 2 + i.  Two
+        // loops in ascending m3, so the terms keep their order.
+        const U below = (U(1) << n1) - U(1);
+        const double* fRow = fS + (int)(__umul24((unsigned)i, (unsigned)(i + 1)) >> 1) - m3lo;
+        for (U p = pm & below; p; p &= p - 1) {
+          const int m3 = dh_ctz(p);
+          term(t, sum, sgnA, m3, fRow[m3]);
+        }
+        const double* fCol = fS + i;
+        for (U p = pm & ~below; p; p &= p - 1) {
+          const int m3 = dh_ctz(p), j = m3 - m3lo;
+          term(t, sum, sgnA, m3, fCol[__umul24((unsigned)j, (unsigned)(j + 1)) >> 1]);
+        }
+      } else {
         for (int q = 0; q < lane; ++q) pm &= pm - 1;
-      while (pm) {
-        const int m3 = dh_ctz(pm);
+        while (pm) {
+          const int m3 = dh_ctz(pm);
 #pragma unroll
-        for (int q = 0; q < LPR; ++q) pm &= pm - 1;  // this term, then the other lanes' next ones
-        const int m4 = sum - m3;
-        const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
-        const auto idx = find(t | (U(1) << m3) | (U(1) << m4));
-        double coef = fS[dh_packed_index(i, m3 - m3lo)];
-        if (sg & 1) coef = -coef;
-        if constexpr (PIPE) {
-          const double* xr = xbase + (size_t)idx * K;
-#pragma unroll
-          for (int v = 0; v < K; ++v) acc[v] = fma(pendCoef, pend[v], acc[v]);
-#pragma unroll
-          for (int v = 0; v < K; ++v) pend[v] = xr[v];
-          pendCoef = coef;
-        } else {
-          dh_gather_fma<K>(acc, xbase, idx, coef);
+          for (int q = 0; q < LPR; ++q) pm &= pm - 1;  // this term, then the other lanes' next ones
+          term(t, sum, sgnA, m3, fS[dh_packed_index(i, m3 - m3lo)]);
         }
       }
     }

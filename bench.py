@@ -349,7 +349,7 @@ def sectors_autotune_leg(capi, arch, n=10, lz=27, nev=10, ks=(1, 2, 4, 8)):
 def kernel_only(capi, arch, stream, name, steps, warmup):
     """One named product repeated `steps` times after `warmup` (for rocprofv3 runs): prints a JSON
     line with its mean time and rates."""
-    k = int(name.rsplit("_k", 1)[1]) if "_k" in name else 1
+    k = int(name.split("_k")[1].split("_")[0]) if "_k" in name else 1
     timer = HipTimer(stream)
     cleanup = []
     free = "_free" in name  # the free-pair row routine for every k (DH_INDEX_RANK_FREE)
@@ -381,6 +381,9 @@ def kernel_only(capi, arch, stream, name, steps, warmup):
                                         [y for x in range(6) for y in range(5)])
         cleanup.append(from_lattice.close)
         from_lattice.set_launch(lpr or (2 if k <= 2 else 1), bs or (512 if k <= 2 else 256))
+        if free or rng:
+            arch.check(capi.gpu().dh_sector_set_row_mode(from_lattice.h, capi.INDEX_RANK_FREE if free
+                                                         else capi.INDEX_RANK_RANGE), "row mode")
         rows, nnz = from_lattice.total, int(from_lattice.count_elements().sum())
         run = lambda dx, dy: from_lattice.hv_device(dx, dy, k)  # noqa: E731
         width = 2 * k

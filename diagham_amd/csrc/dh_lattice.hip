@@ -343,11 +343,10 @@ __device__ __forceinline__ uint64_t lattice_free_pairs(uint64_t t, int N, int ny
 
 // K complex vectors per row: x and y hold 2K doubles per row, (re, im) interleaved.  FREE: the
 // created pairs of each annihilated pair come from lattice_free_pairs (no iteration over a created
-// pair with an occupied orbital) and G terms are gathered together per lane.
+// pair with an occupied orbital); otherwise every listed pair of the momentum sum is tried.
 template <int K, int LPR, bool LO_LDS, bool FREE>
 __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const double* __restrict__ x,
                                                           double* __restrict__ y, int add) {
-  constexpr int G = K == 1 ? 4 : (K <= 2 ? 2 : 1);
   extern __shared__ int64_t smem64[];
   int64_t* sWork = smem64;                                    // 2 nActive + 1
   int64_t* sFacOff = sWork + 2 * a.nActive + 1;               // nbrSums + 1
@@ -398,71 +397,69 @@ __global__ void __launch_bounds__(1024) lattice_hv_kernel(LatticeArgs a, const d
     for (int v = 0; v < 2 * K; ++v) acc[v] = 0.0;
     const double* xs = x + off * 2 * K;
     if constexpr (FREE) {
-      uint64_t r1 = s, r2 = 0, pm = 0, t = 0;
-      int n1 = 0, sgnA = 0, sum = 0;
-      const double* F = a.factors;
-      int skip = lane;
-      bool more = true;
-      const int64_t self = active ? local : 0;
-      while (more) {
-        int64_t idx[G];
-        double cre[G], cim[G];
+      // every annihilated pair n1 < n2, then its free created pairs from the mask of lattice_free_pairs
+      // (lane `lane` of LPR takes every LPR-th of them); for K <= 2 a term's multiply-add waits for
+      // the next term's index loads, as in the list loop below, with the terms in the same order
+      constexpr bool PIPE = K <= 2;
+      double pre = 0.0, pim = 0.0;
+      double2 px[K];
 #pragma unroll
-        for (int q = 0; q < G; ++q) {
-          idx[q] = self;
-          cre[q] = 0.0;
-          cim[q] = 0.0;
-          while (more) {
-            if (!pm) {
-              if (!r2) {
-                if (!r1) {
-                  more = false;
-                  break;
-                }
-                n1 = __builtin_ctzll(r1);  // lower annihilated orbital
-                r1 &= r1 - 1;
-                r2 = s & ~((2ull << n1) - 1ull);
-                continue;
-              }
-              const int n2 = __builtin_ctzll(r2);  // higher annihilated orbital
-              r2 &= r2 - 1;
-              sgnA = __popcll(s >> n2) + __popcll((s ^ (1ull << n2)) >> n1);  // a_n2 first, then a_n1
-              t = s ^ (1ull << n1) ^ (1ull << n2);
-              const int info = sInfo[(n2 * (n2 - 1)) / 2 + n1];
-              sum = info >> 16;
-              const int np = sPairOff[sum + 1] - sPairOff[sum];
-              F = a.factors + 2 * (sFacOff[sum] + (int64_t)(info & 0xffff) * np);
-              pm = lattice_free_pairs(t, a.nbrOrbitals, a.nbrSiteY, sMask, sShift, sum);
-              continue;
-            }
-            if (LPR > 1 && skip > 0) {
-              pm &= pm - 1;
-              --skip;
-              continue;
-            }
+      for (int v = 0; v < K; ++v) px[v] = make_double2(0.0, 0.0);
+      for (uint64_t r1 = s; r1; r1 &= r1 - 1) {
+        const int n1 = __builtin_ctzll(r1);  // lower annihilated orbital
+        for (uint64_t r2 = s & ~((2ull << n1) - 1ull); r2; r2 &= r2 - 1) {
+          const int n2 = __builtin_ctzll(r2);  // higher annihilated orbital
+          // AA(n1, n2): a_n2 first, then a_n1
+          const int sgnA = __popcll(s >> n2) + __popcll((s ^ (1ull << n2)) >> n1);
+          const uint64_t t = s ^ (1ull << n1) ^ (1ull << n2);
+          const int info = sInfo[(n2 * (n2 - 1)) / 2 + n1];
+          const int sum = info >> 16;
+          const int np = sPairOff[sum + 1] - sPairOff[sum];
+          const double* F = a.factors + 2 * (sFacOff[sum] + (int64_t)(info & 0xffff) * np);
+          const int* pp = sPP + sum * a.nbrOrbitals;
+          uint64_t pm = lattice_free_pairs(t, a.nbrOrbitals, a.nbrSiteY, sMask, sShift, sum);
+          if constexpr (LPR > 1)
+            for (int q = 0; q < lane; ++q) pm &= pm - 1;
+          while (pm) {
             const int m1 = __builtin_ctzll(pm);
-            pm &= pm - 1;
-            skip = LPR - 1;
-            const int pp = sPP[sum * a.nbrOrbitals + m1];
-            const int m2 = pp & 0xff, i2 = pp >> 8;  // m1 < m2
-            // AdAd(m1, m2): a+_m2 first, then a+_m1
-            const int sg = sgnA + __popcll(t >> m2) + __popcll((t | (1ull << m2)) >> m1);
-            idx[q] = dh_find_rank(t | (1ull << m1) | (1ull << m2), lo, hi, a.rankBits, dim);
-            const double fr = F[2 * i2], fi = -F[2 * i2 + 1];  // conj(F)
-            cre[q] = (sg & 1) ? -fr : fr;
-            cim[q] = (sg & 1) ? -fi : fi;
-            break;
+#pragma unroll
+            for (int q = 0; q < LPR; ++q) pm &= pm - 1;  // this term, then the other lanes' next ones
+            const int pk = pp[m1];
+            const int m2 = pk & 0xff, i2 = pk >> 8;  // m1 < m2
+            const uint64_t c1 = 1ull << m1, c2 = 1ull << m2;
+            // AdAd(m1, m2), a+_m2 first, then a+_m1: popc(t >> m2) + popc((t | c2) >> m1) has the
+            // parity of 1 + the occupied orbitals of t in [m1, m2)
+            const int par = sgnA + 1 + __popcll(t & (c2 - c1));
+            const int64_t idx = dh_find_rank(t | c1 | c2, lo, hi, a.rankBits, dim);
+            const double2 f = *reinterpret_cast<const double2*>(F + 2 * i2);
+            const double cr = (par & 1) ? -f.x : f.x, ci = (par & 1) ? f.y : -f.y;  // conj(F), signed
+            const double* xr = xs + idx * 2 * K;
+            if constexpr (PIPE) {
+#pragma unroll
+              for (int v = 0; v < K; ++v) {
+                acc[2 * v] += pre * px[v].x - pim * px[v].y;
+                acc[2 * v + 1] += pre * px[v].y + pim * px[v].x;
+              }
+#pragma unroll
+              for (int v = 0; v < K; ++v) px[v] = *reinterpret_cast<const double2*>(xr + 2 * v);
+              pre = cr;
+              pim = ci;
+            } else {
+#pragma unroll
+              for (int v = 0; v < K; ++v) {
+                double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
+                acc[2 * v] += cr * p.x - ci * p.y;
+                acc[2 * v + 1] += cr * p.y + ci * p.x;
+              }
+            }
           }
         }
+      }
+      if constexpr (PIPE) {
 #pragma unroll
-        for (int q = 0; q < G; ++q) {
-          const double* xr = xs + idx[q] * 2 * K;
-#pragma unroll
-          for (int v = 0; v < K; ++v) {
-            double2 p = *reinterpret_cast<const double2*>(xr + 2 * v);
-            acc[2 * v] += cre[q] * p.x - cim[q] * p.y;
-            acc[2 * v + 1] += cre[q] * p.y + cim[q] * p.x;
-          }
+        for (int v = 0; v < K; ++v) {
+          acc[2 * v] += pre * px[v].x - pim * px[v].y;
+          acc[2 * v + 1] += pre * px[v].y + pim * px[v].x;
         }
       }
     } else {
@@ -577,7 +574,9 @@ static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add,
   a.freeMask = S->d_freeMask;
   a.freeShift = S->d_freeShift;
   a.partnerPos = S->d_partnerPos;
-  const bool free = S->rowMode == DH_INDEX_RANK_FREE && S->d_freeMask;
+  // DH_INDEX_RANK: the free-pair masks for one vector (35.7 against 40.3 ms for the 30 sectors of
+  // N=10 on 6 x 5 sites on MI355X), the pair list above (equal at k = 2, faster at k = 4)
+  const bool free = (S->rowMode == DH_INDEX_RANK_FREE || (S->rowMode == DH_INDEX_RANK && K == 1)) && S->d_freeMask;
   size_t base = 8 * (2 * (size_t)na + 1) + 8 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs +
                 4 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs;
   if (free) base += 8 * 3 * (size_t)S->nbrSums + 4 * 2 * (size_t)S->nbrSums + 4 * (size_t)S->nbrSums * S->nbrOrbitals;

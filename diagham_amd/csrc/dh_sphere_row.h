@@ -15,6 +15,13 @@ __device__ __forceinline__ int dh_ctz(uint32_t v) { return __builtin_ctz(v); }
 __device__ __forceinline__ int dh_popc(uint64_t v) { return __popcll(v); }
 __device__ __forceinline__ int dh_popc(uint32_t v) { return __popc(v); }
 
+// dh_packed_index with a 24-bit multiply (positions stay below 2^12): a full-rate multiply-add
+// instead of the quarter-rate 32-bit multiply
+__device__ __forceinline__ int dh_packed_index_dev(int i, int j) {
+  const unsigned mx = (unsigned)max(i, j), mn = (unsigned)min(i, j);
+  return (int)((__umul24(mx, mx + 1u) >> 1) + mn);
+}
+
 // c, negated when the parity p is odd: the sign bit flipped (the same double as -c)
 __device__ __forceinline__ double dh_negate_if(double c, int p) {
   return __hiloint2double(__double2hiint(c) ^ (int)((unsigned)p << 31), __double2loint(c));
@@ -74,7 +81,7 @@ __device__ __forceinline__ void dh_sphere_row(U s, int nbrOrbitals, const double
         if (t & cm) continue;
         const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
         const auto idx = find(t | cm);
-        double coef = fS[dh_packed_index(i, m3 - m3lo)];
+        double coef = fS[dh_packed_index_dev(i, m3 - m3lo)];
         if (sg & 1) coef = -coef;
         if constexpr (PIPE) {
           // the previous term first, then this term's gather into the same registers (no copy,
@@ -128,7 +135,7 @@ __device__ __forceinline__ void dh_sphere_row_x4(U s, int nbrOrbitals, const dou
           const U cm = (U(1) << m3) | (U(1) << m4);
           const bool ok = (m3b + u * LPR <= m3hi) && !(t & cm);
           const int sg = sgnA + dh_popc(t >> m4) + dh_popc(t >> m3);
-          const double c = ok ? fS[dh_packed_index(i, m3 - m3lo)] : 0.0;
+          const double c = ok ? fS[dh_packed_index_dev(i, m3 - m3lo)] : 0.0;
           coef[u] = (sg & 1) ? -c : c;
           idx[u] = find(ok ? (t | cm) : s);
         }
@@ -202,29 +209,13 @@ __device__ __forceinline__ void dh_sphere_row_free(U s, int nbrOrbitals, const d
       const double* fS = sF + sBase[sum];
       const int i = n1 - m3lo;
       U pm = dh_free_pairs(t, nbrOrbitals, sum, m3lo, min(sum, nbrOrbitals - 1));
-      if constexpr (LPR == 1) {
-        // created pairs below the annihilated one (position j < i) read row i of the packed
-        // triangle, i (i + 1) / 2 + j; those at or above it read column i, j (j + 1) / 2 + i.  Two
-        // loops in ascending m3, so the terms keep their order.
-        const U below = (U(1) << n1) - U(1);
-        const double* fRow = fS + (int)(__umul24((unsigned)i, (unsigned)(i + 1)) >> 1) - m3lo;
-        for (U p = pm & below; p; p &= p - 1) {
-          const int m3 = dh_ctz(p);
-          term(t, sum, sgnA, m3, fRow[m3]);
-        }
-        const double* fCol = fS + i;
-        for (U p = pm & ~below; p; p &= p - 1) {
-          const int m3 = dh_ctz(p), j = m3 - m3lo;
-          term(t, sum, sgnA, m3, fCol[__umul24((unsigned)j, (unsigned)(j + 1)) >> 1]);
-        }
-      } else {
+      if constexpr (LPR > 1)
         for (int q = 0; q < lane; ++q) pm &= pm - 1;
-        while (pm) {
-          const int m3 = dh_ctz(pm);
+      while (pm) {
+        const int m3 = dh_ctz(pm);
 #pragma unroll
-          for (int q = 0; q < LPR; ++q) pm &= pm - 1;  // this term, then the other lanes' next ones
-          term(t, sum, sgnA, m3, fS[dh_packed_index(i, m3 - m3lo)]);
-        }
+        for (int q = 0; q < LPR; ++q) pm &= pm - 1;  // this term, then the other lanes' next ones
+        term(t, sum, sgnA, m3, fS[dh_packed_index_dev(i, m3 - m3lo)]);
       }
     }
   }

@@ -114,8 +114,10 @@ void BlockTridiagonalEigen(const std::vector<std::vector<cplx>>& A, const std::v
 // The `count` lowest eigenvalues (ascending) of the symmetric tridiagonal matrix with diagonal d
 // and off-diagonal e, by Sturm-count bisection to adjacent doubles: O(count * n * 60) instead of
 // the O(n^2) of a full QL sweep, for the per-step convergence tests of the one-vector recurrence.
+// hint: the lowest eigenvalues of a leading principal submatrix (an earlier step's matrix), which
+// narrow the starting brackets; the results are the same with or without it.
 void TridiagonalLowest(const std::vector<double>& d, const std::vector<double>& e, int count,
-                       std::vector<double>& evals);
+                       std::vector<double>& evals, const std::vector<double>* hint = nullptr);
 
 template <class T>
 class SegmentedBlockLanczos {
@@ -203,6 +205,7 @@ class SegmentedBlockLanczos {
       Previous[s].clear();
       LastCheck[s] = -1;
     }
+    LowestHint.assign(Dims.size(), std::vector<double>());
     StepCount = 0;
     LaunchedStep = -1;
   }
@@ -314,7 +317,9 @@ class SegmentedBlockLanczos {
     }
     for (int b = 0; b < m; ++b)
       if (DeadCols[s][b][0]) d[b] = 2.0 * bound + 1.0;
-    TridiagonalLowest(d, e, (int)std::min<int64_t>(count, Live(s)), ev);
+    if (LowestHint.size() != Dims.size()) LowestHint.assign(Dims.size(), std::vector<double>());
+    TridiagonalLowest(d, e, (int)std::min<int64_t>(count, Live(s)), ev, &LowestHint[s]);
+    LowestHint[s] = ev;
   }
 
   // Block Lanczos convergence checks cost a band reduction; past the first ones they are spaced
@@ -732,6 +737,7 @@ class SegmentedBlockLanczos {
   std::vector<std::vector<std::vector<char>>> DeadCols;  // per segment, per block X_b, per column
   std::vector<std::vector<double>> Previous;
   std::vector<int> LastCheck;
+  mutable std::vector<std::vector<double>> LowestHint;  // per segment: the last LowestEigen result
 };
 
 }  // namespace dhh

@@ -37,7 +37,7 @@ static int SturmCount(const std::vector<double>& d, const std::vector<double>& e
 }
 
 void TridiagonalLowest(const std::vector<double>& d, const std::vector<double>& e, int count,
-                       std::vector<double>& evals) {
+                       std::vector<double>& evals, const std::vector<double>* hint) {
   const int n = (int)d.size();
   count = std::max(0, std::min(count, n));
   evals.assign(count, 0.0);
@@ -57,21 +57,70 @@ void TridiagonalLowest(const std::vector<double>& d, const std::vector<double>& 
   const double pivmin = std::numeric_limits<double>::min() * std::max(1.0, emax);
   lo -= 2.0 * std::numeric_limits<double>::epsilon() * norm + pivmin;
   hi += 2.0 * std::numeric_limits<double>::epsilon() * norm + pivmin;
-  // bisection per eigenvalue down to adjacent doubles; each one starts from the bracket the
-  // previous ones left (lambda_i >= lambda_{i-1})
-  double floor = lo;
-  for (int i = 0; i < count; ++i) {
-    double a = floor, b = hi;
-    while (true) {
-      const double mid = 0.5 * (a + b);
-      if (mid <= a || mid >= b) break;
-      if (SturmCount(d, e2, mid, pivmin) > i)
-        b = mid;
-      else
-        a = mid;
+  // Bisection per eigenvalue down to adjacent doubles: the result is the pair of adjacent doubles
+  // where the Sturm count passes i, whatever bracket the bisection starts from.
+  std::vector<double> a(count, lo), b(count, hi);
+  if (hint)
+    for (int i = 0; i < count && i < (int)hint->size(); ++i) {
+      // the same eigenvalue of a leading principal submatrix (the tridiagonal matrix of an
+      // earlier step) bounds this one from above (Cauchy interlacing), and a converged value moves
+      // by rounding only: start from [h - w, h], widening w until the Sturm counts confirm the
+      // bracket (62 bisection steps down to adjacent doubles become about 12 for a converged value)
+      const double h = (*hint)[i];
+      if (!(h > lo && h < hi) || SturmCount(d, e2, h, pivmin) <= i) continue;
+      b[i] = h;
+      double w = 1e-13 * std::max(1.0, std::fabs(h));
+      for (int t = 0; t < 8; ++t, w *= 64.0) {
+        const double x = h - w;
+        if (x <= lo) break;
+        if (SturmCount(d, e2, x, pivmin) <= i) {
+          a[i] = x;
+          break;
+        }
+      }
     }
-    evals[i] = b;
-    floor = a;
+  // the bisections of up to 16 eigenvalues advance together: their Sturm sequences are
+  // independent chains of divisions, which the processor overlaps (one chain alone waits on the
+  // latency of every division)
+  constexpr int kBatch = 16;
+  for (int first = 0; first < count; first += kBatch) {
+    const int nb = std::min(kBatch, count - first);
+    int idx[kBatch];
+    double x[kBatch], q[kBatch];
+    int cnt[kBatch];
+    while (true) {
+      int na = 0;
+      for (int j = 0; j < nb; ++j) {
+        const int i = first + j;
+        const double mid = 0.5 * (a[i] + b[i]);
+        if (mid <= a[i] || mid >= b[i]) continue;
+        idx[na] = i;
+        x[na++] = mid;
+      }
+      if (na == 0) break;
+      for (int k = 0; k < na; ++k) {
+        q[k] = d[0] - x[k];
+        if (std::fabs(q[k]) < pivmin) q[k] = -pivmin;
+        cnt[k] = q[k] < 0.0;
+      }
+      for (int r = 1; r < n; ++r) {
+        const double dr = d[r], er = e2[r - 1];
+        for (int k = 0; k < na; ++k) {
+          double v = dr - x[k] - er / q[k];
+          if (std::fabs(v) < pivmin) v = -pivmin;
+          q[k] = v;
+          cnt[k] += v < 0.0;
+        }
+      }
+      for (int k = 0; k < na; ++k) {
+        const int i = idx[k];
+        if (cnt[k] > i)
+          b[i] = x[k];
+        else
+          a[i] = x[k];
+      }
+    }
+    for (int j = 0; j < nb; ++j) evals[first + j] = b[first + j];
   }
 }
 

@@ -6,6 +6,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 
 #include "diagham_host.hpp"
@@ -13,7 +16,22 @@
 namespace dhh {
 
 namespace {
+// n! in 113-bit precision, each entry the running product of the previous one (the same operations,
+// and so the same bits, as multiplying 2 .. n in turn); tabulated below kFactTable
+constexpr int kFactTable = 1700;  // 1700! < 1e4800, inside the __float128 range
+const std::vector<__float128>& FactTable() {
+  static const std::vector<__float128> table = [] {
+    std::vector<__float128> t(kFactTable);
+    t[0] = 1;
+    if (kFactTable > 1) t[1] = 1;
+    for (int i = 2; i < kFactTable; ++i) t[i] = t[i - 1] * i;
+    return t;
+  }();
+  return table;
+}
+
 __float128 fact128(int n) {
+  if (n < kFactTable) return FactTable()[n < 0 ? 0 : n];
   __float128 r = 1;
   for (int i = 2; i <= n; ++i) r *= i;
   return r;
@@ -48,15 +66,29 @@ __float128 binom128(int n, int k) {
 }
 }  // namespace
 
+// The table of (j1, j2) is computed once per process and shared: every Hamiltonian of a sector
+// loop over one sphere builds the same one.
 ClebschGordanCoefficients::ClebschGordanCoefficients(int j1, int j2) : J1(j1), J2(j2) {
+  static std::mutex mutex;
+  static std::map<std::pair<int, int>, std::shared_ptr<const std::vector<double>>> cache;
+  {
+    std::lock_guard<std::mutex> lock(mutex);
+    auto it = cache.find({j1, j2});
+    if (it != cache.end()) {
+      table = it->second;
+      return;
+    }
+  }
   int nj = j1 + j2 + 1;
-  table.assign((size_t)(j1 + 1) * (j2 + 1) * nj, 0.0);
+  auto t = std::make_shared<std::vector<double>>((size_t)(j1 + 1) * (j2 + 1) * nj, 0.0);
   for (int p1 = 0; p1 <= j1; ++p1)
     for (int p2 = 0; p2 <= j2; ++p2) {
       int m1 = 2 * p1 - j1, m2 = 2 * p2 - j2;
-      for (int j = std::abs(j1 - j2); j <= j1 + j2; j += 2)
-        table[((size_t)p1 * (j2 + 1) + p2) * nj + j] = racah(j1, m1, j2, m2, j);
+      for (int j = std::max(std::abs(j1 - j2), std::abs(m1 + m2)); j <= j1 + j2; j += 2)
+        (*t)[((size_t)p1 * (j2 + 1) + p2) * nj + j] = racah(j1, m1, j2, m2, j);
     }
+  std::lock_guard<std::mutex> lock(mutex);
+  table = cache.emplace(std::make_pair(j1, j2), std::move(t)).first->second;
 }
 
 double ClebschGordanCoefficients::Coefficient(int j1, int j2, int m1, int m2, int j) {
@@ -66,7 +98,7 @@ double ClebschGordanCoefficients::Coefficient(int j1, int j2, int m1, int m2, in
 double ClebschGordanCoefficients::GetCoefficient(int m1, int m2, int j) const {
   if (m1 < -J1 || m1 > J1 || m2 < -J2 || m2 > J2 || j < 0 || j > J1 + J2) return 0.0;
   int p1 = (J1 + m1) >> 1, p2 = (J2 + m2) >> 1;
-  return table[((size_t)p1 * (J2 + 1) + p2) * (J1 + J2 + 1) + j];
+  return (*table)[((size_t)p1 * (J2 + 1) + p2) * (J1 + J2 + 1) + j];
 }
 
 std::vector<double> EvaluatePseudopotentials(int nbrFlux) {

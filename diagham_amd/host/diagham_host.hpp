@@ -33,7 +33,7 @@ class ClebschGordanCoefficients {
 
  private:
   int J1, J2;
-  std::vector<double> table;  // [(m1pos * (J2+1) + m2pos) * (J1+J2+1) + j]
+  std::shared_ptr<const std::vector<double>> table;  // [(m1pos * (J2+1) + m2pos) * (J1+J2+1) + j]
 };
 
 // Lowest-Landau-level Coulomb pseudopotentials V_0..V_nbrFlux on the sphere in the units of

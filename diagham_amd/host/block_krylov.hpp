@@ -140,7 +140,8 @@ class SegmentedBlockLanczos {
         B(dims.size()),
         DeadCols(dims.size()),
         Previous(dims.size()),
-        LastCheck(dims.size(), -1) {
+        LastCheck(dims.size(), -1),
+        LowestHint(dims.size()) {
     Chunks.push_back({storage, 0, maxBlocks + 1});
   }
 
@@ -317,7 +318,7 @@ class SegmentedBlockLanczos {
     }
     for (int b = 0; b < m; ++b)
       if (DeadCols[s][b][0]) d[b] = 2.0 * bound + 1.0;
-    if (LowestHint.size() != Dims.size()) LowestHint.assign(Dims.size(), std::vector<double>());
+    // per-segment entries only: the sector solves test their segments in parallel
     TridiagonalLowest(d, e, (int)std::min<int64_t>(count, Live(s)), ev, &LowestHint[s]);
     LowestHint[s] = ev;
   }

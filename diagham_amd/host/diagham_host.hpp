@@ -499,8 +499,8 @@ inline int ExpectedKrylovBlocks(int64_t dim, int k, int nbrEigenvalue, int maxIt
 }
 
 // Waves of a batched sector solve.  Sectors are taken largest first and admitted to a wave while its
-// expected Krylov storage, (expected blocks + 2) x rows x kr doubles plus the basis words, fits the
-// smaller of memoryBytes and the device's free memory less 2 GiB.  A wave's storage starts small
+// Krylov storage with room to grow, (1.5 x expected blocks + 2) x rows x kr doubles plus the basis
+// words, fits the smaller of memoryBytes and the device's free memory less 2 GiB.  A wave's storage starts small
 // and grows as the run needs it (SegmentedBlockLanczos::SetGrowth); sectors stopped because it
 // could not grow go back to the queue expecting 1.5 times the blocks they reached, and after each
 // wave the expectation follows the blocks its sectors used.  solveWave(sectors, blocks) fills out[s]
@@ -519,17 +519,25 @@ int RunSectorWaves(const std::vector<int64_t>& dims, int k, int kr, int nbrEigen
     const size_t avail = freeBytes(), spare = (size_t)2 << 30;
     const size_t budget = std::min(memoryBytes, avail > spare ? avail - spare : (size_t)0);
     std::vector<int> wave, rest;
-    int blocks = 0;
+    int blocks = 0, room = 0;
     int64_t rows = 0;
     for (int s : queue) {
-      const int b = std::max(blocks, std::min(expect[s], (int)((dims[s] + k - 1) / k + 1)));
-      const size_t need = sizeof(double) * ((size_t)(b + 2) * (size_t)(rows + dims[s]) * kr + (size_t)(rows + dims[s]));
+      const int cap = (int)std::min<int64_t>((dims[s] + k - 1) / k + 1, maxIter / k + 1);
+      const int b = std::max(blocks, std::min(expect[s], cap));
+      // admitted with room for half as many blocks again: the storage of a wave is shared by its
+      // sectors (block-major), so when one sector outgrows a full device every sector still running
+      // in that wave stops and starts over later; with the room, growth past the expectation is
+      // the rule and restarts the exception (N=10 2S=27 sectors need 0.5-1.1 x the 500 blocks
+      // expected for 10 eigenvalues at k = 1)
+      const int r = std::max(room, std::min(b + (b + 1) / 2, std::max(b, cap)));
+      const size_t need = sizeof(double) * ((size_t)(r + 2) * (size_t)(rows + dims[s]) * kr + (size_t)(rows + dims[s]));
       if (!wave.empty() && need > budget) {
         rest.push_back(s);
         continue;
       }
       wave.push_back(s);
       blocks = b;
+      room = r;
       rows += dims[s];
     }
     if (std::getenv("DHH_VERBOSE"))

@@ -70,3 +70,15 @@ def test_waves_follow_the_used_blocks(waves, tmp_path, k):
     assert np.all(r[:, 0] == 1)
     assert np.all(r[:, 1] <= 2)
     assert w <= 12
+
+
+def test_growth_within_the_room_needs_no_restart(waves, tmp_path):
+    """Sectors that need 1.3 times the 500 blocks expected for 10 eigenvalues converge in their first
+    wave: a wave is admitted with room for 1.5 times its expected storage, so it holds six of the nine
+    sectors rather than all nine at the expectation (which would stop all nine at 499 blocks)."""
+    dims = [2000] * 9
+    need = [650] * 9
+    free = 8 * (752 * 12000 + 12000)
+    w, r = run(waves, tmp_path, 1, 10, free, dims, need)
+    assert np.all(r[:, 0] == 1) and np.all(r[:, 1] == 1), r
+    assert w == 2

@@ -103,6 +103,7 @@ HOST_SIGNATURES = {
     "dhh_last_error": (c_char_p, []),
     "dhh_last_solve_waves": (c_int, []),
     "dhh_sectors_launch": (c_int, [c_int, c_int, c_int]),
+    "dhh_sectors_alone_rows": (c_int, [c_int64]),
     "dhh_sectors_autotune": (c_int, [c_void_p, c_int, c_int, P_I, c_int, P_D, c_int, P_I, c_int, c_double, c_int,
                                      c_uint64, P_D, P_I]),
     "dhh_arch_create": (c_int, [PP, c_int]),

@@ -356,6 +356,13 @@ struct SectorLaunch {
   int lanesPerRow = 0, blockSize = 256, rowMode = DH_INDEX_RANK;
 };
 
+// Rows from which a sector of a batched sphere solve runs alone, through the single-space solver
+// (0: every sector in the waves).  Such a sector fills the device by itself, so batching saves no
+// per-step overhead on it, and in a wave its vectors compete for the caches with its neighbours':
+// the all-sector product of N=12 2S=33 makes 4.7e11 elements/s against 5.6e11 for the Lz = 0
+// sector alone, and the 133 sectors took 235 s batched against 198 s one after the other.
+constexpr int64_t kSectorAloneRows = 2000000;
+
 // free device memory of an architecture (0 if it cannot be read)
 inline size_t DeviceFreeBytes(GPUArchitecture* arch) {
   size_t freeBytes = 0, totalBytes = 0;
@@ -572,11 +579,14 @@ class MultiSectorBlockLanczos {
   std::vector<SectorResult> Solve(uint64_t seed, double* seconds = nullptr);
   int NbrWaves() const { return Waves; }
   void SetLaunch(const SectorLaunch& launch) { Launch = launch; }
+  // rows from which a sector runs alone (kSectorAloneRows by default; 0: none)
+  void SetAloneRows(int64_t rows) { AloneRows = rows; }
 
  private:
   // expectedBlocks: the Krylov blocks the wave planner expects the sectors to need
   void SolveWave(const std::vector<int>& sectors, uint64_t seed, std::vector<SectorResult>& out, int expectedBlocks);
   SectorLaunch Launch;
+  int64_t AloneRows = kSectorAloneRows;
   GPUArchitecture* Arch;
   int NbrFermions, LzMax, NbrEigenvalue, BlockSize, MaxIter;
   double Precision;

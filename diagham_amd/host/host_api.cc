@@ -57,6 +57,7 @@ struct dhh_problem {
 static thread_local std::string g_error;
 static thread_local int g_lastWaves = 0;
 static thread_local SectorLaunch g_sectorLaunch;  // dhh_sectors_launch
+static thread_local int64_t g_aloneRows = kSectorAloneRows;  // dhh_sectors_alone_rows
 
 template <typename F>
 static int guarded(F&& f) {
@@ -90,6 +91,15 @@ extern "C" int dhh_sectors_launch(int lanesPerRow, int blockSize, int rowMode) {
   g_sectorLaunch.lanesPerRow = lanesPerRow;
   g_sectorLaunch.blockSize = blockSize;
   g_sectorLaunch.rowMode = rowMode;
+  return DH_OK;
+}
+
+extern "C" int dhh_sectors_alone_rows(int64_t rows) {
+  if (rows < 0) {
+    g_error = "alone rows: 0 (every sector batched) or a positive row count";
+    return DH_E_ARG;
+  }
+  g_aloneRows = rows;
   return DH_OK;
 }
 
@@ -409,6 +419,7 @@ extern "C" int dhh_sectors_solve(dhh_arch* a, int n, int lzMax, const int* total
       MultiSectorBlockLanczos solver(a->arch.get(), n, lzMax, lz, v, nbrEigen, blockSize, maxIter, precision,
                                      (size_t)(memoryGB * 1073741824.0));
       solver.SetLaunch(g_sectorLaunch);
+      solver.SetAloneRows(g_aloneRows);
       g_lastWaves = 0;
       std::vector<SectorResult> r = solver.Solve(seed);
       g_lastWaves = solver.NbrWaves();
@@ -524,6 +535,7 @@ extern "C" int dhh_sectors_autotune(dhh_arch* a, int n, int lzMax, const int* to
       // time to the eigenvalues of the calibration sectors with that shape
       MultiSectorBlockLanczos solver(arch, n, lzMax, lz, v, nbrEigen, k, maxIter, 1e-14, 0);
       solver.SetLaunch(shape);
+      solver.SetAloneRows(g_aloneRows);
       double seconds = 0.0;
       std::vector<SectorResult> res = solver.Solve(seed, &seconds);
       int products = 0;

@@ -110,6 +110,24 @@ std::vector<SectorResult> MultiSectorBlockLanczos::Solve(uint64_t seed, double* 
       r.converged = true;
     }
   }
+  // sectors of at least AloneRows rows one at a time through the single-space solver (each fills
+  // the device alone), largest first
+  std::vector<int> batchSectors, aloneSectors;
+  for (int s : lanczosSectors) (AloneRows > 0 && out[s].dimension >= AloneRows ? aloneSectors : batchSectors).push_back(s);
+  std::stable_sort(aloneSectors.begin(), aloneSectors.end(),
+                   [&](int a, int b) { return out[a].dimension > out[b].dimension; });
+  for (int s : aloneSectors) {
+    FermionOnSphere space(Arch, NbrFermions, TotalLz[s], LzMax);
+    ParticleOnSphereGenericHamiltonian h(&space, NbrFermions, LzMax, Pseudo, Arch);
+    FullReorthogonalizedBlockLanczosAlgorithm lanczos(Arch, &h, NbrEigenvalue, BlockSize, MaxIter, Precision);
+    lanczos.InitializeLanczosAlgorithm(seed);
+    LanczosResult r = lanczos.Solve();
+    out[s].eigenvalues = r.eigenvalues;
+    out[s].products = r.iterations;
+    out[s].blocks = r.blockIterations;
+    out[s].converged = r.converged;
+  }
+  lanczosSectors = batchSectors;
   // the rest in waves whose expected Krylov storage fits the device (RunSectorWaves)
   std::vector<int64_t> dims;
   for (int s : lanczosSectors) dims.push_back(out[s].dimension);

@@ -110,6 +110,12 @@ int dhh_autotune(dhh_problem* problem, const int* ks, int nbr_ks, int repeats, d
  * restores the solvers' built-in choice.  No counterpart in the reference. */
 int dhh_sectors_launch(int lanes_per_row, int block_size, int row_mode);
 
+/* Batched sphere sector solves that this thread runs next: sectors of at least `rows` states are
+ * solved one at a time by the single-space solver (they fill the device alone, and batching them
+ * costs cache locality), the smaller ones in waves; 0 batches every sector.  Default 2,000,000.
+ * No counterpart in the reference. */
+int dhh_sectors_alone_rows(int64_t rows);
+
 /* Autotuner of the batched sphere sector solve (no counterpart in the reference).  Calibration
  * sectors: every round(1 / fraction)-th sector by size among those with more than 256 states.  For
  * every block size ks[i]: the launch shape (lanes per row in {1, 2, 4} x workgroup in {256, 1024} x

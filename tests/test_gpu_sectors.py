@@ -272,3 +272,31 @@ def test_sector_autotune_picks_a_shape_that_keeps_the_eigenvalues(arch):
         capi.sectors_launch(0)
     assert np.all(tuned["converged"])
     assert np.max(np.abs(tuned["eigenvalues"] - ref["eigenvalues"])) < 1e-10
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_large_sectors_run_alone_in_a_batched_solve(arch, golden, k):
+    """dhh_sectors_alone_rows: with the threshold at 5,000 states, the eight sectors of N=8 2S=21
+    (2Lz = 0, 4, .., 80) above it go through the single-space solver one at a time and the other
+    thirteen through the waves; every sector's eigenvalues equal those of the all-batched solve and
+    of the reference loop to 1e-10, and Lz = 0 reproduces the reference's printed values."""
+    from diagham_amd import capi
+    sectors = list(range(0, 81, 4))
+    assert capi.host().dhh_sectors_alone_rows(-1) == capi.DH_E_ARG
+    try:
+        assert capi.host().dhh_sectors_alone_rows(5000) == 0
+        ea, pa, ca, _ = solve(arch, 8, 21, sectors, 10, k, True)
+        assert capi.host().dhh_sectors_alone_rows(0) == 0
+        eb, pb, cb, _ = solve(arch, 8, 21, sectors, 10, k, True)
+    finally:
+        capi.host().dhh_sectors_alone_rows(2000000)
+    es, ps, cs, _ = solve(arch, 8, 21, sectors, 10, 1, False)
+    assert ca.all() and cb.all() and cs.all()
+    dims = [O.lib().or_fs_dimension(8, 21, lz) for lz in sectors]
+    assert sum(d >= 5000 for d in dims) == 8 and sum(d < 5000 for d in dims) == 13
+    for i, d in enumerate(dims):
+        m = min(10, d)
+        np.testing.assert_allclose(ea[i, :m], eb[i, :m], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(ea[i, :m], es[i, :m], rtol=0, atol=1e-10)
+    ref = np.array(golden["fermions_sphere_n8_2s21_lz0"]["eigenvalues"])
+    np.testing.assert_allclose(ea[0], ref, rtol=0, atol=1e-10)

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 
@@ -116,7 +117,11 @@ std::vector<SectorResult> MultiSectorBlockLanczos::Solve(uint64_t seed, double* 
   for (int s : lanczosSectors) (AloneRows > 0 && out[s].dimension >= AloneRows ? aloneSectors : batchSectors).push_back(s);
   std::stable_sort(aloneSectors.begin(), aloneSectors.end(),
                    [&](int a, int b) { return out[a].dimension > out[b].dimension; });
-  for (int s : aloneSectors) {
+  ProgressLog log;
+  for (size_t i = 0; i < aloneSectors.size(); ++i) {
+    const int s = aloneSectors[i];
+    if (log.Due())
+      std::fprintf(stderr, "[sectors] alone: sector %zu of %zu (2 Lz = %d)\n", i + 1, aloneSectors.size(), TotalLz[s]);
     FermionOnSphere space(Arch, NbrFermions, TotalLz[s], LzMax);
     ParticleOnSphereGenericHamiltonian h(&space, NbrFermions, LzMax, Pseudo, Arch);
     FullReorthogonalizedBlockLanczosAlgorithm lanczos(Arch, &h, NbrEigenvalue, BlockSize, MaxIter, Precision);

@@ -521,8 +521,7 @@ static int adjoint_pass(dh_hamiltonian* h, int holes, int64_t subsets, int c0, c
   if (lds > 150 * 1024) return dh_fail(c, DH_E_RANGE, "channel table exceeds LDS");
   auto kern = sphere_adjoint_kernel<NC, SAME>;
   if (lds > 64 * 1024) DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int perCU = 1;
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 256, lds);
+  const int perCU = dh_blocks_per_cu(c, kern, 256, lds);
   int nb = (int)std::max<int64_t>(1, std::min<int64_t>((subsets + 255) / 256, (int64_t)c->nbrCU * std::max(perCU, 1)));
   double* scratch;
   int rc = dh_scratch(c, sizeof(double) * ((size_t)nb * NC + NC), &scratch);
@@ -630,8 +629,7 @@ static int csr_launch(dh_hamiltonian* h, const double* x, double* y, int add, in
   dh_context* c = h->ctx;
   constexpr int LPR = K >= 8 ? 16 : 32;
   auto kern = csr_hv_kernel<K, LPR>;
-  int perCU = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, 512, 0) != hipSuccess || perCU < 1) perCU = 1;
+  const int perCU = dh_blocks_per_cu(c, kern, 512, 0);
   const int64_t tiles = (count + 512 / LPR - 1) / (512 / LPR);
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(512), 0, c->stream, h->d_csrRow, h->d_csrCol, h->d_csrVal,

@@ -465,9 +465,7 @@ static int sphere_rank_tables(const dh_hamiltonian* h) {
 
 template <typename Kern>
 static int launch_persistent(dh_context* c, Kern kern, int blockSize, size_t lds, int64_t nbrTiles, dim3& grid) {
-  int perCU = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, blockSize, lds) != hipSuccess || perCU <= 0)
-    perCU = 1;
+  const int perCU = dh_blocks_per_cu(c, kern, blockSize, lds);
   int64_t g = std::min<int64_t>(nbrTiles, (int64_t)c->nbrCU * perCU);
   grid = dim3((unsigned)std::max<int64_t>(g, 1));
   return DH_OK;

@@ -5,11 +5,13 @@
 
 #include <algorithm>
 #include <atomic>
+#include <map>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <type_traits>
 #include <unordered_set>
 #include <vector>
@@ -377,6 +379,26 @@ struct DotsGroup {
 // whether the k = 1 dot products may read rows in pairs
 inline bool dh_pairs_aligned(const double* V, int64_t blockStride, const double* W) {
   return blockStride % 2 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 && reinterpret_cast<uintptr_t>(W) % 16 == 0;
+}
+
+// Workgroups of `kern` that one CU holds at blockSize threads and `lds` bytes of dynamic LDS (at
+// least one).  Asked of the runtime once per kernel, shape and device and then remembered: the
+// query takes about 11 us, as long as a small sector's whole product.
+template <typename Kern>
+inline int dh_blocks_per_cu(const dh_context* c, Kern kern, int blockSize, size_t lds) {
+  static std::mutex mutex;
+  static std::map<std::tuple<const void*, int, size_t, int>, int> cache;
+  const auto key = std::make_tuple(reinterpret_cast<const void*>(kern), blockSize, lds, c->device);
+  {
+    std::lock_guard<std::mutex> lock(mutex);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, blockSize, lds) != hipSuccess || perCU < 1) perCU = 1;
+  std::lock_guard<std::mutex> lock(mutex);
+  cache[key] = perCU;
+  return perCU;
 }
 
 // XCD-aware block remap: MI355X hands consecutive workgroups to its 8 XCDs round robin;

@@ -588,9 +588,7 @@ static int lattice_launch(dh_sector_set* S, const double* x, double* y, int add,
   auto launch = [&](auto kern) -> int {
     if (lds > 64 * 1024)
       DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int perCU = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, S->blockSize, lds) != hipSuccess || perCU < 1)
-      perCU = 1;
+    const int perCU = dh_blocks_per_cu(c, kern, S->blockSize, lds);
     int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(S->blockSize), lds, c->stream, a, x, y, add);
     DH_HIP(c, hipGetLastError());

@@ -179,9 +179,7 @@ static int nbody_launch(dh_hamiltonian* h, const double* x, double* y, int add, 
   auto kern = nbody3_hv_kernel<K, LPR>;
   if (lds > 64 * 1024)
     DH_HIP(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int perCU = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, h->blockSize, lds) != hipSuccess || perCU < 1)
-    perCU = 1;
+  const int perCU = dh_blocks_per_cu(c, kern, h->blockSize, lds);
   const int64_t rowsPerBlock = h->blockSize / LPR;
   const int64_t tiles = (count + rowsPerBlock - 1) / rowsPerBlock;
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));

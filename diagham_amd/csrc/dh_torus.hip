@@ -387,9 +387,7 @@ static int torus_launch(dh_sector_set* S, const double* x, double* y, int add, c
                      4 * (size_t)(S->nbrSums + 1) + 4 * (size_t)S->nbrPairs;
   if (lds > 64 * 1024) return dh_fail(c, DH_E_RANGE, "staged tables exceed LDS");
   auto kern = torus_translation_hv_kernel<K, LPR>;
-  int perCU = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, S->blockSize, lds) != hipSuccess || perCU < 1)
-    perCU = 1;
+  const int perCU = dh_blocks_per_cu(c, kern, S->blockSize, lds);
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->nbrCU * perCU));
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(S->blockSize), lds, c->stream, a, x, y, add);
   DH_HIP(c, hipGetLastError());

@@ -666,6 +666,40 @@ class SegmentedBlockLanczos {
     BasisDots(nb - nb1, nb1, W, mask.data(), na, C.data());
     fromDevice(C.data(), nb1);
     Coefficients(CT, C.data(), nb1, all, out, true, ref);
+    if (local && K == 1 && W == Block(nb)) {
+      // A step's new vector follows the basis in the storage (W = X_nb), so the dots of the second
+      // round and the squared norm of the first round's result come from one pass over blocks
+      // 0 .. nb (the last against itself): one synchronisation less per step.  The dots are those
+      // of W before its normalisation W / |W|; the coefficients follow as (V^H W) / |W|.  Blocks
+      // of k > 1 keep the separate passes: (V^H W) R1^-1 loses accuracy with the conditioning of
+      // R1 when the block's columns are nearly dependent, which the deficiency tests must see.
+      BasisAxpy(nb - nb1, nb1, W, mask.data(), na, C.data());
+      std::vector<double> Cx(na * (size_t)(nb + 1) * KK);
+      BasisDots(0, nb + 1, W, mask.data(), na, Cx.data());
+      for (size_t a = 0; a < na; ++a) {
+        const double* src = &Cx[a * (size_t)(nb + 1) * KK];
+        std::copy(src, src + (size_t)nb * KK, &C[a * (size_t)nb * KK]);
+        std::copy(src + (size_t)nb * KK, src + (size_t)(nb + 1) * KK, &G[a * KK]);
+      }
+      std::vector<std::vector<T>> inv1;
+      Factor(G.data(), all, out, ref, 1e-24, M1.data(), &inv1);
+      for (auto& r : ref) std::fill(r.begin(), r.end(), 0.0);
+      Ops->Rmul(W, mask.data(), M1.data());
+      fromDevice(C.data(), nb);
+      std::vector<T> blk((size_t)kk);
+      for (size_t a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) {
+          T* c = &CT[(a * nb + b) * kk];
+          std::copy(c, c + kk, blk.begin());
+          const std::vector<T> scaled = Mul(K, blk, inv1[a]);
+          std::copy(scaled.begin(), scaled.end(), c);
+        }
+      Coefficients(CT, C.data(), nb, all, out, false, ref);
+      BasisUpdate(0, nb, W, mask.data(), na, C.data(), G.data());
+      Factor(G.data(), all, out, ref, 1e-16, M2.data(), nullptr);
+      Ops->Rmul(W, mask.data(), M2.data());
+      return;
+    }
     BasisUpdate(nb - nb1, nb1, W, mask.data(), na, C.data(), G.data());
     // noise left by one projection is ~1e-15 of the column
     Factor(G.data(), all, out, ref, 1e-24, M1.data(), nullptr);

@@ -141,6 +141,7 @@ class SegmentedBlockLanczos {
         DeadCols(dims.size()),
         Previous(dims.size()),
         LastCheck(dims.size(), -1),
+        LastChange(dims.size(), kFarChange),
         LowestHint(dims.size()) {
     Chunks.push_back({storage, 0, maxBlocks + 1});
   }
@@ -206,6 +207,7 @@ class SegmentedBlockLanczos {
       Previous[s].clear();
       LastCheck[s] = -1;
     }
+    LastChange.assign(Dims.size(), kFarChange);
     LowestHint.assign(Dims.size(), std::vector<double>());
     StepCount = 0;
     LaunchedStep = -1;
@@ -324,14 +326,20 @@ class SegmentedBlockLanczos {
   }
 
   // Block Lanczos convergence checks cost a band reduction; past the first ones they are spaced
-  // so that at most ~10% extra blocks run after convergence.
+  // so that at most ~10% extra blocks run after convergence.  The one-vector recurrence tests every
+  // step once the wanted values moved by at most kNearChange of themselves at the last test, and
+  // every fourth step before: a test (eleven bisections over the tridiagonal matrix) costs more than
+  // a small sector's whole step, and a value still moving by 1e-8 of itself needs more than one
+  // step to settle within the 1e-14 of the test, so at most three steps run past convergence.
+  static constexpr double kNearChange = 1e-8, kFarChange = 1e300;
   bool CheckDue(int s, int nbrEigen) const {
     const int m = NbrBlocks(s);
     if (Exhausted(s)) return true;
     // a second test on the same matrix would compare the Ritz values with themselves
     if (LastCheck[s] == m) return false;
     if ((int64_t)m * K < nbrEigen + 2) return false;
-    return K == 1 || LastCheck[s] < 0 || m - LastCheck[s] >= std::max(1, m / 10);
+    if (K == 1) return LastCheck[s] < 0 || LastChange[s] <= kNearChange || m - LastCheck[s] >= 4;
+    return LastCheck[s] < 0 || m - LastCheck[s] >= std::max(1, m / 10);
   }
 
   // strong test of FullReorthogonalizedLanczosAlgorithm.cc:325: every wanted eigenvalue moved by
@@ -353,6 +361,16 @@ class SegmentedBlockLanczos {
       return true;
     }
     bool ok = !Previous[s].empty() && (int)ev.size() > nbrEigen && (int)Previous[s].size() == want;
+    // the largest relative move of a wanted value since the previous test (values within 3
+    // MACHINE_PRECISION of zero now and before count as still), for the test cadence of CheckDue
+    double change = kFarChange;
+    if (!Previous[s].empty() && (int)Previous[s].size() == want) {
+      change = 0.0;
+      for (int e = 0; e < want; ++e)
+        if (std::fabs(current[e]) > 3 * kMachinePrecision || std::fabs(Previous[s][e]) > 3 * kMachinePrecision)
+          change = std::max(change, std::fabs(current[e] - Previous[s][e]) / std::max(std::fabs(current[e]), 1e-300));
+    }
+    LastChange[s] = change;
     if (ok)
       for (int e = 0; e < want; ++e)
         if (std::fabs(current[e] - Previous[s][e]) > precision * std::fabs(current[e]) &&
@@ -389,6 +407,7 @@ class SegmentedBlockLanczos {
       }
       for (int b = 0; b < nd; ++b) put(DeadCols[s][b].data(), (size_t)K);
       put(Previous[s].data(), sizeof(double) * (size_t)np);
+      put(&LastChange[s], sizeof(double));
     }
     if (!o) throw std::runtime_error("cannot write the Lanczos state");
   }
@@ -428,6 +447,7 @@ class SegmentedBlockLanczos {
       for (int b = 0; b < nd; ++b) get(DeadCols[s][b].data(), (size_t)K);
       Previous[s].assign((size_t)np, 0.0);
       get(Previous[s].data(), sizeof(double) * (size_t)np);
+      get(&LastChange[s], sizeof(double));
       LastCheck[s] = last;
     }
     StepCount = steps;
@@ -772,6 +792,7 @@ class SegmentedBlockLanczos {
   std::vector<std::vector<std::vector<char>>> DeadCols;  // per segment, per block X_b, per column
   std::vector<std::vector<double>> Previous;
   std::vector<int> LastCheck;
+  std::vector<double> LastChange;  // per segment: the largest relative move at the last test
   mutable std::vector<std::vector<double>> LowestHint;  // per segment: the last LowestEigen result
 };
 

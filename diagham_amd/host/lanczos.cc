@@ -353,7 +353,7 @@ void FullReorthogonalizedBlockLanczosAlgorithm::RunLanczosAlgorithm(int nbrIter)
 // lanczos.dat: 8-byte tag, int64 dimension, int32 wanted eigenvalues, double precision, then the
 // engine's state (SaveState).  The reference's lanczos.dat (WriteState, :472) holds the
 // tridiagonal matrix of its three-vector recurrence; this one holds the block recurrence.
-static const char kStateTag[8] = {'D', 'H', 'G', 'P', 'U', 'L', 'Z', '1'};
+static const char kStateTag[8] = {'D', 'H', 'G', 'P', 'U', 'L', 'Z', '2'};  // 2: with the test cadence
 
 static std::string VectorFileName(const std::string& dir, int64_t index) {
   return dir + "/vector." + std::to_string(index);

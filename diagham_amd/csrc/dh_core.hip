@@ -373,8 +373,31 @@ __global__ void __launch_bounds__(256) dh_block_axpy_kernel(const double* __rest
   }
 }
 
+// k = 1 with at most 8 basis vectors (the local round of a one-vector step): the coefficients go by
+// value in the kernel arguments, with no upload; the same products and order as the kernel above
+struct SmallCoefficients {
+  double c[8];
+};
+__global__ void __launch_bounds__(256) dh_block_axpy_small_kernel(const double* __restrict__ V, int nb,
+                                                                  double* __restrict__ W, int64_t dim,
+                                                                  SmallCoefficients C) {
+  const int64_t row = (int64_t)dh_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (row >= dim) return;
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) acc += dh_stream_load(V + (int64_t)b * dim + row) * C.c[b];
+  W[row] += acc;
+}
+
 template <int K>
 static int launch_axpy(dh_context* c, const double* V, int nb, double* W, int64_t dim, const double* h_c) {
+  if (K == 1 && nb <= 8) {
+    SmallCoefficients C;
+    for (int b = 0; b < 8; ++b) C.c[b] = b < nb ? h_c[b] : 0.0;
+    hipLaunchKernelGGL(dh_block_axpy_small_kernel, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, c->stream, V,
+                       nb, W, dim, C);
+    DH_HIP(c, hipGetLastError());
+    return DH_OK;
+  }
   const int KK = K * K;
   size_t bytes = sizeof(double) * (size_t)nb * KK;
   double* scratch;
@@ -423,8 +446,22 @@ __global__ void __launch_bounds__(256) dh_block_rmul_kernel(double* __restrict__
   for (int q = 0; q < K; ++q) W[row * K + q] = o[q];
 }
 
+__global__ void __launch_bounds__(256) dh_block_scale_kernel(double* __restrict__ W, int64_t dim, double m) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= dim) return;
+  double s = 0.0;  // the accumulation of dh_block_rmul_kernel<1>, to the bit
+  s += W[row] * m;
+  W[row] = s;
+}
+
 template <int K>
 static int launch_rmul(dh_context* c, double* W, int64_t dim, const double* h_m) {
+  if (K == 1) {  // a scale: the factor goes by value (W * m, the same product as below)
+    hipLaunchKernelGGL(dh_block_scale_kernel, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, c->stream, W, dim,
+                       h_m[0]);
+    DH_HIP(c, hipGetLastError());
+    return DH_OK;
+  }
   size_t bytes = sizeof(double) * K * K;
   double* scratch;
   int rc = dh_scratch(c, bytes, &scratch);

@@ -82,6 +82,7 @@ GPU_SIGNATURES = {
     "dh_sectors_block_dots": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
     "dh_sectors_block_axpy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_char_p, P_D]),
     "dh_sectors_block_rmul": (c_int, [c_void_p, c_void_p, c_int, c_char_p, P_D]),
+    "dh_momentum_sector_dimension": (c_int64, [c_int, c_int, c_int, c_int, c_int]),
     "dh_momentum_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, c_int, P_I, P_I, c_int]),
     "dh_sector_set_lattice_factors": (c_int, [c_void_p, c_int, P_I, P_I, P_D, c_double]),
     "dh_torus_translation_sector_set_create": (c_int, [c_void_p, PP, c_int, c_int, P_I, P_I, c_int]),

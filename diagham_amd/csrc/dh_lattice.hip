@@ -68,6 +68,17 @@ __global__ void lattice_unrank_kernel(uint64_t* __restrict__ states, int64_t dim
   states[i] = state;
 }
 
+// FermionOnSquareLatticeMomentumSpace::EvaluateHilbertSpaceDimension
+// (FQHE/src/HilbertSpace/FermionOnSquareLatticeMomentumSpace.cc:290): the number of N-subsets of
+// the nx * ny momenta with total momentum (kx, ky), from the class table; no device work.
+extern "C" int64_t dh_momentum_sector_dimension(int N, int nx, int ny, int kx, int ky) {
+  const int nOrb = nx * ny;
+  if (nx <= 0 || ny <= 0 || N <= 0 || N > nOrb || nOrb > 62 || kx < 0 || kx >= nx || ky < 0 || ky >= ny) return -1;
+  Classes cl{nx, ny};
+  const std::vector<int64_t> T = class_table(nOrb, N, cl);
+  return T[((size_t)nOrb * (N + 1) + N) * nOrb + (kx * ny + ky)];
+}
+
 extern "C" int dh_momentum_sector_set_create(dh_context* c, dh_sector_set** out, int N, int nx, int ny,
                                              const int* kx, const int* ky, int nbrSectors) {
   if (!c || !out || !kx || !ky || nbrSectors <= 0) return DH_E_ARG;

@@ -178,9 +178,12 @@ std::vector<SectorResult> LatticeSectorsBlockLanczos::Solve(uint64_t seed, doubl
   const int ns = (int)Kx.size();
   std::vector<SectorResult> out(ns);
   const int k = BlockSize;
-  // sector dimensions from a one-sector set each is wasteful; use a set of all and split
+  // sector dimensions: counted on the host for momentum sectors; the orbits of the torus with
+  // translations come from a set of all sectors
   std::vector<int64_t> dims(ns);
-  {
+  if (!TorusTranslations) {
+    for (int s = 0; s < ns; ++s) dims[s] = dh_momentum_sector_dimension(NbrFermions, NbrSiteX, NbrSiteY, Kx[s], Ky[s]);
+  } else {
     dh_sector_set* S = CreateSet(Kx, Ky);
     std::vector<int64_t> off(ns + 1);
     dh_sector_set_offsets(S, off.data());

@@ -254,6 +254,10 @@ int dh_sectors_block_update(dh_sector_set* set, const double* d_basis, int nbr_b
  * doubles, and dh_sectors_block_* take 2k as their k. */
 int dh_momentum_sector_set_create(dh_context* ctx, dh_sector_set** set, int nbr_fermions, int nbr_site_x,
                                   int nbr_site_y, const int* kx, const int* ky, int nbr_sectors);
+/* FermionOnSquareLatticeMomentumSpace::EvaluateHilbertSpaceDimension (FermionOnSquareLatticeMomentumSpace.cc:290):
+ * the number of states of the momentum sector (kx, ky); no device work; -1 for arguments outside
+ * the ranges above. */
+int64_t dh_momentum_sector_dimension(int nbr_fermions, int nbr_site_x, int nbr_site_y, int kx, int ky);
 /* Two-body factors of ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian
  * (FQHE/src/Hamiltonian/ParticleOnLatticeTimeReversalBreakingSingleBandHamiltonian.h:370):
  * momentum sum j = 0..nbr_sums-1 (nbr_sums = Nx*Ny) lists sum_pair_count[j] orbital pairs

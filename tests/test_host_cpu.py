@@ -65,6 +65,26 @@ def test_dimension_without_gpu(golden):
     assert capi.fermion_sphere_dimension(8, 1, 21) == 0  # wrong parity
 
 
+def test_momentum_sector_dimension_without_gpu():
+    """dh_momentum_sector_dimension against the oracle's basis of FermionOnSquareLatticeMomentumSpace
+    (4 fermions on 4 x 3 sites, every sector), the sum over the Brillouin zone, the sizes of the
+    30 sectors of N=10 on 6 x 5 sites (1,001,400 to 1,001,603 states) and its argument errors."""
+    from math import comb
+    import oracle_lib as O
+    from diagham_amd import capi
+    dim = capi.gpu().dh_momentum_sector_dimension
+    total = 0
+    for kx in range(4):
+        for ky in range(3):
+            d = dim(4, 4, 3, kx, ky)
+            assert d == O.MomentumFermions(4, 4, 3, kx, ky).dim, (kx, ky)
+            total += d
+    assert total == comb(12, 4)
+    big = [dim(10, 6, 5, kx, ky) for kx in range(6) for ky in range(5)]
+    assert sum(big) == comb(30, 10) == 30045015 and min(big) == 1001400 and max(big) == 1001603
+    assert dim(4, 4, 3, 4, 0) == -1 and dim(0, 4, 3, 0, 0) == -1 and dim(13, 4, 3, 0, 0) == -1
+
+
 @pytest.mark.parametrize("lz", [8, 21, 27, 33])
 def test_host_pseudopotentials_match_oracle(lz):
     from diagham_amd import capi

@@ -197,3 +197,18 @@ def test_host_checkerboard_band_and_factors_match_oracle():
         assert np.max(np.abs(F - F.conj().T)) < 1e-14  # Hermitian factor matrix per momentum sum
         pos += m
         fpos += m * m
+
+
+def test_sector_solve_settings_are_checked():
+    """dhh_sectors_launch and dhh_sectors_alone_rows refuse shapes and thresholds outside their
+    ranges with a message, and accept the reset values (no device needed)."""
+    from diagham_amd import capi
+    h = capi.host()
+    assert h.dhh_sectors_launch(3, 256, capi.INDEX_RANK) == capi.DH_E_ARG
+    assert b"lanes per row" in h.dhh_last_error()
+    assert h.dhh_sectors_launch(2, 100, capi.INDEX_RANK) == capi.DH_E_ARG
+    assert h.dhh_sectors_launch(0, 0, 0) == 0
+    assert h.dhh_sectors_alone_rows(-5) == capi.DH_E_ARG
+    assert b"alone rows" in h.dhh_last_error()
+    assert h.dhh_sectors_alone_rows(0) == 0
+    assert h.dhh_sectors_alone_rows(2000000) == 0

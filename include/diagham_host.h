@@ -113,7 +113,8 @@ int dhh_sectors_launch(int lanes_per_row, int block_size, int row_mode);
 /* Batched sphere sector solves that this thread runs next: sectors of at least `rows` states are
  * solved one at a time by the single-space solver (they fill the device alone, and batching them
  * costs cache locality), the smaller ones in waves; 0 batches every sector.  Default 2,000,000.
- * No counterpart in the reference. */
+ * memory_gb bounds the waves only: a sector solved alone grows its Krylov storage within the
+ * device's free memory, as the single-space solver does.  No counterpart in the reference. */
 int dhh_sectors_alone_rows(int64_t rows);
 
 /* Autotuner of the batched sphere sector solve (no counterpart in the reference).  Calibration

@@ -1,4 +1,6 @@
-"""The fermionic-sign identities the product kernels use, checked exhaustively on the CPU against the
+"""Index and sign identities the device kernels rely on, checked exhaustively on the CPU: the row
+ranges of the two-rows-per-lane dot products, and the fermionic-sign identities of the product
+kernels against the
 reference's form of the sign (the parity of the occupied orbitals at and above each operator,
 FermionOnSphere::AdAdAA, FQHE/src/HilbertSpace/FermionOnSphere.cc:468, and the AA / AdAd pair of
 FermionOnSquareLatticeMomentumSpace).
@@ -45,3 +47,20 @@ def test_negation_by_sign_bit_is_negation():
     c = np.concatenate([rng.standard_normal(1000), [0.0, -0.0, 1e-310, np.inf]])
     flipped = (c.view(np.uint64) ^ np.uint64(1 << 63)).view(np.float64)
     assert np.array_equal(flipped.view(np.uint64), (-c).view(np.uint64))
+
+
+def test_row_pair_ranges_cover_every_row_once():
+    """The k = 1 dot products with two rows per lane (dh_dots_rows_pairs): the aligned pairs
+    (2q, 2q + 1) for q in [(b + 1) / 2, e / 2), an odd first row b on lane 0 and an odd last row e - 1
+    on lane 1, cover every row of [b, e) exactly once, for every range of up to 40 rows."""
+    for b in range(0, 40):
+        for e in range(b, 41):
+            rows = []
+            pb, pe = (b + 1) >> 1, e >> 1
+            for q in range(pb, pe):
+                rows += [2 * q, 2 * q + 1]
+            if (b & 1) and b < e:
+                rows.append(b)
+            if (e & 1) and e - 1 >= b:
+                rows.append(e - 1)
+            assert sorted(rows) == list(range(b, e)), (b, e, rows)

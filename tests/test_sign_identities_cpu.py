@@ -1,7 +1,6 @@
 """Index and sign identities the device kernels rely on, checked exhaustively on the CPU: the row
 ranges of the two-rows-per-lane dot products, and the fermionic-sign identities of the product
-kernels against the
-reference's form of the sign (the parity of the occupied orbitals at and above each operator,
+kernels against the reference's form of the sign (the parity of the occupied orbitals at and above each operator,
 FermionOnSphere::AdAdAA, FQHE/src/HilbertSpace/FermionOnSphere.cc:468, and the AA / AdAd pair of
 FermionOnSquareLatticeMomentumSpace).
 
